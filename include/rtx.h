@@ -1,0 +1,167 @@
+/* librtx: MI355X-native BVH build, traversal and path tracing for the
+ * minsuk0519/ray-tracer hot path.  C ABI: plain pointers and sizes.
+ *
+ * Conventions
+ *  - Return 0 on success, a negative RTX_E_* code on failure; rtx_last_error()
+ *    gives the message (thread-local).  Mirrors the reference's bool + "Error : ..."
+ *    convention (e.g. src/bvh/bvh_traverse.cpp:187-241).
+ *  - Vertices use the reference Vertex layout: 6 floats (x,y,z,nx,ny,nz) per vertex
+ *    (src/bvh/bvh.hpp:32-41).  Triangles are 3 uint32 vertex indices (bvh.hpp:10-13).
+ *  - "_d" pointers are device (HBM) pointers; stream is a hipStream_t passed as void*
+ *    (NULL = default stream).  The library owns rtx_bvh handles; the caller owns ray,
+ *    hit and image buffers.  Calls on one handle are not re-entrant across threads.
+ */
+#ifndef RTX_H
+#define RTX_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RTX_OK = 0,
+    RTX_E_ARG = -1,      /* invalid argument (null pointer, zero count, bad index) */
+    RTX_E_HIP = -2,      /* HIP runtime error */
+    RTX_E_IO = -3,       /* file missing, truncated or unwritable */
+    RTX_E_VERSION = -4,  /* .bvh version mismatch */
+    RTX_E_LIMIT = -5,    /* size beyond a documented limit */
+    RTX_E_STATE = -6     /* object not ready (e.g. trace before load) */
+};
+
+typedef struct rtx_bvh rtx_bvh;
+
+/* 32-byte ray: replaces bvh::Ray (src/bvh/bvh_traverse.hpp:10-16; 48 B there). */
+typedef struct rtx_ray {
+    float org[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} rtx_ray;
+
+/* 16-byte closest hit: replaces bvh::Hit (bvh_traverse.hpp:18-26).  tri is the
+ * reference Hit.triIndex (index into the reordered triangle array), 0xFFFFFFFF on a
+ * miss (the reference's hit == false).  Normals are an optional separate output. */
+typedef struct rtx_hit {
+    float t, u, v;
+    uint32_t tri;
+} rtx_hit;
+
+typedef struct rtx_bvh_info {
+    uint32_t node_count;        /* reference BVHNode count (DFS order) */
+    uint32_t tri_count;         /* reordered triangles, spatial-split duplicates included */
+    uint32_t vert_count;
+    uint32_t input_tri_count;
+    uint32_t internal_count;
+    uint32_t leaf_count;
+    uint32_t max_depth;
+    uint32_t tri_index_size;    /* s_triIndexSize at the end of the build */
+    uint32_t sorted_tris_size;  /* s_sortedTrisSize at the end of the build */
+    uint32_t levels;
+    float scene_min[3];
+    float scene_max[3];
+} rtx_bvh_info;
+
+const char* rtx_last_error(void);
+int rtx_device_count(void);
+
+/* ---- build -------------------------------------------------------------------
+ * In-memory bake: replaces bvh::bakeBVH's pipeline (src/bvh/bvh_bake.cpp:180-194:
+ * sortTrisByMorton, buildCentroidCache, initRoot, bfsLoop, reorderNodes,
+ * reorderTriangles) on the GPU.  verts6/tris are host pointers. */
+int rtx_build(const float* verts6, uint32_t nv, const uint32_t* tris, uint32_t nt, rtx_bvh** out);
+/* Same, from device-resident inputs (verts6_d, tris_d stay owned by the caller). */
+int rtx_build_device(const float* verts6_d, uint32_t nv, const uint32_t* tris_d, uint32_t nt,
+                     void* stream, rtx_bvh** out);
+void rtx_bvh_free(rtx_bvh* bvh);
+int rtx_bvh_get_info(const rtx_bvh* bvh, rtx_bvh_info* info);
+/* Copy the tree to host in the reference's layout (BVHNode 48 B each, DFS order;
+ * Triangle 3 x uint32; original triangle id per reordered triangle).  Any pointer may
+ * be NULL.  Replaces the in-memory state writeBakedData serialises (bvh_IO.cpp:253-264). */
+int rtx_bvh_export(const rtx_bvh* bvh, void* nodes48, uint32_t* tris3, uint32_t* prims);
+/* Morton codes per input triangle and the sorted permutation (s_sortedTris[0..N)),
+ * from sortTrisByMorton (src/bvh/bvh_morton.cpp:40-159).  Either pointer may be NULL. */
+int rtx_bvh_morton(const rtx_bvh* bvh, uint64_t* codes, uint32_t* sorted);
+/* Node array before DFS renumbering (the s_nodes order of bfsLoop, bvh_bfs.cpp:72-134). */
+int rtx_bvh_export_bfs(const rtx_bvh* bvh, void* nodes48, uint32_t* count);
+
+/* ---- trace ------------------------------------------------------------------
+ * Closest hit for n rays: replaces bvh::trace (src/bvh/bvh_traverse.cpp:237-246,
+ * traverse :91-178).  rays_d/hits_d device pointers; normals_d optional (3 floats per
+ * ray, the interpolated shading normal of bvh_traverse.cpp:135-139). */
+int rtx_trace_closest(const rtx_bvh* bvh, const rtx_ray* rays_d, size_t n, rtx_hit* hits_d,
+                      float* normals_d, void* stream);
+/* Any hit (occlusion): occluded_d[i] = 1 iff trace(rays[i]).hit.  No reference
+ * counterpart (the reference only has closest hit); pinned by that identity. */
+int rtx_trace_any(const rtx_bvh* bvh, const rtx_ray* rays_d, size_t n, uint8_t* occluded_d, void* stream);
+/* Instrumented closest hit: counts_d[2*i] = internal-node visits, [2*i+1] = triangle
+ * tests of ray i (the per-ray work the roofline's algorithmic bytes are priced on). */
+int rtx_trace_counts(const rtx_bvh* bvh, const rtx_ray* rays_d, size_t n, uint32_t* counts_d, void* stream);
+/* Host-pointer convenience wrapper around rtx_trace_closest (copies in and out). */
+int rtx_trace_closest_host(const rtx_bvh* bvh, const rtx_ray* rays, size_t n, rtx_hit* hits, float* normals);
+
+/* ---- files and scenes ----------------------------------------------------------
+ * .bvh file: version-1 header + raw BVHNode/Triangle/Vertex arrays, bit-compatible
+ * with writeBakedData (src/bvh/bvh_IO.cpp:236-275) and loadBVH (bvh_traverse.cpp:182-235). */
+int rtx_save(const rtx_bvh* bvh, const char* path);
+int rtx_load(const char* path, rtx_bvh** out);
+/* Parse a .scn file's geometry lines (sphere/box; mesh lines are reported and skipped:
+ * assimp import is out of scope) into host arrays, as readScene + initGeos do
+ * (src/bvh/bvh_IO.cpp:72-234, bvh_geo.cpp:22-192).  Free with rtx_free_host. */
+int rtx_scene_load(const char* scn_path, float** verts6, uint32_t* nv, uint32_t** tris, uint32_t* nt);
+void rtx_free_host(void* p);
+/* Drop-in for bvh::bakeBVH(scenePath) (bvh_bake.cpp:180-194): scene -> GPU build ->
+ * <scene>.bvh next to it. */
+int rtx_bake_scene(const char* scn_path);
+/* Reference geometry generators addSphere / addBox (src/bvh/bvh_geo.cpp:22-192).
+ * Pass NULL outputs to query counts.  quat is (x, y, z, w). */
+int rtx_geo_sphere(const float center[3], float radius, int rings, int sectors,
+                   float* verts6, uint32_t* nv, uint32_t* tris, uint32_t* nt);
+int rtx_geo_box(const float center[3], const float half[3], const float quat_xyzw[4],
+                float* verts6, uint32_t* nv, uint32_t* tris, uint32_t* nt);
+
+/* ---- path tracing --------------------------------------------------------------- */
+typedef struct rtx_camera {
+    float pos[3], fwd[3], right[3], up[3];
+    float tan_half_fov_y;
+} rtx_camera;
+
+typedef struct rtx_render_params {
+    int width, height, spp;
+    int max_bounce;   /* segments per path = max_bounce + 1 (src/main.cpp:156) */
+    int rr_bounce;    /* Russian roulette when bounce >= rr_bounce (reference: 3, main.cpp:188) */
+    uint32_t seed;    /* sample-table seed */
+    int env_mode;     /* 0 = sky gradient (main.cpp:169-172), 1 = constant env_color */
+    float env_color[3];
+} rtx_render_params;
+
+/* Wavefront path tracer: replaces the raster loop + pathTrace (src/main.cpp:145-204,
+ * 339-365) with every random number drawn from the counter-based sample table.
+ * albedo_d / emission_d: 3 floats per ORIGINAL triangle (NULL = reference constant 0.8 /
+ * no emission).  image_d: width*height*3 floats, row y = 0 at the bottom like the
+ * reference's pre-flip buffer. */
+int rtx_render(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_params* p,
+               const float* albedo_d, const float* emission_d, float* image_d, void* stream);
+
+/* Path-replay adjoint of rtx_render for the loss sum (I - target)^2: accumulates
+ * dLoss/dalbedo and dLoss/demission (3 floats per original triangle, added to the
+ * buffers) given d_image_d = dLoss/dI.  Paths are re-traced, not stored.  Requires
+ * rr_bounce > max_bounce (no Russian roulette) so the path set does not depend on
+ * albedo.  No reference counterpart. */
+int rtx_render_adjoint(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_params* p,
+                       const float* albedo_d, const float* emission_d, const float* d_image_d,
+                       float* grad_albedo_d, float* grad_emission_d, void* stream);
+
+/* Batched wavefront render of many scenes.  scenes[i] renders with cams[i]/params[i]
+ * into images_d[i].  mode 0 = one launch sequence per scene (plain), 1 = all scenes in
+ * shared persistent launches captured once in a hipGraph (batched).  No reference
+ * counterpart (the reference renders one scene per process). */
+int rtx_render_batched(const rtx_bvh* const* scenes, int nscenes, const rtx_camera* cams,
+                       const rtx_render_params* params, const float* const* albedo_d,
+                       const float* const* emission_d, float* const* images_d, int mode, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,177 @@
+// Shared device/host definitions for librtx: data layouts in HBM and the exact float
+// semantics of the reference (minsuk0519/ray-tracer) that parity depends on.
+//
+// All sources are compiled with -ffp-contract=off (no FMA contraction) and keep HIP's
+// correctly rounded fp32 divide and sqrt.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define RTX_HD __host__ __device__ __forceinline__
+
+namespace rtx {
+
+constexpr float kInf = __builtin_huge_valf();
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr uint32_t kLeafBit = 0x80000000u;
+
+// Builder constants (src/bvh/bvh_defines.hpp:7-21).
+constexpr int      kLbvhThreshold = 64;
+constexpr int      kBinsMin = 8;
+constexpr int      kBinsMax = 32;
+constexpr float    kCTrav = 1.0f;
+constexpr float    kCIsect = 1.2f;
+constexpr float    kCentroidEps = 1e-6f;
+constexpr float    kSpatialEps = 1e-6f;
+constexpr uint32_t kMaxLeaf = 16;
+constexpr uint32_t kMortonMax = (1u << 21) - 1;
+
+struct f3 { float x, y, z; };
+
+RTX_HD float comp(const f3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+RTX_HD void set_comp(f3& v, int a, float s) { if (a == 0) v.x = s; else if (a == 1) v.y = s; else v.z = s; }
+RTX_HD f3 add(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+RTX_HD f3 sub(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+RTX_HD f3 scl(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+// vec3 dot sums lanes as (x+y)+(z+w) with a +0 w lane (src/math/vec3.hpp:41-49).
+RTX_HD float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + (a.z * b.z + 0.0f); }
+RTX_HD f3 cross(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+RTX_HD f3 normalize(f3 v) { float inv = 1.0f / sqrtf(dot(v, v)); return scl(inv, v); }
+
+// SSE minps/maxps (vec3 min/max, vec3.hpp:71-72) and libstdc++ std::min/std::max.
+RTX_HD float sse_min(float a, float b) { return a < b ? a : b; }
+RTX_HD float sse_max(float a, float b) { return a > b ? a : b; }
+RTX_HD float std_min(float a, float b) { return (b < a) ? b : a; }
+RTX_HD float std_max(float a, float b) { return (a < b) ? b : a; }
+
+// x86-64 conversions as the reference binary performs them: float->int is cvttss2si
+// (INT_MIN when out of range or NaN); float->uint32 is the low half of a 64-bit cvttss2si.
+RTX_HD int f2i_x86(float x) {
+    if (!(x >= -2147483648.0f && x < 2147483648.0f)) return (int)0x80000000;
+    return (int)x;
+}
+RTX_HD uint32_t f2u_x86(float x) {
+    if (!(x >= -9223372036854775808.0f && x < 9223372036854775808.0f)) return 0u;
+    return (uint32_t)(uint64_t)(int64_t)x;
+}
+RTX_HD int clampi(int v, int lo, int hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
+// AABB with the reference's empty box (+inf,-inf) and SSE min/max extension
+// (src/bvh/AABB.cpp:5-52). Extension keeps, per component, the last of equal extrema,
+// so an ordered tree reduction combine(earlier, later) equals the serial fold.
+struct Box {
+    f3 lo, hi;
+    RTX_HD static Box empty() { return Box{{kInf, kInf, kInf}, {-kInf, -kInf, -kInf}}; }
+    RTX_HD void extend(f3 p) {
+        lo = {sse_min(lo.x, p.x), sse_min(lo.y, p.y), sse_min(lo.z, p.z)};
+        hi = {sse_max(hi.x, p.x), sse_max(hi.y, p.y), sse_max(hi.z, p.z)};
+    }
+    RTX_HD void extend(const Box& b) {
+        lo = {sse_min(lo.x, b.lo.x), sse_min(lo.y, b.lo.y), sse_min(lo.z, b.lo.z)};
+        hi = {sse_max(hi.x, b.hi.x), sse_max(hi.y, b.hi.y), sse_max(hi.z, b.hi.z)};
+    }
+    RTX_HD float half_area() const {
+        f3 d = sub(hi, lo);
+        return (d.x * d.y + d.y * d.z) + d.z * d.x;
+    }
+    RTX_HD bool valid() const { return lo.x <= hi.x && lo.y <= hi.y && lo.z <= hi.z; }
+};
+
+// ---------------------------------------------------------------------------
+// HBM layouts
+// ---------------------------------------------------------------------------
+
+// Reference BVHNode (src/bvh/bvh.hpp:15-30), 48 bytes, used for export and .bvh files.
+struct Node48 {
+    float bmin[4];
+    float bmax[4];
+    uint8_t is_leaf;
+    uint8_t pad0[3];
+    uint32_t a, b;
+    uint32_t pad1;
+};
+static_assert(sizeof(Node48) == 48, "Node48 layout");
+
+// Traversal node: one 64-byte line per internal node holding both children's boxes.
+// ref = internal node index, or kLeafBit | leaf index into the leaf table.
+struct alignas(64) GNode {
+    float4 l0;   // lmin.x lmin.y lmin.z lmax.x
+    float4 l1;   // lmax.y lmax.z rmin.x rmin.y
+    float4 r1;   // rmin.z rmax.x rmax.y rmax.z
+    uint32_t lref, rref, pad0, pad1;
+};
+static_assert(sizeof(GNode) == 64, "GNode layout");
+
+// Traversal triangle: 48 bytes, v0 and the two edges precomputed (bit-identical to
+// computing v1-v0, v2-v0 at trace time), plus the original triangle id.
+struct alignas(16) GTri {
+    float4 a;    // v0.x v0.y v0.z e1.x
+    float4 b;    // e1.y e1.z e2.x e2.y
+    float4 c;    // e2.z prim(bits) 0 0
+};
+static_assert(sizeof(GTri) == 48, "GTri layout");
+
+struct Leaf { uint32_t begin, count; };
+
+struct Ray32 { float ox, oy, oz, tmin, dx, dy, dz, tmax; };
+struct Hit16 { float t, u, v; uint32_t tri; };
+
+// ---------------------------------------------------------------------------
+// Slab and triangle tests (src/bvh/bvh_traverse.cpp:22-87)
+// ---------------------------------------------------------------------------
+RTX_HD float slab(float ox, float oy, float oz, float ix, float iy, float iz, float tmin,
+                  float x0, float y0, float z0, float x1, float y1, float z1, float tmax) {
+    float t0x = (x0 - ox) * ix, t1x = (x1 - ox) * ix;
+    float t0y = (y0 - oy) * iy, t1y = (y1 - oy) * iy;
+    float t0z = (z0 - oz) * iz, t1z = (z1 - oz) * iz;
+    float tn = std_max(std_max(std_min(t0x, t1x), std_min(t0y, t1y)), std_max(std_min(t0z, t1z), tmin));
+    float tf = std_min(std_min(std_max(t0x, t1x), std_max(t0y, t1y)), std_min(std_max(t0z, t1z), tmax));
+    return tn <= tf ? tn : kInf;
+}
+
+// Moller-Trumbore with back-face culling and no epsilons. Returns t (kInf on a miss).
+RTX_HD float tri_test(f3 o, f3 d, float tmin, float tmax, f3 v0, f3 e1, f3 e2, float& u_out, float& v_out) {
+    f3 h = cross(d, e2);
+    float a = dot(e1, h);
+    if (a >= 0) return kInf;
+    float f = 1.f / a;
+    f3 s = sub(o, v0);
+    float u = f * dot(s, h);
+    if (u < 0.f || u > 1.f) return kInf;
+    f3 q = cross(s, e1);
+    float v = f * dot(d, q);
+    if (v < 0.f || u + v > 1.f) return kInf;
+    float t = f * dot(e2, q);
+    if (t < tmin || t > tmax) return kInf;
+    u_out = u;
+    v_out = v;
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// Sample table and sin/cos shared by the GPU renderer and its CPU checker.
+// ---------------------------------------------------------------------------
+RTX_HD uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+RTX_HD float sample01(uint32_t seed, uint32_t pixel, uint32_t s, uint32_t dim) {
+    uint32_t h = mix32(seed ^ 0x9e3779b9u);
+    h = mix32(h ^ pixel);
+    h = mix32(h ^ s);
+    h = mix32(h ^ dim);
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+// Cody-Waite reduction by pi/2 (3-part constant) and Taylor polynomials on [-pi/4, pi/4].
+RTX_HD void sincos_rtx(float x, float& s, float& c) {
+    float kf = floorf(x * 0.636619747f + 0.5f);
+    float r = ((x - kf * 1.57080078125f) - kf * -4.453584551811218e-06f) - kf * -8.705516307827565e-10f;
+    float r2 = r * r;
+    float sp = r + (r * r2) * (-1.66666672e-1f + r2 * (8.33333377e-3f + r2 * (-1.98412701e-4f + r2 * 2.75573188e-6f)));
+    float cp = 1.0f + r2 * (-0.5f + r2 * (4.16666679e-2f + r2 * (-1.38888892e-3f + r2 * (2.48015876e-5f + r2 * -2.75573188e-7f))));
+    int q = ((int)kf) & 3;
+    s = (q == 0) ? sp : (q == 1) ? cp : (q == 2) ? -sp : -cp;
+    c = (q == 0) ? cp : (q == 1) ? -sp : (q == 2) ? -cp : sp;
+}
+
+}  // namespace rtx

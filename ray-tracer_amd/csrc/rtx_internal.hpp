@@ -1,0 +1,60 @@
+// Internal state of an rtx_bvh handle and helpers shared by the librtx sources.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "rtx_common.hpp"
+#include "../../include/rtx.h"
+
+namespace rtx {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+bool hip_ok(hipError_t e, const char* what);
+
+#define RTX_HIP(call)                                   \
+    do {                                                \
+        if (!::rtx::hip_ok((call), #call)) return RTX_E_HIP; \
+    } while (0)
+
+template <class T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+    return (T*)p;
+}
+
+}  // namespace rtx
+
+struct rtx_bvh {
+    int device = 0;
+    uint32_t nv = 0, nt = 0;          // input vertices / triangles
+    float* verts = nullptr;           // device copy of the 6-float vertices
+    // products, DFS order
+    rtx::Node48* nodes48 = nullptr;   // reference layout
+    uint32_t nn = 0;
+    uint32_t* tris3 = nullptr;        // reordered triangles (vertex indices)
+    uint32_t* prims = nullptr;        // original triangle id per reordered triangle
+    uint32_t ntri = 0;
+    rtx::GNode* gnodes = nullptr;     // traversal layout, internal nodes only
+    uint32_t ninternal = 0;
+    rtx::Leaf* leaves = nullptr;
+    uint32_t nleaves = 0;
+    rtx::GTri* gtris = nullptr;
+    float root_box[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t root_ref = 0;
+    // diagnostics
+    uint64_t* codes = nullptr;        // Morton code per input triangle
+    uint32_t* sorted = nullptr;       // s_sortedTris[0..nt)
+    rtx::Node48* bfs48 = nullptr;     // node array in builder (FIFO) order
+    uint32_t max_depth = 0, levels = 0, tindex_size = 0, sorted_size = 0;
+    float scene[6] = {0, 0, 0, 0, 0, 0};
+};
+
+namespace rtx {
+// Allocates the traversal products of `b` from host arrays in the reference layout
+// (used by rtx_load).  Defined in rtx_build.hip.
+int upload_reference_tree(rtx_bvh* b, const Node48* nodes, uint32_t nn, const uint32_t* tris3,
+                          uint32_t ntri, const float* verts6, uint32_t nv);
+}  // namespace rtx

@@ -1,0 +1,181 @@
+"""Synthetic workloads for the hot path (host side, numpy).
+
+Vertices use the reference Vertex layout (src/bvh/bvh.hpp:32-41): 6 float32 per
+vertex (x, y, z, nx, ny, nz). Triangles are 3 uint32 vertex indices. Rays are 8
+float32: (ox, oy, oz, tmin, dx, dy, dz, tmax).
+
+Every generator is deterministic for a given seed; the GPU path and the CPU oracle
+always consume the same arrays, produced once per run.
+"""
+import math
+
+import numpy as np
+
+F32 = np.float32
+
+
+def _normalize(v):
+    v = np.asarray(v, F32)
+    n = np.atleast_1d(np.sqrt((v * v).sum(-1, dtype=F32)).astype(F32))
+    n[n == 0] = 1
+    return (v / n.reshape(v.shape[:-1] + (1,))).astype(F32)
+
+
+def random_triangles(n, seed=1, mu=-5.0, sigma=0.5):
+    """Triangle soup: centres uniform in the unit cube, Gaussian edge vectors with a
+    lognormal(mu, sigma) scale, unshared vertices, face normals as vertex normals."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    c = rng.random((n, 3))
+    s = np.exp(rng.normal(mu, sigma, n))[:, None]
+    e1 = rng.standard_normal((n, 3)) * s
+    e2 = rng.standard_normal((n, 3)) * s
+    v0 = c - (e1 + e2) / 3.0
+    v1 = v0 + e1
+    v2 = v0 + e2
+    pos = np.stack([v0, v1, v2], 1).astype(F32)            # (n, 3, 3)
+    fn = _normalize(np.cross(pos[:, 1] - pos[:, 0], pos[:, 2] - pos[:, 0]))
+    verts = np.empty((n, 3, 6), F32)
+    verts[:, :, :3] = pos
+    verts[:, :, 3:] = fn[:, None, :]
+    tris = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    return verts.reshape(-1, 6), tris
+
+
+def incoherent_rays(n, seed=1, lo=0.0, hi=1.0):
+    """Origins uniform in [lo,hi]^3, directions uniform on the sphere, tmin 0, tmax inf."""
+    rng = np.random.Generator(np.random.PCG64(seed + 0x5bd1e995))
+    o = lo + (hi - lo) * rng.random((n, 3))
+    z = rng.uniform(-1.0, 1.0, n)
+    phi = rng.uniform(0.0, 2.0 * math.pi, n)
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    d = np.stack([r * np.cos(phi), r * np.sin(phi), z], 1)
+    rays = np.zeros((n, 8), F32)
+    rays[:, 0:3] = o
+    rays[:, 4:7] = d
+    rays[:, 7] = np.inf
+    return rays
+
+
+def camera_lookat(pos, target, fov_deg):
+    """Pinhole camera of makeCameraLookAt (src/main.cpp:55-66) in float32."""
+    pos = np.asarray(pos, F32)
+    target = np.asarray(target, F32)
+    fwd = _normalize(target - pos)
+    world_up = np.array([0, 1, 0], F32) if abs(fwd[1]) < F32(0.99) else np.array([1, 0, 0], F32)
+    right = _normalize(np.cross(fwd, world_up))
+    up = np.cross(right, fwd).astype(F32)
+    tan_h = F32(math.tan(F32(fov_deg) * F32(math.pi) / F32(180.0) * F32(0.5)))
+    return dict(pos=pos, fwd=fwd, right=right, up=up, tan_half_fov_y=tan_h)
+
+
+def pinhole_rays(cam, width, height, jitter=0.5, tmin=1e-3):
+    """One primary ray per pixel through the pixel position (x + jitter, y + jitter),
+    following primaryRay (src/main.cpp:69-83)."""
+    ys, xs = np.meshgrid(np.arange(height, dtype=F32), np.arange(width, dtype=F32), indexing="ij")
+    aspect = F32(width) / F32(height)
+    u = ((xs + F32(jitter)) / F32(width) * F32(2) - F32(1)) * aspect
+    v = (ys + F32(jitter)) / F32(height) * F32(2) - F32(1)
+    th = F32(cam["tan_half_fov_y"])
+    d = (cam["fwd"][None, None, :] + cam["right"][None, None, :] * (u * th)[..., None]
+         + cam["up"][None, None, :] * (v * th)[..., None]).astype(F32)
+    d = _normalize(d.reshape(-1, 3))
+    rays = np.zeros((width * height, 8), F32)
+    rays[:, 0:3] = cam["pos"]
+    rays[:, 3] = tmin
+    rays[:, 4:7] = d
+    rays[:, 7] = np.inf
+    return rays
+
+
+def icosphere(subdiv, radius=1.0, center=(0, 0, 0)):
+    """Geodesic sphere: icosahedron with `subdiv` midpoint subdivisions (20*4^s faces),
+    visible from outside under the reference's back-face test, with radial normals."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+         (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    faces = f
+    for _ in range(subdiv):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[k] = len(verts) - 1
+            return cache[k]
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    P = np.array(verts)
+    out = np.zeros((len(P), 6), F32)
+    out[:, :3] = P * radius + np.asarray(center)
+    out[:, 3:] = P
+    tris = np.array(faces, np.int64)
+    # Visible from outside: e1 x e2 must point inwards (see quad()).
+    e1 = P[tris[:, 1]] - P[tris[:, 0]]
+    e2 = P[tris[:, 2]] - P[tris[:, 0]]
+    outward = (np.cross(e1, e2) * P[tris].mean(1)).sum(-1) > 0
+    tris[outward] = tris[outward][:, [0, 2, 1]]
+    return out, tris.astype(np.uint32)
+
+
+def merge(parts):
+    """Concatenate (verts6, tris) meshes, offsetting indices."""
+    vs, ts, off = [], [], 0
+    for v, t in parts:
+        vs.append(np.asarray(v, F32))
+        ts.append(np.asarray(t, np.uint32) + off)
+        off += len(v)
+    return np.concatenate(vs), np.concatenate(ts)
+
+
+def quad(p0, p1, p2, p3, facing):
+    """Two triangles (p0,p1,p2), (p0,p2,p3) seen from the side `facing` points to.
+
+    The reference culls a = dot(e1, cross(d, e2)) >= 0 (src/bvh/bvh_traverse.cpp:57-60),
+    so a face is hit only by rays travelling along e1 x e2. The winding is chosen so
+    that e1 x e2 points away from the visible side; the shading normal is `facing`."""
+    P = np.array([p0, p1, p2, p3], F32)
+    f = np.asarray(facing, F32)
+    if float(np.dot(np.cross(P[1] - P[0], P[2] - P[0]), f)) > 0:
+        P = P[[0, 3, 2, 1]]
+    v = np.zeros((4, 6), F32)
+    v[:, :3] = P
+    v[:, 3:] = _normalize(f[None])[0]
+    return v, np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+
+
+def cornell_box(seed=0, sphere_subdiv=3, sphere_radius=0.3, light_radiance=15.0, albedo_lo=0.2, albedo_hi=0.8):
+    """Open-front Cornell-style box in [-1,1]^3: 10 wall quads (20 triangles, inward
+    facing), one ceiling light quad (2 triangles) and a centred icosphere.
+    Returns verts6, tris, albedo (3 per triangle), emission (3 per triangle), camera."""
+    parts = []
+    # floor y=-1, ceiling y=1, back z=1, left x=-1, right x=1, each as 2 quads (split in x or z/y)
+    def wall(fn, facing):
+        for i in range(2):
+            parts.append(quad(*fn(i), facing))
+    wall(lambda i: [(-1 + i, -1, -1), (-1 + i, -1, 1), (i, -1, 1), (i, -1, -1)], (0, 1, 0))    # floor
+    wall(lambda i: [(-1 + i, 1, -1), (i, 1, -1), (i, 1, 1), (-1 + i, 1, 1)], (0, -1, 0))      # ceiling
+    wall(lambda i: [(-1 + i, -1, 1), (-1 + i, 1, 1), (i, 1, 1), (i, -1, 1)], (0, 0, -1))      # back
+    wall(lambda i: [(-1, -1, -1 + i), (-1, 1, -1 + i), (-1, 1, i), (-1, -1, i)], (1, 0, 0))   # left
+    wall(lambda i: [(1, -1, -1 + i), (1, -1, i), (1, 1, i), (1, 1, -1 + i)], (-1, 0, 0))      # right
+    # ceiling light slightly below the ceiling, facing down
+    parts.append(quad((-0.25, 0.99, -0.25), (0.25, 0.99, -0.25), (0.25, 0.99, 0.25), (-0.25, 0.99, 0.25),
+                      (0, -1, 0)))
+    sv, st = icosphere(sphere_subdiv, sphere_radius, (0, -0.2, 0.1))
+    parts.append((sv, st))
+    verts, tris = merge(parts)
+    nt = len(tris)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    albedo = rng.uniform(albedo_lo, albedo_hi, (nt, 3)).astype(F32)
+    emission = np.zeros((nt, 3), F32)
+    emission[20:22] = light_radiance
+    cam = camera_lookat((0, 0, -3.2), (0, 0, 0), 40.0)
+    return verts, tris, albedo, emission, cam
