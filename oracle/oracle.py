@@ -261,7 +261,7 @@ def ref_geo_sphere(center, radius, rings=16, sectors=32):
     return _ref_geo(0, args)
 
 
-def ref_geo_box(center, half, quat_xyzw):
+def ref_geo_box(center, half, quat_xyzw=(0, 0, 0, 1)):
     args = np.array([*center, *half, *quat_xyzw], np.float32)
     return _ref_geo(1, args)
 

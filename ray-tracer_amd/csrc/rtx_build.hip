@@ -1,0 +1,1138 @@
+// GPU BVH builder: bit-exact with the reference's bakeBVH pipeline.
+//
+//   sortTrisByMorton  (src/bvh/bvh_morton.cpp:40-159)   -> k_tri_prep, k_scene_box_*, k_morton,
+//                                                         stable 8-bit LSD radix (k_rs_*)
+//   initRoot/bfsLoop  (src/bvh/bvh_bfs.cpp:50-134)      -> level-synchronous frontier:
+//   trySAHSplit       (src/bvh/bvh_sah.cpp:24-460)         k_level_decide / k_level_budget / k_level_apply
+//   reorderNodes/Tris (src/bvh/bvh_bake.cpp:75-149)     -> k_subtree_*, k_dfs_*, k_emit
+//
+// The reference processes one FIFO queue. A FIFO visits the tree level by level, left to
+// right, so one launch per level over the level's contiguous node range reproduces the
+// order in which nodes are numbered and in which the spatial-split budget (1.5 N triangle
+// references) is consumed; k_level_budget resolves that budget in queue order.
+// Node boxes are bit-identical: min/max extension keeps the last of equal extrema, which
+// makes an ordered reduction (earlier, later) equal to the serial fold the reference runs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "rtx_internal.hpp"
+
+namespace rtx {
+
+namespace {
+
+constexpr uint32_t F_LEAF = 1u, F_SAH = 2u, F_PENDING = 4u;
+
+struct BNode {
+    f3 lo, hi;
+    uint32_t a, b;       // children, or {first tri-index position, count}
+    uint32_t flags;
+    uint32_t pad;
+};
+
+enum : uint32_t { D_LEAF = 0, D_MORTON = 1, D_OBJECT = 2, D_SPATIAL = 3 };
+
+struct Dec {
+    uint32_t type;
+    uint32_t mid;           // split position (Morton / object)
+    uint32_t nl, ns, nr;    // spatial: left-only, straddling, right-only counts
+    uint32_t fb_split;      // spatial: the object fallback would split
+    int oAxis, oBin;
+    float oCMin, oScale;
+    float plane;
+    int sAxis;
+    int nb;
+};
+
+struct Counters {
+    uint32_t total;      // s_totalNodeCount
+    uint32_t tsize;      // s_triIndexSize
+    uint32_t ssize;      // s_sortedTrisSize
+    uint32_t err;
+};
+
+struct Ctx {
+    const float* verts;
+    const uint32_t* tris;
+    const float4* tbox;      // 2 per input triangle (lo, hi)
+    const float4* cent;      // centroid per input triangle
+    const uint64_t* scodes;  // Morton codes in sorted order
+    uint32_t* sorted;        // s_sortedTris (capacity cap)
+    uint32_t* tindex;        // s_triIndex (capacity cap)
+    BNode* nodes;
+    Dec* dec;
+    uint32_t* child;         // child base id per frontier item (kInvalid = none)
+    uint32_t* toff;          // appended s_triIndex offset (accepted spatial)
+    uint32_t* soff;          // appended s_sortedTris offset (accepted spatial)
+    uint32_t* acc;           // spatial accepted flag
+    uint32_t cap;
+    uint32_t ncap;           // node array capacity
+    Counters* ctr;
+};
+
+__device__ __forceinline__ f3 vpos(const float* v, uint32_t i) { return {v[6 * i], v[6 * i + 1], v[6 * i + 2]}; }
+
+__device__ __forceinline__ Box load_tbox(const float4* tb, uint32_t t) {
+    float4 a = tb[2 * t], b = tb[2 * t + 1];
+    return Box{{a.x, a.y, a.z}, {b.x, b.y, b.z}};
+}
+
+__device__ __forceinline__ Box combine(const Box& early, const Box& late) {
+    Box r = early;
+    r.extend(late);
+    return r;
+}
+
+// Ordered fold of triangle boxes over s_triIndex positions [b, e) (computeRangeAABB,
+// bvh_bfs.cpp:14-28).
+__device__ Box range_box(const Ctx& C, uint32_t b, uint32_t e) {
+    Box r = Box::empty();
+    for (uint32_t i = b; i < e; ++i) r.extend(load_tbox(C.tbox, C.sorted[C.tindex[i]]));
+    return r;
+}
+
+// Sutherland-Hodgman clip of a triangle to a box, then its AABB (AABB.cpp:86-171).
+__device__ Box clip_tri(f3 v0, f3 v1, f3 v2, const Box& r) {
+    f3 poly[12], tmp[12];
+    int n = 3;
+    poly[0] = v0; poly[1] = v1; poly[2] = v2;
+    for (int axis = 0; axis < 3; ++axis) {
+        for (int side = 0; side < 2; ++side) {
+            float plane = side == 0 ? comp(r.lo, axis) : comp(r.hi, axis);
+            int m = 0;
+            for (int i = 0; i < n; ++i) {
+                int j = (i + 1 == n) ? 0 : i + 1;
+                float pi = comp(poly[i], axis), pj = comp(poly[j], axis);
+                bool ci = side == 0 ? (pi >= plane) : (pi <= plane);
+                bool cj = side == 0 ? (pj >= plane) : (pj <= plane);
+                if (ci) tmp[m++] = poly[i];
+                if (ci != cj) {
+                    float t = (plane - pi) / (pj - pi);
+                    tmp[m++] = add(poly[i], scl(t, sub(poly[j], poly[i])));
+                }
+            }
+            for (int i = 0; i < m; ++i) poly[i] = tmp[i];
+            n = m;
+            if (n == 0) return Box::empty();
+        }
+    }
+    Box b = Box::empty();
+    for (int i = 0; i < n; ++i) b.extend(poly[i]);
+    return b;
+}
+
+__device__ __forceinline__ void tri_verts(const Ctx& C, uint32_t t, f3& a, f3& b, f3& c) {
+    a = vpos(C.verts, C.tris[3 * t]);
+    b = vpos(C.verts, C.tris[3 * t + 1]);
+    c = vpos(C.verts, C.tris[3 * t + 2]);
+}
+
+// libstdc++ bidirectional std::partition (stl_algo.h:1487-1510) over s_triIndex.
+template <class P>
+__device__ uint32_t partition_idx(uint32_t* a, uint32_t first, uint32_t last, P pred) {
+    for (;;) {
+        for (;;) {
+            if (first == last) return first;
+            if (pred(a[first])) ++first; else break;
+        }
+        --last;
+        for (;;) {
+            if (first == last) return first;
+            if (!pred(a[last])) --last; else break;
+        }
+        uint32_t t = a[first]; a[first] = a[last]; a[last] = t;
+        ++first;
+    }
+}
+
+__device__ __forceinline__ float cent_axis(const Ctx& C, uint32_t si, int axis) {
+    float4 c = C.cent[C.sorted[si]];
+    return axis == 0 ? c.x : (axis == 1 ? c.y : c.z);
+}
+
+__device__ __forceinline__ bool obj_pred(const Ctx& C, uint32_t si, int axis, int bin, float cmin, float scale, int nb) {
+    return min(f2i_x86((cent_axis(C, si, axis) - cmin) * scale), nb - 1) < bin;
+}
+
+// Binned SAH evaluation of one node (bvh_sah.cpp:24-243), serial in one lane.
+__device__ void sah_eval(const Ctx& C, uint32_t begin, uint32_t n, const Box& pbox, Dec& d) {
+    const uint32_t end = begin + n;
+    const int nb = clampi((int)n / 4, kBinsMin, kBinsMax);
+    const float invA = 1.0f / pbox.half_area();
+    float best = kCIsect * (float)n;
+    int bAxis = -1, bBin = -1;
+    float bCMin = 0.f, bScale = 0.f;
+    Box bins[kBinsMax], lbox[kBinsMax];
+    uint32_t cnt[kBinsMax], lcnt[kBinsMax], cnt2[kBinsMax];
+
+    for (int axis = 0; axis < 3; ++axis) {
+        float cmin = 3.40282347e+38f, cmax = -3.40282347e+38f;
+        for (uint32_t i = begin; i < end; ++i) {
+            float c = cent_axis(C, C.tindex[i], axis);
+            cmin = std_min(cmin, c);
+            cmax = std_max(cmax, c);
+        }
+        if (cmax - cmin < kCentroidEps) continue;
+        float scale = (float)nb / (cmax - cmin);
+        for (int b = 0; b < nb; ++b) { bins[b] = Box::empty(); cnt[b] = 0; }
+        for (uint32_t i = begin; i < end; ++i) {
+            uint32_t t = C.sorted[C.tindex[i]];
+            float4 c4 = C.cent[t];
+            float c = axis == 0 ? c4.x : (axis == 1 ? c4.y : c4.z);
+            int b = min(f2i_x86((c - cmin) * scale), nb - 1);
+            cnt[b]++;
+            bins[b].extend(load_tbox(C.tbox, t));
+        }
+        Box la = Box::empty();
+        uint32_t lc = 0;
+        for (int b = 0; b < nb - 1; ++b) { la.extend(bins[b]); lc += cnt[b]; lbox[b] = la; lcnt[b] = lc; }
+        Box ra = Box::empty();
+        uint32_t rc = 0;
+        for (int b = nb - 1; b >= 1; --b) {
+            ra.extend(bins[b]); rc += cnt[b];
+            uint32_t lc2 = lcnt[b - 1];
+            if (lc2 == 0 || rc == 0) continue;
+            float cost = kCTrav + kCIsect * (lbox[b - 1].half_area() * (float)lc2 + ra.half_area() * (float)rc) * invA;
+            if (cost < best) { best = cost; bAxis = axis; bBin = b; bCMin = cmin; bScale = scale; }
+        }
+    }
+    d.oAxis = bAxis; d.oBin = bBin; d.oCMin = bCMin; d.oScale = bScale; d.nb = nb;
+
+    bool spatial = false;
+    float sMin = 0.f, sScale = 0.f;
+    for (int axis = 0; axis < 3; ++axis) {
+        float nmin = comp(pbox.lo, axis), nmax = comp(pbox.hi, axis);
+        if (nmax - nmin < kSpatialEps) continue;
+        float scale = (float)nb / (nmax - nmin);
+        for (int b = 0; b < nb; ++b) { bins[b] = Box::empty(); cnt[b] = 0; cnt2[b] = 0; }
+        for (uint32_t i = begin; i < end; ++i) {
+            uint32_t t = C.sorted[C.tindex[i]];
+            Box tb = load_tbox(C.tbox, t);
+            int b0 = clampi(f2i_x86((comp(tb.lo, axis) - nmin) * scale), 0, nb - 1);
+            int b1 = clampi(f2i_x86((comp(tb.hi, axis) - nmin) * scale), 0, nb - 1);
+            cnt[b0]++;
+            cnt2[b1]++;
+            if (b0 > b1) continue;
+            f3 p0, p1, p2;
+            tri_verts(C, t, p0, p1, p2);
+            Box reg = pbox;
+            for (int b = b0; b <= b1; ++b) {
+                set_comp(reg.lo, axis, nmin + (float)b / scale);
+                set_comp(reg.hi, axis, nmin + (float)(b + 1) / scale);
+                Box cb = clip_tri(p0, p1, p2, reg);
+                if (cb.valid()) bins[b].extend(cb);
+            }
+        }
+        Box la = Box::empty();
+        uint32_t lc = 0;
+        for (int b = 0; b < nb - 1; ++b) { la.extend(bins[b]); lc += cnt[b]; lbox[b] = la; lcnt[b] = lc; }
+        Box ra = Box::empty();
+        uint32_t rc = 0;
+        for (int b = nb - 1; b >= 1; --b) {
+            ra.extend(bins[b]); rc += cnt2[b];
+            uint32_t lc2 = lcnt[b - 1];
+            if (lc2 == 0 || rc == 0) continue;
+            float cost = kCTrav + kCIsect * (lbox[b - 1].half_area() * (float)lc2 + ra.half_area() * (float)rc) * invA;
+            if (cost < best) { best = cost; bAxis = axis; bBin = b; spatial = true; sMin = nmin; sScale = scale; }
+        }
+    }
+
+    if (bAxis == -1) { d.type = D_LEAF; return; }
+    if (!spatial) {
+        uint32_t mid = partition_idx(C.tindex, begin, end, [&](uint32_t si) {
+            return obj_pred(C, si, bAxis, bBin, bCMin, bScale, nb);
+        });
+        if (mid == begin || mid == end) { d.type = D_LEAF; return; }
+        d.type = D_OBJECT;
+        d.mid = mid;
+        return;
+    }
+    // Spatial winner: 3-way partition [left-only | straddling | right-only] (bvh_sah.cpp:296-333).
+    const float plane = sMin + (float)bBin / sScale;
+    const int ax = bAxis;
+    uint32_t leftEnd = partition_idx(C.tindex, begin, end, [&](uint32_t si) {
+        return comp(load_tbox(C.tbox, C.sorted[si]).hi, ax) <= plane;
+    });
+    uint32_t strEnd = partition_idx(C.tindex, leftEnd, end, [&](uint32_t si) {
+        return comp(load_tbox(C.tbox, C.sorted[si]).lo, ax) < plane;
+    });
+    uint32_t nl = leftEnd - begin, ns = strEnd - leftEnd, nr = end - strEnd;
+    d.plane = plane;
+    d.sAxis = ax;
+    d.nl = nl; d.ns = ns; d.nr = nr;
+    if (nl + ns == 0 || ns + nr == 0) {
+        // Degenerate spatial partition: object-split fallback (bvh_sah.cpp:335-388).
+        if (d.oAxis == -1) { d.type = D_LEAF; return; }
+        uint32_t mid = partition_idx(C.tindex, begin, end, [&](uint32_t si) {
+            return obj_pred(C, si, d.oAxis, d.oBin, d.oCMin, d.oScale, nb);
+        });
+        if (mid == begin || mid == end) { d.type = D_LEAF; return; }
+        d.type = D_OBJECT;
+        d.mid = mid;
+        return;
+    }
+    // Candidate; the budget decides in queue order. Precompute whether the fallback splits.
+    uint32_t c = 0;
+    if (d.oAxis != -1)
+        for (uint32_t i = begin; i < end; ++i) c += obj_pred(C, C.tindex[i], d.oAxis, d.oBin, d.oCMin, d.oScale, nb) ? 1u : 0u;
+    d.fb_split = (d.oAxis != -1 && c > 0 && c < n) ? 1u : 0u;
+    d.type = D_SPATIAL;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels: triangle preparation, scene box, Morton codes
+// ---------------------------------------------------------------------------
+__global__ void k_check_indices(const uint32_t* tris, uint32_t n3, uint32_t nv, uint32_t* err) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3 && tris[i] >= nv) atomicOr(err, 1u);
+}
+
+__global__ void k_tri_prep(const float* verts, const uint32_t* tris, uint32_t nt, float4* tbox, float4* cent) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    f3 a = vpos(verts, tris[3 * t]), b = vpos(verts, tris[3 * t + 1]), c = vpos(verts, tris[3 * t + 2]);
+    Box bx = Box::empty();
+    bx.extend(a); bx.extend(b); bx.extend(c);
+    tbox[2 * t] = make_float4(bx.lo.x, bx.lo.y, bx.lo.z, 0.f);
+    tbox[2 * t + 1] = make_float4(bx.hi.x, bx.hi.y, bx.hi.z, 0.f);
+    // Centroid exactly as computeMortonCodes / buildCentroidCache (bvh_morton.cpp:88-90, bvh_bake.cpp:51-55).
+    cent[t] = make_float4(((a.x + b.x) + c.x) / 3.0f, ((a.y + b.y) + c.y) / 3.0f, ((a.z + b.z) + c.z) / 3.0f, 0.f);
+}
+
+constexpr int kRedThreads = 256;
+
+// Ordered reduction of triangle boxes in input order (the scene AABB loop, bvh_morton.cpp:49-59).
+__global__ void k_scene_box_partial(const float4* tbox, uint32_t nt, uint32_t chunk, float4* part) {
+    __shared__ Box sh[kRedThreads];
+    uint32_t blk_b = blockIdx.x * chunk * kRedThreads;
+    uint32_t b = blk_b + threadIdx.x * chunk;
+    Box acc = Box::empty();
+    for (uint32_t i = b; i < b + chunk && i < nt; ++i) acc.extend(load_tbox(tbox, i));
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 1; s < kRedThreads; s <<= 1) {
+        if ((threadIdx.x & (2 * s - 1)) == 0) sh[threadIdx.x] = combine(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = make_float4(sh[0].lo.x, sh[0].lo.y, sh[0].lo.z, 0.f);
+        part[2 * blockIdx.x + 1] = make_float4(sh[0].hi.x, sh[0].hi.y, sh[0].hi.z, 0.f);
+    }
+}
+
+__global__ void k_scene_box_final(const float4* part, uint32_t np, float* out6) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Box acc = Box::empty();
+    for (uint32_t i = 0; i < np; ++i) acc.extend(load_tbox(part, i));
+    out6[0] = acc.lo.x; out6[1] = acc.lo.y; out6[2] = acc.lo.z;
+    out6[3] = acc.hi.x; out6[4] = acc.hi.y; out6[5] = acc.hi.z;
+}
+
+__device__ __forceinline__ uint64_t spread21(uint32_t v) {
+    uint64_t x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t quant(float c, float lo, float size) {
+    uint32_t q = f2u_x86(((c - lo) * 2097151.0f) / size);
+    return q > kMortonMax ? kMortonMax : q;
+}
+
+__global__ void k_morton(const float4* cent, uint32_t nt, const float* box6, uint64_t* codes, uint32_t* idx,
+                         unsigned long long* orand) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t code = 0;
+    bool valid = t < nt;
+    if (valid) {
+        float sx = box6[3] - box6[0], sy = box6[4] - box6[1], sz = box6[5] - box6[2];
+        if (sx == 0.f) sx = 1.f;
+        if (sy == 0.f) sy = 1.f;
+        if (sz == 0.f) sz = 1.f;
+        float4 c = cent[t];
+        code = (spread21(quant(c.x, box6[0], sx)) << 2) | (spread21(quant(c.y, box6[1], sy)) << 1) |
+               spread21(quant(c.z, box6[2], sz));
+        codes[t] = code;
+        idx[t] = t;
+    }
+    // OR and AND of all codes, to skip radix passes whose digit never varies.
+    unsigned long long o = valid ? code : 0ull, a = valid ? code : ~0ull;
+    for (int d = 32; d >= 1; d >>= 1) {
+        o |= __shfl_xor(o, d);
+        a &= __shfl_xor(a, d);
+    }
+    if (__lane_id() == 0) {
+        atomicOr(&orand[0], o);
+        atomicAnd(&orand[1], a);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (code, index), 8-bit digits (bvh_morton.cpp:121-156).
+// Tiles of 4096 keys; per-tile LDS histograms; one scan; stable scatter ranked by
+// wave ballots (lanes with the same digit) and per-wave LDS counts.
+// ---------------------------------------------------------------------------
+constexpr int kRsThreads = 256, kRsItems = 16, kRsTile = kRsThreads * kRsItems;
+
+__global__ void k_rs_hist(const uint64_t* keys, uint32_t n, int shift, uint32_t* hist, uint32_t ntiles) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsItems; ++r) {
+        uint32_t i = base + r * kRsThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of m counters with one block of 1024 threads.
+__global__ void k_scan_excl(uint32_t* data, uint32_t m) {
+    __shared__ uint32_t sums[1024];
+    uint32_t chunk = (m + 1023) / 1024;
+    uint32_t b = threadIdx.x * chunk, e = min(m, b + chunk);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += data[i];
+    sums[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < 1024; ++i) { uint32_t v = sums[i]; sums[i] = run; run += v; }
+    }
+    __syncthreads();
+    uint32_t run = sums[threadIdx.x];
+    for (uint32_t i = b; i < e; ++i) { uint32_t v = data[i]; data[i] = run; run += v; }
+}
+
+__global__ void k_rs_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint32_t n,
+                             int shift, const uint32_t* hist, uint32_t ntiles) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[kRsThreads / 64][256];
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    run[threadIdx.x] = hist[threadIdx.x * ntiles + blockIdx.x];
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t base = blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsItems; ++r) {
+        for (int ww = 0; ww < kRsThreads / 64; ++ww) wcnt[ww][threadIdx.x] = 0;
+        __syncthreads();
+        uint32_t i = base + r * kRsThreads + threadIdx.x;
+        bool valid = i < n;
+        uint64_t k = valid ? kin[i] : 0ull;
+        uint32_t v = valid ? vin[i] : 0u;
+        uint32_t d = (uint32_t)(k >> shift) & 255u;
+        uint64_t mask = __ballot(valid);
+        for (int bit = 0; bit < 8; ++bit) {
+            uint64_t bb = __ballot(valid && ((d >> bit) & 1u));
+            mask &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        uint32_t rank = __popcll(mask & lt);
+        if (valid && rank == 0) wcnt[w][d] = __popcll(mask);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + rank;
+            for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][d];
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+        uint32_t add_ = 0;
+        for (int ww = 0; ww < kRsThreads / 64; ++ww) add_ += wcnt[ww][threadIdx.x];
+        run[threadIdx.x] += add_;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Level-synchronous top-down build (the FIFO of bvh_bfs.cpp:72-134)
+// ---------------------------------------------------------------------------
+__global__ void k_init_root(BNode* nodes, const float* box6, uint32_t nt, uint32_t* tindex, Counters* ctr) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        BNode r;
+        r.lo = {box6[0], box6[1], box6[2]};
+        r.hi = {box6[3], box6[4], box6[5]};
+        r.a = 0;
+        r.b = nt;
+        r.flags = F_LEAF | (nt <= (uint32_t)kLbvhThreshold ? F_SAH : 0u);
+        r.pad = 0;
+        nodes[0] = r;
+        ctr->total = 1;
+        ctr->tsize = nt;
+        ctr->ssize = nt;
+        ctr->err = 0;
+    }
+    (void)tindex;
+}
+
+__global__ void k_iota(uint32_t* a, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+__global__ void k_level_decide(Ctx C, uint32_t ls, uint32_t cnt) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    const uint32_t ni = ls + k;
+    BNode nd = C.nodes[ni];
+    Dec d;
+    d.type = D_LEAF;
+    d.mid = 0; d.nl = d.ns = d.nr = 0; d.fb_split = 0;
+    d.oAxis = -1; d.oBin = -1; d.oCMin = 0.f; d.oScale = 0.f; d.plane = 0.f; d.sAxis = -1; d.nb = 0;
+    const uint32_t begin = nd.a, n = nd.b;
+    if (n > kMaxLeaf) {
+        bool sah = (nd.flags & F_SAH) != 0;
+        if (!sah) {
+            uint64_t first = C.scodes[begin], last = C.scodes[begin + n - 1];
+            uint64_t x = first ^ last;
+            if (x == 0) {
+                sah = true;  // identical codes: SAH fallback (bvh_bfs.cpp:102-107)
+            } else {
+                uint64_t mask = 1ull << (63 - __clzll(x));
+                uint32_t lo = begin, hi = begin + n - 1;   // code[lo] lacks the bit, code[hi] has it
+                while (hi - lo > 1) {
+                    uint32_t m = lo + (hi - lo) / 2;
+                    if (C.scodes[m] & mask) hi = m; else lo = m;
+                }
+                d.type = D_MORTON;
+                d.mid = hi;
+            }
+        }
+        if (sah) sah_eval(C, begin, n, Box{nd.lo, nd.hi}, d);
+    }
+    C.dec[k] = d;
+}
+
+// Block-wide exclusive scan for 1024 threads (16 waves). Returns the exclusive prefix of
+// v and the block total; ws needs 17 entries.
+__device__ uint32_t block_scan(uint32_t v, uint32_t* ws, uint32_t& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t s = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(s, o);
+        if (lane >= o) s += y;
+    }
+    if (lane == 63) ws[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < 16; ++i) { uint32_t x = ws[i]; ws[i] = run; run += x; }
+        ws[16] = run;
+    }
+    __syncthreads();
+    uint32_t ex = ws[w] + s - v;
+    total = ws[16];
+    __syncthreads();
+    return ex;
+}
+
+// Queue-order resolution of the spatial-split budget (bvh_sah.cpp:390-394) and child
+// numbering (s_totalNodeCount++ in FIFO order). One block of 1024 threads walks the
+// level in chunks; a chunk whose spatial candidates all fit is accepted with a scan,
+// otherwise lane 0 runs the greedy in order (a rejected candidate costs nothing).
+__global__ void __launch_bounds__(1024) k_level_budget(Ctx C, uint32_t cnt) {
+    __shared__ uint32_t ws[17];
+    __shared__ uint32_t base[3];     // total nodes, s_triIndexSize, s_sortedTrisSize
+    __shared__ uint32_t slow;
+    const int tid = threadIdx.x;
+    if (tid == 0) { base[0] = C.ctr->total; base[1] = C.ctr->tsize; base[2] = C.ctr->ssize; }
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < cnt; c0 += 1024) {
+        const uint32_t k = c0 + tid;
+        const bool in = k < cnt;
+        Dec d;
+        d.type = D_LEAF; d.nr = d.ns = 0; d.fb_split = 0;
+        if (in) d = C.dec[k];
+        const bool sp = in && d.type == D_SPATIAL;
+        const uint32_t need_t = sp ? d.nr + d.ns : 0u, need_s = sp ? d.ns : 0u;
+        uint32_t tot_t, tot_s;
+        uint32_t ex_t = block_scan(need_t, ws, tot_t);
+        uint32_t ex_s = block_scan(need_s, ws, tot_s);
+        if (tid == 0) slow = (base[1] + tot_t > C.cap || base[2] + tot_s > C.cap) ? 1u : 0u;
+        __syncthreads();
+        uint32_t accepted = 0;
+        if (!slow) {
+            if (sp) {
+                accepted = 1;
+                C.toff[k] = base[1] + ex_t;
+                C.soff[k] = base[2] + ex_s;
+            }
+            __syncthreads();
+            if (tid == 0) { base[1] += tot_t; base[2] += tot_s; }
+        } else {
+            if (tid == 0) {
+                uint32_t ts = base[1], ss = base[2];
+                const uint32_t e = min(cnt, c0 + 1024);
+                for (uint32_t q = c0; q < e; ++q) {
+                    Dec dq = C.dec[q];
+                    uint32_t ok = 0;
+                    if (dq.type == D_SPATIAL && !(ts + dq.nr + dq.ns > C.cap || ss + dq.ns > C.cap)) {
+                        ok = 1;
+                        C.toff[q] = ts;
+                        C.soff[q] = ss;
+                        ts += dq.nr + dq.ns;
+                        ss += dq.ns;
+                    }
+                    C.acc[q] = ok;
+                }
+                base[1] = ts;
+                base[2] = ss;
+            }
+            __syncthreads();
+            if (sp) accepted = C.acc[k];
+        }
+        __syncthreads();
+        uint32_t nch = 0;
+        if (in) {
+            if (d.type == D_MORTON || d.type == D_OBJECT) nch = 2;
+            else if (d.type == D_SPATIAL) nch = (accepted || d.fb_split) ? 2u : 0u;
+        }
+        uint32_t tot_n;
+        uint32_t ex_n = block_scan(nch, ws, tot_n);
+        if (in) {
+            C.acc[k] = accepted;
+            C.child[k] = nch ? base[0] + ex_n : kInvalid;
+        }
+        __syncthreads();
+        if (tid == 0) base[0] += tot_n;
+        __syncthreads();
+    }
+    if (tid == 0) { C.ctr->total = base[0]; C.ctr->tsize = base[1]; C.ctr->ssize = base[2]; }
+}
+
+__device__ void set_child(const Ctx& C, uint32_t id, const Box& b, uint32_t begin, uint32_t n, uint32_t flags) {
+    BNode c;
+    c.lo = b.lo;
+    c.hi = b.hi;
+    c.a = begin;
+    c.b = n;
+    c.flags = flags;
+    c.pad = 0;
+    C.nodes[id] = c;
+}
+
+__device__ void object_children(const Ctx& C, uint32_t ni, uint32_t begin, uint32_t mid, uint32_t end, uint32_t c) {
+    set_child(C, c, range_box(C, begin, mid), begin, mid - begin, F_LEAF | F_SAH);
+    set_child(C, c + 1, range_box(C, mid, end), mid, end - mid, F_LEAF | F_SAH);
+    BNode& p = C.nodes[ni];
+    p.flags &= ~F_LEAF;
+    p.a = c;
+    p.b = c + 1;
+}
+
+__global__ void k_level_apply(Ctx C, uint32_t ls, uint32_t cnt) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    const uint32_t ni = ls + k;
+    const Dec d = C.dec[k];
+    const BNode nd = C.nodes[ni];
+    const uint32_t begin = nd.a, n = nd.b, end = begin + n, c = C.child[k];
+    if (d.type == D_LEAF || c == kInvalid) return;
+    if (c + 1 >= C.ncap) {  // cannot happen for a valid build (nodes <= 2 * cap - 1)
+        atomicOr(&C.ctr->err, 1u);
+        return;
+    }
+    if (d.type == D_MORTON) {
+        // Children of a Morton split (bvh_bfs.cpp:109-130). A child that will itself be
+        // split by Morton bits gets its box from the bottom-up refit; the others now.
+        const uint32_t cb[2] = {begin, d.mid}, ce[2] = {d.mid, end};
+        for (int s = 0; s < 2; ++s) {
+            uint32_t sz = ce[s] - cb[s];
+            uint32_t fl = F_LEAF | (sz <= (uint32_t)kLbvhThreshold ? F_SAH : 0u);
+            bool direct = sz <= (uint32_t)kLbvhThreshold || C.scodes[cb[s]] == C.scodes[ce[s] - 1];
+            Box b = Box::empty();
+            if (direct) b = range_box(C, cb[s], ce[s]);
+            else fl |= F_PENDING;
+            set_child(C, c + s, b, cb[s], sz, fl);
+        }
+        BNode& p = C.nodes[ni];
+        p.flags &= ~F_LEAF;
+        p.a = c;
+        p.b = c + 1;
+        return;
+    }
+    if (d.type == D_OBJECT) { object_children(C, ni, begin, d.mid, end, c); return; }
+    // D_SPATIAL
+    if (!C.acc[k]) {
+        // Budget exhausted: object-split fallback on the spatially partitioned range.
+        uint32_t mid = partition_idx(C.tindex, begin, end, [&](uint32_t si) {
+            return obj_pred(C, si, d.oAxis, d.oBin, d.oCMin, d.oScale, d.nb);
+        });
+        object_children(C, ni, begin, mid, end, c);
+        return;
+    }
+    // Accepted spatial split: append right-only references, duplicate straddlers
+    // (bvh_sah.cpp:396-414), and clip children to the split plane (:427-445).
+    const uint32_t tOff = C.toff[k], sOff = C.soff[k];
+    const uint32_t leftEnd = begin + d.nl, strEnd = leftEnd + d.ns;
+    for (uint32_t q = 0; q < d.nr; ++q) C.tindex[tOff + q] = C.tindex[strEnd + q];
+    for (uint32_t q = 0; q < d.ns; ++q) {
+        uint32_t old = C.tindex[leftEnd + q];
+        C.sorted[sOff + q] = C.sorted[old];
+        C.tindex[tOff + d.nr + q] = sOff + q;
+    }
+    const uint32_t lsize = d.nl + d.ns, rsize = d.nr + d.ns;
+    Box pbox{nd.lo, nd.hi};
+    Box lclip = pbox, rclip = pbox;
+    set_comp(lclip.hi, d.sAxis, d.plane);
+    set_comp(rclip.lo, d.sAxis, d.plane);
+    Box lb = Box::empty(), rb = Box::empty();
+    for (uint32_t q = begin; q < begin + lsize; ++q) {
+        f3 p0, p1, p2;
+        tri_verts(C, C.sorted[C.tindex[q]], p0, p1, p2);
+        Box cb = clip_tri(p0, p1, p2, lclip);
+        if (cb.valid()) lb.extend(cb);
+    }
+    for (uint32_t q = tOff; q < tOff + rsize; ++q) {
+        f3 p0, p1, p2;
+        tri_verts(C, C.sorted[C.tindex[q]], p0, p1, p2);
+        Box cb = clip_tri(p0, p1, p2, rclip);
+        if (cb.valid()) rb.extend(cb);
+    }
+    set_child(C, c, lb, begin, lsize, F_LEAF | F_SAH);
+    set_child(C, c + 1, rb, tOff, rsize, F_LEAF | F_SAH);
+    BNode& p = C.nodes[ni];
+    p.flags &= ~F_LEAF;
+    p.a = c;
+    p.b = c + 1;
+}
+
+// Bottom-up boxes for Morton-phase nodes: box(parent range) = ordered combine of the
+// boxes of its two consecutive child ranges.
+__global__ void k_refit(BNode* nodes, uint32_t ls, uint32_t cnt) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    BNode& n = nodes[ls + k];
+    if (!(n.flags & F_PENDING) || (n.flags & F_LEAF)) return;
+    const BNode& l = nodes[n.a];
+    const BNode& r = nodes[n.b];
+    Box b{l.lo, l.hi};
+    b.extend(Box{r.lo, r.hi});
+    n.lo = b.lo;
+    n.hi = b.hi;
+    n.flags &= ~F_PENDING;
+}
+
+struct Dfs {
+    uint32_t *size, *tcnt, *icnt, *lcnt;
+    uint32_t *dfs, *toff, *ioff, *loff;
+};
+
+__global__ void k_subtree(const BNode* nodes, uint32_t ls, uint32_t cnt, Dfs D) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    uint32_t i = ls + k;
+    const BNode n = nodes[i];
+    if (n.flags & F_LEAF) {
+        D.size[i] = 1; D.tcnt[i] = n.b; D.icnt[i] = 0; D.lcnt[i] = 1;
+    } else {
+        D.size[i] = 1 + D.size[n.a] + D.size[n.b];
+        D.tcnt[i] = D.tcnt[n.a] + D.tcnt[n.b];
+        D.icnt[i] = 1 + D.icnt[n.a] + D.icnt[n.b];
+        D.lcnt[i] = D.lcnt[n.a] + D.lcnt[n.b];
+    }
+}
+
+// DFS pre-order, left first (reorderNodes, bvh_bake.cpp:75-118); leaves own consecutive
+// triangle runs in that order (reorderTriangles, :120-149).
+__global__ void k_dfs(const BNode* nodes, uint32_t ls, uint32_t cnt, Dfs D) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    uint32_t p = ls + k;
+    if (p == 0) { D.dfs[0] = 0; D.toff[0] = 0; D.ioff[0] = 0; D.loff[0] = 0; }
+    const BNode n = nodes[p];
+    if (n.flags & F_LEAF) return;
+    uint32_t dp = (p == 0) ? 0 : D.dfs[p], tp = (p == 0) ? 0 : D.toff[p];
+    uint32_t ip = (p == 0) ? 0 : D.ioff[p], lp = (p == 0) ? 0 : D.loff[p];
+    D.dfs[n.a] = dp + 1;
+    D.dfs[n.b] = dp + 1 + D.size[n.a];
+    D.toff[n.a] = tp;
+    D.toff[n.b] = tp + D.tcnt[n.a];
+    D.ioff[n.a] = ip + 1;
+    D.ioff[n.b] = ip + 1 + D.icnt[n.a];
+    D.loff[n.a] = lp;
+    D.loff[n.b] = lp + D.lcnt[n.a];
+}
+
+struct Out {
+    Node48* nodes48;
+    Node48* bfs48;
+    uint32_t* tris3;
+    uint32_t* prims;
+    GNode* gnodes;
+    Leaf* leaves;
+    GTri* gtris;
+};
+
+__device__ __forceinline__ Node48 to48(const BNode& n, uint32_t a, uint32_t b) {
+    Node48 o;
+    o.bmin[0] = n.lo.x; o.bmin[1] = n.lo.y; o.bmin[2] = n.lo.z; o.bmin[3] = 0.f;
+    o.bmax[0] = n.hi.x; o.bmax[1] = n.hi.y; o.bmax[2] = n.hi.z; o.bmax[3] = 0.f;
+    o.is_leaf = (n.flags & F_LEAF) ? 1 : 0;
+    o.pad0[0] = o.pad0[1] = o.pad0[2] = 0;
+    o.a = a;
+    o.b = b;
+    o.pad1 = 0;
+    return o;
+}
+
+__global__ void k_emit(Ctx C, uint32_t total, Dfs D, Out O) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const BNode n = C.nodes[i];
+    const bool leaf = (n.flags & F_LEAF) != 0;
+    O.bfs48[i] = to48(n, n.a, n.b);
+    O.nodes48[D.dfs[i]] = to48(n, leaf ? D.toff[i] : D.dfs[n.a], leaf ? n.b : D.dfs[n.b]);
+    if (!leaf) {
+        const BNode l = C.nodes[n.a], r = C.nodes[n.b];
+        GNode g;
+        g.l0 = make_float4(l.lo.x, l.lo.y, l.lo.z, l.hi.x);
+        g.l1 = make_float4(l.hi.y, l.hi.z, r.lo.x, r.lo.y);
+        g.r1 = make_float4(r.lo.z, r.hi.x, r.hi.y, r.hi.z);
+        g.lref = (l.flags & F_LEAF) ? (kLeafBit | D.loff[n.a]) : D.ioff[n.a];
+        g.rref = (r.flags & F_LEAF) ? (kLeafBit | D.loff[n.b]) : D.ioff[n.b];
+        g.pad0 = g.pad1 = 0;
+        O.gnodes[D.ioff[i]] = g;
+        return;
+    }
+    const uint32_t o0 = D.toff[i];
+    O.leaves[D.loff[i]] = Leaf{o0, n.b};
+    for (uint32_t k = 0; k < n.b; ++k) {
+        uint32_t t = C.sorted[C.tindex[n.a + k]];
+        uint32_t o = o0 + k;
+        uint32_t i0 = C.tris[3 * t], i1 = C.tris[3 * t + 1], i2 = C.tris[3 * t + 2];
+        O.tris3[3 * o] = i0; O.tris3[3 * o + 1] = i1; O.tris3[3 * o + 2] = i2;
+        O.prims[o] = t;
+        f3 v0 = vpos(C.verts, i0), v1 = vpos(C.verts, i1), v2 = vpos(C.verts, i2);
+        f3 e1 = sub(v1, v0), e2 = sub(v2, v0);
+        GTri g;
+        g.a = make_float4(v0.x, v0.y, v0.z, e1.x);
+        g.b = make_float4(e1.y, e1.z, e2.x, e2.y);
+        g.c = make_float4(e2.z, __uint_as_float(t), 0.f, 0.f);
+        O.gtris[o] = g;
+    }
+}
+
+// Traversal layout from a reference-layout tree (a loaded .bvh file).
+__global__ void k_from_ref(const Node48* nodes, uint32_t nn, const uint32_t* iidx, const uint32_t* lidx,
+                           GNode* gnodes, Leaf* leaves) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const Node48 n = nodes[i];
+    if (n.is_leaf) {
+        leaves[lidx[i]] = Leaf{n.a, n.b};
+        return;
+    }
+    const Node48 l = nodes[n.a], r = nodes[n.b];
+    GNode g;
+    g.l0 = make_float4(l.bmin[0], l.bmin[1], l.bmin[2], l.bmax[0]);
+    g.l1 = make_float4(l.bmax[1], l.bmax[2], r.bmin[0], r.bmin[1]);
+    g.r1 = make_float4(r.bmin[2], r.bmax[0], r.bmax[1], r.bmax[2]);
+    g.lref = l.is_leaf ? (kLeafBit | lidx[n.a]) : iidx[n.a];
+    g.rref = r.is_leaf ? (kLeafBit | lidx[n.b]) : iidx[n.b];
+    g.pad0 = g.pad1 = 0;
+    gnodes[iidx[i]] = g;
+}
+
+__global__ void k_gtris_from_ref(const uint32_t* tris3, const uint32_t* prims, uint32_t ntri, const float* verts,
+                                 GTri* gtris) {
+    uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= ntri) return;
+    f3 v0 = vpos(verts, tris3[3 * o]), v1 = vpos(verts, tris3[3 * o + 1]), v2 = vpos(verts, tris3[3 * o + 2]);
+    f3 e1 = sub(v1, v0), e2 = sub(v2, v0);
+    GTri g;
+    g.a = make_float4(v0.x, v0.y, v0.z, e1.x);
+    g.b = make_float4(e1.y, e1.z, e2.x, e2.y);
+    g.c = make_float4(e2.z, __uint_as_float(prims ? prims[o] : o), 0.f, 0.f);
+    gtris[o] = g;
+}
+
+inline uint32_t nblk(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
+
+// Scratch buffers of one build; freed on destruction.
+struct Scratch {
+    std::vector<void*> ptrs;
+    template <class T>
+    T* get(size_t n) {
+        T* p = dalloc<T>(n);
+        if (p) ptrs.push_back(p);
+        return p;
+    }
+    ~Scratch() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+void free_products(rtx_bvh* b) {
+    void* ps[] = {b->verts, b->nodes48, b->tris3, b->prims, b->gnodes, b->leaves, b->gtris, b->codes, b->sorted, b->bfs48};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host orchestration
+// ---------------------------------------------------------------------------
+static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d, uint32_t nt, hipStream_t st,
+                      rtx_bvh* B) {
+    if (nt == 0 || nv == 0) return fail(RTX_E_ARG, "no triangles or vertices on rtx_build()");
+    if (nt > (1u << 27)) return fail(RTX_E_LIMIT, "more than 2^27 triangles on rtx_build()");
+    Scratch S;
+    const uint32_t N = nt;
+    const uint32_t cap = std::max<uint32_t>((uint32_t)((float)N * 1.5f), N);
+    const uint32_t ncap = 2 * cap + 1;
+
+    uint32_t* err = S.get<uint32_t>(1);
+    float4* tbox = S.get<float4>(2 * (size_t)N);
+    float4* cent = S.get<float4>(N);
+    float* box6 = S.get<float>(8);
+    uint64_t* keysB = S.get<uint64_t>(N);
+    uint64_t* keysC = S.get<uint64_t>(N);
+    uint32_t* valsA = S.get<uint32_t>(N);
+    uint32_t* valsB = S.get<uint32_t>(N);
+    uint32_t* valsC = S.get<uint32_t>(N);
+    unsigned long long* orand = S.get<unsigned long long>(2);
+    uint32_t* sorted = S.get<uint32_t>(cap);
+    uint32_t* tindex = S.get<uint32_t>(cap);
+    BNode* nodes = S.get<BNode>(ncap);
+    Dec* dec = S.get<Dec>(cap + 1);
+    uint32_t* child = S.get<uint32_t>(cap + 1);
+    uint32_t* toff = S.get<uint32_t>(cap + 1);
+    uint32_t* soff = S.get<uint32_t>(cap + 1);
+    uint32_t* acc = S.get<uint32_t>(cap + 1);
+    Counters* ctr = S.get<Counters>(1);
+    Dfs D;
+    uint32_t* dbuf = S.get<uint32_t>((size_t)ncap * 8);
+    B->codes = dalloc<uint64_t>(N);
+    B->sorted = dalloc<uint32_t>(N);
+    B->verts = dalloc<float>((size_t)nv * 6);
+    if (!err || !tbox || !cent || !box6 || !keysB || !keysC || !valsA || !valsB || !valsC || !orand || !sorted ||
+        !tindex || !nodes || !dec || !child || !toff || !soff || !acc || !ctr || !dbuf || !B->codes || !B->sorted ||
+        !B->verts)
+        return fail(RTX_E_HIP, "device allocation failed on rtx_build()");
+    D.size = dbuf; D.tcnt = dbuf + ncap; D.icnt = dbuf + 2 * (size_t)ncap; D.lcnt = dbuf + 3 * (size_t)ncap;
+    D.dfs = dbuf + 4 * (size_t)ncap; D.toff = dbuf + 5 * (size_t)ncap; D.ioff = dbuf + 6 * (size_t)ncap;
+    D.loff = dbuf + 7 * (size_t)ncap;
+
+    RTX_HIP(hipMemcpyAsync(B->verts, verts_d, sizeof(float) * 6 * nv, hipMemcpyDeviceToDevice, st));
+    RTX_HIP(hipMemsetAsync(err, 0, 4, st));
+    k_check_indices<<<nblk(3ull * N, 256), 256, 0, st>>>(tris_d, 3 * N, nv, err);
+    uint32_t herr = 0;
+    RTX_HIP(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st));
+    RTX_HIP(hipStreamSynchronize(st));
+    if (herr) return fail(RTX_E_ARG, "triangle has out-of-bounds vertex index on rtx_build()");
+
+    const float* V = B->verts;
+    k_tri_prep<<<nblk(N, 256), 256, 0, st>>>(V, tris_d, N, tbox, cent);
+    // scene box: ordered two-level reduction
+    const uint32_t chunk = 16;
+    const uint32_t nparts = nblk(N, chunk * kRedThreads);
+    float4* part = S.get<float4>(2 * (size_t)nparts);
+    if (!part) return fail(RTX_E_HIP, "device allocation failed on rtx_build()");
+    k_scene_box_partial<<<nparts, kRedThreads, 0, st>>>(tbox, N, chunk, part);
+    k_scene_box_final<<<1, 64, 0, st>>>(part, nparts, box6);
+    unsigned long long orand_init[2] = {0ull, ~0ull};
+    RTX_HIP(hipMemcpyAsync(orand, orand_init, sizeof(orand_init), hipMemcpyHostToDevice, st));
+    k_morton<<<nblk(N, 256), 256, 0, st>>>(cent, N, box6, B->codes, valsA, orand);
+    unsigned long long hor[2];
+    RTX_HIP(hipMemcpyAsync(hor, orand, sizeof(hor), hipMemcpyDeviceToHost, st));
+    RTX_HIP(hipMemcpyAsync(B->scene, box6, 6 * sizeof(float), hipMemcpyDeviceToHost, st));
+    RTX_HIP(hipStreamSynchronize(st));
+
+    // radix sort: skip passes whose digit is the same for every key
+    const uint32_t ntiles = nblk(N, kRsTile);
+    uint32_t* hist = S.get<uint32_t>(256 * (size_t)ntiles);
+    if (!hist) return fail(RTX_E_HIP, "device allocation failed on rtx_build()");
+    const uint64_t vary = hor[0] ^ hor[1];
+    const uint64_t* kin = B->codes;
+    const uint32_t* vin = valsA;
+    uint64_t* kbuf[2] = {keysB, keysC};
+    uint32_t* vbuf[2] = {valsB, valsC};
+    int flip = 0;
+    for (int pass = 0; pass < 8; ++pass) {
+        int shift = 8 * pass;
+        if (((vary >> shift) & 255u) == 0) continue;
+        k_rs_hist<<<ntiles, kRsThreads, 0, st>>>(kin, N, shift, hist, ntiles);
+        k_scan_excl<<<1, 1024, 0, st>>>(hist, 256 * ntiles);
+        k_rs_scatter<<<ntiles, kRsThreads, 0, st>>>(kin, vin, kbuf[flip], vbuf[flip], N, shift, hist, ntiles);
+        kin = kbuf[flip];
+        vin = vbuf[flip];
+        flip ^= 1;
+    }
+    const uint64_t* scodes = kin;
+    RTX_HIP(hipMemcpyAsync(sorted, vin, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, st));
+    RTX_HIP(hipMemcpyAsync(B->sorted, vin, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, st));
+    k_iota<<<nblk(N, 256), 256, 0, st>>>(tindex, N);
+    k_init_root<<<1, 64, 0, st>>>(nodes, box6, N, tindex, ctr);
+
+    Ctx C{V, tris_d, tbox, cent, scodes, sorted, tindex, nodes, dec, child, toff, soff, acc, cap, ncap, ctr};
+    std::vector<std::pair<uint32_t, uint32_t>> levels;
+    uint32_t ls = 0, le = 1;
+    Counters hc{};
+    while (ls < le) {
+        uint32_t cnt = le - ls;
+        levels.push_back({ls, cnt});
+        k_level_decide<<<nblk(cnt, 64), 64, 0, st>>>(C, ls, cnt);
+        k_level_budget<<<1, 1024, 0, st>>>(C, cnt);
+        k_level_apply<<<nblk(cnt, 64), 64, 0, st>>>(C, ls, cnt);
+        RTX_HIP(hipMemcpyAsync(&hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
+        RTX_HIP(hipStreamSynchronize(st));
+        if (hc.err || hc.total > ncap) return fail(RTX_E_LIMIT, "node capacity exceeded on rtx_build()");
+        ls = le;
+        le = hc.total;
+    }
+    const uint32_t total = hc.total;
+    for (size_t l = levels.size(); l-- > 0;)
+        k_refit<<<nblk(levels[l].second, 256), 256, 0, st>>>(nodes, levels[l].first, levels[l].second);
+    for (size_t l = levels.size(); l-- > 0;)
+        k_subtree<<<nblk(levels[l].second, 256), 256, 0, st>>>(nodes, levels[l].first, levels[l].second, D);
+    for (size_t l = 0; l < levels.size(); ++l)
+        k_dfs<<<nblk(levels[l].second, 256), 256, 0, st>>>(nodes, levels[l].first, levels[l].second, D);
+    uint32_t root_counts[4];  // tcnt, icnt, lcnt
+    RTX_HIP(hipMemcpyAsync(&root_counts[0], D.tcnt, 4, hipMemcpyDeviceToHost, st));
+    RTX_HIP(hipMemcpyAsync(&root_counts[1], D.icnt, 4, hipMemcpyDeviceToHost, st));
+    RTX_HIP(hipMemcpyAsync(&root_counts[2], D.lcnt, 4, hipMemcpyDeviceToHost, st));
+    RTX_HIP(hipStreamSynchronize(st));
+    B->nn = total;
+    B->ntri = root_counts[0];
+    B->ninternal = root_counts[1];
+    B->nleaves = root_counts[2];
+    B->nodes48 = dalloc<Node48>(total);
+    B->bfs48 = dalloc<Node48>(total);
+    B->tris3 = dalloc<uint32_t>(3 * (size_t)B->ntri);
+    B->prims = dalloc<uint32_t>(B->ntri);
+    B->gnodes = dalloc<GNode>(std::max<uint32_t>(B->ninternal, 1));
+    B->leaves = dalloc<Leaf>(B->nleaves);
+    B->gtris = dalloc<GTri>(B->ntri);
+    if (!B->nodes48 || !B->bfs48 || !B->tris3 || !B->prims || !B->gnodes || !B->leaves || !B->gtris)
+        return fail(RTX_E_HIP, "device allocation failed on rtx_build()");
+    Out O{B->nodes48, B->bfs48, B->tris3, B->prims, B->gnodes, B->leaves, B->gtris};
+    k_emit<<<nblk(total, 128), 128, 0, st>>>(C, total, D, O);
+    BNode root;
+    RTX_HIP(hipMemcpyAsync(&root, nodes, sizeof(root), hipMemcpyDeviceToHost, st));
+    RTX_HIP(hipStreamSynchronize(st));
+    B->root_box[0] = root.lo.x; B->root_box[1] = root.lo.y; B->root_box[2] = root.lo.z;
+    B->root_box[3] = root.hi.x; B->root_box[4] = root.hi.y; B->root_box[5] = root.hi.z;
+    B->root_ref = (root.flags & F_LEAF) ? kLeafBit : 0u;
+    B->nv = nv;
+    B->nt = nt;
+    B->levels = (uint32_t)levels.size();
+    B->max_depth = B->levels - 1;
+    B->tindex_size = hc.tsize;
+    B->sorted_size = hc.ssize;
+    return RTX_OK;
+}
+
+int upload_reference_tree(rtx_bvh* B, const Node48* nodes, uint32_t nn, const uint32_t* tris3, uint32_t ntri,
+                          const float* verts6, uint32_t nv) {
+    if (nn == 0) return fail(RTX_E_ARG, "empty tree");
+    std::vector<uint32_t> iidx(nn), lidx(nn);
+    uint32_t ni = 0, nl = 0;
+    for (uint32_t i = 0; i < nn; ++i) {
+        if (nodes[i].is_leaf) {
+            lidx[i] = nl++;
+            if ((uint64_t)nodes[i].a + nodes[i].b > ntri) return fail(RTX_E_ARG, "leaf triangle range out of bounds");
+        } else {
+            iidx[i] = ni++;
+            if (nodes[i].a >= nn || nodes[i].b >= nn) return fail(RTX_E_ARG, "child index out of bounds");
+        }
+    }
+    for (uint64_t i = 0; i < 3ull * ntri; ++i)
+        if (tris3[i] >= nv) return fail(RTX_E_ARG, "triangle vertex index out of bounds");
+    B->nn = nn; B->ntri = ntri; B->nv = nv; B->nt = ntri;
+    B->ninternal = ni; B->nleaves = nl;
+    B->nodes48 = dalloc<Node48>(nn);
+    B->tris3 = dalloc<uint32_t>(3 * (size_t)std::max<uint32_t>(ntri, 1));
+    B->prims = dalloc<uint32_t>(std::max<uint32_t>(ntri, 1));
+    B->verts = dalloc<float>(6 * (size_t)nv);
+    B->gnodes = dalloc<GNode>(std::max<uint32_t>(ni, 1));
+    B->leaves = dalloc<Leaf>(std::max<uint32_t>(nl, 1));
+    B->gtris = dalloc<GTri>(std::max<uint32_t>(ntri, 1));
+    uint32_t* d_i = dalloc<uint32_t>(nn);
+    uint32_t* d_l = dalloc<uint32_t>(nn);
+    if (!B->nodes48 || !B->tris3 || !B->prims || !B->verts || !B->gnodes || !B->leaves || !B->gtris || !d_i || !d_l)
+        return fail(RTX_E_HIP, "device allocation failed on rtx_load()");
+    std::vector<uint32_t> prims(ntri);
+    for (uint32_t i = 0; i < ntri; ++i) prims[i] = i;  // a .bvh file keeps no original ids
+    bool ok = hip_ok(hipMemcpy(B->nodes48, nodes, sizeof(Node48) * nn, hipMemcpyHostToDevice), "copy") &&
+              hip_ok(hipMemcpy(B->tris3, tris3, 12ull * ntri, hipMemcpyHostToDevice), "copy") &&
+              hip_ok(hipMemcpy(B->prims, prims.data(), 4ull * ntri, hipMemcpyHostToDevice), "copy") &&
+              hip_ok(hipMemcpy(B->verts, verts6, 24ull * nv, hipMemcpyHostToDevice), "copy") &&
+              hip_ok(hipMemcpy(d_i, iidx.data(), 4ull * nn, hipMemcpyHostToDevice), "copy") &&
+              hip_ok(hipMemcpy(d_l, lidx.data(), 4ull * nn, hipMemcpyHostToDevice), "copy");
+    if (ok) {
+        k_from_ref<<<nblk(nn, 256), 256>>>(B->nodes48, nn, d_i, d_l, B->gnodes, B->leaves);
+        if (ntri) k_gtris_from_ref<<<nblk(ntri, 256), 256>>>(B->tris3, B->prims, ntri, B->verts, B->gtris);
+        ok = hip_ok(hipDeviceSynchronize(), "k_from_ref");
+    }
+    (void)hipFree(d_i);
+    (void)hipFree(d_l);
+    if (!ok) return RTX_E_HIP;
+    for (int k = 0; k < 3; ++k) { B->root_box[k] = nodes[0].bmin[k]; B->root_box[3 + k] = nodes[0].bmax[k]; }
+    B->root_ref = nodes[0].is_leaf ? kLeafBit : 0u;
+    // Depth by a host walk; every node must be reached exactly once (a tree, no cycles).
+    std::vector<uint8_t> seen(nn, 0);
+    std::vector<std::pair<uint32_t, uint32_t>> st{{0u, 0u}};
+    uint32_t depth = 0, reached = 0;
+    while (!st.empty()) {
+        auto [i, d] = st.back();
+        st.pop_back();
+        if (seen[i]) return fail(RTX_E_ARG, "node reached twice: not a tree");
+        seen[i] = 1;
+        ++reached;
+        depth = std::max(depth, d);
+        if (!nodes[i].is_leaf) { st.push_back({nodes[i].a, d + 1}); st.push_back({nodes[i].b, d + 1}); }
+    }
+    B->max_depth = depth;
+    return RTX_OK;
+}
+
+}  // namespace rtx
+
+using namespace rtx;
+
+extern "C" int rtx_build_device(const float* verts6_d, uint32_t nv, const uint32_t* tris_d, uint32_t nt, void* stream,
+                                rtx_bvh** out) {
+    if (!out || !verts6_d || !tris_d) return fail(RTX_E_ARG, "null argument on rtx_build_device()");
+    *out = nullptr;
+    rtx_bvh* b = new rtx_bvh;
+    (void)hipGetDevice(&b->device);
+    int rc = build_impl(verts6_d, nv, tris_d, nt, (hipStream_t)stream, b);
+    if (rc != RTX_OK) {
+        free_products(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return RTX_OK;
+}
+
+extern "C" int rtx_build(const float* verts6, uint32_t nv, const uint32_t* tris, uint32_t nt, rtx_bvh** out) {
+    if (!out || !verts6 || !tris) return fail(RTX_E_ARG, "null argument on rtx_build()");
+    if (nt == 0 || nv == 0) return fail(RTX_E_ARG, "no triangles or vertices on rtx_build()");
+    float* dv = dalloc<float>(6 * (size_t)nv);
+    uint32_t* dt = dalloc<uint32_t>(3 * (size_t)nt);
+    int rc = RTX_OK;
+    if (!dv || !dt) rc = fail(RTX_E_HIP, "device allocation failed on rtx_build()");
+    if (rc == RTX_OK && !(hip_ok(hipMemcpy(dv, verts6, 24ull * nv, hipMemcpyHostToDevice), "copy") &&
+                          hip_ok(hipMemcpy(dt, tris, 12ull * nt, hipMemcpyHostToDevice), "copy")))
+        rc = RTX_E_HIP;
+    if (rc == RTX_OK) rc = rtx_build_device(dv, nv, dt, nt, nullptr, out);
+    if (dv) (void)hipFree(dv);
+    if (dt) (void)hipFree(dt);
+    return rc;
+}
+
+extern "C" void rtx_bvh_free(rtx_bvh* b) {
+    if (!b) return;
+    free_products(b);
+    delete b;
+}

@@ -1,0 +1,268 @@
+"""Python host side of librtx (MI355X BVH build, traversal and path tracing).
+
+Mirrors the reference API of minsuk0519/ray-tracer for the hot path:
+    bake_bvh(scene_path)   ~ bvh::bakeBVH   (src/bvh/bvh_bake.cpp:180-194)
+    load_bvh(path)         ~ bvh::loadBVH   (src/bvh/bvh_traverse.cpp:182-235)
+    BVH.trace(rays)        ~ bvh::trace     (src/bvh/bvh_traverse.cpp:237-246), batched
+Everything runs through the C ABI of librtx.so (include/rtx.h) on the GPU; there is no
+CPU fallback. Importing works without a GPU; calls that need one raise RtxError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(HERE, "..", "lib", "librtx.so"))
+
+NODE48 = np.dtype([("bmin", "<f4", 4), ("bmax", "<f4", 4), ("is_leaf", "u1"), ("pad0", "u1", 3),
+                   ("a", "<u4"), ("b", "<u4"), ("pad1", "<u4")])
+
+# Every symbol include/rtx.h declares.
+EXPORTS = ["rtx_last_error", "rtx_device_count", "rtx_build", "rtx_build_device", "rtx_bvh_free",
+           "rtx_bvh_get_info", "rtx_bvh_export", "rtx_bvh_morton", "rtx_bvh_export_bfs", "rtx_trace_closest",
+           "rtx_trace_any", "rtx_trace_counts", "rtx_trace_closest_host", "rtx_save", "rtx_load",
+           "rtx_scene_load", "rtx_free_host", "rtx_bake_scene", "rtx_geo_sphere", "rtx_geo_box", "rtx_render",
+           "rtx_render_adjoint", "rtx_render_batched"]
+
+
+class RtxError(RuntimeError):
+    pass
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("node_count", C.c_uint32), ("tri_count", C.c_uint32), ("vert_count", C.c_uint32),
+                ("input_tri_count", C.c_uint32), ("internal_count", C.c_uint32), ("leaf_count", C.c_uint32),
+                ("max_depth", C.c_uint32), ("tri_index_size", C.c_uint32), ("sorted_tris_size", C.c_uint32),
+                ("levels", C.c_uint32), ("scene_min", C.c_float * 3), ("scene_max", C.c_float * 3)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("fwd", C.c_float * 3), ("right", C.c_float * 3),
+                ("up", C.c_float * 3), ("tan_half_fov_y", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("max_bounce", C.c_int),
+                ("rr_bounce", C.c_int), ("seed", C.c_uint32), ("env_mode", C.c_int),
+                ("env_color", C.c_float * 3)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load librtx.so; raise if it has not been built (no fallback path exists)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RtxError(f"librtx.so not built at {LIB_PATH}; run ray-tracer_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp, u32, sz, i32 = C.c_void_p, C.c_uint32, C.c_size_t, C.c_int
+        L.rtx_last_error.restype = C.c_char_p
+        L.rtx_build.argtypes = [vp, u32, vp, u32, C.POINTER(vp)]
+        L.rtx_build_device.argtypes = [vp, u32, vp, u32, vp, C.POINTER(vp)]
+        L.rtx_bvh_free.argtypes = [vp]
+        L.rtx_bvh_free.restype = None
+        L.rtx_bvh_get_info.argtypes = [vp, C.POINTER(BvhInfo)]
+        L.rtx_bvh_export.argtypes = [vp, vp, vp, vp]
+        L.rtx_bvh_morton.argtypes = [vp, vp, vp]
+        L.rtx_bvh_export_bfs.argtypes = [vp, vp, C.POINTER(u32)]
+        L.rtx_trace_closest.argtypes = [vp, vp, sz, vp, vp, vp]
+        L.rtx_trace_any.argtypes = [vp, vp, sz, vp, vp]
+        L.rtx_trace_counts.argtypes = [vp, vp, sz, vp, vp]
+        L.rtx_trace_closest_host.argtypes = [vp, vp, sz, vp, vp]
+        L.rtx_save.argtypes = [vp, C.c_char_p]
+        L.rtx_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.rtx_scene_load.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(u32),
+                                     C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(u32)]
+        L.rtx_free_host.argtypes = [vp]
+        L.rtx_free_host.restype = None
+        L.rtx_bake_scene.argtypes = [C.c_char_p]
+        L.rtx_geo_sphere.argtypes = [vp, C.c_float, i32, i32, vp, C.POINTER(u32), vp, C.POINTER(u32)]
+        L.rtx_geo_box.argtypes = [vp, vp, vp, vp, C.POINTER(u32), vp, C.POINTER(u32)]
+        if hasattr(L, "rtx_render"):
+            L.rtx_render.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp]
+            L.rtx_render_adjoint.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp, vp,
+                                             vp]
+            L.rtx_render_batched.argtypes = [vp, i32, vp, vp, vp, vp, vp, i32, vp]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().rtx_last_error()
+        raise RtxError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _dptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return C.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+
+class BVH:
+    """A tree resident in HBM (owned by librtx)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.rtx_bvh_free(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        inf = BvhInfo()
+        _check(lib().rtx_bvh_get_info(self._h, C.byref(inf)), "rtx_bvh_get_info")
+        d = {k: getattr(inf, k) for k, _ in BvhInfo._fields_ if k not in ("scene_min", "scene_max")}
+        d["scene_min"] = list(inf.scene_min)
+        d["scene_max"] = list(inf.scene_max)
+        return d
+
+    def export(self):
+        """(nodes [reference BVHNode layout, DFS order], reordered triangles, original ids)."""
+        inf = self.info()
+        nodes = np.zeros(inf["node_count"], NODE48)
+        tris = np.zeros((inf["tri_count"], 3), np.uint32)
+        prims = np.zeros(inf["tri_count"], np.uint32)
+        _check(lib().rtx_bvh_export(self._h, _p(nodes), _p(tris), _p(prims)), "rtx_bvh_export")
+        return nodes, tris, prims
+
+    def export_bfs(self):
+        n = C.c_uint32()
+        _check(lib().rtx_bvh_export_bfs(self._h, None, C.byref(n)), "rtx_bvh_export_bfs")
+        nodes = np.zeros(n.value, NODE48)
+        _check(lib().rtx_bvh_export_bfs(self._h, _p(nodes), C.byref(n)), "rtx_bvh_export_bfs")
+        return nodes
+
+    def morton(self):
+        n = self.info()["input_tri_count"]
+        codes = np.zeros(n, np.uint64)
+        srt = np.zeros(n, np.uint32)
+        _check(lib().rtx_bvh_morton(self._h, _p(codes), _p(srt)), "rtx_bvh_morton")
+        return codes, srt
+
+    def save(self, path):
+        _check(lib().rtx_save(self._h, str(path).encode()), "rtx_save")
+
+    # ---- tracing --------------------------------------------------------------
+    def trace(self, rays, normals=False):
+        """Closest hit for host rays (N, 8) float32. Returns dict t/u/v/tri[/normal]."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = len(rays)
+        hits = np.zeros((n, 4), np.float32)
+        nrm = np.zeros((n, 3), np.float32) if normals else None
+        _check(lib().rtx_trace_closest_host(self._h, _p(rays), n, _p(hits), _p(nrm)), "rtx_trace_closest_host")
+        out = {"t": hits[:, 0].copy(), "u": hits[:, 1].copy(), "v": hits[:, 2].copy(),
+               "tri": hits[:, 3].view(np.uint32).copy()}
+        if normals:
+            out["normal"] = nrm
+        return out
+
+    def trace_closest_device(self, rays_d, hits_d, normals_d=None, stream=None):
+        """Device-resident closest hit: rays_d (N, 8) float32, hits_d (N, 4) 16-byte records."""
+        n = rays_d.shape[0]
+        _check(lib().rtx_trace_closest(self._h, _dptr(rays_d), n, _dptr(hits_d), _dptr(normals_d),
+                                       _stream_ptr(stream)), "rtx_trace_closest")
+
+    def trace_any_device(self, rays_d, occl_d, stream=None):
+        _check(lib().rtx_trace_any(self._h, _dptr(rays_d), rays_d.shape[0], _dptr(occl_d), _stream_ptr(stream)),
+               "rtx_trace_any")
+
+    def trace_counts_device(self, rays_d, counts_d, stream=None):
+        _check(lib().rtx_trace_counts(self._h, _dptr(rays_d), rays_d.shape[0], _dptr(counts_d),
+                                      _stream_ptr(stream)), "rtx_trace_counts")
+
+
+def build(verts6, tris):
+    """GPU build from host arrays (verts (V,6) float32, tris (N,3) uint32)."""
+    v = np.ascontiguousarray(verts6, np.float32).reshape(-1, 6)
+    t = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+    h = C.c_void_p()
+    _check(lib().rtx_build(_p(v), len(v), _p(t), len(t), C.byref(h)), "rtx_build")
+    return BVH(h)
+
+
+def build_device(verts_d, tris_d, stream=None):
+    """GPU build from device tensors (verts (V,6) float32, tris (N,3) int32/uint32)."""
+    h = C.c_void_p()
+    _check(lib().rtx_build_device(_dptr(verts_d), verts_d.shape[0], _dptr(tris_d), tris_d.shape[0],
+                                  _stream_ptr(stream), C.byref(h)), "rtx_build_device")
+    return BVH(h)
+
+
+def load_bvh(path):
+    h = C.c_void_p()
+    _check(lib().rtx_load(str(path).encode(), C.byref(h)), "rtx_load")
+    return BVH(h)
+
+
+def bake_bvh(scene_path):
+    """Scene file -> GPU build -> <scene>.bvh, like bvh::bakeBVH."""
+    _check(lib().rtx_bake_scene(str(scene_path).encode()), "rtx_bake_scene")
+
+
+def load_scene(scene_path):
+    v = C.POINTER(C.c_float)()
+    t = C.POINTER(C.c_uint32)()
+    nv, nt = C.c_uint32(), C.c_uint32()
+    _check(lib().rtx_scene_load(str(scene_path).encode(), C.byref(v), C.byref(nv), C.byref(t), C.byref(nt)),
+           "rtx_scene_load")
+    verts = np.ctypeslib.as_array(v, shape=(nv.value, 6)).copy()
+    tris = np.ctypeslib.as_array(t, shape=(nt.value, 3)).copy()
+    lib().rtx_free_host(C.cast(v, C.c_void_p))
+    lib().rtx_free_host(C.cast(t, C.c_void_p))
+    return verts, tris
+
+
+def geo_sphere(center, radius, rings=16, sectors=32):
+    c = np.asarray(center, np.float32)
+    nv, nt = C.c_uint32(), C.c_uint32()
+    _check(lib().rtx_geo_sphere(_p(c), radius, rings, sectors, None, C.byref(nv), None, C.byref(nt)), "geo_sphere")
+    v = np.zeros((nv.value, 6), np.float32)
+    t = np.zeros((nt.value, 3), np.uint32)
+    _check(lib().rtx_geo_sphere(_p(c), radius, rings, sectors, _p(v), C.byref(nv), _p(t), C.byref(nt)), "geo_sphere")
+    return v, t
+
+
+def geo_box(center, half, quat_xyzw=(0, 0, 0, 1)):
+    c = np.asarray(center, np.float32)
+    h = np.asarray(half, np.float32)
+    q = np.asarray(quat_xyzw, np.float32)
+    nv, nt = C.c_uint32(), C.c_uint32()
+    _check(lib().rtx_geo_box(_p(c), _p(h), _p(q), None, C.byref(nv), None, C.byref(nt)), "geo_box")
+    v = np.zeros((nv.value, 6), np.float32)
+    t = np.zeros((nt.value, 3), np.uint32)
+    _check(lib().rtx_geo_box(_p(c), _p(h), _p(q), _p(v), C.byref(nv), _p(t), C.byref(nt)), "geo_box")
+    return v, t
+
+
+def make_camera(cam):
+    c = Camera()
+    for k in ("pos", "fwd", "right", "up"):
+        getattr(c, k)[:] = [float(x) for x in cam[k]]
+    c.tan_half_fov_y = float(cam["tan_half_fov_y"])
+    return c
+
+
+def make_params(width, height, spp, max_bounce, rr_bounce=3, seed=0, env_mode=0, env_color=(0, 0, 0)):
+    p = RenderParams()
+    p.width, p.height, p.spp, p.max_bounce, p.rr_bounce, p.seed = width, height, spp, max_bounce, rr_bounce, seed
+    p.env_mode = env_mode
+    p.env_color[:] = [float(x) for x in env_color]
+    return p

@@ -1,0 +1,104 @@
+"""GPU traversal parity: same hit triangle (reference triIndex), bit-identical t/u/v,
+any-hit == closest-hit.hit, identical per-ray work counts, on the GPU-built tree."""
+import numpy as np
+import pytest
+
+import rtx
+from oracle import oracle as O
+from rtx import scenes as S
+from scenes_util import scene_suite
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def suite():
+    return scene_suite(rtx.geo_sphere, rtx.geo_box)
+
+
+def _rays_for(v, n, seed):
+    lo = v[:, :3].min(0) - 0.1
+    hi = v[:, :3].max(0) + 0.1
+    r = S.incoherent_rays(n, seed=seed)
+    r[:, 0:3] = lo + (hi - lo) * r[:, 0:3]
+    return r
+
+
+def _device_trace(g, rays):
+    import torch
+    rd = torch.from_numpy(rays).cuda()
+    hits = torch.zeros((len(rays), 4), dtype=torch.float32, device="cuda")
+    nrm = torch.zeros((len(rays), 3), dtype=torch.float32, device="cuda")
+    occ = torch.zeros(len(rays), dtype=torch.uint8, device="cuda")
+    cnt = torch.zeros((len(rays), 2), dtype=torch.int32, device="cuda")
+    g.trace_closest_device(rd, hits, nrm)
+    g.trace_any_device(rd, occ)
+    g.trace_counts_device(rd, cnt)
+    torch.cuda.synchronize()
+    h = hits.cpu().numpy()
+    return h, nrm.cpu().numpy(), occ.cpu().numpy(), cnt.cpu().numpy().view(np.uint32)
+
+
+@pytest.mark.parametrize("name", ["rand1", "rand17", "rand1000", "rand20000", "cornell", "test_traverse",
+                                  "straddle", "dupcodes"])
+def test_trace_matches_oracle(suite, name):
+    v, t = suite[name]
+    print(f"[trace parity] {name}", flush=True)
+    g = rtx.build(v, t)
+    o = O.build(v, t)
+    rays = _rays_for(v, 20000, seed=len(t))
+    h, nrm, occ, cnt = _device_trace(g, rays)
+    ref = O.trace(o, rays, counts=True, normals=True, nthreads=8)
+    tri = h[:, 3].view(np.uint32)
+    assert np.array_equal(tri, ref["tri"]), f"{name}: hit ids differ on {(tri != ref['tri']).sum()} rays"
+    for k, col in (("t", 0), ("u", 1), ("v", 2)):
+        assert np.array_equal(h[:, col].view(np.uint32), ref[k].view(np.uint32)), f"{name}: {k} differs"
+    assert np.array_equal(occ.astype(bool), ref["tri"] != 0xFFFFFFFF), f"{name}: any-hit != closest hit"
+    assert np.array_equal(cnt, ref["counts"]), f"{name}: per-ray work counts differ"
+    np.testing.assert_allclose(nrm, ref["normal"], rtol=0, atol=1e-6)
+    hit = tri != 0xFFFFFFFF
+    assert hit.any() or len(t) < 20
+
+
+def test_trace_host_wrapper_and_reference_rays(suite):
+    """The 12 rays of the reference's tests/test_traverse.cpp:117-236 through the GPU."""
+    v, t = suite["test_traverse"]
+    g = rtx.build(v, t)
+
+    def ray(o, d):
+        d = np.asarray(d, np.float32)
+        d = d / np.float32(np.sqrt(np.float32((d * d).sum())))
+        return np.array([*o, 0.0, *d, np.inf], np.float32)
+    rays = np.stack([ray((0, 0, 0), (0, 0, 1)), ray((0, 0, 8), (0, 0, 1)), ray((0, 0, 15), (0, 0, -1)),
+                     ray((-5, 0, 5), (1, 0, 0)), ray((0, 0, 8), (0, 0, -1)), ray((3, 0, 0), (0, 0, 1)),
+                     ray((0, 10, 0), (0, 0, 1)), ray((0, 0, 0), (0, 0, -1)), ray((0, 0, 0), (0, 1, 0)),
+                     ray((3, 0, 0), (0, 1, 0)), ray((0, -5, 0), (0, 1, 0)), ray((25, 0, 0), (0, -1, 0))])
+    h = g.trace(rays, normals=True)
+    hit = h["tri"] != 0xFFFFFFFF
+    assert list(hit) == [True] * 5 + [False] * 3 + [True, False, True, False]
+    t_ = h["t"]
+    assert 3.9 <= t_[0] <= 4.1 and h["normal"][0, 2] < 0
+    assert 2.9 <= t_[1] <= 3.1 and h["normal"][1, 2] < 0
+    assert 1.9 <= t_[2] <= 2.1 and h["normal"][2, 2] > 0
+    assert 3.9 <= t_[3] <= 4.1 and h["normal"][3, 0] < 0
+    assert 1.9 <= t_[4] <= 2.1 and h["normal"][4, 2] > 0
+    assert 2.5 <= t_[8] <= 3.5 and h["normal"][8, 1] < 0
+    assert 1.9 <= t_[10] <= 2.2 and h["normal"][10, 1] > 0
+
+
+def test_trace_1m_subset():
+    """1M-triangle scene: a 65k-ray subset of both benchmark ray sets against the oracle."""
+    v, t = S.random_triangles(1_000_000, seed=1)
+    g = rtx.build(v, t)
+    o = O.build(v, t)
+    inc = S.incoherent_rays(1 << 16, seed=1)
+    cam = S.camera_lookat((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), 45.0)
+    coh = S.pinhole_rays(cam, 2048, 2048)[:: 64]
+    for name, rays in (("incoherent", inc), ("coherent", coh)):
+        print(f"[trace 1M] {name}", flush=True)
+        h, _, occ, cnt = _device_trace(g, rays)
+        ref = O.trace(o, rays, counts=True, nthreads=16)
+        assert np.array_equal(h[:, 3].view(np.uint32), ref["tri"]), name
+        assert np.array_equal(h[:, 0].view(np.uint32), ref["t"].view(np.uint32)), name
+        assert np.array_equal(occ.astype(bool), ref["tri"] != 0xFFFFFFFF), name
+        assert np.array_equal(cnt, ref["counts"]), name
