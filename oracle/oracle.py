@@ -243,12 +243,16 @@ def ref_build(verts6, tris):
 
 
 def ref_trace(tree, rays, normals=False):
+    """bvh::trace over `rays`; tree=None reuses the tree loaded by the previous call."""
     rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
     n = len(rays)
     hits = np.zeros((n, 4), np.float32)
     nrm = np.zeros((n, 3), np.float32) if normals else None
-    ref().ref_trace(_p(tree.nodes), len(tree.nodes), _p(tree.tris), len(tree.tris), _p(tree.verts),
-                    len(tree.verts), _p(rays), n, _p(hits), _p(nrm))
+    if tree is None:
+        ref().ref_trace(None, 0, None, 0, None, 0, _p(rays), n, _p(hits), _p(nrm))
+    else:
+        ref().ref_trace(_p(tree.nodes), len(tree.nodes), _p(tree.tris), len(tree.tris), _p(tree.verts),
+                        len(tree.verts), _p(rays), n, _p(hits), _p(nrm))
     out = {"t": hits[:, 0].copy(), "u": hits[:, 1].copy(), "v": hits[:, 2].copy(),
            "tri": hits[:, 3].view(np.uint32).copy()}
     if normals:

@@ -92,16 +92,19 @@ void ref_scene_box(float* out) { std::memcpy(out, g_res.scene, sizeof(g_res.scen
 
 // Load reference-layout arrays into the tracer's state and run bvh::trace per ray.
 // rays: 8 floats {ox,oy,oz,tmin,dx,dy,dz,tmax}. hits: {t,u,v,triIndex|0xFFFFFFFF}.
+// nodes == NULL traces against the arrays loaded by the previous call.
 int ref_trace(const void* nodes, uint32_t nn, const uint32_t* tris, uint32_t ntri,
               const float* verts6, uint32_t nv, const float* rays, size_t n,
               float* hits4, float* normals3) {
     using namespace bvh;
-    t_nodes.resize(nn);
-    std::memcpy(t_nodes.data(), nodes, sizeof(BVHNode) * nn);
-    t_tris.resize(ntri);
-    std::memcpy(t_tris.data(), tris, sizeof(Triangle) * ntri);
-    t_verts.resize(nv);
-    std::memcpy(t_verts.data(), verts6, sizeof(Vertex) * nv);
+    if (nodes) {
+        t_nodes.resize(nn);
+        std::memcpy(t_nodes.data(), nodes, sizeof(BVHNode) * nn);
+        t_tris.resize(ntri);
+        std::memcpy(t_tris.data(), tris, sizeof(Triangle) * ntri);
+        t_verts.resize(nv);
+        std::memcpy(t_verts.data(), verts6, sizeof(Vertex) * nv);
+    }
     for (size_t i = 0; i < n; ++i) {
         const float* r = rays + 8 * i;
         Ray ray;

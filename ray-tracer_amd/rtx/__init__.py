@@ -57,6 +57,13 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RtxError(f"librtx.so not built at {LIB_PATH}; run ray-tracer_amd/build.py")
+        # torch ships its own HIP/HSA runtime and asks for it under unversioned names. Load
+        # torch first so librtx's libamdhip64.so.7 / libhsa-runtime64.so.1 resolve (by
+        # SONAME) to that same runtime: one HIP runtime per process, shared device memory.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp, u32, sz, i32 = C.c_void_p, C.c_uint32, C.c_size_t, C.c_int
         L.rtx_last_error.restype = C.c_char_p
