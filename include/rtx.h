@@ -161,6 +161,28 @@ int rtx_render_batched(const rtx_bvh* const* scenes, int nscenes, const rtx_came
                        const rtx_render_params* params, const float* const* albedo_d,
                        const float* const* emission_d, float* const* images_d, int mode, void* stream);
 
+/* Prepared multi-scene render.  mode 0 = one launch sequence per scene (plain),
+ * 1 = all scenes share every launch and the whole sequence is captured once into a
+ * hipGraph at creation (batched), 2 = shared launches without the graph.  Buffers passed
+ * at creation must stay valid; launch may be repeated (each launch re-renders). */
+typedef struct rtx_batch rtx_batch;
+int rtx_batch_create(const rtx_bvh* const* scenes, int nscenes, const rtx_camera* cams,
+                     const rtx_render_params* params, const float* const* albedo_d,
+                     const float* const* emission_d, float* const* images_d, int mode, rtx_batch** out);
+int rtx_batch_launch(rtx_batch* batch, void* stream);
+void rtx_batch_free(rtx_batch* batch);
+
+/* Launch shape of the renderer (process-wide; chosen by the project autotuner).
+ * batch_paths: paths in flight per wavefront batch (>= 1024); grid_blocks: blocks of the
+ * persistent trace/shade kernels; shade_threads: threads per shade block (multiple of 64). */
+typedef struct rtx_tuning {
+    uint32_t batch_paths;
+    uint32_t grid_blocks;
+    uint32_t shade_threads;
+} rtx_tuning;
+int rtx_set_tuning(const rtx_tuning* t);
+int rtx_get_tuning(rtx_tuning* t);
+
 #ifdef __cplusplus
 }
 #endif

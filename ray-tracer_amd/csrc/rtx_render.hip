@@ -1,0 +1,546 @@
+// Wavefront path tracer, multi-scene batching and the path-replay adjoint.
+//
+// Replaces the raster loop + pathTrace of the reference (src/main.cpp:145-204, 339-365).
+// Every random number comes from the counter-based sample table (seed, pixel, sample,
+// dim): dims 0/1 jitter, then per bounce b: 2+3b, 3+3b (direction), 4+3b (roulette).
+//
+// Paths of all scenes are numbered globally: scene-major, then sample, then pixel. They
+// are processed in batches of `batch_paths`; per batch:
+//   k_camera -> { k_wtrace -> k_shade } x (max_bounce + 1) -> k_accum
+// k_shade appends surviving paths to the next queue (wave ballot + one atomic per wave).
+// k_accum adds each pixel's samples into the image in sample order, so the sum is the
+// reference's serial `accum += pathTrace(...)`; k_finalize divides by spp.
+// No stage needs the host: queue lengths stay in device memory, so a whole render can be
+// captured into one hipGraph (counters are cleared by a kernel, never by a memset).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "rtx_trace_impl.hpp"
+
+namespace rtx {
+namespace {
+
+struct SceneDesc {
+    TraceArgs T;
+    const uint32_t* prims;
+    const float* albedo;     // 3 per original triangle, or null (0.8)
+    const float* emission;   // 3 per original triangle, or null
+    float pos[3], fwd[3], right[3], up[3], tanh_;
+    int W, H, spp, max_bounce, rr_bounce, env_mode;
+    uint32_t seed;
+    float env[3];
+    float* image;
+    uint64_t path_begin;
+    uint32_t npix;
+};
+
+struct PathState {
+    float4* ray0;      // ox oy oz tmin
+    float4* ray1;      // dx dy dz tmax
+    float4* hit;       // t u v tri
+    float4* tp;
+    float4* L;
+    uint32_t* sid;
+    uint32_t* pix;
+    uint32_t* smp;
+    uint32_t* q[2];
+    uint32_t* cnt;     // [0], [1] queue lengths
+};
+
+rtx_tuning g_tuning = {1u << 22, 2048, 256};
+
+__device__ __forceinline__ uint32_t find_scene(const uint64_t* off, int n, uint64_t g) {
+    int lo = 0, hi = n;   // off[lo] <= g < off[hi]
+    while (hi - lo > 1) {
+        int m = (lo + hi) >> 1;
+        if (off[m] <= g) lo = m; else hi = m;
+    }
+    return (uint32_t)lo;
+}
+
+// primaryRay (main.cpp:69-83)
+__device__ __forceinline__ void primary_ray(const SceneDesc& S, int px, int py, float jx, float jy, f3& o, f3& d) {
+    float aspect = (float)S.W / (float)S.H;
+    float u = (((float)px + jx) / (float)S.W * 2.f - 1.f) * aspect;
+    float v = ((float)py + jy) / (float)S.H * 2.f - 1.f;
+    f3 fwd{S.fwd[0], S.fwd[1], S.fwd[2]}, rt{S.right[0], S.right[1], S.right[2]}, up{S.up[0], S.up[1], S.up[2]};
+    d = normalize(add(add(fwd, scl(u * S.tanh_, rt)), scl(v * S.tanh_, up)));
+    o = f3{S.pos[0], S.pos[1], S.pos[2]};
+}
+
+// sampleCosineHemisphere + toWorld (main.cpp:123-141)
+__device__ __forceinline__ f3 bounce_dir(float u1, float u2, f3 n) {
+    float r = sqrtf(u1);
+    float theta = 6.28318548f * u2;
+    float s, c;
+    sincos_rtx(theta, s, c);
+    f3 l{r * c, sqrtf(std_max(0.f, 1.f - u1)), r * s};
+    f3 up = fabsf(n.x) < 0.9f ? f3{1.f, 0.f, 0.f} : f3{0.f, 1.f, 0.f};
+    f3 t = normalize(cross(n, up));
+    f3 b = cross(n, t);
+    return add(add(scl(l.x, t), scl(l.y, n)), scl(l.z, b));
+}
+
+// Miss shading: the sky gradient of main.cpp:169-172, or a constant environment.
+__device__ __forceinline__ f3 env_color(const SceneDesc& S, f3 d) {
+    if (S.env_mode == 1) return f3{S.env[0], S.env[1], S.env[2]};
+    float t = d.y * 0.5f + 0.5f;
+    float omt = 1.f - t;
+    return f3{1.0f * omt + 0.5f * t, 1.0f * omt + 0.7f * t, 1.0f * omt + 1.0f * t};
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+__global__ void k_clear1(uint32_t* p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = 0u;
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t a, uint32_t b) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) { p[0] = a; p[1] = b; }
+}
+
+__global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* __restrict__ off, int ns, uint64_t g0,
+                         uint32_t cnt, PathState P) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const uint64_t g = g0 + i;
+        const uint32_t s = find_scene(off, ns, g);
+        const SceneDesc& S = scenes[s];
+        const uint64_t q = g - S.path_begin;
+        const uint32_t sample = (uint32_t)(q / S.npix), pixel = (uint32_t)(q % S.npix);
+        const int x = (int)(pixel % (uint32_t)S.W), y = (int)(pixel / (uint32_t)S.W);
+        f3 o, d;
+        primary_ray(S, x, y, sample01(S.seed, pixel, sample, 0), sample01(S.seed, pixel, sample, 1), o, d);
+        P.ray0[i] = make_float4(o.x, o.y, o.z, 1e-3f);
+        P.ray1[i] = make_float4(d.x, d.y, d.z, kInf);
+        P.tp[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+        P.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        P.sid[i] = s;
+        P.pix[i] = pixel;
+        P.smp[i] = sample;
+        P.q[0][i] = i;
+    }
+}
+
+__global__ void __launch_bounds__(kTraceThreads) k_wtrace(const SceneDesc* __restrict__ scenes, PathState P, int qi) {
+    __shared__ uint2 lst[kLdsDepth][kTraceThreads];
+    const uint32_t n = P.cnt[qi];
+    const uint32_t* Q = P.q[qi];
+    for (uint32_t k = xcd_block(blockIdx.x) * kTraceThreads + threadIdx.x; k < n; k += gridDim.x * kTraceThreads) {
+        const uint32_t i = Q[k];
+        const float4 r0 = P.ray0[i], r1 = P.ray1[i];
+        const TraceOut h = trace_ray<false, false>(scenes[P.sid[i]].T, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z},
+                                                   r0.w, r1.w, lst, threadIdx.x);
+        P.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
+    }
+}
+
+// One bounce of pathTrace (main.cpp:158-200) for every queued path.
+__global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int qi, int bounce) {
+    const uint32_t n = P.cnt[qi];
+    const uint32_t* Q = P.q[qi];
+    uint32_t* Qo = P.q[qi ^ 1];
+    const uint64_t lt = (1ull << __lane_id()) - 1ull;
+    // uniform trip count per wave so the ballot below sees every lane
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t base0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    for (uint32_t wb = base0; wb < n; wb += stride) {
+        const uint32_t k = wb + __lane_id();
+        bool cont = false;
+        uint32_t i = 0;
+        if (k < n) {
+            i = Q[k];
+            const SceneDesc& S = scenes[P.sid[i]];
+            const float4 h = P.hit[i];
+            const uint32_t tri = __float_as_uint(h.w);
+            float4 tp = P.tp[i], L = P.L[i];
+            const float4 r1 = P.ray1[i];
+            const f3 d{r1.x, r1.y, r1.z};
+            if (tri == kInvalid) {
+                f3 e = env_color(S, d);
+                L = make_float4(L.x + tp.x * e.x, L.y + tp.y * e.y, L.z + tp.z * e.z, 0.f);
+            } else {
+                const uint32_t prim = S.prims[tri];
+                if (S.emission) {
+                    const float* e = S.emission + 3 * prim;
+                    L = make_float4(L.x + tp.x * e[0], L.y + tp.y * e[1], L.z + tp.z * e[2], 0.f);
+                }
+                if (bounce < S.max_bounce) {
+                    float ax = 0.8f, ay = 0.8f, az = 0.8f;
+                    if (S.albedo) { ax = S.albedo[3 * prim]; ay = S.albedo[3 * prim + 1]; az = S.albedo[3 * prim + 2]; }
+                    const uint32_t px = P.pix[i], sm = P.smp[i];
+                    const float u1 = sample01(S.seed, px, sm, 2 + 3 * bounce);
+                    const float u2 = sample01(S.seed, px, sm, 3 + 3 * bounce);
+                    const f3 nrm = hit_normal(S.T, tri, h.y, h.z);
+                    const f3 wd = bounce_dir(u1, u2, nrm);
+                    tp = make_float4(tp.x * ax, tp.y * ay, tp.z * az, 0.f);
+                    cont = true;
+                    if (bounce >= S.rr_bounce) {
+                        float pr = tp.x;
+                        if (pr < tp.y) pr = tp.y;
+                        if (pr < tp.z) pr = tp.z;
+                        if (sample01(S.seed, px, sm, 4 + 3 * bounce) > pr) cont = false;
+                        else tp = make_float4(tp.x / pr, tp.y / pr, tp.z / pr, 0.f);
+                    }
+                    if (cont) {
+                        const float4 r0 = P.ray0[i];
+                        const f3 o = add(f3{r0.x, r0.y, r0.z}, scl(h.x, d));
+                        P.ray0[i] = make_float4(o.x, o.y, o.z, 1e-4f);
+                        P.ray1[i] = make_float4(wd.x, wd.y, wd.z, kInf);
+                        P.tp[i] = tp;
+                    }
+                }
+            }
+            P.L[i] = L;
+        }
+        const uint64_t m = __ballot(cont);
+        if (m) {
+            uint32_t base = 0;
+            if (__lane_id() == 0) base = atomicAdd(&P.cnt[qi ^ 1], (uint32_t)__popcll(m));
+            base = __shfl(base, 0);
+            if (cont) Qo[base + __popcll(m & lt)] = i;
+        }
+    }
+}
+
+// In-order accumulation: the first path of each pixel in the batch adds that pixel's
+// samples in increasing sample order.
+__global__ void k_accum(const SceneDesc* __restrict__ scenes, const uint64_t* __restrict__ off, int ns, uint64_t g0,
+                        uint32_t cnt, PathState P) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const uint32_t s = P.sid[i];
+        const SceneDesc& S = scenes[s];
+        const uint32_t sample = P.smp[i], pixel = P.pix[i];
+        if (sample != 0 && i >= S.npix) continue;   // an earlier sample of this pixel is in the batch
+        float* img = S.image + 3 * (size_t)pixel;
+        float ax = img[0], ay = img[1], az = img[2];
+        for (uint32_t j = i, sm = sample; j < cnt && sm < (uint32_t)S.spp; j += S.npix, ++sm) {
+            const float4 L = P.L[j];
+            ax += L.x; ay += L.y; az += L.z;
+        }
+        img[0] = ax; img[1] = ay; img[2] = az;
+    }
+    (void)off; (void)ns; (void)g0;
+}
+
+__global__ void k_finalize(const SceneDesc* __restrict__ scenes, int s) {
+    const SceneDesc& S = scenes[s];
+    const float fs = (float)S.spp;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < S.npix * 3u; p += gridDim.x * blockDim.x)
+        S.image[p] = S.image[p] / fs;
+}
+
+// Path-replay adjoint for per-triangle albedo and emission. One lane per path re-traces
+// the path (same traversal, same sample table) and keeps only its vertices' triangle ids
+// and the radiance gathered so far; with L = sum_m (prod_{j<m} a_j) X_m,
+//   dL/da_k = (L - L_upto_k) / a_k      and      dL/dLe_k = throughput arriving at k.
+constexpr int kMaxAdjVerts = 16;
+
+__global__ void __launch_bounds__(kTraceThreads) k_adjoint(const SceneDesc* __restrict__ scenes,
+                                                           const float* __restrict__ dimg, float* __restrict__ ga,
+                                                           float* __restrict__ ge) {
+    __shared__ uint2 lst[kLdsDepth][kTraceThreads];
+    const SceneDesc& S = scenes[0];
+    const uint64_t total = (uint64_t)S.npix * S.spp;
+    for (uint64_t g = (uint64_t)xcd_block(blockIdx.x) * kTraceThreads + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * kTraceThreads) {
+        const uint32_t sample = (uint32_t)(g / S.npix), pixel = (uint32_t)(g % S.npix);
+        const float fs = (float)S.spp;
+        const f3 delta{dimg[3 * pixel] / fs, dimg[3 * pixel + 1] / fs, dimg[3 * pixel + 2] / fs};
+        f3 o, d;
+        primary_ray(S, (int)(pixel % (uint32_t)S.W), (int)(pixel / (uint32_t)S.W),
+                    sample01(S.seed, pixel, sample, 0), sample01(S.seed, pixel, sample, 1), o, d);
+        float tmin = 1e-3f;
+        f3 tp{1.f, 1.f, 1.f}, L{0.f, 0.f, 0.f};
+        uint32_t vprim[kMaxAdjVerts];
+        f3 vL[kMaxAdjVerts];
+        int nv = 0;
+        for (int bounce = 0; bounce <= S.max_bounce; ++bounce) {
+            const TraceOut h = trace_ray<false, false>(S.T, o, d, tmin, kInf, lst, threadIdx.x);
+            if (h.tri == kInvalid) {
+                f3 e = env_color(S, d);
+                L = f3{L.x + tp.x * e.x, L.y + tp.y * e.y, L.z + tp.z * e.z};
+                break;
+            }
+            const uint32_t prim = S.prims[h.tri];
+            if (S.emission) {
+                const float* e = S.emission + 3 * prim;
+                if (ge) {
+                    atomicAdd(&ge[3 * prim], delta.x * tp.x);
+                    atomicAdd(&ge[3 * prim + 1], delta.y * tp.y);
+                    atomicAdd(&ge[3 * prim + 2], delta.z * tp.z);
+                }
+                L = f3{L.x + tp.x * e[0], L.y + tp.y * e[1], L.z + tp.z * e[2]};
+            }
+            if (bounce == S.max_bounce) break;
+            const float* a = S.albedo + 3 * prim;
+            vprim[nv] = prim;
+            vL[nv] = L;
+            ++nv;
+            const float u1 = sample01(S.seed, pixel, sample, 2 + 3 * bounce);
+            const float u2 = sample01(S.seed, pixel, sample, 3 + 3 * bounce);
+            const f3 wd = bounce_dir(u1, u2, hit_normal(S.T, h.tri, h.u, h.v));
+            tp = f3{tp.x * a[0], tp.y * a[1], tp.z * a[2]};
+            o = add(o, scl(h.t, d));
+            d = wd;
+            tmin = 1e-4f;
+        }
+        if (ga) {
+            for (int k = 0; k < nv; ++k) {
+                const uint32_t prim = vprim[k];
+                const float* a = S.albedo + 3 * prim;
+                atomicAdd(&ga[3 * prim], delta.x * (L.x - vL[k].x) / a[0]);
+                atomicAdd(&ga[3 * prim + 1], delta.y * (L.y - vL[k].y) / a[1]);
+                atomicAdd(&ga[3 * prim + 2], delta.z * (L.z - vL[k].z) / a[2]);
+            }
+        }
+    }
+}
+
+SceneDesc make_desc(const rtx_bvh* b, const rtx_camera* c, const rtx_render_params* p, const float* albedo,
+                    const float* emission, float* image) {
+    SceneDesc S;
+    S.T = make_trace_args(b);
+    S.prims = b->prims;
+    S.albedo = albedo;
+    S.emission = emission;
+    for (int k = 0; k < 3; ++k) {
+        S.pos[k] = c->pos[k]; S.fwd[k] = c->fwd[k]; S.right[k] = c->right[k]; S.up[k] = c->up[k];
+        S.env[k] = p->env_color[k];
+    }
+    S.tanh_ = c->tan_half_fov_y;
+    S.W = p->width; S.H = p->height; S.spp = p->spp; S.max_bounce = p->max_bounce; S.rr_bounce = p->rr_bounce;
+    S.env_mode = p->env_mode;
+    S.seed = p->seed;
+    S.image = image;
+    S.path_begin = 0;
+    S.npix = (uint32_t)(p->width * p->height);
+    return S;
+}
+
+int check_scene(const rtx_bvh* b, const rtx_camera* c, const rtx_render_params* p, const void* image,
+                const char* fn) {
+    if (!b || !c || !p || !image) return fail(RTX_E_ARG, std::string("null argument on ") + fn);
+    if (!b->gnodes) return fail(RTX_E_STATE, std::string("BVH not loaded on ") + fn);
+    if (p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->max_bounce < 0)
+        return fail(RTX_E_ARG, std::string("bad render parameters on ") + fn);
+    if ((uint64_t)p->width * p->height > 0xFFFFFFFFull / 3)
+        return fail(RTX_E_LIMIT, std::string("image too large on ") + fn);
+    if (b->max_depth + 2 > (uint32_t)(kLdsDepth + kSpillDepth))
+        return fail(RTX_E_LIMIT, std::string("tree deeper than the traversal stack on ") + fn);
+    return RTX_OK;
+}
+
+}  // namespace
+
+// A prepared render of one or more scenes: device scene table, path state, and (for the
+// batched mode) the captured graph.
+struct Batch {
+    std::vector<SceneDesc> descs;
+    std::vector<uint64_t> off;
+    SceneDesc* d_scenes = nullptr;
+    uint64_t* d_off = nullptr;
+    PathState P{};
+    std::vector<void*> allocs;
+    uint32_t B = 0;
+    int mode = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int max_b = 0;
+
+    ~Batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void* p : allocs) (void)hipFree(p);
+    }
+    template <class T>
+    T* alloc(size_t n) {
+        T* p = dalloc<T>(n);
+        if (p) allocs.push_back(p);
+        return p;
+    }
+    int setup(uint32_t batch_paths);
+    void enqueue_range(int s_first, int s_last, hipStream_t st);   // scenes [s_first, s_last)
+    int launch(hipStream_t st);
+};
+
+int Batch::setup(uint32_t batch_paths) {
+    const int ns = (int)descs.size();
+    off.assign(ns + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+        descs[s].path_begin = off[s];
+        off[s + 1] = off[s] + (uint64_t)descs[s].npix * descs[s].spp;
+        max_b = std::max(max_b, descs[s].max_bounce);
+    }
+    uint64_t total = off[ns];
+    B = (uint32_t)std::min<uint64_t>(std::max<uint32_t>(batch_paths, 1024u), total);
+    d_scenes = alloc<SceneDesc>(ns);
+    d_off = alloc<uint64_t>(ns + 1);
+    P.ray0 = alloc<float4>(B); P.ray1 = alloc<float4>(B); P.hit = alloc<float4>(B);
+    P.tp = alloc<float4>(B); P.L = alloc<float4>(B);
+    P.sid = alloc<uint32_t>(B); P.pix = alloc<uint32_t>(B); P.smp = alloc<uint32_t>(B);
+    P.q[0] = alloc<uint32_t>(B); P.q[1] = alloc<uint32_t>(B);
+    P.cnt = alloc<uint32_t>(4);
+    if (!d_scenes || !d_off || !P.ray0 || !P.ray1 || !P.hit || !P.tp || !P.L || !P.sid || !P.pix || !P.smp ||
+        !P.q[0] || !P.q[1] || !P.cnt)
+        return fail(RTX_E_HIP, "device allocation failed on render setup");
+    RTX_HIP(hipMemcpy(d_scenes, descs.data(), sizeof(SceneDesc) * ns, hipMemcpyHostToDevice));
+    RTX_HIP(hipMemcpy(d_off, off.data(), sizeof(uint64_t) * (ns + 1), hipMemcpyHostToDevice));
+    return RTX_OK;
+}
+
+// Kernel sequence for the paths of scenes [s_first, s_last): batches of B paths.
+void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
+    const uint32_t grid = g_tuning.grid_blocks, sh = g_tuning.shade_threads;
+    int mb = 0;
+    for (int s = s_first; s < s_last; ++s) {
+        const SceneDesc& S = descs[s];
+        uint64_t n3 = 3ull * S.npix;
+        k_zero_u32<<<std::min<uint64_t>((n3 + 255) / 256, 4096), 256, 0, st>>>(reinterpret_cast<uint32_t*>(S.image), n3);
+        mb = std::max(mb, S.max_bounce);
+    }
+    const int ns = (int)descs.size();
+    for (uint64_t g0 = off[s_first]; g0 < off[s_last]; g0 += B) {
+        const uint32_t cnt = (uint32_t)std::min<uint64_t>(B, off[s_last] - g0);
+        const uint32_t cgrid = std::min<uint32_t>((cnt + 255) / 256, 65535);
+        k_set_u32<<<1, 64, 0, st>>>(P.cnt, cnt, 0u);
+        k_camera<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
+        // cnt[qi] is the input queue's length; shade appends to q[qi^1] (cnt[qi^1] == 0),
+        // then the consumed queue becomes the next output and its length is cleared.
+        int qi = 0;
+        for (int b = 0; b <= mb; ++b) {
+            k_wtrace<<<grid, kTraceThreads, 0, st>>>(d_scenes, P, qi);
+            k_shade<<<grid, sh, 0, st>>>(d_scenes, P, qi, b);
+            k_clear1<<<1, 64, 0, st>>>(P.cnt + qi);
+            qi ^= 1;
+        }
+        k_accum<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
+    }
+    for (int s = s_first; s < s_last; ++s)
+        k_finalize<<<std::min<uint32_t>((3 * descs[s].npix + 255) / 256, 4096), 256, 0, st>>>(d_scenes, s);
+}
+
+int Batch::launch(hipStream_t st) {
+    if (mode == 1) {
+        RTX_HIP(hipGraphLaunch(exec, st));
+        return RTX_OK;
+    }
+    if (mode == 2) {
+        enqueue_range(0, (int)descs.size(), st);
+    } else {
+        for (int s = 0; s < (int)descs.size(); ++s) enqueue_range(s, s + 1, st);
+    }
+    RTX_HIP(hipGetLastError());
+    return RTX_OK;
+}
+
+}  // namespace rtx
+
+using namespace rtx;
+
+struct rtx_batch {
+    rtx::Batch b;
+};
+
+extern "C" int rtx_set_tuning(const rtx_tuning* t) {
+    if (!t || t->batch_paths < 1024 || t->grid_blocks == 0 || t->shade_threads == 0 || t->shade_threads % 64 ||
+        t->shade_threads > 1024)
+        return fail(RTX_E_ARG, "bad tuning on rtx_set_tuning()");
+    g_tuning = *t;
+    return RTX_OK;
+}
+
+extern "C" int rtx_get_tuning(rtx_tuning* t) {
+    if (!t) return fail(RTX_E_ARG, "null argument on rtx_get_tuning()");
+    *t = g_tuning;
+    return RTX_OK;
+}
+
+extern "C" int rtx_batch_create(const rtx_bvh* const* scenes, int n, const rtx_camera* cams,
+                                const rtx_render_params* params, const float* const* albedo_d,
+                                const float* const* emission_d, float* const* images_d, int mode, rtx_batch** out) {
+    if (!scenes || n <= 0 || !cams || !params || !images_d || !out || mode < 0 || mode > 2)
+        return fail(RTX_E_ARG, "bad argument on rtx_batch_create()");
+    *out = nullptr;
+    rtx_batch* bt = new rtx_batch;
+    Batch& B = bt->b;
+    B.mode = mode;
+    for (int s = 0; s < n; ++s) {
+        int rc = check_scene(scenes[s], &cams[s], &params[s], images_d[s], "rtx_batch_create()");
+        if (rc) { delete bt; return rc; }
+        B.descs.push_back(make_desc(scenes[s], &cams[s], &params[s], albedo_d ? albedo_d[s] : nullptr,
+                                    emission_d ? emission_d[s] : nullptr, images_d[s]));
+    }
+    int rc = B.setup(g_tuning.batch_paths);
+    if (rc) { delete bt; return rc; }
+    if (mode == 1) {
+        hipStream_t cs;
+        if (!hip_ok(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "stream")) { delete bt; return RTX_E_HIP; }
+        bool ok = hip_ok(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
+        if (ok) {
+            B.enqueue_range(0, n, cs);
+            ok = hip_ok(hipStreamEndCapture(cs, &B.graph), "end capture");
+        }
+        if (ok) ok = hip_ok(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0), "instantiate");
+        (void)hipStreamDestroy(cs);
+        if (!ok) { delete bt; return RTX_E_HIP; }
+    }
+    *out = bt;
+    return RTX_OK;
+}
+
+extern "C" int rtx_batch_launch(rtx_batch* bt, void* stream) {
+    if (!bt) return fail(RTX_E_ARG, "null argument on rtx_batch_launch()");
+    return bt->b.launch((hipStream_t)stream);
+}
+
+extern "C" void rtx_batch_free(rtx_batch* bt) { delete bt; }
+
+extern "C" int rtx_render_batched(const rtx_bvh* const* scenes, int n, const rtx_camera* cams,
+                                  const rtx_render_params* params, const float* const* albedo_d,
+                                  const float* const* emission_d, float* const* images_d, int mode, void* stream) {
+    rtx_batch* bt = nullptr;
+    int rc = rtx_batch_create(scenes, n, cams, params, albedo_d, emission_d, images_d, mode, &bt);
+    if (rc) return rc;
+    rc = rtx_batch_launch(bt, stream);
+    if (rc == RTX_OK && !hip_ok(hipStreamSynchronize((hipStream_t)stream), "sync")) rc = RTX_E_HIP;
+    rtx_batch_free(bt);
+    return rc;
+}
+
+extern "C" int rtx_render(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params* p, const float* albedo_d,
+                          const float* emission_d, float* image_d, void* stream) {
+    const rtx_bvh* sc[1] = {b};
+    const float* al[1] = {albedo_d};
+    const float* em[1] = {emission_d};
+    float* im[1] = {image_d};
+    return rtx_render_batched(sc, 1, cam, p, al, em, im, 0, stream);
+}
+
+extern "C" int rtx_render_adjoint(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params* p,
+                                  const float* albedo_d, const float* emission_d, const float* d_image_d,
+                                  float* grad_albedo_d, float* grad_emission_d, void* stream) {
+    int rc = check_scene(b, cam, p, d_image_d, "rtx_render_adjoint()");
+    if (rc) return rc;
+    if (!albedo_d) return fail(RTX_E_ARG, "albedo is required on rtx_render_adjoint()");
+    if (p->rr_bounce <= p->max_bounce)
+        return fail(RTX_E_ARG, "the adjoint needs rr_bounce > max_bounce (no Russian roulette)");
+    if (p->max_bounce >= kMaxAdjVerts) return fail(RTX_E_LIMIT, "max_bounce too large on rtx_render_adjoint()");
+    SceneDesc S = make_desc(b, cam, p, albedo_d, emission_d, nullptr);
+    SceneDesc* d = dalloc<SceneDesc>(1);
+    if (!d) return fail(RTX_E_HIP, "device allocation failed on rtx_render_adjoint()");
+    hipStream_t st = (hipStream_t)stream;
+    rc = RTX_OK;
+    if (!hip_ok(hipMemcpyAsync(d, &S, sizeof(S), hipMemcpyHostToDevice, st), "copy")) rc = RTX_E_HIP;
+    if (rc == RTX_OK) {
+        k_adjoint<<<g_tuning.grid_blocks, kTraceThreads, 0, st>>>(d, d_image_d, grad_albedo_d,
+                                                                  emission_d ? grad_emission_d : nullptr);
+        if (!hip_ok(hipGetLastError(), "k_adjoint") || !hip_ok(hipStreamSynchronize(st), "sync")) rc = RTX_E_HIP;
+    }
+    (void)hipFree(d);
+    return rc;
+}

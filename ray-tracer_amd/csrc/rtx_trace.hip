@@ -13,31 +13,10 @@
 // leaf order with a strict t < best update, so exact-t ties resolve as in the reference.
 #include <hip/hip_runtime.h>
 
-#include "rtx_internal.hpp"
+#include "rtx_trace_impl.hpp"
 
 namespace rtx {
 namespace {
-
-constexpr int kTraceThreads = 256;
-constexpr int kLdsDepth = 16;          // power of two
-constexpr int kSpillDepth = 128;
-
-struct TraceArgs {
-    const GNode* nodes;
-    const Leaf* leaves;
-    const GTri* tris;
-    const uint32_t* tris3;
-    const float* verts;
-    float rb[6];
-    uint32_t root_ref;
-};
-
-// Map hardware workgroup ids (dealt round-robin over the 8 XCDs) to logical ids so that
-// each XCD, with its own L2, works on one contiguous range of rays.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t g) {
-    const uint32_t G = gridDim.x, per = G / 8, rem = G % 8, x = g % 8, j = g / 8;
-    return x < rem ? x * (per + 1) + j : rem * (per + 1) + (x - rem) * per + j;
-}
 
 template <bool ANY, bool COUNT>
 __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray32* __restrict__ rays, uint32_t n,
@@ -49,105 +28,23 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray3
     if (i >= n) return;
     const float4 r0 = reinterpret_cast<const float4*>(rays)[2 * i];
     const float4 r1 = reinterpret_cast<const float4*>(rays)[2 * i + 1];
-    const f3 o{r0.x, r0.y, r0.z}, d{r1.x, r1.y, r1.z};
-    const float tmin = r0.w, tmax = r1.w;
-    const float ix = 1.f / d.x, iy = 1.f / d.y, iz = 1.f / d.z;
-    uint2 spill[kSpillDepth];
-    uint32_t top = 0, lo = 0;
-    float bestT = tmax, bu = 0.f, bv = 0.f;
-    uint32_t best = kInvalid;
-    uint32_t nI = 0, nT = 0;
-
-    uint32_t ref = A.root_ref;
-    bool live = slab(o.x, o.y, o.z, ix, iy, iz, tmin, A.rb[0], A.rb[1], A.rb[2], A.rb[3], A.rb[4], A.rb[5], bestT) < bestT;
-    while (live) {
-        if (ref & kLeafBit) {
-            const Leaf lf = A.leaves[ref & ~kLeafBit];
-            for (uint32_t k = lf.begin; k < lf.begin + lf.count; ++k) {
-                const float4 a = A.tris[k].a, b = A.tris[k].b, c = A.tris[k].c;
-                if (COUNT) ++nT;
-                float u, v;
-                float t = tri_test(o, d, tmin, tmax, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y}, f3{b.z, b.w, c.x}, u, v);
-                if (t < bestT) {
-                    bestT = t; bu = u; bv = v; best = k;
-                    if (ANY) break;
-                }
-            }
-            if (ANY && best != kInvalid) break;
-        } else {
-            if (COUNT) ++nI;
-            const GNode& g = A.nodes[ref];
-            const float4 l0 = g.l0, l1 = g.l1, r1b = g.r1;
-            const uint2 refs = *reinterpret_cast<const uint2*>(&g.lref);
-            const float tl = slab(o.x, o.y, o.z, ix, iy, iz, tmin, l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, bestT);
-            const float tr = slab(o.x, o.y, o.z, ix, iy, iz, tmin, l1.z, l1.w, r1b.x, r1b.y, r1b.z, r1b.w, bestT);
-            uint32_t nearRef, farRef;
-            float tn, tf;
-            if (tl <= tr) { nearRef = refs.x; tn = tl; farRef = refs.y; tf = tr; }
-            else { nearRef = refs.y; tn = tr; farRef = refs.x; tf = tl; }
-            if (tn < bestT) {
-                if (tf < bestT) {
-                    if (top - lo == (uint32_t)kLdsDepth) { spill[lo] = lst[lo & (kLdsDepth - 1)][tid]; ++lo; }
-                    lst[top & (kLdsDepth - 1)][tid] = make_uint2(farRef, __float_as_uint(tf));
-                    ++top;
-                }
-                ref = nearRef;
-                continue;
-            }
-        }
-        // pop the next entry that can still hold a closer hit
-        live = false;
-        while (top > 0) {
-            --top;
-            uint2 e;
-            if (top < lo) { --lo; e = spill[top]; }
-            else e = lst[top & (kLdsDepth - 1)][tid];
-            if (__uint_as_float(e.y) < bestT) { ref = e.x; live = true; break; }
-        }
-    }
-
+    const TraceOut h = trace_ray<ANY, COUNT>(A, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w, lst, tid);
     if (ANY) {
-        occl[i] = best != kInvalid ? 1 : 0;
+        occl[i] = h.tri != kInvalid ? 1 : 0;
         return;
     }
-    Hit16 h;
-    h.t = best != kInvalid ? bestT : 0.f;
-    h.u = bu;
-    h.v = bv;
-    h.tri = best;
     reinterpret_cast<float4*>(hits)[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
     if (COUNT) {
-        counts[2 * i] = nI;
-        counts[2 * i + 1] = nT;
+        counts[2 * i] = h.nI;
+        counts[2 * i + 1] = h.nT;
     }
     if (normals) {
         f3 nn{0.f, 0.f, 0.f};
-        if (best != kInvalid) {
-            // Interpolated shading normal of the final hit (bvh_traverse.cpp:135-139).
-            const float* V = A.verts;
-            uint32_t i0 = A.tris3[3 * best], i1 = A.tris3[3 * best + 1], i2 = A.tris3[3 * best + 2];
-            f3 n0{V[6 * i0 + 3], V[6 * i0 + 4], V[6 * i0 + 5]};
-            f3 n1{V[6 * i1 + 3], V[6 * i1 + 4], V[6 * i1 + 5]};
-            f3 n2{V[6 * i2 + 3], V[6 * i2 + 4], V[6 * i2 + 5]};
-            float w = 1.f - bu - bv;
-            nn = normalize(add(add(scl(w, n0), scl(bu, n1)), scl(bv, n2)));
-        }
+        if (h.tri != kInvalid) nn = hit_normal(A, h.tri, h.u, h.v);
         normals[3 * i] = nn.x;
         normals[3 * i + 1] = nn.y;
         normals[3 * i + 2] = nn.z;
     }
-}
-
-TraceArgs make_args(const rtx_bvh* b) {
-    TraceArgs A;
-    A.nodes = b->gnodes;
-    A.leaves = b->leaves;
-    A.tris = b->gtris;
-    A.tris3 = b->tris3;
-    A.verts = b->verts;
-    for (int k = 0; k < 6; ++k) A.rb[k] = b->root_box[k];
-    A.root_ref = b->root_ref;
-    return A;
 }
 
 int check(const rtx_bvh* b, const void* rays, size_t n, const void* out, const char* fn) {
@@ -164,7 +61,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
            uint32_t* counts, hipStream_t s) {
     if (n == 0) return RTX_OK;
     uint32_t blocks = (uint32_t)((n + kTraceThreads - 1) / kTraceThreads);
-    k_trace<ANY, COUNT><<<blocks, kTraceThreads, 0, s>>>(make_args(b), reinterpret_cast<const Ray32*>(rays),
+    k_trace<ANY, COUNT><<<blocks, kTraceThreads, 0, s>>>(make_trace_args(b), reinterpret_cast<const Ray32*>(rays),
                                                          (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
                                                          counts);
     RTX_HIP(hipGetLastError());

@@ -23,7 +23,8 @@ EXPORTS = ["rtx_last_error", "rtx_device_count", "rtx_build", "rtx_build_device"
            "rtx_bvh_get_info", "rtx_bvh_export", "rtx_bvh_morton", "rtx_bvh_export_bfs", "rtx_trace_closest",
            "rtx_trace_any", "rtx_trace_counts", "rtx_trace_closest_host", "rtx_save", "rtx_load",
            "rtx_scene_load", "rtx_free_host", "rtx_bake_scene", "rtx_geo_sphere", "rtx_geo_box", "rtx_render",
-           "rtx_render_adjoint", "rtx_render_batched"]
+           "rtx_render_adjoint", "rtx_render_batched", "rtx_batch_create", "rtx_batch_launch", "rtx_batch_free",
+           "rtx_set_tuning", "rtx_get_tuning"]
 
 
 class RtxError(RuntimeError):
@@ -46,6 +47,10 @@ class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("max_bounce", C.c_int),
                 ("rr_bounce", C.c_int), ("seed", C.c_uint32), ("env_mode", C.c_int),
                 ("env_color", C.c_float * 3)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("batch_paths", C.c_uint32), ("grid_blocks", C.c_uint32), ("shade_threads", C.c_uint32)]
 
 
 _LIB = None
@@ -88,11 +93,15 @@ def lib():
         L.rtx_bake_scene.argtypes = [C.c_char_p]
         L.rtx_geo_sphere.argtypes = [vp, C.c_float, i32, i32, vp, C.POINTER(u32), vp, C.POINTER(u32)]
         L.rtx_geo_box.argtypes = [vp, vp, vp, vp, C.POINTER(u32), vp, C.POINTER(u32)]
-        if hasattr(L, "rtx_render"):
-            L.rtx_render.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp]
-            L.rtx_render_adjoint.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp, vp,
-                                             vp]
-            L.rtx_render_batched.argtypes = [vp, i32, vp, vp, vp, vp, vp, i32, vp]
+        L.rtx_render.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp]
+        L.rtx_render_adjoint.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp, vp, vp]
+        L.rtx_render_batched.argtypes = [vp, i32, vp, vp, vp, vp, vp, i32, vp]
+        L.rtx_batch_create.argtypes = [vp, i32, vp, vp, vp, vp, vp, i32, C.POINTER(vp)]
+        L.rtx_batch_launch.argtypes = [vp, vp]
+        L.rtx_batch_free.argtypes = [vp]
+        L.rtx_batch_free.restype = None
+        L.rtx_set_tuning.argtypes = [C.POINTER(Tuning)]
+        L.rtx_get_tuning.argtypes = [C.POINTER(Tuning)]
         _LIB = L
     return _LIB
 
@@ -273,3 +282,103 @@ def make_params(width, height, spp, max_bounce, rr_bounce=3, seed=0, env_mode=0,
     p.env_mode = env_mode
     p.env_color[:] = [float(x) for x in env_color]
     return p
+
+
+# ---- path tracing ------------------------------------------------------------------
+
+def get_tuning():
+    t = Tuning()
+    _check(lib().rtx_get_tuning(C.byref(t)), "rtx_get_tuning")
+    return dict(batch_paths=t.batch_paths, grid_blocks=t.grid_blocks, shade_threads=t.shade_threads)
+
+
+def set_tuning(batch_paths=None, grid_blocks=None, shade_threads=None):
+    cur = get_tuning()
+    t = Tuning(batch_paths or cur["batch_paths"], grid_blocks or cur["grid_blocks"],
+               shade_threads or cur["shade_threads"])
+    _check(lib().rtx_set_tuning(C.byref(t)), "rtx_set_tuning")
+
+
+def render(bvh, cam, params, albedo=None, emission=None, out=None, stream=None):
+    """Render into a (H, W, 3) float32 CUDA tensor. albedo/emission: (ntri, 3) CUDA tensors
+    indexed by original triangle id (None = constant 0.8 / no emission)."""
+    import torch
+    if out is None:
+        out = torch.empty((params.height, params.width, 3), dtype=torch.float32, device="cuda")
+    _check(lib().rtx_render(bvh.handle, C.byref(make_camera(cam)), C.byref(params), _dptr(albedo), _dptr(emission),
+                            _dptr(out), _stream_ptr(stream)), "rtx_render")
+    return out
+
+
+def render_adjoint(bvh, cam, params, albedo, emission, d_image, grad_albedo=None, grad_emission=None, stream=None):
+    """Accumulate dLoss/dalbedo and dLoss/demission given dLoss/dimage (path replay)."""
+    import torch
+    if grad_albedo is None:
+        grad_albedo = torch.zeros_like(albedo)
+    if grad_emission is None and emission is not None:
+        grad_emission = torch.zeros_like(emission)
+    _check(lib().rtx_render_adjoint(bvh.handle, C.byref(make_camera(cam)), C.byref(params), _dptr(albedo),
+                                    _dptr(emission), _dptr(d_image.contiguous()), _dptr(grad_albedo),
+                                    _dptr(grad_emission), _stream_ptr(stream)), "rtx_render_adjoint")
+    return grad_albedo, grad_emission
+
+
+class Batch:
+    """Several scenes rendered together; mode 0 plain (one launch sequence per scene),
+    1 batched and captured once in a hipGraph, 2 batched without the graph."""
+
+    def __init__(self, bvhs, cams, params, albedos, emissions, images, mode):
+        n = len(bvhs)
+        self._keep = (bvhs, albedos, emissions, images)
+        hs = (C.c_void_p * n)(*[b.handle for b in bvhs])
+        cs = (Camera * n)(*[make_camera(c) for c in cams])
+        ps = (RenderParams * n)(*params)
+        al = (C.c_void_p * n)(*[None if a is None else a.data_ptr() for a in albedos])
+        em = (C.c_void_p * n)(*[None if e is None else e.data_ptr() for e in emissions])
+        im = (C.c_void_p * n)(*[i.data_ptr() for i in images])
+        h = C.c_void_p()
+        _check(lib().rtx_batch_create(hs, n, cs, ps, al, em, im, mode, C.byref(h)), "rtx_batch_create")
+        self._h = h
+
+    def launch(self, stream=None):
+        _check(lib().rtx_batch_launch(self._h, _stream_ptr(stream)), "rtx_batch_launch")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.rtx_batch_free(self._h)
+            self._h = None
+
+
+def _autograd_fn():
+    import torch
+
+    class RenderFunction(torch.autograd.Function):
+        """image = render(albedo, emission); backward re-traces the paths (path replay)."""
+
+        @staticmethod
+        def forward(ctx, albedo, emission, bvh, cam, params):
+            img = render(bvh, cam, params, albedo, emission)
+            ctx.save_for_backward(albedo, emission)
+            ctx.scene = (bvh, cam, params)
+            return img
+
+        @staticmethod
+        def backward(ctx, grad_img):
+            albedo, emission = ctx.saved_tensors
+            bvh, cam, params = ctx.scene
+            ga, ge = render_adjoint(bvh, cam, params, albedo, emission, grad_img.contiguous())
+            return ga, ge, None, None, None
+
+    return RenderFunction
+
+
+_RF = None
+
+
+def differentiable_render(albedo, emission, bvh, cam, params):
+    """torch.autograd entry: gradients flow to albedo and emission ((ntri, 3) CUDA tensors).
+    Requires params.rr_bounce > params.max_bounce."""
+    global _RF
+    if _RF is None:
+        _RF = _autograd_fn()
+    return _RF.apply(albedo, emission, bvh, cam, params)

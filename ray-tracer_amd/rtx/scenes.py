@@ -179,3 +179,98 @@ def cornell_box(seed=0, sphere_subdiv=3, sphere_radius=0.3, light_radiance=15.0,
     emission[20:22] = light_radiance
     cam = camera_lookat((0, 0, -3.2), (0, 0, 0), 40.0)
     return verts, tris, albedo, emission, cam
+
+
+def fbm(p, octaves=4, seed=0):
+    """Smooth deterministic fBm on 3-D points: a sum of octaves of randomly oriented sine
+    waves with halving amplitude and doubling frequency. Returns values in about [-1, 1]."""
+    rng = np.random.Generator(np.random.PCG64(seed + 77))
+    out = np.zeros(len(p))
+    amp, freq, norm = 1.0, 2.0, 0.0
+    for _ in range(octaves):
+        for _ in range(3):
+            k = rng.standard_normal(3)
+            k /= np.linalg.norm(k)
+            out += amp * np.sin(freq * (p @ k) + rng.uniform(0, 2 * math.pi)) / 3.0
+        norm += amp
+        amp *= 0.5
+        freq *= 2.0
+    return out / norm
+
+
+def uv_sphere(rings, sectors, radius=1.0, center=(0, 0, 0), displace=0.0, seed=0):
+    """UV sphere with the reference's vertex/triangle layout (src/bvh/bvh_geo.cpp:22-112:
+    2 + (rings-1)*sectors vertices, 2*sectors*(rings-1) triangles), optionally displaced
+    along the normal by displace * fBm. Shading normals are the undisplaced radial ones."""
+    th = np.pi * np.arange(1, rings) / rings
+    ph = 2 * np.pi * np.arange(sectors) / sectors
+    T, Pp = np.meshgrid(th, ph, indexing="ij")
+    n = np.stack([np.sin(T) * np.cos(Pp), np.cos(T), np.sin(T) * np.sin(Pp)], -1).reshape(-1, 3)
+    n = np.concatenate([[[0, 1, 0]], n, [[0, -1, 0]]])
+    r = radius * (1.0 + displace * fbm(n, seed=seed)) if displace else np.full(len(n), radius)
+    verts = np.zeros((len(n), 6), F32)
+    verts[:, :3] = n * r[:, None] + np.asarray(center)
+    verts[:, 3:] = n
+    south = len(n) - 1
+
+    def ring(rr, s):
+        return 1 + (rr - 1) * sectors + (s % sectors)
+    tris = []
+    for s in range(sectors):
+        tris.append((0, ring(1, s + 1), ring(1, s)))
+    for rr in range(1, rings - 1):
+        for s in range(sectors):
+            a, b, c, d = ring(rr, s), ring(rr, s + 1), ring(rr + 1, s + 1), ring(rr + 1, s)
+            tris += [(a, b, c), (a, c, d)]
+    for s in range(sectors):
+        tris.append((south, ring(rings - 1, s + 1), ring(rings - 1, s)))
+    tris = np.array(tris, np.int64)
+    # visible from outside under the reference's culling rule (see quad())
+    P = verts[:, :3].astype(np.float64)
+    cr = np.cross(P[tris[:, 1]] - P[tris[:, 0]], P[tris[:, 2]] - P[tris[:, 0]])
+    out = (cr * (P[tris].mean(1) - np.asarray(center))).sum(-1) > 0
+    tris[out] = tris[out][:, [0, 2, 1]]
+    return verts, tris.astype(np.uint32)
+
+
+def diff_scene(seed=3, rings=33, sectors=64, displace=0.08, albedo_lo=0.1, albedo_hi=0.9):
+    """Differentiable-render scene: the Cornell box walls, a displaced UV sphere of
+    2*sectors*(rings-1) triangles (4096 for 33 x 64) and a 2-triangle ceiling emitter.
+    Returns verts, tris, albedo (per triangle, seeded), emission, camera."""
+    parts = []
+
+    def wall(fn, facing):
+        for i in range(2):
+            parts.append(quad(*fn(i), facing))
+    wall(lambda i: [(-1 + i, -1, -1), (-1 + i, -1, 1), (i, -1, 1), (i, -1, -1)], (0, 1, 0))
+    wall(lambda i: [(-1 + i, 1, -1), (i, 1, -1), (i, 1, 1), (-1 + i, 1, 1)], (0, -1, 0))
+    wall(lambda i: [(-1 + i, -1, 1), (-1 + i, 1, 1), (i, 1, 1), (i, -1, 1)], (0, 0, -1))
+    wall(lambda i: [(-1, -1, -1 + i), (-1, 1, -1 + i), (-1, 1, i), (-1, -1, i)], (1, 0, 0))
+    wall(lambda i: [(1, -1, -1 + i), (1, -1, i), (1, 1, i), (1, 1, -1 + i)], (-1, 0, 0))
+    parts.append(quad((-0.3, 0.98, -0.3), (0.3, 0.98, -0.3), (0.3, 0.98, 0.3), (-0.3, 0.98, 0.3), (0, -1, 0)))
+    parts.append(uv_sphere(rings, sectors, 0.45, (0, -0.3, 0.2), displace, seed))
+    verts, tris = merge(parts)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    albedo = rng.uniform(albedo_lo, albedo_hi, (len(tris), 3)).astype(F32)
+    emission = np.zeros((len(tris), 3), F32)
+    emission[20:22] = 12.0
+    cam = camera_lookat((0, 0, -3.0), (0, -0.1, 0), 42.0)
+    return verts, tris, albedo, emission, cam
+
+
+def batched_scenes(n=64, seed=2, lo=1000, hi=500_000, displace=0.05):
+    """Scenes of uneven size: fBm-displaced spheres whose triangle counts follow a
+    lognormal clamped to [lo, hi]; resolutions from {128, 256, 512}^2; spp uniform in
+    8..64; max_bounce 6. Returns a list of dicts (verts, tris, cam, width, height, spp)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n):
+        ntri = int(np.clip(np.exp(rng.normal(math.log(20000), 1.2)), lo, hi))
+        rings = max(3, int(round(math.sqrt(ntri / 4.0))))
+        sectors = 2 * rings
+        res = int(rng.choice([128, 256, 512]))
+        spp = int(rng.integers(8, 65))
+        v, t = uv_sphere(rings, sectors, 1.0, (0, 0, 0), displace, seed=seed * 1000 + i)
+        cam = camera_lookat((0.3 * math.sin(i), 0.4, -3.2), (0, 0, 0), 45.0)
+        out.append(dict(verts=v, tris=t, cam=cam, width=res, height=res, spp=spp, max_bounce=6))
+    return out

@@ -1,0 +1,187 @@
+"""GPU path tracer parity: images equal the oracle's restatement of the reference loop
+(same sample table) within 1e-4 relative per pixel; batched and graph-replayed renders
+equal single-scene renders; adjoint gradients match central finite differences."""
+import numpy as np
+import pytest
+
+import rtx
+from oracle import oracle as O
+from rtx import scenes as S
+from scenes_util import reference_test_scene
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-4   # per-pixel relative tolerance of the image parity contract
+
+
+def _tensor(a):
+    import torch
+    return None if a is None else torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def _close(gpu, orc, what):
+    bit = np.mean(gpu.view(np.uint32) == orc.view(np.uint32))
+    print(f"[render parity] {what}: {bit * 100:.3f}% of pixel channels bit-identical", flush=True)
+    np.testing.assert_allclose(gpu, orc, rtol=RTOL, atol=1e-6, err_msg=what)
+
+
+def _render_both(v, t, albedo, emission, cam, params, oparams):
+    g = rtx.build(v, t)
+    o = O.build(v, t)
+    img = rtx.render(g, cam, params, _tensor(albedo), _tensor(emission)).cpu().numpy()
+    ref = O.render(o, cam, oparams, albedo, emission, nthreads=16)
+    return img, ref
+
+
+@pytest.mark.parametrize("spp,max_bounce,rr", [(4, 5, 3), (2, 8, 3), (3, 2, 99)])
+def test_cornell_matches_oracle(spp, max_bounce, rr):
+    v, t, albedo, emission, cam = S.cornell_box()
+    p = rtx.make_params(64, 48, spp, max_bounce, rr, seed=0)
+    op = O.make_params(64, 48, spp, max_bounce, rr, seed=0)
+    img, ref = _render_both(v, t, albedo, emission, cam, p, op)
+    assert img.shape == (48, 64, 3) and np.isfinite(img).all()
+    assert img.max() > 0.05
+    _close(img, ref, f"cornell spp={spp} maxBounce={max_bounce} rr={rr}")
+
+
+def test_reference_shading_defaults():
+    """No materials: the reference's constant 0.8 albedo and sky gradient (main.cpp:160-179)."""
+    v, t = reference_test_scene(rtx.geo_sphere, rtx.geo_box)
+    cam = S.camera_lookat((0, 2, -12), (0, 0, 4), 55.0)
+    p = rtx.make_params(80, 60, 3, 8, 3, seed=5)
+    op = O.make_params(80, 60, 3, 8, 3, seed=5)
+    img, ref = _render_both(v, t, None, None, cam, p, op)
+    _close(img, ref, "reference defaults")
+
+
+def test_constant_environment():
+    v, t, albedo, _, cam = S.cornell_box(seed=4)
+    p = rtx.make_params(40, 40, 2, 4, 3, seed=9, env_mode=1, env_color=(0.3, 0.5, 0.9))
+    op = O.make_params(40, 40, 2, 4, 3, seed=9, env_mode=1, env_color=(0.3, 0.5, 0.9))
+    img, ref = _render_both(v, t, albedo, None, cam, p, op)
+    _close(img, ref, "constant environment")
+
+
+def test_sky_closed_form():
+    """One triangle behind the camera: every pixel is the mean over samples of the sky
+    gradient (1,1,1)(1-s) + (.5,.7,1)s with s = dir.y/2 + 1/2 of its primary ray."""
+    v = np.array([[0, 0, 10, 0, 0, -1], [1, 0, 10, 0, 0, -1], [0, 1, 10, 0, 0, -1]], np.float32)
+    t = np.array([[0, 1, 2]], np.uint32)
+    cam = S.camera_lookat((0, 0, 0), (0, 0, -1), 60.0)
+    W, H, spp = 24, 16, 5
+    g = rtx.build(v, t)
+    img = rtx.render(g, cam, rtx.make_params(W, H, spp, 3, 3, seed=2)).cpu().numpy().astype(np.float64)
+    exp = np.zeros((H, W, 3))
+    for y in range(H):
+        for x in range(W):
+            for s in range(spp):
+                pix = y * W + x
+                jx, jy = O.sample(2, pix, s, 0), O.sample(2, pix, s, 1)
+                u = ((x + jx) / W * 2 - 1) * (W / H)
+                vv = (y + jy) / H * 2 - 1
+                d = cam["fwd"] + cam["right"] * u * cam["tan_half_fov_y"] + cam["up"] * vv * cam["tan_half_fov_y"]
+                d = d / np.linalg.norm(d)
+                sy = d[1] * 0.5 + 0.5
+                exp[y, x] += np.array([1 - sy + 0.5 * sy, 1 - sy + 0.7 * sy, 1.0]) / spp
+    np.testing.assert_allclose(img, exp, rtol=2e-6, atol=2e-6)
+
+
+def _small_batch_scenes():
+    out = []
+    for i, (rings, res, spp) in enumerate([(8, 24, 3), (20, 32, 2), (12, 16, 5)]):
+        v, t = S.uv_sphere(rings, 2 * rings, 1.0, (0, 0, 0), 0.05, seed=i)
+        cam = S.camera_lookat((0.3 * i, 0.4, -3.2), (0, 0, 0), 45.0)
+        out.append((v, t, cam, rtx.make_params(res, res, spp, 6, 3, seed=i), O.make_params(res, res, spp, 6, 3, seed=i)))
+    return out
+
+
+def test_batched_modes_match_single_and_oracle():
+    import torch
+    sc = _small_batch_scenes()
+    bvhs = [rtx.build(v, t) for v, t, _, _, _ in sc]
+    singles = [rtx.render(b, c, p).cpu().numpy() for b, (_, _, c, p, _) in zip(bvhs, sc)]
+    for (v, t, c, _, op), img in zip(sc, singles):
+        _close(img, O.render(O.build(v, t), c, op, nthreads=16), "batch scene vs oracle")
+    for mode in (0, 1, 2):
+        imgs = [torch.zeros((p.height, p.width, 3), device="cuda") for _, _, _, p, _ in sc]
+        b = rtx.Batch(bvhs, [c for _, _, c, _, _ in sc], [p for _, _, _, p, _ in sc], [None] * 3, [None] * 3,
+                      imgs, mode)
+        b.launch()
+        torch.cuda.synchronize()
+        for a, s in zip(imgs, singles):
+            assert np.array_equal(a.cpu().numpy(), s), f"mode {mode} differs from single-scene render"
+
+
+def test_graph_replays_match_eager():
+    """The captured batch is replayed several times with eager work in between; every
+    replay must reproduce the eager render."""
+    import torch
+    sc = _small_batch_scenes()
+    bvhs = [rtx.build(v, t) for v, t, _, _, _ in sc]
+    eager = [rtx.render(b, c, p).cpu().numpy() for b, (_, _, c, p, _) in zip(bvhs, sc)]
+    imgs = [torch.zeros((p.height, p.width, 3), device="cuda") for _, _, _, p, _ in sc]
+    b = rtx.Batch(bvhs, [c for _, _, c, _, _ in sc], [p for _, _, _, p, _ in sc], [None] * 3, [None] * 3, imgs, 1)
+    for rep in range(3):
+        for im in imgs:
+            im.fill_(-1.0)
+        b.launch()
+        torch.cuda.synchronize()
+        for a, e in zip(imgs, eager):
+            assert np.array_equal(a.cpu().numpy(), e), f"replay {rep} differs"
+        _ = rtx.render(bvhs[0], sc[0][2], sc[0][3])   # eager work between replays
+        torch.cuda.synchronize()
+
+
+def test_adjoint_matches_finite_differences():
+    """Path-replay gradients of sum((I - T)^2) w.r.t. per-triangle albedo and emission
+    against central differences (eps 1e-3) of the oracle renderer, 1e-2 relative."""
+    import torch
+    v, t, albedo_true, emission, cam = S.diff_scene(seed=3)
+    W, H, spp, mb = 40, 40, 4, 4
+    params = rtx.make_params(W, H, spp, mb, 99, seed=3)
+    oparams = O.make_params(W, H, spp, mb, 99, seed=3)
+    g = rtx.build(v, t)
+    o = O.build(v, t)
+    target = rtx.render(g, cam, params, _tensor(albedo_true), _tensor(emission))
+    rng = np.random.default_rng(0)
+    albedo = np.clip(albedo_true + rng.normal(0, 0.1, albedo_true.shape), 0.05, 0.95).astype(np.float32)
+    a_t = _tensor(albedo).requires_grad_(True)
+    e_t = _tensor(emission).requires_grad_(True)
+    img = rtx.differentiable_render(a_t, e_t, g, cam, params)
+    loss = ((img - target) ** 2).sum()
+    loss.backward()
+    ga = a_t.grad.cpu().numpy()
+    ge = e_t.grad.cpu().numpy()
+    tgt = target.cpu().numpy()
+    mag = np.abs(ga).ravel()
+    cand = np.argsort(-mag)[:400]
+    picks = list(rng.choice(cand, 14, replace=False))
+    eps = 1e-3
+    worst = 0.0
+    for flat in picks:
+        ti, c = divmod(int(flat), 3)
+        ap, am = albedo.copy(), albedo.copy()
+        ap[ti, c] += eps
+        am[ti, c] -= eps
+        fd = (O.render_loss_f64(o, cam, oparams, ap, emission, tgt, nthreads=16) -
+              O.render_loss_f64(o, cam, oparams, am, emission, tgt, nthreads=16)) / (2 * eps)
+        err = abs(ga[ti, c] - fd) / max(abs(fd), 1e-12)
+        worst = max(worst, err)
+        assert err < 1e-2, f"albedo[{ti},{c}]: adjoint {ga[ti, c]:.6g} vs FD {fd:.6g}"
+    for ti, c in [(20, 0), (21, 1)]:
+        ep, em = emission.copy(), emission.copy()
+        ep[ti, c] += eps
+        em[ti, c] -= eps
+        fd = (O.render_loss_f64(o, cam, oparams, albedo, ep, tgt, nthreads=16) -
+              O.render_loss_f64(o, cam, oparams, albedo, em, tgt, nthreads=16)) / (2 * eps)
+        assert abs(ge[ti, c] - fd) <= 1e-2 * abs(fd), f"emission[{ti},{c}]: {ge[ti, c]} vs FD {fd}"
+    print(f"[adjoint] worst relative error over {len(picks)} albedo parameters: {worst:.2e}", flush=True)
+
+
+def test_adjoint_rejects_roulette():
+    import torch
+    v, t, albedo, emission, cam = S.diff_scene(seed=3)
+    g = rtx.build(v, t)
+    p = rtx.make_params(8, 8, 1, 4, 3, seed=1)
+    with pytest.raises(rtx.RtxError, match="rr_bounce"):
+        rtx.render_adjoint(g, cam, p, _tensor(albedo), _tensor(emission), torch.zeros((8, 8, 3), device="cuda"))

@@ -1,0 +1,113 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports every
+declared symbol, the scene front-end equals the reference generators, file errors map to
+the reference's failure cases, and the oracle renderer meets closed forms."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import rtx
+from oracle import oracle as O
+from rtx import scenes as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "rtx.h")).read()
+    declared = sorted(set(re.findall(r"\b(rtx_[a-z_]+)\s*\(", hdr)))
+    L = rtx.lib()
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, f"declared but not exported: {missing}"
+    assert sorted(rtx.EXPORTS) == declared
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("args", [((0, 5, 0), 2.0, 16, 32), ((1.5, -2, 3), 0.7, 33, 64), ((0, 0, 0), 1.0, 2, 3)])
+def test_sphere_generator_matches_reference(args):
+    a, b = rtx.geo_sphere(*args), O.ref_geo_sphere(*args)
+    assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("args", [((0, 0, 5), (1, 1, 1), (0, 0, 0, 1)), ((0, -3, 0), (20, 0.1, 20), (0.1, 0.2, 0.3, 0.927))])
+def test_box_generator_matches_reference(args):
+    a, b = rtx.geo_box(*args), O.ref_geo_box(*args)
+    assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)) and np.array_equal(a[1], b[1])
+
+
+def test_scene_file_front_end(tmp_path):
+    scn = tmp_path / "s.scn"
+    scn.write_text("screen 80 60\ncamera 0 1 -5 lookat 0 0 0 60\nsphere 0 1 0 1\n"
+                   "box -2 1 2 1 1 1 q 0 0 0 1\nsphere 0, 0, 0, 1000\nbox 1 2 3\n")
+    v, t = rtx.load_scene(scn)
+    sv, st = rtx.geo_sphere((0, 1, 0), 1.0)
+    bv, bt = rtx.geo_box((-2, 1, 2), (1, 1, 1))
+    assert len(t) == len(st) + len(bt)       # malformed lines are skipped, as readScene does
+    n = np.linalg.norm(v[:, 3:], axis=1)
+    np.testing.assert_allclose(n, 1.0, atol=1e-6)
+
+
+def test_scene_file_errors(tmp_path):
+    with pytest.raises(rtx.RtxError, match="failed to open"):
+        rtx.load_scene(tmp_path / "missing.scn")
+    empty = tmp_path / "empty.scn"
+    empty.write_text("screen 10 10\n")
+    with pytest.raises(rtx.RtxError, match="no vertices"):
+        rtx.load_scene(empty)
+    mesh = tmp_path / "mesh.scn"
+    mesh.write_text("mesh bunny.ply 0 0 0 1 q 0 0 0 1\n")
+    with pytest.raises(rtx.RtxError, match="mesh import"):
+        rtx.load_scene(mesh)
+
+
+def _write_bvh(path, version=1, truncate=0):
+    nodes = np.zeros(1, O.NODE48)
+    nodes["is_leaf"] = 1
+    nodes["b"] = 1
+    tris = np.array([[0, 1, 2]], np.uint32)
+    verts = np.zeros((3, 6), np.float32)
+    blob = np.array([version, 1, 1, 3], np.uint32).tobytes() + nodes.tobytes() + tris.tobytes() + verts.tobytes()
+    path.write_bytes(blob[: len(blob) - truncate])
+
+
+def test_bvh_file_errors(tmp_path):
+    with pytest.raises(rtx.RtxError, match="failed to open"):
+        rtx.load_bvh(tmp_path / "none.bvh")
+    _write_bvh(tmp_path / "v2.bvh", version=2)
+    with pytest.raises(rtx.RtxError, match="version mismatch"):
+        rtx.load_bvh(tmp_path / "v2.bvh")
+    _write_bvh(tmp_path / "short.bvh", truncate=10)
+    with pytest.raises(rtx.RtxError, match="payload"):
+        rtx.load_bvh(tmp_path / "short.bvh")
+    (tmp_path / "hdr.bvh").write_bytes(b"\x01\x00")
+    with pytest.raises(rtx.RtxError, match="header"):
+        rtx.load_bvh(tmp_path / "hdr.bvh")
+
+
+def _closed_box():
+    parts = []
+    for axis in range(3):
+        for sgn in (-1, 1):
+            c = np.zeros(3)
+            c[axis] = sgn
+            u = np.zeros(3); u[(axis + 1) % 3] = 1
+            w = np.zeros(3); w[(axis + 2) % 3] = 1
+            pts = [c - u - w, c + u - w, c + u + w, c - u + w]
+            parts.append(S.quad(*pts, -c))
+    return S.merge(parts)
+
+
+@pytest.mark.parametrize("mb", [0, 1, 4])
+def test_oracle_white_furnace(mb):
+    """Closed box, albedo a and emission E everywhere, no roulette: every path hits a wall
+    at every bounce, so each pixel is E * sum_{k<=mb} a^k exactly."""
+    v, t = _closed_box()
+    o = O.build(v, t)
+    a, E = 0.6, 0.5
+    albedo = np.full((len(t), 3), a, np.float32)
+    emission = np.full((len(t), 3), E, np.float32)
+    cam = S.camera_lookat((0.1, 0.05, -0.2), (0.3, 0.2, 1.0), 60.0)
+    img = O.render(o, cam, O.make_params(16, 12, 3, mb, 99, seed=1), albedo, emission)
+    np.testing.assert_allclose(img, E * sum(a ** k for k in range(mb + 1)), rtol=2e-6)
