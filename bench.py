@@ -45,6 +45,8 @@ def main():
     ap.add_argument("--rays", type=int, default=1 << 22)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-render", action="store_true", help="skip the batched (C3) and differentiable (C4) legs")
+    ap.add_argument("--c3-scenes", type=int, default=64)
     ap.add_argument("--skip-extras", action="store_true",
                     help="time only the headline launch (used under the profiler so k_trace's average is "
                          "the incoherent launch alone)")
@@ -125,6 +127,12 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(bvh, v, inc[: 1 << 21].cpu().numpy(), args.cpu_seconds)
 
+    render = None
+    if not (args.no_render or args.skip_extras):
+        del inc, coh, hits, occ, counts
+        torch.cuda.empty_cache()
+        render = {"differentiable": bench_differentiable(stream), "batched": bench_batched(stream, args.c3_scenes)}
+
     mrays = n / (ms_inc * 1e-3) / 1e6
     out = {
         "metric": "Mrays/s incoherent closest-hit (batched vs plain); fwd+adjoint ms/iter",
@@ -156,8 +164,82 @@ def main():
             "algorithmic_bytes_per_launch": alg_bytes,
             "timed_wall_s": round(wall, 3),
         },
+        "render": render,
     }
     print(json.dumps(out), flush=True)
+
+
+def _cuda_ms(fn, reps, stream):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def bench_differentiable(stream, reps=3):
+    """Differentiable render: Cornell box + 4096-triangle displaced sphere, per-triangle
+    albedo, 2-triangle emitter; 512x512, 32 spp, max_bounce 4, no Russian roulette (the
+    adjoint needs a path set independent of albedo). Loss = sum (I - T)^2."""
+    import numpy as np
+    import torch
+
+    import rtx
+    from rtx import scenes as S
+    v, t, albedo_true, emission, cam = S.diff_scene(seed=3)
+    g = rtx.build(v, t)
+    p = rtx.make_params(512, 512, 32, 4, 99, seed=3)
+    e_t = torch.from_numpy(emission).cuda()
+    target = rtx.render(g, cam, p, torch.from_numpy(albedo_true).cuda(), e_t, stream=stream)
+    rng = np.random.default_rng(0)
+    albedo = torch.from_numpy(np.clip(albedo_true + rng.normal(0, 0.1, albedo_true.shape), 0.05, 0.95)
+                              .astype(np.float32)).cuda()
+    img = torch.empty_like(target)
+
+    def fwd():
+        rtx.render(g, cam, p, albedo, e_t, out=img, stream=stream)
+
+    def fwd_adj():
+        a = albedo.detach().clone().requires_grad_(True)
+        e = e_t.detach().clone().requires_grad_(True)
+        loss = ((rtx.differentiable_render(a, e, g, cam, p) - target) ** 2).sum()
+        loss.backward()
+    ms_f = _cuda_ms(fwd, reps, stream)
+    ms_fa = _cuda_ms(fwd_adj, reps, stream)
+    return {"workload": "512x512, 32 spp, max_bounce 4, 4118 triangles", "forward_ms": round(ms_f, 3),
+            "forward_adjoint_ms_per_iter": round(ms_fa, 3), "ratio": round(ms_fa / ms_f, 3),
+            "paths": 512 * 512 * 32}
+
+
+def bench_batched(stream, nscenes, reps=2):
+    """Batched wavefront render of fBm-displaced spheres of uneven size (seed 2): one
+    launch sequence per scene (plain) vs all scenes in shared launches replayed from one
+    hipGraph (batched)."""
+    import torch
+
+    import rtx
+    from rtx import scenes as S
+    sc = S.batched_scenes(nscenes, seed=2)
+    bvhs = [rtx.build(s["verts"], s["tris"]) for s in sc]
+    params = [rtx.make_params(s["width"], s["height"], s["spp"], s["max_bounce"], 3, seed=2 + i)
+              for i, s in enumerate(sc)]
+    imgs = [torch.empty((s["height"], s["width"], 3), device="cuda") for s in sc]
+    paths = sum(s["width"] * s["height"] * s["spp"] for s in sc)
+    out = {"workload": f"{nscenes} scenes, {sum(len(s['tris']) for s in sc)} triangles, {paths} paths, "
+                       "max_bounce 6", "paths": paths}
+    for mode, name in ((0, "plain"), (1, "batched_graph")):
+        b = rtx.Batch(bvhs, [s["cam"] for s in sc], params, [None] * nscenes, [None] * nscenes, imgs, mode)
+        ms = _cuda_ms(lambda: b.launch(stream), reps, stream)
+        out[name + "_ms"] = round(ms, 2)
+        out[name + "_mpaths_s"] = round(paths / ms / 1e3, 2)
+        del b
+    out["speedup"] = round(out["plain_ms"] / out["batched_graph_ms"], 3)
+    return out
 
 
 def cpu_baseline(bvh, verts, rays, seconds):
