@@ -172,13 +172,17 @@ int rtx_batch_create(const rtx_bvh* const* scenes, int nscenes, const rtx_camera
 int rtx_batch_launch(rtx_batch* batch, void* stream);
 void rtx_batch_free(rtx_batch* batch);
 
-/* Launch shape of the renderer (process-wide; chosen by the project autotuner).
+/* Launch shapes (process-wide; chosen by the project autotuner, none changes results).
  * batch_paths: paths in flight per wavefront batch (>= 1024); grid_blocks: blocks of the
- * persistent trace/shade kernels; shade_threads: threads per shade block (multiple of 64). */
+ * persistent trace/shade kernels; shade_threads: threads per shade block (multiple of 64);
+ * trace_lds_depth: LDS stack entries per lane (8, 12 or 16); trace_loop: 0 = one
+ * node/leaf loop, 1 = while-while (triangle tests run with the wave converged). */
 typedef struct rtx_tuning {
     uint32_t batch_paths;
     uint32_t grid_blocks;
     uint32_t shade_threads;
+    uint32_t trace_lds_depth;
+    uint32_t trace_loop;
 } rtx_tuning;
 int rtx_set_tuning(const rtx_tuning* t);
 int rtx_get_tuning(rtx_tuning* t);

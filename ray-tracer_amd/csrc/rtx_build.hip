@@ -53,6 +53,7 @@ struct Counters {
     uint32_t tsize;      // s_triIndexSize
     uint32_t ssize;      // s_sortedTrisSize
     uint32_t err;
+    uint32_t nwave;      // SAH nodes of this level queued for the wave evaluator
 };
 
 struct Ctx {
@@ -72,6 +73,7 @@ struct Ctx {
     uint32_t cap;
     uint32_t ncap;           // node array capacity
     Counters* ctr;
+    uint32_t* wlist;         // frontier items for k_sah_wave
 };
 
 __device__ __forceinline__ f3 vpos(const float* v, uint32_t i) { return {v[6 * i], v[6 * i + 1], v[6 * i + 2]}; }
@@ -468,8 +470,13 @@ __global__ void k_init_root(BNode* nodes, const float* box6, uint32_t nt, uint32
         ctr->tsize = nt;
         ctr->ssize = nt;
         ctr->err = 0;
+        ctr->nwave = 0;
     }
     (void)tindex;
+}
+
+__global__ void k_clear_nwave(Counters* c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->nwave = 0;
 }
 
 __global__ void k_iota(uint32_t* a, uint32_t n) {
@@ -505,8 +512,213 @@ __global__ void k_level_decide(Ctx C, uint32_t ls, uint32_t cnt) {
                 d.mid = hi;
             }
         }
-        if (sah) sah_eval(C, begin, n, Box{nd.lo, nd.hi}, d);
+        if (sah) {
+            if (n <= 64) {
+                C.wlist[atomicAdd(&C.ctr->nwave, 1u)] = k;   // one wave per node (k_sah_wave)
+                return;
+            }
+            sah_eval(C, begin, n, Box{nd.lo, nd.hi}, d);
+        }
     }
+    C.dec[k] = d;
+}
+
+// Binned SAH of one node of at most 64 triangles per 64-lane block (bvh_sah.cpp:24-243).
+// The triangles are staged in LDS in s_triIndex order. Lane (axis, bin) folds, in that
+// order, the triangles falling into its bin -- object bins by centroid, spatial bins by
+// clipping (AABB.cpp:86-171) -- so every bin box is the reference's serial fold. Each
+// lane then prices split candidate `bin` of its axis; the winner is the minimum cost, and
+// on equal cost the first one the reference's loops meet (lower axis, higher bin).
+__global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
+    __shared__ float tv[64][9];
+    __shared__ float tb[64][6];
+    __shared__ float tc[64][3];
+    __shared__ uint32_t tsi[64];
+    __shared__ float bbox[3 * 16][6];
+    __shared__ uint32_t bc0[3 * 16], bc1[3 * 16];
+    if (blockIdx.x >= C.ctr->nwave) return;
+    const uint32_t k = C.wlist[blockIdx.x];
+    const uint32_t ni = ls + k;
+    const BNode nd = C.nodes[ni];
+    const uint32_t begin = nd.a, n = nd.b;
+    const int lane = threadIdx.x;
+    const int nb = clampi((int)n / 4, kBinsMin, kBinsMax);   // n <= 64 -> nb <= 16
+    const Box pbox{nd.lo, nd.hi};
+    if ((uint32_t)lane < n) {
+        const uint32_t si = C.tindex[begin + lane];
+        const uint32_t t = C.sorted[si];
+        tsi[lane] = si;
+        f3 a, b, c;
+        tri_verts(C, t, a, b, c);
+        tv[lane][0] = a.x; tv[lane][1] = a.y; tv[lane][2] = a.z;
+        tv[lane][3] = b.x; tv[lane][4] = b.y; tv[lane][5] = b.z;
+        tv[lane][6] = c.x; tv[lane][7] = c.y; tv[lane][8] = c.z;
+        const Box bx = load_tbox(C.tbox, t);
+        tb[lane][0] = bx.lo.x; tb[lane][1] = bx.lo.y; tb[lane][2] = bx.lo.z;
+        tb[lane][3] = bx.hi.x; tb[lane][4] = bx.hi.y; tb[lane][5] = bx.hi.z;
+        const float4 ce = C.cent[t];
+        tc[lane][0] = ce.x; tc[lane][1] = ce.y; tc[lane][2] = ce.z;
+    }
+    __syncthreads();
+    const float invA = 1.0f / pbox.half_area();
+    const float leafCost = kCIsect * (float)n;
+    const int axis = lane / nb, bin = lane % nb;
+    const bool act = lane < 3 * nb;
+    auto tbox_i = [&](int i) { return Box{{tb[i][0], tb[i][1], tb[i][2]}, {tb[i][3], tb[i][4], tb[i][5]}}; };
+    auto save_bin = [&](int slot, const Box& b) {
+        bbox[slot][0] = b.lo.x; bbox[slot][1] = b.lo.y; bbox[slot][2] = b.lo.z;
+        bbox[slot][3] = b.hi.x; bbox[slot][4] = b.hi.y; bbox[slot][5] = b.hi.z;
+    };
+    auto bin_box = [&](int slot) {
+        return Box{{bbox[slot][0], bbox[slot][1], bbox[slot][2]}, {bbox[slot][3], bbox[slot][4], bbox[slot][5]}};
+    };
+    // price candidate `bin` (> 0) of this lane's axis from the bin table; returns cost or +inf
+    auto price = [&](bool spatialCounts) {
+        if (!act || bin == 0) return kInf;
+        Box la = Box::empty();
+        uint32_t lc = 0;
+        for (int b = 0; b < bin; ++b) { la.extend(bin_box(axis * nb + b)); lc += bc0[axis * nb + b]; }
+        Box ra = Box::empty();
+        uint32_t rc = 0;
+        for (int b = nb - 1; b >= bin; --b) {
+            ra.extend(bin_box(axis * nb + b));
+            rc += spatialCounts ? bc1[axis * nb + b] : bc0[axis * nb + b];
+        }
+        if (lc == 0 || rc == 0) return kInf;
+        return kCTrav + kCIsect * (la.half_area() * (float)lc + ra.half_area() * (float)rc) * invA;
+    };
+    // (cost, order) minimum over the wave; order = axis * 64 + (63 - bin)
+    auto argmin = [&](float cost, int& wAxis, int& wBin) {
+        float bc = cost;
+        int key = (cost < kInf) ? axis * 64 + (63 - bin) : 0x7fffffff;
+        for (int o = 32; o >= 1; o >>= 1) {
+            float oc = __shfl_xor(bc, o);
+            int ok = __shfl_xor(key, o);
+            if (oc < bc || (oc == bc && ok < key)) { bc = oc; key = ok; }
+        }
+        if (key == 0x7fffffff) { wAxis = -1; wBin = -1; return kInf; }
+        wAxis = key / 64;
+        wBin = 63 - key % 64;
+        return bc;
+    };
+
+    // ---- object split: centroid bins (bvh_sah.cpp:58-126)
+    float cmin = 3.40282347e+38f, cmax = -3.40282347e+38f;
+    if (act) {
+        for (uint32_t i = 0; i < n; ++i) {
+            float c = tc[i][axis];
+            cmin = std_min(cmin, c);
+            cmax = std_max(cmax, c);
+        }
+    }
+    const bool axisOk = act && !(cmax - cmin < kCentroidEps);
+    const float oscale = axisOk ? (float)nb / (cmax - cmin) : 0.f;
+    if (act) {
+        Box bx = Box::empty();
+        uint32_t cnt = 0;
+        if (axisOk) {
+            for (uint32_t i = 0; i < n; ++i) {
+                if (min(f2i_x86((tc[i][axis] - cmin) * oscale), nb - 1) == bin) { ++cnt; bx.extend(tbox_i(i)); }
+            }
+        }
+        save_bin(lane, bx);
+        bc0[lane] = cnt;
+    }
+    __syncthreads();
+    int oAxis, oBin;
+    float oCost = argmin(axisOk ? price(false) : kInf, oAxis, oBin);
+    if (!(oCost < leafCost)) { oAxis = -1; oBin = -1; oCost = leafCost; }
+    const int wl = oAxis >= 0 ? oAxis * nb + 1 : 0;          // a lane of the winning axis
+    const float oCMin = __shfl(cmin, wl), oScale = __shfl(oscale, wl);
+    __syncthreads();
+
+    // ---- spatial split: node-extent bins with per-bin clipping (bvh_sah.cpp:157-243)
+    const float nmin = act ? comp(pbox.lo, axis) : 0.f, nmax = act ? comp(pbox.hi, axis) : 0.f;
+    const bool sOk = act && !(nmax - nmin < kSpatialEps);
+    const float sscale = sOk ? (float)nb / (nmax - nmin) : 0.f;
+    if (act) {
+        Box bx = Box::empty();
+        uint32_t cin = 0, cout = 0;
+        if (sOk) {
+            Box reg = pbox;
+            set_comp(reg.lo, axis, nmin + (float)bin / sscale);
+            set_comp(reg.hi, axis, nmin + (float)(bin + 1) / sscale);
+            for (uint32_t i = 0; i < n; ++i) {
+                int b0 = clampi(f2i_x86((tb[i][axis] - nmin) * sscale), 0, nb - 1);
+                int b1 = clampi(f2i_x86((tb[i][3 + axis] - nmin) * sscale), 0, nb - 1);
+                cin += (b0 == bin);
+                cout += (b1 == bin);
+                if (b0 <= bin && bin <= b1) {
+                    Box cb = clip_tri(f3{tv[i][0], tv[i][1], tv[i][2]}, f3{tv[i][3], tv[i][4], tv[i][5]},
+                                      f3{tv[i][6], tv[i][7], tv[i][8]}, reg);
+                    if (cb.valid()) bx.extend(cb);
+                }
+            }
+        }
+        save_bin(lane, bx);
+        bc0[lane] = cin;
+        bc1[lane] = cout;
+    }
+    __syncthreads();
+    int sAxis, sBin;
+    float sCost = argmin(sOk ? price(true) : kInf, sAxis, sBin);
+    const bool spatial = sAxis >= 0 && sCost < oCost;
+    const int ws = sAxis >= 0 ? sAxis * nb + 1 : 0;
+    const float sMin = __shfl(nmin, ws), sScale = __shfl(sscale, ws);
+
+    if (lane != 0) return;
+    // ---- apply the partitions serially, as std::partition orders them (bvh_sah.cpp:247-388)
+    Dec d;
+    d.mid = 0; d.nl = d.ns = d.nr = 0; d.fb_split = 0;
+    d.oAxis = oAxis; d.oBin = oBin; d.oCMin = oCMin; d.oScale = oScale; d.nb = nb;
+    d.plane = 0.f; d.sAxis = -1;
+    d.type = D_LEAF;
+    uint32_t slot[64];
+    for (uint32_t i = 0; i < n; ++i) slot[i] = i;
+    auto opred = [&](uint32_t s_) { return min(f2i_x86((tc[s_][oAxis] - oCMin) * oScale), nb - 1) < oBin; };
+    auto part = [&](uint32_t first, uint32_t last, auto pred) {
+        for (;;) {
+            for (;;) {
+                if (first == last) return first;
+                if (pred(slot[first])) ++first; else break;
+            }
+            --last;
+            for (;;) {
+                if (first == last) return first;
+                if (!pred(slot[last])) --last; else break;
+            }
+            uint32_t t_ = slot[first]; slot[first] = slot[last]; slot[last] = t_;
+            ++first;
+        }
+    };
+    if (oAxis == -1 && !spatial) {
+        C.dec[k] = d;
+        return;
+    }
+    if (!spatial) {
+        uint32_t mid = part(0, n, opred);
+        if (mid != 0 && mid != n) { d.type = D_OBJECT; d.mid = begin + mid; }
+    } else {
+        const float plane = sMin + (float)sBin / sScale;
+        const int ax = sAxis;
+        uint32_t le = part(0, n, [&](uint32_t s_) { return tb[s_][3 + ax] <= plane; });
+        uint32_t se = part(le, n, [&](uint32_t s_) { return tb[s_][ax] < plane; });
+        d.plane = plane; d.sAxis = ax;
+        d.nl = le; d.ns = se - le; d.nr = n - se;
+        if (d.nl + d.ns == 0 || d.ns + d.nr == 0) {
+            if (oAxis != -1) {
+                uint32_t mid = part(0, n, opred);
+                if (mid != 0 && mid != n) { d.type = D_OBJECT; d.mid = begin + mid; }
+            }
+        } else {
+            uint32_t c = 0;
+            if (oAxis != -1)
+                for (uint32_t i = 0; i < n; ++i) c += opred(i) ? 1u : 0u;
+            d.fb_split = (oAxis != -1 && c > 0 && c < n) ? 1u : 0u;
+            d.type = D_SPATIAL;
+        }
+    }
+    for (uint32_t i = 0; i < n; ++i) C.tindex[begin + i] = tsi[slot[i]];
     C.dec[k] = d;
 }
 
@@ -909,6 +1121,7 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
     uint32_t* soff = S.get<uint32_t>(cap + 1);
     uint32_t* acc = S.get<uint32_t>(cap + 1);
     Counters* ctr = S.get<Counters>(1);
+    uint32_t* wlist = S.get<uint32_t>(cap + 1);
     Dfs D;
     uint32_t* dbuf = S.get<uint32_t>((size_t)ncap * 8);
     B->codes = dalloc<uint64_t>(N);
@@ -973,14 +1186,16 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
     k_iota<<<nblk(N, 256), 256, 0, st>>>(tindex, N);
     k_init_root<<<1, 64, 0, st>>>(nodes, box6, N, tindex, ctr);
 
-    Ctx C{V, tris_d, tbox, cent, scodes, sorted, tindex, nodes, dec, child, toff, soff, acc, cap, ncap, ctr};
+    Ctx C{V, tris_d, tbox, cent, scodes, sorted, tindex, nodes, dec, child, toff, soff, acc, cap, ncap, ctr, wlist};
     std::vector<std::pair<uint32_t, uint32_t>> levels;
     uint32_t ls = 0, le = 1;
     Counters hc{};
     while (ls < le) {
         uint32_t cnt = le - ls;
         levels.push_back({ls, cnt});
+        k_clear_nwave<<<1, 64, 0, st>>>(ctr);
         k_level_decide<<<nblk(cnt, 64), 64, 0, st>>>(C, ls, cnt);
+        k_sah_wave<<<cnt, 64, 0, st>>>(C, ls);
         k_level_budget<<<1, 1024, 0, st>>>(C, cnt);
         k_level_apply<<<nblk(cnt, 64), 64, 0, st>>>(C, ls, cnt);
         RTX_HIP(hipMemcpyAsync(&hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));

@@ -47,9 +47,13 @@ struct PathState {
     uint32_t* smp;
     uint32_t* q[2];
     uint32_t* cnt;     // [0], [1] queue lengths
+    // adjoint only: per surface hit k of path i, at [k * B + i]
+    float4* vlup;      // radiance gathered up to and including hit k (xyz), triangle id (w bits)
+    float4* vtp;       // throughput arriving at hit k
+    uint32_t* nv;      // hits recorded per path
+    uint32_t B;
 };
 
-rtx_tuning g_tuning = {1u << 22, 2048, 256};
 
 __device__ __forceinline__ uint32_t find_scene(const uint64_t* off, int n, uint64_t g) {
     int lo = 0, hi = n;   // off[lo] <= g < off[hi]
@@ -123,23 +127,38 @@ __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* _
         P.pix[i] = pixel;
         P.smp[i] = sample;
         P.q[0][i] = i;
+        if (P.nv) P.nv[i] = 0;
     }
 }
 
+template <int DEPTH, int LOOP>
 __global__ void __launch_bounds__(kTraceThreads) k_wtrace(const SceneDesc* __restrict__ scenes, PathState P, int qi) {
-    __shared__ uint2 lst[kLdsDepth][kTraceThreads];
+    __shared__ uint2 lst[DEPTH][kTraceThreads];
     const uint32_t n = P.cnt[qi];
     const uint32_t* Q = P.q[qi];
-    for (uint32_t k = xcd_block(blockIdx.x) * kTraceThreads + threadIdx.x; k < n; k += gridDim.x * kTraceThreads) {
-        const uint32_t i = Q[k];
-        const float4 r0 = P.ray0[i], r1 = P.ray1[i];
-        const TraceOut h = trace_ray<false, false>(scenes[P.sid[i]].T, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z},
-                                                   r0.w, r1.w, lst, threadIdx.x);
-        P.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
+    // wave-uniform trip count (the while-while loop votes across the wave)
+    for (uint32_t wb = xcd_block(blockIdx.x) * kTraceThreads + (threadIdx.x & ~63u); wb < n;
+         wb += gridDim.x * kTraceThreads) {
+        const uint32_t k = wb + __lane_id();
+        const bool in = k < n;
+        uint32_t i = 0;
+        float4 r0 = make_float4(0.f, 0.f, 0.f, 1.f), r1 = make_float4(0.f, 0.f, 1.f, 0.f);
+        const SceneDesc* S = scenes;
+        if (in) {
+            i = Q[k];
+            r0 = P.ray0[i];
+            r1 = P.ray1[i];
+            S = scenes + P.sid[i];
+        }
+        const TraceOut h = trace_ray<false, false, DEPTH, LOOP>(S->T, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z},
+                                                                r0.w, r1.w, lst, threadIdx.x);
+        if (in) P.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
     }
 }
 
-// One bounce of pathTrace (main.cpp:158-200) for every queued path.
+// One bounce of pathTrace (main.cpp:158-200) for every queued path. ADJ additionally
+// records each surface hit for the adjoint (triangle, radiance so far, throughput).
+template <bool ADJ>
 __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int qi, int bounce) {
     const uint32_t n = P.cnt[qi];
     const uint32_t* Q = P.q[qi];
@@ -165,9 +184,14 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
                 L = make_float4(L.x + tp.x * e.x, L.y + tp.y * e.y, L.z + tp.z * e.z, 0.f);
             } else {
                 const uint32_t prim = S.prims[tri];
+                if (ADJ) P.vtp[(size_t)bounce * P.B + i] = tp;
                 if (S.emission) {
                     const float* e = S.emission + 3 * prim;
                     L = make_float4(L.x + tp.x * e[0], L.y + tp.y * e[1], L.z + tp.z * e[2], 0.f);
+                }
+                if (ADJ) {
+                    P.vlup[(size_t)bounce * P.B + i] = make_float4(L.x, L.y, L.z, __uint_as_float(prim));
+                    P.nv[i] = (uint32_t)bounce + 1;
                 }
                 if (bounce < S.max_bounce) {
                     float ax = 0.8f, ay = 0.8f, az = 0.8f;
@@ -234,69 +258,78 @@ __global__ void k_finalize(const SceneDesc* __restrict__ scenes, int s) {
         S.image[p] = S.image[p] / fs;
 }
 
-// Path-replay adjoint for per-triangle albedo and emission. One lane per path re-traces
-// the path (same traversal, same sample table) and keeps only its vertices' triangle ids
-// and the radiance gathered so far; with L = sum_m (prod_{j<m} a_j) X_m,
-//   dL/da_k = (L - L_upto_k) / a_k      and      dL/dLe_k = throughput arriving at k.
-constexpr int kMaxAdjVerts = 16;
+// Atomic accumulation for the adjoint: before adding, the wave merges lanes that target
+// the same triangle (one merge round per distinct triangle, up to kMergeRounds); lanes
+// left after that add their own contributions.
+constexpr int kMergeRounds = 8;
 
-__global__ void __launch_bounds__(kTraceThreads) k_adjoint(const SceneDesc* __restrict__ scenes,
-                                                           const float* __restrict__ dimg, float* __restrict__ ga,
-                                                           float* __restrict__ ge) {
-    __shared__ uint2 lst[kLdsDepth][kTraceThreads];
-    const SceneDesc& S = scenes[0];
-    const uint64_t total = (uint64_t)S.npix * S.spp;
-    for (uint64_t g = (uint64_t)xcd_block(blockIdx.x) * kTraceThreads + threadIdx.x; g < total;
-         g += (uint64_t)gridDim.x * kTraceThreads) {
-        const uint32_t sample = (uint32_t)(g / S.npix), pixel = (uint32_t)(g % S.npix);
-        const float fs = (float)S.spp;
-        const f3 delta{dimg[3 * pixel] / fs, dimg[3 * pixel + 1] / fs, dimg[3 * pixel + 2] / fs};
-        f3 o, d;
-        primary_ray(S, (int)(pixel % (uint32_t)S.W), (int)(pixel / (uint32_t)S.W),
-                    sample01(S.seed, pixel, sample, 0), sample01(S.seed, pixel, sample, 1), o, d);
-        float tmin = 1e-3f;
-        f3 tp{1.f, 1.f, 1.f}, L{0.f, 0.f, 0.f};
-        uint32_t vprim[kMaxAdjVerts];
-        f3 vL[kMaxAdjVerts];
-        int nv = 0;
-        for (int bounce = 0; bounce <= S.max_bounce; ++bounce) {
-            const TraceOut h = trace_ray<false, false>(S.T, o, d, tmin, kInf, lst, threadIdx.x);
-            if (h.tri == kInvalid) {
-                f3 e = env_color(S, d);
-                L = f3{L.x + tp.x * e.x, L.y + tp.y * e.y, L.z + tp.z * e.z};
-                break;
-            }
-            const uint32_t prim = S.prims[h.tri];
-            if (S.emission) {
-                const float* e = S.emission + 3 * prim;
-                if (ge) {
-                    atomicAdd(&ge[3 * prim], delta.x * tp.x);
-                    atomicAdd(&ge[3 * prim + 1], delta.y * tp.y);
-                    atomicAdd(&ge[3 * prim + 2], delta.z * tp.z);
-                }
-                L = f3{L.x + tp.x * e[0], L.y + tp.y * e[1], L.z + tp.z * e[2]};
-            }
-            if (bounce == S.max_bounce) break;
-            const float* a = S.albedo + 3 * prim;
-            vprim[nv] = prim;
-            vL[nv] = L;
-            ++nv;
-            const float u1 = sample01(S.seed, pixel, sample, 2 + 3 * bounce);
-            const float u2 = sample01(S.seed, pixel, sample, 3 + 3 * bounce);
-            const f3 wd = bounce_dir(u1, u2, hit_normal(S.T, h.tri, h.u, h.v));
-            tp = f3{tp.x * a[0], tp.y * a[1], tp.z * a[2]};
-            o = add(o, scl(h.t, d));
-            d = wd;
-            tmin = 1e-4f;
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ void merged_add(float* __restrict__ buf, bool has, uint32_t prim, f3 g) {
+    uint64_t pending = __ballot(has);
+    const uint64_t me = 1ull << __lane_id();
+    for (int round = 0; pending && round < kMergeRounds; ++round) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t p = __shfl(prim, leader);
+        const bool in = has && (pending & me) && prim == p;
+        const uint64_t m = __ballot(in);
+        const float sx = wave_sum(in ? g.x : 0.f), sy = wave_sum(in ? g.y : 0.f), sz = wave_sum(in ? g.z : 0.f);
+        if (__lane_id() == leader) {
+            atomicAdd(&buf[3 * p], sx);
+            atomicAdd(&buf[3 * p + 1], sy);
+            atomicAdd(&buf[3 * p + 2], sz);
         }
-        if (ga) {
-            for (int k = 0; k < nv; ++k) {
-                const uint32_t prim = vprim[k];
+        pending &= ~m;
+    }
+    if (has && (pending & me)) {
+        atomicAdd(&buf[3 * prim], g.x);
+        atomicAdd(&buf[3 * prim + 1], g.y);
+        atomicAdd(&buf[3 * prim + 2], g.z);
+    }
+}
+
+// Gradient scatter of the path-replay adjoint. The replay itself is the forward
+// wavefront (k_wtrace + k_shade<true>), which recorded every surface hit k of path i.
+// With L = sum_m (prod_{j<m} a_j) X_m:
+//   dL/da_k = (L - L_upto_k) / a_k        dL/dLe_k = throughput arriving at k.
+// Paths are in pixel order, so neighbouring lanes mostly hit the same triangles and the
+// wave merge cuts the atomics.
+__global__ void k_grad(const SceneDesc* __restrict__ scenes, PathState P, uint32_t cnt,
+                       const float* __restrict__ dimg, float* __restrict__ ga, float* __restrict__ ge) {
+    const SceneDesc& S = scenes[0];
+    const float fs = (float)S.spp;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t wb = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wb < cnt; wb += stride) {
+        const uint32_t i = wb + __lane_id();
+        const bool live = i < cnt;
+        uint32_t nv = 0;
+        f3 L{0.f, 0.f, 0.f}, delta{0.f, 0.f, 0.f};
+        if (live) {
+            nv = P.nv[i];
+            const float4 l = P.L[i];
+            L = f3{l.x, l.y, l.z};
+            const uint32_t pixel = P.pix[i];
+            delta = f3{dimg[3 * pixel] / fs, dimg[3 * pixel + 1] / fs, dimg[3 * pixel + 2] / fs};
+        }
+        uint32_t maxnv = nv;
+        for (int off = 32; off >= 1; off >>= 1) maxnv = max(maxnv, (uint32_t)__shfl_xor((int)maxnv, off));
+        for (uint32_t k = 0; k < maxnv; ++k) {
+            const bool has = k < nv;
+            uint32_t prim = 0;
+            f3 gA{0.f, 0.f, 0.f}, gE{0.f, 0.f, 0.f};
+            if (has) {
+                const float4 lu = P.vlup[(size_t)k * P.B + i];
+                const float4 tp = P.vtp[(size_t)k * P.B + i];
+                prim = __float_as_uint(lu.w);
                 const float* a = S.albedo + 3 * prim;
-                atomicAdd(&ga[3 * prim], delta.x * (L.x - vL[k].x) / a[0]);
-                atomicAdd(&ga[3 * prim + 1], delta.y * (L.y - vL[k].y) / a[1]);
-                atomicAdd(&ga[3 * prim + 2], delta.z * (L.z - vL[k].z) / a[2]);
+                gA = f3{delta.x * (L.x - lu.x) / a[0], delta.y * (L.y - lu.y) / a[1], delta.z * (L.z - lu.z) / a[2]};
+                gE = f3{delta.x * tp.x, delta.y * tp.y, delta.z * tp.z};
             }
+            if (ga) merged_add(ga, has, prim, gA);
+            if (ge) merged_add(ge, has, prim, gE);
         }
     }
 }
@@ -330,7 +363,7 @@ int check_scene(const rtx_bvh* b, const rtx_camera* c, const rtx_render_params* 
         return fail(RTX_E_ARG, std::string("bad render parameters on ") + fn);
     if ((uint64_t)p->width * p->height > 0xFFFFFFFFull / 3)
         return fail(RTX_E_LIMIT, std::string("image too large on ") + fn);
-    if (b->max_depth + 2 > (uint32_t)(kLdsDepth + kSpillDepth))
+    if (b->max_depth > kMaxTraceDepth)
         return fail(RTX_E_LIMIT, std::string("tree deeper than the traversal stack on ") + fn);
     return RTX_OK;
 }
@@ -363,12 +396,16 @@ struct Batch {
         if (p) allocs.push_back(p);
         return p;
     }
-    int setup(uint32_t batch_paths);
+    // adjoint: dLoss/dimage in, gradients out (scene 0 only)
+    const float* dimg = nullptr;
+    float* ga = nullptr;
+    float* ge = nullptr;
+    int setup(uint32_t batch_paths, bool adjoint = false);
     void enqueue_range(int s_first, int s_last, hipStream_t st);   // scenes [s_first, s_last)
     int launch(hipStream_t st);
 };
 
-int Batch::setup(uint32_t batch_paths) {
+int Batch::setup(uint32_t batch_paths, bool adjoint) {
     const int ns = (int)descs.size();
     off.assign(ns + 1, 0);
     for (int s = 0; s < ns; ++s) {
@@ -385,6 +422,14 @@ int Batch::setup(uint32_t batch_paths) {
     P.sid = alloc<uint32_t>(B); P.pix = alloc<uint32_t>(B); P.smp = alloc<uint32_t>(B);
     P.q[0] = alloc<uint32_t>(B); P.q[1] = alloc<uint32_t>(B);
     P.cnt = alloc<uint32_t>(4);
+    P.B = B;
+    P.vlup = nullptr; P.vtp = nullptr; P.nv = nullptr;
+    if (adjoint) {
+        P.vlup = alloc<float4>((size_t)B * (max_b + 1));
+        P.vtp = alloc<float4>((size_t)B * (max_b + 1));
+        P.nv = alloc<uint32_t>(B);
+        if (!P.vlup || !P.vtp || !P.nv) return fail(RTX_E_HIP, "device allocation failed on render setup");
+    }
     if (!d_scenes || !d_off || !P.ray0 || !P.ray1 || !P.hit || !P.tp || !P.L || !P.sid || !P.pix || !P.smp ||
         !P.q[0] || !P.q[1] || !P.cnt)
         return fail(RTX_E_HIP, "device allocation failed on render setup");
@@ -396,11 +441,14 @@ int Batch::setup(uint32_t batch_paths) {
 // Kernel sequence for the paths of scenes [s_first, s_last): batches of B paths.
 void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
     const uint32_t grid = g_tuning.grid_blocks, sh = g_tuning.shade_threads;
+    const bool adj = dimg != nullptr;
     int mb = 0;
     for (int s = s_first; s < s_last; ++s) {
         const SceneDesc& S = descs[s];
         uint64_t n3 = 3ull * S.npix;
-        k_zero_u32<<<std::min<uint64_t>((n3 + 255) / 256, 4096), 256, 0, st>>>(reinterpret_cast<uint32_t*>(S.image), n3);
+        if (!adj)
+            k_zero_u32<<<std::min<uint64_t>((n3 + 255) / 256, 4096), 256, 0, st>>>(reinterpret_cast<uint32_t*>(S.image),
+                                                                                 n3);
         mb = std::max(mb, S.max_bounce);
     }
     const int ns = (int)descs.size();
@@ -413,15 +461,20 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
         // then the consumed queue becomes the next output and its length is cleared.
         int qi = 0;
         for (int b = 0; b <= mb; ++b) {
-            k_wtrace<<<grid, kTraceThreads, 0, st>>>(d_scenes, P, qi);
-            k_shade<<<grid, sh, 0, st>>>(d_scenes, P, qi, b);
+            with_trace_variant([&](auto D, auto L) {
+                k_wtrace<decltype(D)::value, decltype(L)::value><<<grid, kTraceThreads, 0, st>>>(d_scenes, P, qi);
+            });
+            if (adj) k_shade<true><<<grid, sh, 0, st>>>(d_scenes, P, qi, b);
+            else k_shade<false><<<grid, sh, 0, st>>>(d_scenes, P, qi, b);
             k_clear1<<<1, 64, 0, st>>>(P.cnt + qi);
             qi ^= 1;
         }
-        k_accum<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
+        if (adj) k_grad<<<cgrid, 256, 0, st>>>(d_scenes, P, cnt, dimg, ga, ge);
+        else k_accum<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
     }
-    for (int s = s_first; s < s_last; ++s)
-        k_finalize<<<std::min<uint32_t>((3 * descs[s].npix + 255) / 256, 4096), 256, 0, st>>>(d_scenes, s);
+    if (!adj)
+        for (int s = s_first; s < s_last; ++s)
+            k_finalize<<<std::min<uint32_t>((3 * descs[s].npix + 255) / 256, 4096), 256, 0, st>>>(d_scenes, s);
 }
 
 int Batch::launch(hipStream_t st) {
@@ -448,7 +501,8 @@ struct rtx_batch {
 
 extern "C" int rtx_set_tuning(const rtx_tuning* t) {
     if (!t || t->batch_paths < 1024 || t->grid_blocks == 0 || t->shade_threads == 0 || t->shade_threads % 64 ||
-        t->shade_threads > 1024)
+        t->shade_threads > 1024 || !(t->trace_lds_depth == 8 || t->trace_lds_depth == 12 || t->trace_lds_depth == 16) ||
+        t->trace_loop > 1)
         return fail(RTX_E_ARG, "bad tuning on rtx_set_tuning()");
     g_tuning = *t;
     return RTX_OK;
@@ -529,18 +583,20 @@ extern "C" int rtx_render_adjoint(const rtx_bvh* b, const rtx_camera* cam, const
     if (!albedo_d) return fail(RTX_E_ARG, "albedo is required on rtx_render_adjoint()");
     if (p->rr_bounce <= p->max_bounce)
         return fail(RTX_E_ARG, "the adjoint needs rr_bounce > max_bounce (no Russian roulette)");
-    if (p->max_bounce >= kMaxAdjVerts) return fail(RTX_E_LIMIT, "max_bounce too large on rtx_render_adjoint()");
-    SceneDesc S = make_desc(b, cam, p, albedo_d, emission_d, nullptr);
-    SceneDesc* d = dalloc<SceneDesc>(1);
-    if (!d) return fail(RTX_E_HIP, "device allocation failed on rtx_render_adjoint()");
+    if (p->max_bounce > 64) return fail(RTX_E_LIMIT, "max_bounce above 64 on rtx_render_adjoint()");
+    Batch B;
+    B.descs.push_back(make_desc(b, cam, p, albedo_d, emission_d, nullptr));
+    B.dimg = d_image_d;
+    B.ga = grad_albedo_d;
+    B.ge = emission_d ? grad_emission_d : nullptr;
+    // the vertex records cost 32 B per hit per path: keep the batch within ~2 GiB of them
+    const uint64_t per_path = 32ull * (p->max_bounce + 1) + 96;
+    const uint32_t bp = (uint32_t)std::max<uint64_t>(1024, std::min<uint64_t>(g_tuning.batch_paths,
+                                                                               (2ull << 30) / per_path));
+    rc = B.setup(bp, true);
+    if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
-    rc = RTX_OK;
-    if (!hip_ok(hipMemcpyAsync(d, &S, sizeof(S), hipMemcpyHostToDevice, st), "copy")) rc = RTX_E_HIP;
-    if (rc == RTX_OK) {
-        k_adjoint<<<g_tuning.grid_blocks, kTraceThreads, 0, st>>>(d, d_image_d, grad_albedo_d,
-                                                                  emission_d ? grad_emission_d : nullptr);
-        if (!hip_ok(hipGetLastError(), "k_adjoint") || !hip_ok(hipStreamSynchronize(st), "sync")) rc = RTX_E_HIP;
-    }
-    (void)hipFree(d);
-    return rc;
+    B.enqueue_range(0, 1, st);
+    if (!hip_ok(hipGetLastError(), "adjoint launch") || !hip_ok(hipStreamSynchronize(st), "sync")) return RTX_E_HIP;
+    return RTX_OK;
 }

@@ -1,7 +1,7 @@
 // Closest-hit and any-hit traversal (src/bvh/bvh_traverse.cpp:22-178) for gfx950.
 //
 // One lane per ray. The node being visited lives in registers; the stack of pending
-// (node, entry distance) pairs is a per-lane ring of kLdsDepth entries in LDS, with the
+// (node, entry distance) pairs is a per-lane ring of 8-16 entries in LDS, with the
 // oldest entries spilled to private memory when it fills, so any tree depth the builder
 // can produce is handled (the reference's fixed stack[64] has no overflow check).
 //
@@ -18,17 +18,23 @@
 namespace rtx {
 namespace {
 
-template <bool ANY, bool COUNT>
+template <bool ANY, bool COUNT, int DEPTH, int LOOP>
 __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray32* __restrict__ rays, uint32_t n,
                                                          Hit16* __restrict__ hits, float* __restrict__ normals,
                                                          uint8_t* __restrict__ occl, uint32_t* __restrict__ counts) {
-    __shared__ uint2 lst[kLdsDepth][kTraceThreads];
+    __shared__ uint2 lst[DEPTH][kTraceThreads];
     const uint32_t tid = threadIdx.x;
     const uint32_t i = xcd_block(blockIdx.x) * kTraceThreads + tid;
-    if (i >= n) return;
-    const float4 r0 = reinterpret_cast<const float4*>(rays)[2 * i];
-    const float4 r1 = reinterpret_cast<const float4*>(rays)[2 * i + 1];
-    const TraceOut h = trace_ray<ANY, COUNT>(A, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w, lst, tid);
+    // out-of-range lanes still run (with a ray that misses) so wave-wide votes see them
+    const bool in = i < n;
+    float4 r0 = make_float4(0.f, 0.f, 0.f, 1.f), r1 = make_float4(0.f, 0.f, 1.f, 0.f);
+    if (in) {
+        r0 = reinterpret_cast<const float4*>(rays)[2 * i];
+        r1 = reinterpret_cast<const float4*>(rays)[2 * i + 1];
+    }
+    const TraceOut h = trace_ray<ANY, COUNT, DEPTH, LOOP>(A, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w,
+                                                          lst, tid);
+    if (!in) return;
     if (ANY) {
         occl[i] = h.tri != kInvalid ? 1 : 0;
         return;
@@ -51,7 +57,7 @@ int check(const rtx_bvh* b, const void* rays, size_t n, const void* out, const c
     if (!b || !out || (!rays && n)) return fail(RTX_E_ARG, std::string("null argument on ") + fn);
     if (!b->gnodes || !b->leaves || !b->gtris) return fail(RTX_E_STATE, std::string("BVH not loaded on ") + fn);
     if (n > 0xFFFFFFFFull - kTraceThreads) return fail(RTX_E_LIMIT, std::string("too many rays on ") + fn);
-    if (b->max_depth + 2 > (uint32_t)(kLdsDepth + kSpillDepth))
+    if (b->max_depth > kMaxTraceDepth)
         return fail(RTX_E_LIMIT, std::string("tree deeper than the traversal stack on ") + fn);
     return RTX_OK;
 }
@@ -61,14 +67,20 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
            uint32_t* counts, hipStream_t s) {
     if (n == 0) return RTX_OK;
     uint32_t blocks = (uint32_t)((n + kTraceThreads - 1) / kTraceThreads);
-    k_trace<ANY, COUNT><<<blocks, kTraceThreads, 0, s>>>(make_trace_args(b), reinterpret_cast<const Ray32*>(rays),
-                                                         (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
-                                                         counts);
+    const TraceArgs A = make_trace_args(b);
+    with_trace_variant([&](auto D, auto L) {
+        k_trace<ANY, COUNT, decltype(D)::value, decltype(L)::value><<<blocks, kTraceThreads, 0, s>>>(
+            A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
+            counts);
+    });
     RTX_HIP(hipGetLastError());
     return RTX_OK;
 }
 
 }  // namespace
+
+rtx_tuning g_tuning = {1u << 22, 2048, 256, 16, 0};
+
 }  // namespace rtx
 
 using namespace rtx;

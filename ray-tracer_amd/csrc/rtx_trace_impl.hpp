@@ -1,12 +1,14 @@
 // Traversal device code shared by the trace entry points and the path tracer.
 // See rtx_trace.hip for the visiting-order argument.
 #pragma once
+#include <type_traits>
+
 #include "rtx_internal.hpp"
 
 namespace rtx {
 
 constexpr int kTraceThreads = 256;
-constexpr int kLdsDepth = 16;          // power of two
+constexpr int kLdsDepthMax = 16;       // LDS ring entries per lane: 8, 12 or 16 (a tuning choice)
 constexpr int kSpillDepth = 128;
 
 struct TraceArgs {
@@ -18,6 +20,25 @@ struct TraceArgs {
     float rb[6];
     uint32_t root_ref;
 };
+
+extern rtx_tuning g_tuning;
+
+// Calls f(integral_constant<DEPTH>, integral_constant<LOOP>) for the tuned variant.
+template <class F>
+void with_trace_variant(F&& f) {
+    using I8 = std::integral_constant<int, 8>;
+    using I12 = std::integral_constant<int, 12>;
+    using I16 = std::integral_constant<int, 16>;
+    using L0 = std::integral_constant<int, 0>;
+    using L1 = std::integral_constant<int, 1>;
+    const uint32_t d = g_tuning.trace_lds_depth, l = g_tuning.trace_loop;
+    if (d == 8) { if (l) f(I8{}, L1{}); else f(I8{}, L0{}); }
+    else if (d == 12) { if (l) f(I12{}, L1{}); else f(I12{}, L0{}); }
+    else { if (l) f(I16{}, L1{}); else f(I16{}, L0{}); }
+}
+
+// Deepest tree every variant can traverse (smallest LDS ring + spill).
+constexpr uint32_t kMaxTraceDepth = 8 + kSpillDepth - 2;
 
 inline TraceArgs make_trace_args(const rtx_bvh* b) {
     TraceArgs A;
@@ -44,9 +65,11 @@ struct TraceOut {
     uint32_t nI, nT;    // internal visits, triangle tests
 };
 
-// One ray against one tree. lst is this block's LDS stack [kLdsDepth][kTraceThreads];
-// the lane uses column tid.
-template <bool ANY, bool COUNT>
+// One ray against one tree. lst is this block's LDS stack, DEPTH entries per lane in
+// column tid. LOOP 0 visits nodes and leaves in one loop; LOOP 1 ("while-while") lets a
+// lane that reaches a leaf wait until every lane of the wave is at a leaf, so triangle
+// tests run with the wave converged. Both visit the same nodes in the same order per ray.
+template <bool ANY, bool COUNT, int DEPTH, int LOOP>
 __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, float tmin, float tmax,
                                               uint2 (*lst)[kTraceThreads], uint32_t tid) {
     const float ix = 1.f / d.x, iy = 1.f / d.y, iz = 1.f / d.z;
@@ -58,49 +81,77 @@ __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, fl
 
     uint32_t ref = A.root_ref;
     bool live = slab(o.x, o.y, o.z, ix, iy, iz, tmin, A.rb[0], A.rb[1], A.rb[2], A.rb[3], A.rb[4], A.rb[5], bestT) < bestT;
-    while (live) {
-        if (ref & kLeafBit) {
-            const Leaf lf = A.leaves[ref & ~kLeafBit];
-            for (uint32_t k = lf.begin; k < lf.begin + lf.count; ++k) {
-                const float4 a = A.tris[k].a, b = A.tris[k].b, c = A.tris[k].c;
-                if (COUNT) ++nT;
-                float u, v;
-                float t = tri_test(o, d, tmin, tmax, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y}, f3{b.z, b.w, c.x}, u, v);
-                if (t < bestT) {
-                    bestT = t; bu = u; bv = v; best = k;
-                    if (ANY) break;
-                }
-            }
-            if (ANY && best != kInvalid) break;
-        } else {
-            if (COUNT) ++nI;
-            const GNode& g = A.nodes[ref];
-            const float4 l0 = g.l0, l1 = g.l1, r1b = g.r1;
-            const uint2 refs = *reinterpret_cast<const uint2*>(&g.lref);
-            const float tl = slab(o.x, o.y, o.z, ix, iy, iz, tmin, l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, bestT);
-            const float tr = slab(o.x, o.y, o.z, ix, iy, iz, tmin, l1.z, l1.w, r1b.x, r1b.y, r1b.z, r1b.w, bestT);
-            uint32_t nearRef, farRef;
-            float tn, tf;
-            if (tl <= tr) { nearRef = refs.x; tn = tl; farRef = refs.y; tf = tr; }
-            else { nearRef = refs.y; tn = tr; farRef = refs.x; tf = tl; }
-            if (tn < bestT) {
-                if (tf < bestT) {
-                    if (top - lo == (uint32_t)kLdsDepth) { spill[lo] = lst[lo & (kLdsDepth - 1)][tid]; ++lo; }
-                    lst[top & (kLdsDepth - 1)][tid] = make_uint2(farRef, __float_as_uint(tf));
-                    ++top;
-                }
-                ref = nearRef;
-                continue;
-            }
-        }
-        // pop the next entry that can still hold a closer hit
+
+    // ring slot of stack entry e (DEPTH need not be a power of two)
+    auto slot = [&](uint32_t e) { return DEPTH == 16 || DEPTH == 8 ? (e & (DEPTH - 1)) : (e % DEPTH); };
+    auto push = [&](uint32_t r, float t) {
+        if (top - lo == (uint32_t)DEPTH) { spill[lo] = lst[slot(lo)][tid]; ++lo; }
+        lst[slot(top)][tid] = make_uint2(r, __float_as_uint(t));
+        ++top;
+    };
+    // pop the next entry that can still hold a closer hit
+    auto pop = [&]() {
         live = false;
         while (top > 0) {
             --top;
             uint2 e;
             if (top < lo) { --lo; e = spill[top]; }
-            else e = lst[top & (kLdsDepth - 1)][tid];
+            else e = lst[slot(top)][tid];
             if (__uint_as_float(e.y) < bestT) { ref = e.x; live = true; break; }
+        }
+    };
+    // one internal node: test both children, push the far one, continue with the near one
+    auto node_step = [&]() {
+        if (COUNT) ++nI;
+        const GNode& g = A.nodes[ref];
+        const float4 l0 = g.l0, l1 = g.l1, r1b = g.r1;
+        const uint2 refs = *reinterpret_cast<const uint2*>(&g.lref);
+        const float tl = slab(o.x, o.y, o.z, ix, iy, iz, tmin, l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, bestT);
+        const float tr = slab(o.x, o.y, o.z, ix, iy, iz, tmin, l1.z, l1.w, r1b.x, r1b.y, r1b.z, r1b.w, bestT);
+        uint32_t nearRef, farRef;
+        float tn, tf;
+        if (tl <= tr) { nearRef = refs.x; tn = tl; farRef = refs.y; tf = tr; }
+        else { nearRef = refs.y; tn = tr; farRef = refs.x; tf = tl; }
+        if (tn < bestT) {
+            if (tf < bestT) push(farRef, tf);
+            ref = nearRef;
+        } else {
+            pop();
+        }
+    };
+    // all triangles of the current leaf; returns true when an any-hit query is answered
+    auto leaf_step = [&]() {
+        const Leaf lf = A.leaves[ref & ~kLeafBit];
+        for (uint32_t k = lf.begin; k < lf.begin + lf.count; ++k) {
+            const float4 a = A.tris[k].a, b = A.tris[k].b, c = A.tris[k].c;
+            if (COUNT) ++nT;
+            float u, v;
+            float t = tri_test(o, d, tmin, tmax, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y}, f3{b.z, b.w, c.x}, u, v);
+            if (t < bestT) {
+                bestT = t; bu = u; bv = v; best = k;
+                if (ANY) return true;
+            }
+        }
+        return false;
+    };
+
+    if (LOOP == 0) {
+        while (live) {
+            if (ref & kLeafBit) {
+                if (leaf_step()) break;
+                pop();
+            } else {
+                node_step();
+            }
+        }
+    } else {
+        for (;;) {
+            while (live && !(ref & kLeafBit)) node_step();
+            if (!__any(live)) break;
+            if (live) {
+                if (leaf_step()) live = false;
+                else pop();
+            }
         }
     }
     TraceOut r;

@@ -50,7 +50,8 @@ class RenderParams(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("batch_paths", C.c_uint32), ("grid_blocks", C.c_uint32), ("shade_threads", C.c_uint32)]
+    _fields_ = [("batch_paths", C.c_uint32), ("grid_blocks", C.c_uint32), ("shade_threads", C.c_uint32),
+                ("trace_lds_depth", C.c_uint32), ("trace_loop", C.c_uint32)]
 
 
 _LIB = None
@@ -289,13 +290,17 @@ def make_params(width, height, spp, max_bounce, rr_bounce=3, seed=0, env_mode=0,
 def get_tuning():
     t = Tuning()
     _check(lib().rtx_get_tuning(C.byref(t)), "rtx_get_tuning")
-    return dict(batch_paths=t.batch_paths, grid_blocks=t.grid_blocks, shade_threads=t.shade_threads)
+    return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
 
-def set_tuning(batch_paths=None, grid_blocks=None, shade_threads=None):
+def set_tuning(**kw):
+    """Change launch shapes (see rtx_tuning in include/rtx.h); results never change."""
     cur = get_tuning()
-    t = Tuning(batch_paths or cur["batch_paths"], grid_blocks or cur["grid_blocks"],
-               shade_threads or cur["shade_threads"])
+    unknown = set(kw) - set(cur)
+    if unknown:
+        raise ValueError(f"unknown tuning keys {unknown}")
+    cur.update({k: v for k, v in kw.items() if v is not None})
+    t = Tuning(*[cur[k] for k, _ in Tuning._fields_])
     _check(lib().rtx_set_tuning(C.byref(t)), "rtx_set_tuning")
 
 
