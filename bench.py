@@ -47,6 +47,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-render", action="store_true", help="skip the batched (C3) and differentiable (C4) legs")
     ap.add_argument("--c3-scenes", type=int, default=64)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (the reference point that scales HBM "
+                         "counters under the profiler)")
+    ap.add_argument("--autotune", action="store_true", help="tune launch shapes on this GPU before timing")
     ap.add_argument("--skip-extras", action="store_true",
                     help="time only the headline launch (used under the profiler so k_trace's average is "
                          "the incoherent launch alone)")
@@ -64,6 +68,12 @@ def main():
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     v, t = S.random_triangles(args.tris, seed=1)
     vd = torch.from_numpy(v).to(dev)

@@ -34,6 +34,7 @@ struct SceneDesc {
     float* image;
     uint64_t path_begin;
     uint32_t npix;
+    uint32_t nprim;      // original triangle count (gradient array length / 3)
 };
 
 struct PathState {
@@ -295,12 +296,25 @@ __device__ __forceinline__ void merged_add(float* __restrict__ buf, bool has, ui
 // wavefront (k_wtrace + k_shade<true>), which recorded every surface hit k of path i.
 // With L = sum_m (prod_{j<m} a_j) X_m:
 //   dL/da_k = (L - L_upto_k) / a_k        dL/dLe_k = throughput arriving at k.
-// Paths are in pixel order, so neighbouring lanes mostly hit the same triangles and the
-// wave merge cuts the atomics.
-__global__ void k_grad(const SceneDesc* __restrict__ scenes, PathState P, uint32_t cnt,
-                       const float* __restrict__ dimg, float* __restrict__ ga, float* __restrict__ ge) {
+// LDS=true: each block accumulates into a private copy of ONE gradient array in LDS
+// (scenes of up to kLdsGradTris triangles; ga or ge is null, the kernel runs once per
+// array) and flushes it once with global atomics; otherwise lanes add to global memory
+// after merging equal triangles within the wave.
+constexpr uint32_t kLdsGradTris = 5120;   // 3 floats each: 60 KiB of LDS
+constexpr int kGradThreads = 512;
+
+template <bool LDS>
+__global__ void __launch_bounds__(kGradThreads) k_grad(const SceneDesc* __restrict__ scenes, PathState P,
+                                                       uint32_t cnt, const float* __restrict__ dimg,
+                                                       float* __restrict__ ga, float* __restrict__ ge,
+                                                       uint32_t nprim) {
+    extern __shared__ float lg[];   // LDS: [nprim * 3] gradient of the one non-null array
     const SceneDesc& S = scenes[0];
     const float fs = (float)S.spp;
+    if (LDS) {
+        for (uint32_t j = threadIdx.x; j < 3 * nprim; j += blockDim.x) lg[j] = 0.f;
+        __syncthreads();
+    }
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t wb = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wb < cnt; wb += stride) {
         const uint32_t i = wb + __lane_id();
@@ -315,7 +329,8 @@ __global__ void k_grad(const SceneDesc* __restrict__ scenes, PathState P, uint32
             delta = f3{dimg[3 * pixel] / fs, dimg[3 * pixel + 1] / fs, dimg[3 * pixel + 2] / fs};
         }
         uint32_t maxnv = nv;
-        for (int off = 32; off >= 1; off >>= 1) maxnv = max(maxnv, (uint32_t)__shfl_xor((int)maxnv, off));
+        if (!LDS)
+            for (int off = 32; off >= 1; off >>= 1) maxnv = max(maxnv, (uint32_t)__shfl_xor((int)maxnv, off));
         for (uint32_t k = 0; k < maxnv; ++k) {
             const bool has = k < nv;
             uint32_t prim = 0;
@@ -328,8 +343,25 @@ __global__ void k_grad(const SceneDesc* __restrict__ scenes, PathState P, uint32
                 gA = f3{delta.x * (L.x - lu.x) / a[0], delta.y * (L.y - lu.y) / a[1], delta.z * (L.z - lu.z) / a[2]};
                 gE = f3{delta.x * tp.x, delta.y * tp.y, delta.z * tp.z};
             }
-            if (ga) merged_add(ga, has, prim, gA);
-            if (ge) merged_add(ge, has, prim, gE);
+            if (LDS) {
+                if (has) {
+                    const f3 g = ga ? gA : gE;
+                    atomicAdd(&lg[3 * prim], g.x);
+                    atomicAdd(&lg[3 * prim + 1], g.y);
+                    atomicAdd(&lg[3 * prim + 2], g.z);
+                }
+            } else {
+                if (ga) merged_add(ga, has, prim, gA);
+                if (ge) merged_add(ge, has, prim, gE);
+            }
+        }
+    }
+    if (LDS) {
+        __syncthreads();
+        float* out = ga ? ga : ge;
+        for (uint32_t j = threadIdx.x; j < 3 * nprim; j += blockDim.x) {
+            const float x = lg[j];
+            if (x != 0.f) atomicAdd(&out[j], x);
         }
     }
 }
@@ -352,6 +384,7 @@ SceneDesc make_desc(const rtx_bvh* b, const rtx_camera* c, const rtx_render_para
     S.image = image;
     S.path_begin = 0;
     S.npix = (uint32_t)(p->width * p->height);
+    S.nprim = b->nt;
     return S;
 }
 
@@ -469,7 +502,17 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
             k_clear1<<<1, 64, 0, st>>>(P.cnt + qi);
             qi ^= 1;
         }
-        if (adj) k_grad<<<cgrid, 256, 0, st>>>(d_scenes, P, cnt, dimg, ga, ge);
+        if (adj) {
+            const uint32_t nprim = descs[0].nprim;
+            if (nprim <= kLdsGradTris) {
+                const uint32_t gb = std::min<uint32_t>(512, (cnt + kGradThreads - 1) / kGradThreads);
+                const size_t lds = 3 * nprim * sizeof(float);
+                if (ga) k_grad<true><<<gb, kGradThreads, lds, st>>>(d_scenes, P, cnt, dimg, ga, nullptr, nprim);
+                if (ge) k_grad<true><<<gb, kGradThreads, lds, st>>>(d_scenes, P, cnt, dimg, nullptr, ge, nprim);
+            } else
+                k_grad<false><<<std::min<uint32_t>(4096, (cnt + kGradThreads - 1) / kGradThreads), kGradThreads, 0,
+                                st>>>(d_scenes, P, cnt, dimg, ga, ge, nprim);
+        }
         else k_accum<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
     }
     if (!adj)

@@ -79,7 +79,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
 
 }  // namespace
 
-rtx_tuning g_tuning = {1u << 22, 2048, 256, 16, 0};
+rtx_tuning g_tuning = {1u << 22, 2048, 256, 8, 1};
 
 }  // namespace rtx
 

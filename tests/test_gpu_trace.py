@@ -102,3 +102,26 @@ def test_trace_1m_subset():
         assert np.array_equal(h[:, 0].view(np.uint32), ref["t"].view(np.uint32)), name
         assert np.array_equal(occ.astype(bool), ref["tri"] != 0xFFFFFFFF), name
         assert np.array_equal(cnt, ref["counts"]), name
+
+
+def test_tuning_variants_give_identical_hits():
+    """Every launch-shape variant (LDS stack depth, loop structure) returns the same hits."""
+    import torch
+    v, t = S.random_triangles(200_000, seed=5)
+    g = rtx.build(v, t)
+    rays = torch.from_numpy(S.incoherent_rays(1 << 16, seed=6)).cuda()
+    hits = torch.zeros((len(rays), 4), dtype=torch.float32, device="cuda")
+    saved = rtx.get_tuning()
+    ref = None
+    try:
+        for depth in (8, 12, 16):
+            for loop in (0, 1):
+                rtx.set_tuning(trace_lds_depth=depth, trace_loop=loop)
+                g.trace_closest_device(rays, hits)
+                torch.cuda.synchronize()
+                h = hits.view(torch.int32).cpu().numpy().copy()
+                if ref is None:
+                    ref = h
+                assert np.array_equal(h, ref), f"variant depth={depth} loop={loop} differs"
+    finally:
+        rtx.set_tuning(**saved)
