@@ -106,11 +106,18 @@ int rtx_trace_closest_host(const rtx_bvh* bvh, const rtx_ray* rays, size_t n, rt
  * with writeBakedData (src/bvh/bvh_IO.cpp:236-275) and loadBVH (bvh_traverse.cpp:182-235). */
 int rtx_save(const rtx_bvh* bvh, const char* path);
 int rtx_load(const char* path, rtx_bvh** out);
-/* Parse a .scn file's geometry lines (sphere/box; mesh lines are reported and skipped:
- * assimp import is out of scope) into host arrays, as readScene + initGeos do
- * (src/bvh/bvh_IO.cpp:72-234, bvh_geo.cpp:22-192).  Free with rtx_free_host. */
+/* Parse a .scn file's geometry lines (sphere / box / mesh) into host arrays, as
+ * readScene + initGeos do (src/bvh/bvh_IO.cpp:72-234, bvh_geo.cpp:22-192).  `mesh` reads
+ * ASCII PLY in place of the assimp import of loadMesh (bvh_IO.cpp:24-70); other mesh
+ * formats fail with RTX_E_IO.  Free the arrays with rtx_free_host. */
 int rtx_scene_load(const char* scn_path, float** verts6, uint32_t* nv, uint32_t** tris, uint32_t* nt);
 void rtx_free_host(void* p);
+/* Radiance .hdr files (the stbi_write_hdr output and stbi_loadf input of main.cpp:95-104,
+ * 378).  Write: rgb is width*height*3 floats; flip_y != 0 writes row height-1 first, as
+ * main.cpp:367-376 does for its bottom-up image.  Read: flat and run-length encoded
+ * scanlines; *rgb is allocated (free with rtx_free_host), top row first. */
+int rtx_hdr_write(const char* path, int width, int height, const float* rgb, int flip_y);
+int rtx_hdr_read(const char* path, int* width, int* height, float** rgb);
 /* Drop-in for bvh::bakeBVH(scenePath) (bvh_bake.cpp:180-194): scene -> GPU build ->
  * <scene>.bvh next to it. */
 int rtx_bake_scene(const char* scn_path);
@@ -132,8 +139,11 @@ typedef struct rtx_render_params {
     int max_bounce;   /* segments per path = max_bounce + 1 (src/main.cpp:156) */
     int rr_bounce;    /* Russian roulette when bounce >= rr_bounce (reference: 3, main.cpp:188) */
     uint32_t seed;    /* sample-table seed */
-    int env_mode;     /* 0 = sky gradient (main.cpp:169-172), 1 = constant env_color */
+    int env_mode;     /* 0 = sky gradient (main.cpp:169-172), 1 = constant env_color,
+                         2 = equirectangular IBL map ibl_d (IBL::sample, main.cpp:107-117) */
     float env_color[3];
+    const float* ibl_d;   /* env_mode 2: ibl_width * ibl_height * 3 floats in HBM, top row first */
+    int ibl_width, ibl_height;
 } rtx_render_params;
 
 /* Wavefront path tracer: replaces the raster loop + pathTrace (src/main.cpp:145-204,

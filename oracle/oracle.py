@@ -26,7 +26,7 @@ class Camera(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("max_bounce", C.c_int),
                 ("rr_bounce", C.c_int), ("seed", C.c_uint32), ("env_mode", C.c_int),
-                ("env_color", C.c_float * 3)]
+                ("env_color", C.c_float * 3), ("ibl", C.c_void_p), ("ibl_w", C.c_int), ("ibl_h", C.c_int)]
 
 
 def _p(a):
@@ -173,11 +173,17 @@ def make_camera(cam):
     return c
 
 
-def make_params(width, height, spp, max_bounce, rr_bounce=3, seed=0, env_mode=0, env_color=(0, 0, 0)):
+def make_params(width, height, spp, max_bounce, rr_bounce=3, seed=0, env_mode=0, env_color=(0, 0, 0), ibl=None):
+    """ibl: (h, w, 3) float32 array, top row first (env_mode 2); kept alive on the params."""
     p = RenderParams()
     p.width, p.height, p.spp, p.max_bounce, p.rr_bounce, p.seed = width, height, spp, max_bounce, rr_bounce, seed
     p.env_mode = env_mode
     p.env_color[:] = [float(x) for x in env_color]
+    if ibl is not None:
+        arr = np.ascontiguousarray(ibl, np.float32)
+        p._ibl_keep = arr
+        p.ibl = arr.ctypes.data
+        p.ibl_h, p.ibl_w = arr.shape[:2]
     return p
 
 

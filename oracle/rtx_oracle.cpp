@@ -760,9 +760,43 @@ inline V3 to_world(V3 l, V3 n) {
     return add(add(scl(l.x, t), scl(l.y, n)), scl(l.z, b));
 }
 
+// atan2/acos of the sample-table renderer: octant reduction and a degree-7 polynomial in
+// a^2 for atan(a)/a on [0,1] (the same specification the GPU renderer implements).
+inline float atan2_spec(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    const float a = mx == 0.f ? 0.f : mn / mx;
+    const float s = a * a;
+    const float c[8] = {0.9999998211860657f, -0.33331647515296936f, 0.19965273141860962f, -0.13995960354804993f,
+                        0.09853174537420273f, -0.058659493923187256f, 0.023694388568401337f, -0.004545152187347412f};
+    float p = c[7];
+    for (int k = 6; k >= 0; --k) p = p * s + c[k];
+    float r = p * a;
+    if (ay > ax) r = 1.57079637f - r;
+    if (x < 0.f) r = 3.14159274f - r;
+    if (y < 0.f) r = -r;
+    return r;
+}
+
+// IBL::sample (main.cpp:107-117) with the atan2/acos above.
+inline V3 ibl_sample(const orc_render_params* p, V3 d) {
+    float phi = atan2_spec(d.z, d.x);
+    float cy = d.y < -1.f ? -1.f : (1.f < d.y ? 1.f : d.y);
+    float theta = atan2_spec(sqrtf((1.f - cy) * (1.f + cy)), cy);
+    float u = (phi + 3.14159274f) / 6.28318548f;
+    float v = theta / 3.14159274f;
+    int px = std::min(f2i(u * (float)p->ibl_w), p->ibl_w - 1);
+    int py = std::min(f2i(v * (float)p->ibl_h), p->ibl_h - 1);
+    px = std::max(px, 0);
+    py = std::max(py, 0);
+    const float* t = p->ibl + 3 * ((size_t)py * p->ibl_w + px);
+    return {t[0], t[1], t[2]};
+}
+
 template <class R>
 inline Vec3T<R> env_color(const orc_render_params* p, V3 d) {
     if (p->env_mode == 1) return {(R)p->env_color[0], (R)p->env_color[1], (R)p->env_color[2]};
+    if (p->env_mode == 2) { V3 e = ibl_sample(p, d); return {(R)e.x, (R)e.y, (R)e.z}; }
     float t = d.y * 0.5f + 0.5f;
     float omt = 1.f - t;
     return {(R)(1.0f * omt + 0.5f * t), (R)(1.0f * omt + 0.7f * t), (R)(1.0f * omt + 1.0f * t)};

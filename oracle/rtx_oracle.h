@@ -82,8 +82,10 @@ typedef struct orc_render_params {
     int width, height, spp, max_bounce;
     int rr_bounce;      /* Russian roulette applies when bounce >= rr_bounce (reference: 3) */
     uint32_t seed;
-    int env_mode;       /* 0 = reference sky gradient, 1 = constant env_color */
+    int env_mode;       /* 0 = reference sky gradient, 1 = constant env_color, 2 = IBL map */
     float env_color[3];
+    const float* ibl;   /* env_mode 2: ibl_w * ibl_h * 3 floats, top row first */
+    int ibl_w, ibl_h;
 } orc_render_params;
 
 /* Path-trace loop of main.cpp:145-204 / 339-365 driven by the sample table.

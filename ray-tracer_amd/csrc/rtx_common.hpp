@@ -174,4 +174,45 @@ RTX_HD void sincos_rtx(float x, float& s, float& c) {
     c = (q == 0) ? cp : (q == 1) ? -sp : (q == 2) ? -cp : sp;
 }
 
+// atan2 and acos shared by the GPU renderer and its CPU checker (so environment-map texel
+// choices agree bit for bit): octant reduction to a = min/max in [0,1], then a degree-7
+// polynomial in a^2 fitted to atan(a)/a (max error 2.5e-7 rad).
+RTX_HD float atan2_rtx(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    const float a = mx == 0.f ? 0.f : mn / mx;
+    const float s = a * a;
+    float p = -0.004545152187347412f;
+    p = p * s + 0.023694388568401337f;
+    p = p * s + -0.058659493923187256f;
+    p = p * s + 0.09853174537420273f;
+    p = p * s + -0.13995960354804993f;
+    p = p * s + 0.19965273141860962f;
+    p = p * s + -0.33331647515296936f;
+    p = p * s + 0.9999998211860657f;
+    float r = p * a;
+    if (ay > ax) r = 1.57079637f - r;
+    if (x < 0.f) r = 3.14159274f - r;
+    if (y < 0.f) r = -r;
+    return r;
+}
+RTX_HD float acos_rtx(float y) { return atan2_rtx(sqrtf((1.f - y) * (1.f + y)), y); }
+
+// Equirectangular environment lookup of IBL::sample (src/main.cpp:107-117): nearest texel,
+// row 0 = straight up. rgb holds w*h*3 floats, top row first.
+RTX_HD f3 ibl_lookup(const float* rgb, int w, int h, f3 d) {
+    const float phi = atan2_rtx(d.z, d.x);
+    const float cy = d.y < -1.f ? -1.f : (1.f < d.y ? 1.f : d.y);
+    const float theta = acos_rtx(cy);
+    const float u = (phi + 3.14159274f) / 6.28318548f;
+    const float v = theta / 3.14159274f;
+    int px = f2i_x86(u * (float)w), py = f2i_x86(v * (float)h);
+    px = px < w - 1 ? px : w - 1;
+    py = py < h - 1 ? py : h - 1;
+    px = px < 0 ? 0 : px;
+    py = py < 0 ? 0 : py;
+    const float* p = rgb + 3 * ((size_t)py * w + px);
+    return f3{p[0], p[1], p[2]};
+}
+
 }  // namespace rtx

@@ -124,6 +124,100 @@ int init_geos(Mesh& m) {
     return RTX_OK;
 }
 
+// ASCII PLY mesh for `mesh` lines, in place of the assimp import of loadMesh
+// (bvh_IO.cpp:24-70): polygons are split into fans (0,i,i+1) as aiProcess_Triangulate does
+// for convex faces; missing normals are the per-vertex average of the unit face normals
+// (aiProcess_GenSmoothNormals without its angle limit). The transform is
+// translate(t) * scale(s) * rotate(q) for positions and the rotation for normals, which
+// initGeos then renormalises; for a uniform scale this is the inverse-transpose direction.
+int load_ply(const std::string& path, const f3& t, float scale, const Q& q, Mesh& m) {
+    std::ifstream f(path);
+    if (!f.is_open()) return fail(RTX_E_IO, "failed to open mesh " + path + " on loadMesh()");
+    std::string line, tok;
+    std::getline(f, line);
+    if (line.rfind("ply", 0) != 0) return fail(RTX_E_IO, "not a PLY file: " + path + " on loadMesh()");
+    uint64_t nvert = 0, nface = 0;
+    int cur = 0;   // 1 = vertex element, 2 = face element
+    std::vector<std::string> vprops;
+    while (std::getline(f, line)) {
+        std::istringstream is(line);
+        if (!(is >> tok)) continue;
+        if (tok == "format") {
+            std::string fmt;
+            is >> fmt;
+            if (fmt != "ascii") return fail(RTX_E_IO, "only ASCII PLY is supported: " + path + " on loadMesh()");
+        } else if (tok == "element") {
+            std::string name;
+            uint64_t n = 0;
+            is >> name >> n;
+            if (name == "vertex") { nvert = n; cur = 1; }
+            else if (name == "face") { nface = n; cur = 2; }
+            else cur = 0;
+        } else if (tok == "property" && cur == 1) {
+            std::string type, name;
+            is >> type >> name;
+            vprops.push_back(name);
+        } else if (tok == "end_header") {
+            break;
+        }
+    }
+    int ix = -1, iy = -1, iz = -1, inx = -1, iny = -1, inz = -1;
+    for (int i = 0; i < (int)vprops.size(); ++i) {
+        const std::string& n = vprops[i];
+        if (n == "x") ix = i; else if (n == "y") iy = i; else if (n == "z") iz = i;
+        else if (n == "nx") inx = i; else if (n == "ny") iny = i; else if (n == "nz") inz = i;
+    }
+    if (ix < 0 || iy < 0 || iz < 0) return fail(RTX_E_IO, "PLY without x/y/z: " + path + " on loadMesh()");
+    const bool has_n = inx >= 0 && iny >= 0 && inz >= 0;
+    // rotation matrix of toMat4 (src/math/quat.hpp:44-59)
+    const float xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z, xy = q.x * q.y, xz = q.x * q.z, yz = q.y * q.z;
+    const float wx = q.w * q.x, wy = q.w * q.y, wz = q.w * q.z;
+    const float R[3][3] = {{1.f - 2.f * (yy + zz), 2.f * (xy - wz), 2.f * (xz + wy)},
+                           {2.f * (xy + wz), 1.f - 2.f * (xx + zz), 2.f * (yz - wx)},
+                           {2.f * (xz - wy), 2.f * (yz + wx), 1.f - 2.f * (xx + yy)}};
+    auto rot = [&](f3 v) {
+        return f3{R[0][0] * v.x + R[0][1] * v.y + R[0][2] * v.z, R[1][0] * v.x + R[1][1] * v.y + R[1][2] * v.z,
+                  R[2][0] * v.x + R[2][1] * v.y + R[2][2] * v.z};
+    };
+    const uint32_t off = m.nv();
+    std::vector<f3> pos(nvert), nrm(nvert, f3{0.f, 0.f, 0.f});
+    std::vector<double> val(vprops.size());
+    for (uint64_t i = 0; i < nvert; ++i) {
+        for (size_t k = 0; k < vprops.size(); ++k)
+            if (!(f >> val[k])) return fail(RTX_E_IO, "truncated PLY vertex list: " + path + " on loadMesh()");
+        pos[i] = f3{(float)val[ix], (float)val[iy], (float)val[iz]};
+        if (has_n) nrm[i] = f3{(float)val[inx], (float)val[iny], (float)val[inz]};
+    }
+    std::vector<uint32_t> tri;
+    for (uint64_t i = 0; i < nface; ++i) {
+        uint32_t c = 0;
+        if (!(f >> c)) return fail(RTX_E_IO, "truncated PLY face list: " + path + " on loadMesh()");
+        std::vector<uint32_t> idx(c);
+        for (uint32_t k = 0; k < c; ++k) {
+            if (!(f >> idx[k])) return fail(RTX_E_IO, "truncated PLY face list: " + path + " on loadMesh()");
+            if (idx[k] >= nvert) return fail(RTX_E_ARG, "PLY face index out of range: " + path + " on loadMesh()");
+        }
+        for (uint32_t k = 1; k + 1 < c; ++k) tri.insert(tri.end(), {idx[0], idx[k], idx[k + 1]});
+    }
+    if (!has_n) {
+        for (size_t i = 0; i < tri.size(); i += 3) {
+            f3 a = pos[tri[i]], b = pos[tri[i + 1]], c = pos[tri[i + 2]];
+            f3 fn = cross(sub(b, a), sub(c, a));
+            float len = std::sqrt(dot(fn, fn));
+            if (!(len > 0.f)) continue;
+            fn = scl(1.f / len, fn);
+            for (int k = 0; k < 3; ++k) nrm[tri[i + k]] = add(nrm[tri[i + k]], fn);
+        }
+    }
+    for (uint64_t i = 0; i < nvert; ++i) {
+        f3 p = rot(pos[i]);
+        f3 n = rot(nrm[i]);
+        m.vert(t.x + scale * p.x, t.y + scale * p.y, t.z + scale * p.z, n.x, n.y, n.z);
+    }
+    for (uint32_t v : tri) m.t.push_back(off + v);
+    return RTX_OK;
+}
+
 // Geometry lines of a .scn file (readScene, bvh_IO.cpp:125-234).
 int read_scene(const std::string& path, Mesh& m) {
     if (path.empty()) return fail(RTX_E_ARG, "empty scene path on readScene()");
@@ -158,9 +252,23 @@ int read_scene(const std::string& path, Mesh& m) {
             continue;
         }
         if (tok == "mesh") {
-            std::string mp;
-            if (!(iss >> mp)) continue;
-            return fail(RTX_E_IO, "mesh import (assimp) is not supported: " + mp + " on readScene()");
+            std::string mp, ot;
+            float tx, ty, tz, sc, qx = 0.f, qy = 0.f, qz = 0.f, qw = 1.f;
+            if (!(iss >> mp >> tx >> ty >> tz >> sc >> ot)) continue;
+            if (ot == "q") iss >> qx >> qy >> qz >> qw;
+            // the reference opens the path as given; fall back to the scene file's directory
+            std::string full = mp;
+            if (!std::ifstream(full).good()) {
+                auto slash = path.find_last_of('/');
+                if (slash != std::string::npos) full = path.substr(0, slash + 1) + mp;
+            }
+            auto dot_ = mp.rfind('.');
+            std::string ext = dot_ == std::string::npos ? "" : mp.substr(dot_);
+            if (ext != ".ply")
+                return fail(RTX_E_IO, "mesh format not supported (ASCII .ply only): " + mp + " on readScene()");
+            int rc = load_ply(full, {tx, ty, tz}, sc, Q{qx, qy, qz, qw}, m);
+            if (rc) return rc;
+            continue;
         }
     }
     return RTX_OK;
@@ -297,6 +405,104 @@ extern "C" int rtx_scene_load(const char* scn, float** verts6, uint32_t* nv, uin
 }
 
 extern "C" void rtx_free_host(void* p) { std::free(p); }
+
+// ---- Radiance .hdr (RGBE) -------------------------------------------------------------
+extern "C" int rtx_hdr_write(const char* path, int w, int h, const float* rgb, int flip_y) {
+    if (!path || !rgb || w <= 0 || h <= 0) return fail(RTX_E_ARG, "bad argument on rtx_hdr_write()");
+    std::ofstream out(path, std::ios::binary);
+    if (!out.is_open()) return fail(RTX_E_IO, std::string("failed to open ") + path + " on rtx_hdr_write()");
+    out << "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y " << h << " +X " << w << "\n";
+    std::vector<uint8_t> row(4 * (size_t)w);
+    for (int y = 0; y < h; ++y) {
+        const int src = flip_y ? h - 1 - y : y;
+        for (int x = 0; x < w; ++x) {
+            const float* p = rgb + 3 * ((size_t)src * w + x);
+            float v = p[0] > p[1] ? p[0] : p[1];
+            v = v > p[2] ? v : p[2];
+            uint8_t* o = &row[4 * (size_t)x];
+            if (!(v >= 1e-32f)) { o[0] = o[1] = o[2] = o[3] = 0; continue; }
+            int e;
+            const float m = std::frexp(v, &e);      // v = m * 2^e, m in [0.5, 1)
+            const float f = m * 256.0f / v;
+            o[0] = (uint8_t)(p[0] > 0.f ? p[0] * f : 0.f);
+            o[1] = (uint8_t)(p[1] > 0.f ? p[1] * f : 0.f);
+            o[2] = (uint8_t)(p[2] > 0.f ? p[2] * f : 0.f);
+            o[3] = (uint8_t)(e + 128);
+        }
+        out.write(reinterpret_cast<const char*>(row.data()), (std::streamsize)row.size());
+    }
+    if (!out.good()) return fail(RTX_E_IO, std::string("write error for ") + path + " on rtx_hdr_write()");
+    return RTX_OK;
+}
+
+extern "C" int rtx_hdr_read(const char* path, int* wp, int* hp, float** rgbp) {
+    if (!path || !wp || !hp || !rgbp) return fail(RTX_E_ARG, "null argument on rtx_hdr_read()");
+    std::ifstream in(path, std::ios::binary);
+    if (!in.is_open()) return fail(RTX_E_IO, std::string("failed to open ") + path + " on rtx_hdr_read()");
+    std::string line;
+    std::getline(in, line);
+    if (line.rfind("#?", 0) != 0) return fail(RTX_E_IO, std::string("not a Radiance file: ") + path);
+    while (std::getline(in, line) && !line.empty()) {
+        if (line.rfind("FORMAT=", 0) == 0 && line != "FORMAT=32-bit_rle_rgbe")
+            return fail(RTX_E_IO, std::string("unsupported Radiance format in ") + path);
+    }
+    std::getline(in, line);
+    char ys[8], xs[8];
+    int h = 0, w = 0;
+    if (std::sscanf(line.c_str(), "%7s %d %7s %d", ys, &h, xs, &w) != 4 || std::string(ys) != "-Y" ||
+        std::string(xs) != "+X" || w <= 0 || h <= 0 || (uint64_t)w * h > (1ull << 28))
+        return fail(RTX_E_IO, std::string("unsupported Radiance resolution line in ") + path);
+    float* rgb = (float*)std::malloc(12ull * w * h);
+    if (!rgb) return fail(RTX_E_ARG, "host allocation failed on rtx_hdr_read()");
+    std::vector<uint8_t> row(4 * (size_t)w);
+    auto bad = [&](const char* what) {
+        std::free(rgb);
+        return fail(RTX_E_IO, std::string(what) + " in " + path + " on rtx_hdr_read()");
+    };
+    for (int y = 0; y < h; ++y) {
+        uint8_t hdr4[4];
+        if (!in.read(reinterpret_cast<char*>(hdr4), 4)) return bad("truncated scanline");
+        const bool rle = w >= 8 && w < 32768 && hdr4[0] == 2 && hdr4[1] == 2 && ((hdr4[2] << 8) | hdr4[3]) == w;
+        if (!rle) {
+            if (hdr4[0] == 1 && hdr4[1] == 1 && hdr4[2] == 1) return bad("old-style run-length encoding");
+            std::memcpy(row.data(), hdr4, 4);
+            if (w > 1 && !in.read(reinterpret_cast<char*>(row.data() + 4), 4 * ((std::streamsize)w - 1)))
+                return bad("truncated scanline");
+        } else {
+            // channel-planar runs: count > 128 repeats the next byte (count - 128) times
+            std::vector<uint8_t> plane((size_t)w);
+            for (int c = 0; c < 4; ++c) {
+                int x = 0;
+                while (x < w) {
+                    int cnt = in.get();
+                    if (cnt == EOF) return bad("truncated run");
+                    if (cnt > 128) {
+                        cnt -= 128;
+                        int v = in.get();
+                        if (v == EOF || x + cnt > w) return bad("bad run");
+                        std::memset(&plane[x], v, cnt);
+                    } else {
+                        if (cnt == 0 || x + cnt > w) return bad("bad run");
+                        if (!in.read(reinterpret_cast<char*>(&plane[x]), cnt)) return bad("truncated run");
+                    }
+                    x += cnt;
+                }
+                for (int i = 0; i < w; ++i) row[4 * (size_t)i + c] = plane[i];
+            }
+        }
+        for (int x = 0; x < w; ++x) {
+            const uint8_t* q = &row[4 * (size_t)x];
+            float* o = rgb + 3 * ((size_t)y * w + x);
+            if (q[3] == 0) { o[0] = o[1] = o[2] = 0.f; continue; }
+            const float f = std::ldexp(1.0f, (int)q[3] - (128 + 8));
+            o[0] = q[0] * f; o[1] = q[1] * f; o[2] = q[2] * f;
+        }
+    }
+    *wp = w;
+    *hp = h;
+    *rgbp = rgb;
+    return RTX_OK;
+}
 
 extern "C" int rtx_bake_scene(const char* scn) {
     float* v = nullptr;

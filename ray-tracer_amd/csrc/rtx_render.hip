@@ -35,6 +35,8 @@ struct SceneDesc {
     uint64_t path_begin;
     uint32_t npix;
     uint32_t nprim;      // original triangle count (gradient array length / 3)
+    const float* ibl;
+    int ibl_w, ibl_h;
 };
 
 struct PathState {
@@ -88,9 +90,10 @@ __device__ __forceinline__ f3 bounce_dir(float u1, float u2, f3 n) {
     return add(add(scl(l.x, t), scl(l.y, n)), scl(l.z, b));
 }
 
-// Miss shading: the sky gradient of main.cpp:169-172, or a constant environment.
+// Miss shading: the sky gradient of main.cpp:169-172, a constant, or the IBL map.
 __device__ __forceinline__ f3 env_color(const SceneDesc& S, f3 d) {
     if (S.env_mode == 1) return f3{S.env[0], S.env[1], S.env[2]};
+    if (S.env_mode == 2) return ibl_lookup(S.ibl, S.ibl_w, S.ibl_h, d);
     float t = d.y * 0.5f + 0.5f;
     float omt = 1.f - t;
     return f3{1.0f * omt + 0.5f * t, 1.0f * omt + 0.7f * t, 1.0f * omt + 1.0f * t};
@@ -385,6 +388,9 @@ SceneDesc make_desc(const rtx_bvh* b, const rtx_camera* c, const rtx_render_para
     S.path_begin = 0;
     S.npix = (uint32_t)(p->width * p->height);
     S.nprim = b->nt;
+    S.ibl = p->ibl_d;
+    S.ibl_w = p->ibl_width;
+    S.ibl_h = p->ibl_height;
     return S;
 }
 
@@ -392,8 +398,10 @@ int check_scene(const rtx_bvh* b, const rtx_camera* c, const rtx_render_params* 
                 const char* fn) {
     if (!b || !c || !p || !image) return fail(RTX_E_ARG, std::string("null argument on ") + fn);
     if (!b->gnodes) return fail(RTX_E_STATE, std::string("BVH not loaded on ") + fn);
-    if (p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->max_bounce < 0)
+    if (p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->max_bounce < 0 || p->env_mode < 0 || p->env_mode > 2)
         return fail(RTX_E_ARG, std::string("bad render parameters on ") + fn);
+    if (p->env_mode == 2 && (!p->ibl_d || p->ibl_width <= 0 || p->ibl_height <= 0))
+        return fail(RTX_E_ARG, std::string("env_mode 2 needs an IBL map on ") + fn);
     if ((uint64_t)p->width * p->height > 0xFFFFFFFFull / 3)
         return fail(RTX_E_LIMIT, std::string("image too large on ") + fn);
     if (b->max_depth > kMaxTraceDepth)

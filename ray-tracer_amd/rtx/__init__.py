@@ -24,7 +24,7 @@ EXPORTS = ["rtx_last_error", "rtx_device_count", "rtx_build", "rtx_build_device"
            "rtx_trace_any", "rtx_trace_counts", "rtx_trace_closest_host", "rtx_save", "rtx_load",
            "rtx_scene_load", "rtx_free_host", "rtx_bake_scene", "rtx_geo_sphere", "rtx_geo_box", "rtx_render",
            "rtx_render_adjoint", "rtx_render_batched", "rtx_batch_create", "rtx_batch_launch", "rtx_batch_free",
-           "rtx_set_tuning", "rtx_get_tuning"]
+           "rtx_set_tuning", "rtx_get_tuning", "rtx_hdr_write", "rtx_hdr_read"]
 
 
 class RtxError(RuntimeError):
@@ -46,7 +46,7 @@ class Camera(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("max_bounce", C.c_int),
                 ("rr_bounce", C.c_int), ("seed", C.c_uint32), ("env_mode", C.c_int),
-                ("env_color", C.c_float * 3)]
+                ("env_color", C.c_float * 3), ("ibl_d", C.c_void_p), ("ibl_width", C.c_int), ("ibl_height", C.c_int)]
 
 
 class Tuning(C.Structure):
@@ -101,6 +101,8 @@ def lib():
         L.rtx_batch_launch.argtypes = [vp, vp]
         L.rtx_batch_free.argtypes = [vp]
         L.rtx_batch_free.restype = None
+        L.rtx_hdr_write.argtypes = [C.c_char_p, i32, i32, vp, i32]
+        L.rtx_hdr_read.argtypes = [C.c_char_p, C.POINTER(i32), C.POINTER(i32), C.POINTER(C.POINTER(C.c_float))]
         L.rtx_set_tuning.argtypes = [C.POINTER(Tuning)]
         L.rtx_get_tuning.argtypes = [C.POINTER(Tuning)]
         _LIB = L
@@ -277,12 +279,34 @@ def make_camera(cam):
     return c
 
 
-def make_params(width, height, spp, max_bounce, rr_bounce=3, seed=0, env_mode=0, env_color=(0, 0, 0)):
+def make_params(width, height, spp, max_bounce, rr_bounce=3, seed=0, env_mode=0, env_color=(0, 0, 0), ibl=None):
+    """ibl: (h, w, 3) float32 CUDA tensor, top row first, for env_mode 2 (kept alive on the
+    returned params)."""
     p = RenderParams()
     p.width, p.height, p.spp, p.max_bounce, p.rr_bounce, p.seed = width, height, spp, max_bounce, rr_bounce, seed
     p.env_mode = env_mode
     p.env_color[:] = [float(x) for x in env_color]
+    if ibl is not None:
+        p._ibl_keep = ibl
+        p.ibl_d = ibl.data_ptr()
+        p.ibl_height, p.ibl_width = int(ibl.shape[0]), int(ibl.shape[1])
     return p
+
+
+def write_hdr(path, img, flip_y=True):
+    """Radiance .hdr of an (h, w, 3) image; flip_y writes the bottom-up render top row first."""
+    a = np.ascontiguousarray(img, np.float32)
+    _check(lib().rtx_hdr_write(str(path).encode(), a.shape[1], a.shape[0], _p(a), int(flip_y)), "rtx_hdr_write")
+
+
+def read_hdr(path):
+    """(h, w, 3) float32 array of a Radiance .hdr file, top row first."""
+    w, h = C.c_int(), C.c_int()
+    ptr = C.POINTER(C.c_float)()
+    _check(lib().rtx_hdr_read(str(path).encode(), C.byref(w), C.byref(h), C.byref(ptr)), "rtx_hdr_read")
+    out = np.ctypeslib.as_array(ptr, shape=(h.value, w.value, 3)).copy()
+    lib().rtx_free_host(C.cast(ptr, C.c_void_p))
+    return out
 
 
 # ---- path tracing ------------------------------------------------------------------

@@ -185,3 +185,15 @@ def test_adjoint_rejects_roulette():
     p = rtx.make_params(8, 8, 1, 4, 3, seed=1)
     with pytest.raises(rtx.RtxError, match="rr_bounce"):
         rtx.render_adjoint(g, cam, p, _tensor(albedo), _tensor(emission), torch.zeros((8, 8, 3), device="cuda"))
+
+
+def test_ibl_environment_matches_oracle():
+    """Equirectangular IBL miss shading (IBL::sample, main.cpp:107-117) on both sides."""
+    import torch
+    v, t, albedo, _, cam = S.cornell_box(seed=6)
+    rng = np.random.default_rng(3)
+    ibl = (rng.random((32, 64, 3)) * 2).astype(np.float32)
+    p = rtx.make_params(48, 40, 2, 4, 3, seed=4, env_mode=2, ibl=torch.from_numpy(ibl).cuda())
+    op = O.make_params(48, 40, 2, 4, 3, seed=4, env_mode=2, ibl=ibl)
+    img, ref = _render_both(v, t, albedo, None, cam, p, op)
+    _close(img, ref, "IBL environment")

@@ -57,9 +57,39 @@ def test_scene_file_errors(tmp_path):
     with pytest.raises(rtx.RtxError, match="no vertices"):
         rtx.load_scene(empty)
     mesh = tmp_path / "mesh.scn"
-    mesh.write_text("mesh bunny.ply 0 0 0 1 q 0 0 0 1\n")
-    with pytest.raises(rtx.RtxError, match="mesh import"):
+    mesh.write_text("mesh nothere.ply 0 0 0 1 q 0 0 0 1\n")
+    with pytest.raises(rtx.RtxError, match="failed to open mesh"):
         rtx.load_scene(mesh)
+    mesh.write_text("mesh dwarf.x 0 0 0 1 q 0 0 0 1\n")
+    with pytest.raises(rtx.RtxError, match="ASCII .ply only"):
+        rtx.load_scene(mesh)
+    (tmp_path / "bin.ply").write_text("ply\nformat binary_little_endian 1.0\nelement vertex 0\nend_header\n")
+    mesh.write_text("mesh bin.ply 0 0 0 1 q 0 0 0 1\n")
+    with pytest.raises(rtx.RtxError, match="only ASCII PLY"):
+        rtx.load_scene(mesh)
+
+
+def test_ply_mesh_lines(tmp_path):
+    """A quad and a triangle in ASCII PLY, translated, scaled and rotated 90 degrees about z:
+    fan triangulation, smooth normals from face normals, transform translate*scale*rotate."""
+    (tmp_path / "q.ply").write_text(
+        "ply\nformat ascii 1.0\nelement vertex 5\nproperty float x\nproperty float y\nproperty float z\n"
+        "element face 2\nproperty list uchar int vertex_indices\nend_header\n"
+        "0 0 0\n1 0 0\n1 1 0\n0 1 0\n2 2 0\n4 0 1 2 3\n3 1 4 2\n")
+    s = float(np.sqrt(0.5))
+    (tmp_path / "m.scn").write_text(f"mesh q.ply 1 2 3 2 q 0 0 {s} {s}\n")
+    v, t = rtx.load_scene(tmp_path / "m.scn")
+    assert v.shape == (5, 6) and t.tolist() == [[0, 1, 2], [0, 2, 3], [1, 4, 2]]
+    # rotation by 90 degrees about z maps (x, y) -> (-y, x); then scale 2, translate (1, 2, 3)
+    np.testing.assert_allclose(v[:, :3], [[1, 2, 3], [1, 4, 3], [-1, 4, 3], [-1, 2, 3], [-3, 6, 3]], atol=1e-5)
+    np.testing.assert_allclose(v[:, 3:], np.tile([0, 0, 1], (5, 1)), atol=1e-6)
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/bunny.ply"), reason="reference assets not present")
+def test_reference_bunny_loads(tmp_path):
+    (tmp_path / "b.scn").write_text("mesh /root/reference/bunny.ply 0 0 0 10 q 0 0 0 1\n")
+    v, t = rtx.load_scene(tmp_path / "b.scn")
+    assert v.shape == (35947, 6) and t.shape == (69451, 3)
 
 
 def _write_bvh(path, version=1, truncate=0):
@@ -111,3 +141,54 @@ def test_oracle_white_furnace(mb):
     cam = S.camera_lookat((0.1, 0.05, -0.2), (0.3, 0.2, 1.0), 60.0)
     img = O.render(o, cam, O.make_params(16, 12, 3, mb, 99, seed=1), albedo, emission)
     np.testing.assert_allclose(img, E * sum(a ** k for k in range(mb + 1)), rtol=2e-6)
+
+
+def test_hdr_round_trip_and_flip(tmp_path):
+    rng = np.random.default_rng(1)
+    img = (rng.random((7, 9, 3)) * 4).astype(np.float32)
+    img[0, 0] = 0
+    rtx.write_hdr(tmp_path / "a.hdr", img, flip_y=False)
+    back = rtx.read_hdr(tmp_path / "a.hdr")
+    assert back.shape == img.shape
+    # RGBE shares one exponent per pixel: error <= (largest channel) / 128, truncated
+    assert (np.abs(back - img) <= img.max(-1, keepdims=True) / 128 + 1e-7).all()
+    rtx.write_hdr(tmp_path / "b.hdr", img, flip_y=True)
+    np.testing.assert_array_equal(rtx.read_hdr(tmp_path / "b.hdr"), back[::-1])
+
+
+def test_hdr_reads_run_length_scanlines(tmp_path):
+    """New-style RLE scanlines (channel planes, runs > 128 repeat the next byte)."""
+    w, h = 10, 2
+    body = b""
+    for _ in range(h):
+        body += bytes([2, 2, 0, w])
+        body += bytes([128 + 10, 100])                    # R: run of 10
+        body += bytes([10]) + bytes(range(10))             # G: 10 literals
+        body += bytes([128 + 10, 0])                       # B: run of zeros
+        body += bytes([128 + 10, 129])                     # E: exponent 1
+    (tmp_path / "r.hdr").write_bytes(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 2 +X 10\n" + body)
+    img = rtx.read_hdr(tmp_path / "r.hdr")
+    f = 2.0 ** (129 - 136)
+    np.testing.assert_allclose(img[:, :, 0], 100 * f)
+    np.testing.assert_allclose(img[1, :, 1], np.arange(10) * f)
+    assert (img[:, :, 2] == 0).all()
+
+
+def test_oracle_ibl_lookup():
+    """A constant map equals the constant environment; a map whose rows differ picks the
+    row of the ray's polar angle (row 0 = straight up, IBL::sample main.cpp:107-117)."""
+    v = np.array([[0, 0, 10, 0, 0, -1], [1, 0, 10, 0, 0, -1], [0, 1, 10, 0, 0, -1]], np.float32)
+    t = np.array([[0, 1, 2]], np.uint32)
+    o = O.build(v, t)
+    cam = S.camera_lookat((0, 0, 0), (0, 0, -1), 60.0)
+    const = np.full((8, 16, 3), 0.25, np.float32)
+    a = O.render(o, cam, O.make_params(12, 8, 2, 3, 3, seed=1, env_mode=2, ibl=const))
+    b = O.render(o, cam, O.make_params(12, 8, 2, 3, 3, seed=1, env_mode=1, env_color=(0.25, 0.25, 0.25)))
+    np.testing.assert_array_equal(a, b)
+    rows = np.repeat(np.arange(4, dtype=np.float32)[:, None, None], 16, 1).repeat(3, 2)
+    up = S.camera_lookat((0, 0, 0), (0, 1, 0.0001), 10.0)
+    img = O.render(o, up, O.make_params(4, 4, 1, 0, 3, seed=1, env_mode=2, ibl=rows))
+    assert (img == 0).all()                   # looking straight up: the top row
+    down = S.camera_lookat((0, 0, 0), (0, -1, 0.0001), 10.0)
+    img = O.render(o, down, O.make_params(4, 4, 1, 0, 3, seed=1, env_mode=2, ibl=rows))
+    assert (img == 3).all()                   # straight down: the bottom row
