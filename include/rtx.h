@@ -186,7 +186,9 @@ void rtx_batch_free(rtx_batch* batch);
  * batch_paths: paths in flight per wavefront batch (>= 1024); grid_blocks: blocks of the
  * persistent trace/shade kernels; shade_threads: threads per shade block (multiple of 64);
  * trace_lds_depth: LDS stack entries per lane (8, 12 or 16); trace_loop: 0 = one
- * node/leaf loop, 1 = while-while (triangle tests run with the wave converged). */
+ * node/leaf loop, 1 = while-while (triangle tests run with the wave converged), 2 = the
+ * while-while loop in a persistent grid where idle lanes fetch new rays (plain trace
+ * entry points; the renderer runs loop 1 for it). */
 typedef struct rtx_tuning {
     uint32_t batch_paths;
     uint32_t grid_blocks;

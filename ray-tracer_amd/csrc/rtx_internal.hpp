@@ -44,6 +44,7 @@ struct rtx_bvh {
     rtx::GTri* gtris = nullptr;
     float root_box[6] = {0, 0, 0, 0, 0, 0};
     uint32_t root_ref = 0;
+    uint32_t* work = nullptr;         // ray counter of the persistent trace kernel
     // diagnostics
     uint64_t* codes = nullptr;        // Morton code per input triangle
     uint32_t* sorted = nullptr;       // s_sortedTris[0..nt)

@@ -13,6 +13,8 @@
 // leaf order with a strict t < best update, so exact-t ties resolve as in the reference.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "rtx_trace_impl.hpp"
 
 namespace rtx {
@@ -53,10 +55,79 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray3
     }
 }
 
+// Persistent variant (tuning trace_loop = 2): a fixed grid of waves; when at least
+// kRefill lanes of a wave are idle, one atomic on `work` hands them the next rays. Each
+// ray is still traversed exactly as trace_ray's while-while loop does.
+constexpr int kRefill = 16;
+
+template <bool ANY, bool COUNT, int DEPTH>
+__global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, const Ray32* __restrict__ rays,
+                                                                 uint32_t n, Hit16* __restrict__ hits,
+                                                                 float* __restrict__ normals,
+                                                                 uint8_t* __restrict__ occl,
+                                                                 uint32_t* __restrict__ counts, uint32_t* work) {
+    __shared__ uint2 lst[DEPTH][kTraceThreads];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t lt = (1ull << __lane_id()) - 1ull;
+    Walker<DEPTH> w;
+    w.live = false;
+    w.best = kInvalid;
+    uint32_t id = kInvalid;
+    bool more = true;
+    for (;;) {
+        if (!w.live && id != kInvalid) {
+            if (ANY) {
+                occl[id] = w.best != kInvalid ? 1 : 0;
+            } else {
+                reinterpret_cast<float4*>(hits)[id] =
+                    make_float4(w.best != kInvalid ? w.bestT : 0.f, w.bu, w.bv, __uint_as_float(w.best));
+                if (COUNT) { counts[2 * id] = w.nI; counts[2 * id + 1] = w.nT; }
+                if (normals) {
+                    f3 nn{0.f, 0.f, 0.f};
+                    if (w.best != kInvalid) nn = hit_normal(A, w.best, w.bu, w.bv);
+                    normals[3 * id] = nn.x; normals[3 * id + 1] = nn.y; normals[3 * id + 2] = nn.z;
+                }
+            }
+            id = kInvalid;
+        }
+        const uint64_t idle = __ballot(!w.live);
+        if (more && __popcll(idle) >= kRefill) {
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if (__lane_id() == leader) base = atomicAdd(work, (uint32_t)__popcll(idle));
+            base = __shfl(base, leader);
+            if (base + __popcll(idle) >= n) more = false;
+            if (!w.live) {
+                const uint32_t k = base + __popcll(idle & lt);
+                if (k < n) {
+                    id = k;
+                    const float4 r0 = reinterpret_cast<const float4*>(rays)[2 * k];
+                    const float4 r1 = reinterpret_cast<const float4*>(rays)[2 * k + 1];
+                    w.start(A, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w);
+                    continue;   // a ray that misses the root is written at the loop top
+                }
+            }
+        }
+        if (!__any(w.live)) {
+            if (!more && !__any(id != kInvalid)) break;
+            continue;
+        }
+        while (w.live && !(w.ref & kLeafBit)) w.template node_step<COUNT>(A, lst, tid);
+        if (w.live) {
+            if (w.template leaf_step<ANY, COUNT>(A)) w.live = false;
+            else w.pop(lst, tid);
+        }
+    }
+}
+
+__global__ void k_zero_work(uint32_t* w) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *w = 0u;
+}
+
 int check(const rtx_bvh* b, const void* rays, size_t n, const void* out, const char* fn) {
     if (!b || !out || (!rays && n)) return fail(RTX_E_ARG, std::string("null argument on ") + fn);
     if (!b->gnodes || !b->leaves || !b->gtris) return fail(RTX_E_STATE, std::string("BVH not loaded on ") + fn);
-    if (n > 0xFFFFFFFFull - kTraceThreads) return fail(RTX_E_LIMIT, std::string("too many rays on ") + fn);
+    if (n > (1ull << 31)) return fail(RTX_E_LIMIT, std::string("more than 2^31 rays on ") + fn);
     if (b->max_depth > kMaxTraceDepth)
         return fail(RTX_E_LIMIT, std::string("tree deeper than the traversal stack on ") + fn);
     return RTX_OK;
@@ -68,6 +139,17 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
     if (n == 0) return RTX_OK;
     uint32_t blocks = (uint32_t)((n + kTraceThreads - 1) / kTraceThreads);
     const TraceArgs A = make_trace_args(b);
+    if (g_tuning.trace_loop == 2 && b->work) {
+        const uint32_t pb = std::min<uint32_t>(blocks, g_tuning.grid_blocks);
+        k_zero_work<<<1, 64, 0, s>>>(b->work);
+        with_trace_variant([&](auto D, auto) {
+            k_trace_persist<ANY, COUNT, decltype(D)::value><<<pb, kTraceThreads, 0, s>>>(
+                A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
+                counts, b->work);
+        });
+        RTX_HIP(hipGetLastError());
+        return RTX_OK;
+    }
     with_trace_variant([&](auto D, auto L) {
         k_trace<ANY, COUNT, decltype(D)::value, decltype(L)::value><<<blocks, kTraceThreads, 0, s>>>(
             A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,

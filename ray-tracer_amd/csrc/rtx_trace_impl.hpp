@@ -31,6 +31,8 @@ void with_trace_variant(F&& f) {
     using I16 = std::integral_constant<int, 16>;
     using L0 = std::integral_constant<int, 0>;
     using L1 = std::integral_constant<int, 1>;
+    // loop 2 (persistent, dynamic fetch) exists for the plain trace only; elsewhere it
+    // runs as the while-while loop it is built on
     const uint32_t d = g_tuning.trace_lds_depth, l = g_tuning.trace_loop;
     if (d == 8) { if (l) f(I8{}, L1{}); else f(I8{}, L0{}); }
     else if (d == 12) { if (l) f(I12{}, L1{}); else f(I12{}, L0{}); }
@@ -65,32 +67,35 @@ struct TraceOut {
     uint32_t nI, nT;    // internal visits, triangle tests
 };
 
-// One ray against one tree. lst is this block's LDS stack, DEPTH entries per lane in
-// column tid. LOOP 0 visits nodes and leaves in one loop; LOOP 1 ("while-while") lets a
-// lane that reaches a leaf wait until every lane of the wave is at a leaf, so triangle
-// tests run with the wave converged. Both visit the same nodes in the same order per ray.
-template <bool ANY, bool COUNT, int DEPTH, int LOOP>
-__device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, float tmin, float tmax,
-                                              uint2 (*lst)[kTraceThreads], uint32_t tid) {
-    const float ix = 1.f / d.x, iy = 1.f / d.y, iz = 1.f / d.z;
+// Traversal state of one lane. lst is the block's LDS stack, DEPTH entries per lane in
+// column tid; entries beyond that spill (oldest first) to private memory.
+template <int DEPTH>
+struct Walker {
+    f3 o, d;
+    float ix, iy, iz, tmin, tmax;
+    float bestT, bu, bv;
+    uint32_t best, ref, top, lo, nI, nT;
+    bool live;
     uint2 spill[kSpillDepth];
-    uint32_t top = 0, lo = 0;
-    float bestT = tmax, bu = 0.f, bv = 0.f;
-    uint32_t best = kInvalid;
-    uint32_t nI = 0, nT = 0;
 
-    uint32_t ref = A.root_ref;
-    bool live = slab(o.x, o.y, o.z, ix, iy, iz, tmin, A.rb[0], A.rb[1], A.rb[2], A.rb[3], A.rb[4], A.rb[5], bestT) < bestT;
-
-    // ring slot of stack entry e (DEPTH need not be a power of two)
-    auto slot = [&](uint32_t e) { return DEPTH == 16 || DEPTH == 8 ? (e & (DEPTH - 1)) : (e % DEPTH); };
-    auto push = [&](uint32_t r, float t) {
+    __device__ __forceinline__ uint32_t slot(uint32_t e) const {
+        return DEPTH == 16 || DEPTH == 8 ? (e & (DEPTH - 1)) : (e % DEPTH);
+    }
+    __device__ __forceinline__ void start(const TraceArgs& A, f3 o_, f3 d_, float tmin_, float tmax_) {
+        o = o_; d = d_; tmin = tmin_; tmax = tmax_;
+        ix = 1.f / d.x; iy = 1.f / d.y; iz = 1.f / d.z;
+        bestT = tmax; bu = 0.f; bv = 0.f;
+        best = kInvalid; top = 0; lo = 0; nI = 0; nT = 0;
+        ref = A.root_ref;
+        live = slab(o.x, o.y, o.z, ix, iy, iz, tmin, A.rb[0], A.rb[1], A.rb[2], A.rb[3], A.rb[4], A.rb[5], bestT) < bestT;
+    }
+    __device__ __forceinline__ void push(uint2 (*lst)[kTraceThreads], uint32_t tid, uint32_t r, float t) {
         if (top - lo == (uint32_t)DEPTH) { spill[lo] = lst[slot(lo)][tid]; ++lo; }
         lst[slot(top)][tid] = make_uint2(r, __float_as_uint(t));
         ++top;
-    };
+    }
     // pop the next entry that can still hold a closer hit
-    auto pop = [&]() {
+    __device__ __forceinline__ void pop(uint2 (*lst)[kTraceThreads], uint32_t tid) {
         live = false;
         while (top > 0) {
             --top;
@@ -99,9 +104,10 @@ __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, fl
             else e = lst[slot(top)][tid];
             if (__uint_as_float(e.y) < bestT) { ref = e.x; live = true; break; }
         }
-    };
+    }
     // one internal node: test both children, push the far one, continue with the near one
-    auto node_step = [&]() {
+    template <bool COUNT>
+    __device__ __forceinline__ void node_step(const TraceArgs& A, uint2 (*lst)[kTraceThreads], uint32_t tid) {
         if (COUNT) ++nI;
         const GNode& g = A.nodes[ref];
         const float4 l0 = g.l0, l1 = g.l1, r1b = g.r1;
@@ -113,14 +119,15 @@ __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, fl
         if (tl <= tr) { nearRef = refs.x; tn = tl; farRef = refs.y; tf = tr; }
         else { nearRef = refs.y; tn = tr; farRef = refs.x; tf = tl; }
         if (tn < bestT) {
-            if (tf < bestT) push(farRef, tf);
+            if (tf < bestT) push(lst, tid, farRef, tf);
             ref = nearRef;
         } else {
-            pop();
+            pop(lst, tid);
         }
-    };
-    // all triangles of the current leaf; returns true when an any-hit query is answered
-    auto leaf_step = [&]() {
+    }
+    // all triangles of the current leaf; true when an any-hit query is answered
+    template <bool ANY, bool COUNT>
+    __device__ __forceinline__ bool leaf_step(const TraceArgs& A) {
         const Leaf lf = A.leaves[ref & ~kLeafBit];
         for (uint32_t k = lf.begin; k < lf.begin + lf.count; ++k) {
             const float4 a = A.tris[k].a, b = A.tris[k].b, c = A.tris[k].c;
@@ -133,34 +140,44 @@ __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, fl
             }
         }
         return false;
-    };
+    }
+};
 
+// One ray against one tree. LOOP 0 visits nodes and leaves in one loop; LOOP 1
+// ("while-while") lets a lane that reaches a leaf wait until every lane of the wave is at
+// a leaf, so triangle tests run with the wave converged. Both visit the same nodes in the
+// same order per ray.
+template <bool ANY, bool COUNT, int DEPTH, int LOOP>
+__device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, float tmin, float tmax,
+                                              uint2 (*lst)[kTraceThreads], uint32_t tid) {
+    Walker<DEPTH> w;
+    w.start(A, o, d, tmin, tmax);
     if (LOOP == 0) {
-        while (live) {
-            if (ref & kLeafBit) {
-                if (leaf_step()) break;
-                pop();
+        while (w.live) {
+            if (w.ref & kLeafBit) {
+                if (w.template leaf_step<ANY, COUNT>(A)) break;
+                w.pop(lst, tid);
             } else {
-                node_step();
+                w.template node_step<COUNT>(A, lst, tid);
             }
         }
     } else {
         for (;;) {
-            while (live && !(ref & kLeafBit)) node_step();
-            if (!__any(live)) break;
-            if (live) {
-                if (leaf_step()) live = false;
-                else pop();
+            while (w.live && !(w.ref & kLeafBit)) w.template node_step<COUNT>(A, lst, tid);
+            if (!__any(w.live)) break;
+            if (w.live) {
+                if (w.template leaf_step<ANY, COUNT>(A)) w.live = false;
+                else w.pop(lst, tid);
             }
         }
     }
     TraceOut r;
-    r.t = best != kInvalid ? bestT : 0.f;
-    r.u = bu;
-    r.v = bv;
-    r.tri = best;
-    r.nI = nI;
-    r.nT = nT;
+    r.t = w.best != kInvalid ? w.bestT : 0.f;
+    r.u = w.bu;
+    r.v = w.bv;
+    r.tri = w.best;
+    r.nI = w.nI;
+    r.nT = w.nT;
     return r;
 }
 

@@ -115,7 +115,7 @@ def test_tuning_variants_give_identical_hits():
     ref = None
     try:
         for depth in (8, 12, 16):
-            for loop in (0, 1):
+            for loop in (0, 1, 2):
                 rtx.set_tuning(trace_lds_depth=depth, trace_loop=loop)
                 g.trace_closest_device(rays, hits)
                 torch.cuda.synchronize()
@@ -123,5 +123,26 @@ def test_tuning_variants_give_identical_hits():
                 if ref is None:
                     ref = h
                 assert np.array_equal(h, ref), f"variant depth={depth} loop={loop} differs"
+    finally:
+        rtx.set_tuning(**saved)
+
+
+def test_persistent_variant_counts_and_any_hit(suite):
+    """The persistent dynamic-fetch kernel also serves counts and any-hit exactly."""
+    saved = rtx.get_tuning()
+    try:
+        rtx.set_tuning(trace_loop=2, trace_lds_depth=8)
+        for name in ("rand20000", "straddle", "cornell"):
+            v, t = suite[name]
+            g = rtx.build(v, t)
+            o = O.build(v, t)
+            rays = _rays_for(v, 30001, seed=11)
+            h, nrm, occ, cnt = _device_trace(g, rays)
+            ref = O.trace(o, rays, counts=True, normals=True, nthreads=8)
+            assert np.array_equal(h[:, 3].view(np.uint32), ref["tri"]), name
+            assert np.array_equal(h[:, 0].view(np.uint32), ref["t"].view(np.uint32)), name
+            assert np.array_equal(occ.astype(bool), ref["tri"] != 0xFFFFFFFF), name
+            assert np.array_equal(cnt, ref["counts"]), name
+            np.testing.assert_allclose(nrm, ref["normal"], atol=1e-6)
     finally:
         rtx.set_tuning(**saved)
