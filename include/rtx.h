@@ -186,15 +186,19 @@ void rtx_batch_free(rtx_batch* batch);
  * batch_paths: paths in flight per wavefront batch (>= 1024); grid_blocks: blocks of the
  * persistent trace/shade kernels; shade_threads: threads per shade block (multiple of 64);
  * trace_lds_depth: LDS stack entries per lane (8, 12 or 16); trace_loop: 0 = one
- * node/leaf loop, 1 = while-while (triangle tests run with the wave converged), 2 = the
- * while-while loop in a persistent grid where idle lanes fetch new rays (plain trace
- * entry points; the renderer runs loop 1 for it). */
+ * node/leaf loop, 1 = while-while (triangle tests run with the wave converged); the value
+ * 2 (a persistent dynamic-fetch grid) is reserved and refused; ray_sort: 1 = the trace entry points
+ * first sort rays (>= 65536 of them) by a Morton key of origin and direction and trace in
+ * that order, writing results back at each ray's index. The sort's scratch lives on the
+ * BVH handle and grows to the largest ray count seen, so with ray_sort on, trace calls
+ * that overlap on one handle (other streams or threads) race. */
 typedef struct rtx_tuning {
     uint32_t batch_paths;
     uint32_t grid_blocks;
     uint32_t shade_threads;
     uint32_t trace_lds_depth;
     uint32_t trace_loop;
+    uint32_t ray_sort;
 } rtx_tuning;
 int rtx_set_tuning(const rtx_tuning* t);
 int rtx_get_tuning(rtx_tuning* t);

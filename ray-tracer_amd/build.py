@@ -13,7 +13,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "librtx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["rtx_build.hip", "rtx_trace.hip", "rtx_render.hip", "rtx_host.cpp"]
+SOURCES = ["rtx_build.hip", "rtx_sort.hip", "rtx_trace.hip", "rtx_render.hip", "rtx_host.cpp"]
 
 # -ffp-contract=off: no FMA contraction, the reference's float results depend on it.
 # Packed fp32 ops are disabled for all device code.

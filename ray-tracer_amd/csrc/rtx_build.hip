@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "rtx_internal.hpp"
+#include "rtx_sort.hpp"
 
 namespace rtx {
 
@@ -374,82 +375,6 @@ __global__ void k_morton(const float4* cent, uint32_t nt, const float* box6, uin
     if (__lane_id() == 0) {
         atomicOr(&orand[0], o);
         atomicAnd(&orand[1], a);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Stable LSD radix sort of (code, index), 8-bit digits (bvh_morton.cpp:121-156).
-// Tiles of 4096 keys; per-tile LDS histograms; one scan; stable scatter ranked by
-// wave ballots (lanes with the same digit) and per-wave LDS counts.
-// ---------------------------------------------------------------------------
-constexpr int kRsThreads = 256, kRsItems = 16, kRsTile = kRsThreads * kRsItems;
-
-__global__ void k_rs_hist(const uint64_t* keys, uint32_t n, int shift, uint32_t* hist, uint32_t ntiles) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    uint32_t base = blockIdx.x * kRsTile;
-    for (int r = 0; r < kRsItems; ++r) {
-        uint32_t i = base + r * kRsThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
-}
-
-// Exclusive scan of m counters with one block of 1024 threads.
-__global__ void k_scan_excl(uint32_t* data, uint32_t m) {
-    __shared__ uint32_t sums[1024];
-    uint32_t chunk = (m + 1023) / 1024;
-    uint32_t b = threadIdx.x * chunk, e = min(m, b + chunk);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += data[i];
-    sums[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (int i = 0; i < 1024; ++i) { uint32_t v = sums[i]; sums[i] = run; run += v; }
-    }
-    __syncthreads();
-    uint32_t run = sums[threadIdx.x];
-    for (uint32_t i = b; i < e; ++i) { uint32_t v = data[i]; data[i] = run; run += v; }
-}
-
-__global__ void k_rs_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint32_t n,
-                             int shift, const uint32_t* hist, uint32_t ntiles) {
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[kRsThreads / 64][256];
-    const int lane = __lane_id(), w = threadIdx.x >> 6;
-    run[threadIdx.x] = hist[threadIdx.x * ntiles + blockIdx.x];
-    const uint64_t lt = (1ull << lane) - 1ull;
-    uint32_t base = blockIdx.x * kRsTile;
-    for (int r = 0; r < kRsItems; ++r) {
-        for (int ww = 0; ww < kRsThreads / 64; ++ww) wcnt[ww][threadIdx.x] = 0;
-        __syncthreads();
-        uint32_t i = base + r * kRsThreads + threadIdx.x;
-        bool valid = i < n;
-        uint64_t k = valid ? kin[i] : 0ull;
-        uint32_t v = valid ? vin[i] : 0u;
-        uint32_t d = (uint32_t)(k >> shift) & 255u;
-        uint64_t mask = __ballot(valid);
-        for (int bit = 0; bit < 8; ++bit) {
-            uint64_t bb = __ballot(valid && ((d >> bit) & 1u));
-            mask &= ((d >> bit) & 1u) ? bb : ~bb;
-        }
-        uint32_t rank = __popcll(mask & lt);
-        if (valid && rank == 0) wcnt[w][d] = __popcll(mask);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = run[d] + rank;
-            for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][d];
-            kout[pos] = k;
-            vout[pos] = v;
-        }
-        __syncthreads();
-        uint32_t add_ = 0;
-        for (int ww = 0; ww < kRsThreads / 64; ++ww) add_ += wcnt[ww][threadIdx.x];
-        run[threadIdx.x] += add_;
-        __syncthreads();
     }
 }
 
@@ -1084,7 +1009,8 @@ struct Scratch {
 
 void free_products(rtx_bvh* b) {
     void* ps[] = {b->verts, b->nodes48, b->tris3, b->prims, b->gnodes, b->leaves, b->gtris, b->codes, b->sorted,
-                  b->bfs48, b->work};
+                  b->bfs48, b->rs_keys[0], b->rs_keys[1], b->rs_vals[0], b->rs_vals[1], b->rs_hist,
+                  b->rs_part};
     for (void* p : ps)
         if (p) (void)hipFree(p);
 }
@@ -1111,7 +1037,6 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
     uint64_t* keysC = S.get<uint64_t>(N);
     uint32_t* valsA = S.get<uint32_t>(N);
     uint32_t* valsB = S.get<uint32_t>(N);
-    uint32_t* valsC = S.get<uint32_t>(N);
     unsigned long long* orand = S.get<unsigned long long>(2);
     uint32_t* sorted = S.get<uint32_t>(cap);
     uint32_t* tindex = S.get<uint32_t>(cap);
@@ -1128,7 +1053,7 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
     B->codes = dalloc<uint64_t>(N);
     B->sorted = dalloc<uint32_t>(N);
     B->verts = dalloc<float>((size_t)nv * 6);
-    if (!err || !tbox || !cent || !box6 || !keysB || !keysC || !valsA || !valsB || !valsC || !orand || !sorted ||
+    if (!err || !tbox || !cent || !box6 || !keysB || !keysC || !valsA || !valsB || !orand || !sorted ||
         !tindex || !nodes || !dec || !child || !toff || !soff || !acc || !ctr || !dbuf || !B->codes || !B->sorted ||
         !B->verts)
         return fail(RTX_E_HIP, "device allocation failed on rtx_build()");
@@ -1161,27 +1086,19 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
     RTX_HIP(hipMemcpyAsync(B->scene, box6, 6 * sizeof(float), hipMemcpyDeviceToHost, st));
     RTX_HIP(hipStreamSynchronize(st));
 
-    // radix sort: skip passes whose digit is the same for every key
-    const uint32_t ntiles = nblk(N, kRsTile);
-    uint32_t* hist = S.get<uint32_t>(256 * (size_t)ntiles);
-    if (!hist) return fail(RTX_E_HIP, "device allocation failed on rtx_build()");
+    // radix sort (rtx_sort.hip): skip passes whose digit is the same for every key
+    uint32_t* hist = S.get<uint32_t>(sort_hist_entries(N));
+    uint32_t* spart = S.get<uint32_t>(sort_part_entries(N));
+    if (!hist || !spart) return fail(RTX_E_HIP, "device allocation failed on rtx_build()");
     const uint64_t vary = hor[0] ^ hor[1];
-    const uint64_t* kin = B->codes;
-    const uint32_t* vin = valsA;
-    uint64_t* kbuf[2] = {keysB, keysC};
-    uint32_t* vbuf[2] = {valsB, valsC};
-    int flip = 0;
-    for (int pass = 0; pass < 8; ++pass) {
-        int shift = 8 * pass;
-        if (((vary >> shift) & 255u) == 0) continue;
-        k_rs_hist<<<ntiles, kRsThreads, 0, st>>>(kin, N, shift, hist, ntiles);
-        k_scan_excl<<<1, 1024, 0, st>>>(hist, 256 * ntiles);
-        k_rs_scatter<<<ntiles, kRsThreads, 0, st>>>(kin, vin, kbuf[flip], vbuf[flip], N, shift, hist, ntiles);
-        kin = kbuf[flip];
-        vin = vbuf[flip];
-        flip ^= 1;
-    }
-    const uint64_t* scodes = kin;
+    uint32_t pass_mask = 0;
+    for (int pass = 0; pass < 8; ++pass)
+        if ((vary >> (8 * pass)) & 255u) pass_mask |= 1u << pass;
+    RTX_HIP(hipMemcpyAsync(keysB, B->codes, sizeof(uint64_t) * N, hipMemcpyDeviceToDevice, st));
+    SortBuffers SB{{keysB, keysC}, {valsA, valsB}, hist, spart, 0};
+    radix_sort_pairs(SB, N, pass_mask, st);
+    const uint64_t* scodes = SB.keys[SB.cur];
+    const uint32_t* vin = SB.vals[SB.cur];
     RTX_HIP(hipMemcpyAsync(sorted, vin, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, st));
     RTX_HIP(hipMemcpyAsync(B->sorted, vin, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, st));
     k_iota<<<nblk(N, 256), 256, 0, st>>>(tindex, N);
@@ -1228,8 +1145,7 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
     B->gnodes = dalloc<GNode>(std::max<uint32_t>(B->ninternal, 1));
     B->leaves = dalloc<Leaf>(B->nleaves);
     B->gtris = dalloc<GTri>(B->ntri);
-    B->work = dalloc<uint32_t>(64);
-    if (!B->nodes48 || !B->bfs48 || !B->tris3 || !B->prims || !B->gnodes || !B->leaves || !B->gtris || !B->work)
+    if (!B->nodes48 || !B->bfs48 || !B->tris3 || !B->prims || !B->gnodes || !B->leaves || !B->gtris)
         return fail(RTX_E_HIP, "device allocation failed on rtx_build()");
     Out O{B->nodes48, B->bfs48, B->tris3, B->prims, B->gnodes, B->leaves, B->gtris};
     k_emit<<<nblk(total, 128), 128, 0, st>>>(C, total, D, O);
@@ -1273,10 +1189,9 @@ int upload_reference_tree(rtx_bvh* B, const Node48* nodes, uint32_t nn, const ui
     B->gnodes = dalloc<GNode>(std::max<uint32_t>(ni, 1));
     B->leaves = dalloc<Leaf>(std::max<uint32_t>(nl, 1));
     B->gtris = dalloc<GTri>(std::max<uint32_t>(ntri, 1));
-    B->work = dalloc<uint32_t>(64);
     uint32_t* d_i = dalloc<uint32_t>(nn);
     uint32_t* d_l = dalloc<uint32_t>(nn);
-    if (!B->nodes48 || !B->tris3 || !B->prims || !B->verts || !B->gnodes || !B->leaves || !B->gtris || !B->work ||
+    if (!B->nodes48 || !B->tris3 || !B->prims || !B->verts || !B->gnodes || !B->leaves || !B->gtris ||
         !d_i || !d_l)
         return fail(RTX_E_HIP, "device allocation failed on rtx_load()");
     std::vector<uint32_t> prims(ntri);

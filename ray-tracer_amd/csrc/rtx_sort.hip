@@ -1,0 +1,152 @@
+// Stable LSD radix sort of (uint64 key, uint32 value) pairs on 8-bit digits, shared by
+// the builder (Morton codes, bvh_morton.cpp:109-159) and the ray reordering of traces.
+//
+// Per pass: k_rs_hist counts digits per 4096-key tile in LDS; a three-kernel scan turns
+// the digit-major histogram into global offsets; k_rs_scatter ranks keys stably inside a
+// tile with wave ballots (lanes holding the same digit) and per-wave LDS counts.
+#include <hip/hip_runtime.h>
+
+#include "rtx_sort.hpp"
+
+namespace rtx {
+namespace {
+
+constexpr int kRsThreads = 256, kRsItems = 16, kRsTile = kRsThreads * kRsItems;
+constexpr int kScanThreads = 1024, kScanItems = 4, kScanSeg = kScanThreads * kScanItems;
+
+__global__ void k_rs_hist(const uint64_t* keys, uint32_t n, int shift, uint32_t* hist, uint32_t ntiles) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsItems; ++r) {
+        uint32_t i = base + r * kRsThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Block-wide exclusive scan (1024 threads); returns the block total in *total.
+__device__ uint32_t block_excl(uint32_t v, uint32_t* ws, uint32_t* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t s = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(s, o);
+        if (lane >= o) s += y;
+    }
+    if (lane == 63) ws[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < kScanThreads / 64; ++i) { uint32_t x = ws[i]; ws[i] = run; run += x; }
+        ws[kScanThreads / 64] = run;
+    }
+    __syncthreads();
+    uint32_t ex = ws[w] + s - v;
+    *total = ws[kScanThreads / 64];
+    __syncthreads();
+    return ex;
+}
+
+// Exclusive scan of segments of kScanSeg entries in place; segment totals to part[].
+__global__ void __launch_bounds__(kScanThreads) k_scan_seg(uint32_t* data, uint32_t m, uint32_t* part) {
+    __shared__ uint32_t ws[kScanThreads / 64 + 1];
+    const uint32_t b = blockIdx.x * kScanSeg + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems], s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (b + k < m) ? data[b + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl(s, ws, &tot);
+    for (int k = 0; k < kScanItems; ++k) {
+        if (b + k < m) data[b + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// Exclusive scan of the segment totals (any count, one block, serial per thread chunk).
+__global__ void __launch_bounds__(kScanThreads) k_scan_parts(uint32_t* part, uint32_t np) {
+    __shared__ uint32_t ws[kScanThreads / 64 + 1];
+    const uint32_t chunk = (np + kScanThreads - 1) / kScanThreads;
+    const uint32_t b = threadIdx.x * chunk, e = min(np, b + chunk);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += part[i];
+    uint32_t tot;
+    uint32_t run = block_excl(s, ws, &tot);
+    for (uint32_t i = b; i < e; ++i) { uint32_t x = part[i]; part[i] = run; run += x; }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_add(uint32_t* data, uint32_t m, const uint32_t* part) {
+    const uint32_t add = part[blockIdx.x];
+    const uint32_t b = blockIdx.x * kScanSeg + threadIdx.x * kScanItems;
+    for (int k = 0; k < kScanItems; ++k)
+        if (b + k < m) data[b + k] += add;
+}
+
+__global__ void k_rs_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint32_t n,
+                             int shift, const uint32_t* hist, uint32_t ntiles) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[kRsThreads / 64][256];
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    run[threadIdx.x] = hist[threadIdx.x * ntiles + blockIdx.x];
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t base = blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsItems; ++r) {
+        for (int ww = 0; ww < kRsThreads / 64; ++ww) wcnt[ww][threadIdx.x] = 0;
+        __syncthreads();
+        uint32_t i = base + r * kRsThreads + threadIdx.x;
+        bool valid = i < n;
+        uint64_t k = valid ? kin[i] : 0ull;
+        uint32_t v = valid ? vin[i] : 0u;
+        uint32_t d = (uint32_t)(k >> shift) & 255u;
+        uint64_t mask = __ballot(valid);
+        for (int bit = 0; bit < 8; ++bit) {
+            uint64_t bb = __ballot(valid && ((d >> bit) & 1u));
+            mask &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        uint32_t rank = __popcll(mask & lt);
+        if (valid && rank == 0) wcnt[w][d] = __popcll(mask);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + rank;
+            for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][d];
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+        uint32_t add_ = 0;
+        for (int ww = 0; ww < kRsThreads / 64; ++ww) add_ += wcnt[ww][threadIdx.x];
+        run[threadIdx.x] += add_;
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+size_t sort_hist_entries(uint32_t n) { return 256ull * ((n + kRsTile - 1) / kRsTile); }
+size_t sort_part_entries(uint32_t n) { return (sort_hist_entries(n) + kScanSeg - 1) / kScanSeg + 1; }
+
+int radix_sort_pairs(SortBuffers& S, uint32_t n, uint32_t pass_mask, hipStream_t st) {
+    S.cur = 0;
+    if (n == 0) return 0;
+    const uint32_t ntiles = (n + kRsTile - 1) / kRsTile;
+    const uint32_t m = 256 * ntiles;
+    const uint32_t nseg = (m + kScanSeg - 1) / kScanSeg;
+    for (int pass = 0; pass < 8; ++pass) {
+        if (!((pass_mask >> pass) & 1u)) continue;
+        const int shift = 8 * pass;
+        k_rs_hist<<<ntiles, kRsThreads, 0, st>>>(S.keys[S.cur], n, shift, S.hist, ntiles);
+        k_scan_seg<<<nseg, kScanThreads, 0, st>>>(S.hist, m, S.part);
+        k_scan_parts<<<1, kScanThreads, 0, st>>>(S.part, nseg);
+        k_scan_add<<<nseg, kScanThreads, 0, st>>>(S.hist, m, S.part);
+        k_rs_scatter<<<ntiles, kRsThreads, 0, st>>>(S.keys[S.cur], S.vals[S.cur], S.keys[S.cur ^ 1],
+                                                    S.vals[S.cur ^ 1], n, shift, S.hist, ntiles);
+        S.cur ^= 1;
+    }
+    return 0;
+}
+
+}  // namespace rtx

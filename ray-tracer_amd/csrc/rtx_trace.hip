@@ -15,6 +15,7 @@
 
 #include <algorithm>
 
+#include "rtx_sort.hpp"
 #include "rtx_trace_impl.hpp"
 
 namespace rtx {
@@ -23,12 +24,14 @@ namespace {
 template <bool ANY, bool COUNT, int DEPTH, int LOOP>
 __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray32* __restrict__ rays, uint32_t n,
                                                          Hit16* __restrict__ hits, float* __restrict__ normals,
-                                                         uint8_t* __restrict__ occl, uint32_t* __restrict__ counts) {
+                                                         uint8_t* __restrict__ occl, uint32_t* __restrict__ counts,
+                                                         const uint32_t* __restrict__ perm) {
     __shared__ uint2 lst[DEPTH][kTraceThreads];
     const uint32_t tid = threadIdx.x;
-    const uint32_t i = xcd_block(blockIdx.x) * kTraceThreads + tid;
+    const uint32_t k = xcd_block(blockIdx.x) * kTraceThreads + tid;
     // out-of-range lanes still run (with a ray that misses) so wave-wide votes see them
-    const bool in = i < n;
+    const bool in = k < n;
+    const uint32_t i = in && perm ? perm[k] : k;   // work item k traces ray i
     float4 r0 = make_float4(0.f, 0.f, 0.f, 1.f), r1 = make_float4(0.f, 0.f, 1.f, 0.f);
     if (in) {
         r0 = reinterpret_cast<const float4*>(rays)[2 * i];
@@ -55,73 +58,51 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray3
     }
 }
 
-// Persistent variant (tuning trace_loop = 2): a fixed grid of waves; when at least
-// kRefill lanes of a wave are idle, one atomic on `work` hands them the next rays. Each
-// ray is still traversed exactly as trace_ray's while-while loop does.
-constexpr int kRefill = 16;
-
-template <bool ANY, bool COUNT, int DEPTH>
-__global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, const Ray32* __restrict__ rays,
-                                                                 uint32_t n, Hit16* __restrict__ hits,
-                                                                 float* __restrict__ normals,
-                                                                 uint8_t* __restrict__ occl,
-                                                                 uint32_t* __restrict__ counts, uint32_t* work) {
-    __shared__ uint2 lst[DEPTH][kTraceThreads];
-    const uint32_t tid = threadIdx.x;
-    const uint64_t lt = (1ull << __lane_id()) - 1ull;
-    Walker<DEPTH> w;
-    w.live = false;
-    w.best = kInvalid;
-    uint32_t id = kInvalid;
-    bool more = true;
-    for (;;) {
-        if (!w.live && id != kInvalid) {
-            if (ANY) {
-                occl[id] = w.best != kInvalid ? 1 : 0;
-            } else {
-                reinterpret_cast<float4*>(hits)[id] =
-                    make_float4(w.best != kInvalid ? w.bestT : 0.f, w.bu, w.bv, __uint_as_float(w.best));
-                if (COUNT) { counts[2 * id] = w.nI; counts[2 * id + 1] = w.nT; }
-                if (normals) {
-                    f3 nn{0.f, 0.f, 0.f};
-                    if (w.best != kInvalid) nn = hit_normal(A, w.best, w.bu, w.bv);
-                    normals[3 * id] = nn.x; normals[3 * id + 1] = nn.y; normals[3 * id + 2] = nn.z;
-                }
-            }
-            id = kInvalid;
-        }
-        const uint64_t idle = __ballot(!w.live);
-        if (more && __popcll(idle) >= kRefill) {
-            const int leader = __ffsll((unsigned long long)idle) - 1;
-            uint32_t base = 0;
-            if (__lane_id() == leader) base = atomicAdd(work, (uint32_t)__popcll(idle));
-            base = __shfl(base, leader);
-            if (base + __popcll(idle) >= n) more = false;
-            if (!w.live) {
-                const uint32_t k = base + __popcll(idle & lt);
-                if (k < n) {
-                    id = k;
-                    const float4 r0 = reinterpret_cast<const float4*>(rays)[2 * k];
-                    const float4 r1 = reinterpret_cast<const float4*>(rays)[2 * k + 1];
-                    w.start(A, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w);
-                    continue;   // a ray that misses the root is written at the loop top
-                }
-            }
-        }
-        if (!__any(w.live)) {
-            if (!more && !__any(id != kInvalid)) break;
-            continue;
-        }
-        while (w.live && !(w.ref & kLeafBit)) w.template node_step<COUNT>(A, lst, tid);
-        if (w.live) {
-            if (w.template leaf_step<ANY, COUNT>(A)) w.live = false;
-            else w.pop(lst, tid);
-        }
-    }
+// Ray order key: 24-bit Morton code of the origin (8 bits per axis inside the scene box)
+// above a 24-bit Morton code of the direction ((d + 1) / 2, 8 bits per axis). Rays with
+// nearby origins and directions then share nodes and triangles in cache and in a wave.
+__device__ __forceinline__ uint64_t spread8(uint32_t v) {
+    uint64_t r = 0;
+    for (int b = 0; b < 8; ++b) r |= (uint64_t)((v >> b) & 1u) << (3 * b);
+    return r;
+}
+__device__ __forceinline__ uint32_t q8(float x) {
+    float f = x * 255.0f;
+    return f > 0.f ? (f < 255.f ? (uint32_t)f : 255u) : 0u;
+}
+__global__ void k_ray_keys(TraceArgs A, const Ray32* __restrict__ rays, uint32_t n, uint64_t* keys, uint32_t* vals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 r0 = reinterpret_cast<const float4*>(rays)[2 * i];
+    const float4 r1 = reinterpret_cast<const float4*>(rays)[2 * i + 1];
+    const float sx = A.rb[3] - A.rb[0], sy = A.rb[4] - A.rb[1], sz = A.rb[5] - A.rb[2];
+    const uint64_t o = (spread8(q8(sx > 0.f ? (r0.x - A.rb[0]) / sx : 0.f)) << 2) |
+                       (spread8(q8(sy > 0.f ? (r0.y - A.rb[1]) / sy : 0.f)) << 1) |
+                       spread8(q8(sz > 0.f ? (r0.z - A.rb[2]) / sz : 0.f));
+    const uint64_t d = (spread8(q8((r1.x + 1.f) * 0.5f)) << 2) | (spread8(q8((r1.y + 1.f) * 0.5f)) << 1) |
+                       spread8(q8((r1.z + 1.f) * 0.5f));
+    keys[i] = (o << 24) | d;
+    vals[i] = i;
 }
 
-__global__ void k_zero_work(uint32_t* w) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *w = 0u;
+
+constexpr uint32_t kSortMinRays = 1u << 16;
+
+// Grow-only scratch of the ray sort, kept on the handle.
+bool reserve_sort(rtx_bvh* b, uint32_t n) {
+    if (b->rs_cap >= n) return true;
+    void* ps[] = {b->rs_keys[0], b->rs_keys[1], b->rs_vals[0], b->rs_vals[1], b->rs_hist, b->rs_part};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    b->rs_keys[0] = dalloc<uint64_t>(n);
+    b->rs_keys[1] = dalloc<uint64_t>(n);
+    b->rs_vals[0] = dalloc<uint32_t>(n);
+    b->rs_vals[1] = dalloc<uint32_t>(n);
+    b->rs_hist = dalloc<uint32_t>(sort_hist_entries(n));
+    b->rs_part = dalloc<uint32_t>(sort_part_entries(n));
+    const bool ok = b->rs_keys[0] && b->rs_keys[1] && b->rs_vals[0] && b->rs_vals[1] && b->rs_hist && b->rs_part;
+    b->rs_cap = ok ? n : 0;
+    return ok;
 }
 
 int check(const rtx_bvh* b, const void* rays, size_t n, const void* out, const char* fn) {
@@ -139,21 +120,20 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
     if (n == 0) return RTX_OK;
     uint32_t blocks = (uint32_t)((n + kTraceThreads - 1) / kTraceThreads);
     const TraceArgs A = make_trace_args(b);
-    if (g_tuning.trace_loop == 2 && b->work) {
-        const uint32_t pb = std::min<uint32_t>(blocks, g_tuning.grid_blocks);
-        k_zero_work<<<1, 64, 0, s>>>(b->work);
-        with_trace_variant([&](auto D, auto) {
-            k_trace_persist<ANY, COUNT, decltype(D)::value><<<pb, kTraceThreads, 0, s>>>(
-                A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
-                counts, b->work);
-        });
-        RTX_HIP(hipGetLastError());
-        return RTX_OK;
+    const uint32_t* perm = nullptr;
+    if (g_tuning.ray_sort && n >= kSortMinRays) {
+        rtx_bvh* mb = const_cast<rtx_bvh*>(b);
+        if (!reserve_sort(mb, (uint32_t)n)) return fail(RTX_E_HIP, "device allocation failed for ray sorting");
+        SortBuffers S{{mb->rs_keys[0], mb->rs_keys[1]}, {mb->rs_vals[0], mb->rs_vals[1]}, mb->rs_hist, mb->rs_part, 0};
+        k_ray_keys<<<blocks, kTraceThreads, 0, s>>>(A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, S.keys[0],
+                                                    S.vals[0]);
+        radix_sort_pairs(S, (uint32_t)n, 0x3Fu, s);
+        perm = S.vals[S.cur];
     }
     with_trace_variant([&](auto D, auto L) {
         k_trace<ANY, COUNT, decltype(D)::value, decltype(L)::value><<<blocks, kTraceThreads, 0, s>>>(
             A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
-            counts);
+            counts, perm);
     });
     RTX_HIP(hipGetLastError());
     return RTX_OK;
@@ -161,7 +141,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
 
 }  // namespace
 
-rtx_tuning g_tuning = {1u << 22, 2048, 256, 8, 1};
+rtx_tuning g_tuning = {1u << 22, 2048, 256, 8, 1, 0};
 
 }  // namespace rtx
 

@@ -105,7 +105,8 @@ def test_trace_1m_subset():
 
 
 def test_tuning_variants_give_identical_hits():
-    """Every launch-shape variant (LDS stack depth, loop structure) returns the same hits."""
+    """Every launch-shape variant (LDS stack depth, loop structure, ray sorting) returns the
+    same hits at each ray's own index."""
     import torch
     v, t = S.random_triangles(200_000, seed=5)
     g = rtx.build(v, t)
@@ -114,24 +115,53 @@ def test_tuning_variants_give_identical_hits():
     saved = rtx.get_tuning()
     ref = None
     try:
-        for depth in (8, 12, 16):
-            for loop in (0, 1, 2):
-                rtx.set_tuning(trace_lds_depth=depth, trace_loop=loop)
-                g.trace_closest_device(rays, hits)
-                torch.cuda.synchronize()
-                h = hits.view(torch.int32).cpu().numpy().copy()
-                if ref is None:
-                    ref = h
-                assert np.array_equal(h, ref), f"variant depth={depth} loop={loop} differs"
+        for sort in (0, 1):
+            for depth in (8, 12, 16):
+                for loop in (0, 1):
+                    rtx.set_tuning(trace_lds_depth=depth, trace_loop=loop, ray_sort=sort)
+                    hits.fill_(-1.0)
+                    g.trace_closest_device(rays, hits)
+                    torch.cuda.synchronize()
+                    h = hits.view(torch.int32).cpu().numpy().copy()
+                    if ref is None:
+                        ref = h
+                    assert np.array_equal(h, ref), f"variant depth={depth} loop={loop} sort={sort} differs"
     finally:
         rtx.set_tuning(**saved)
 
 
-def test_persistent_variant_counts_and_any_hit(suite):
-    """The persistent dynamic-fetch kernel also serves counts and any-hit exactly."""
+def test_ray_sort_counts_any_hit_and_normals(suite):
+    """With ray sorting on, closest hit, any-hit, counts and normals still land at each
+    ray's own index (70001 rays: above the sorting threshold, ragged tail)."""
     saved = rtx.get_tuning()
     try:
-        rtx.set_tuning(trace_loop=2, trace_lds_depth=8)
+        rtx.set_tuning(ray_sort=1)
+        for name in ("rand20000", "straddle", "cornell"):
+            v, t = suite[name]
+            g = rtx.build(v, t)
+            o = O.build(v, t)
+            rays = _rays_for(v, 70001, seed=11)
+            print(f"[ray_sort] {name}", flush=True)
+            h, nrm, occ, cnt = _device_trace(g, rays)
+            ref = O.trace(o, rays, counts=True, normals=True, nthreads=8)
+            assert np.array_equal(h[:, 3].view(np.uint32), ref["tri"]), name
+            assert np.array_equal(h[:, 0].view(np.uint32), ref["t"].view(np.uint32)), name
+            assert np.array_equal(occ.astype(bool), ref["tri"] != 0xFFFFFFFF), name
+            assert np.array_equal(cnt, ref["counts"]), name
+            np.testing.assert_allclose(nrm, ref["normal"], atol=1e-6)
+    finally:
+        rtx.set_tuning(**saved)
+
+
+@pytest.mark.skip(reason="persistent dynamic-fetch traversal (trace_loop 2) is disabled: a 4M-ray launch on "
+                         "the 1M-triangle tree did not finish; cause not yet found")
+def test_persistent_variant_counts_and_any_hit(suite):
+    """The persistent dynamic-fetch kernel serves closest hit, counts and any-hit exactly
+    when lanes take second rays (30001 rays per 118 blocks is one ray per lane; the
+    grid here is forced smaller so lanes refill)."""
+    saved = rtx.get_tuning()
+    try:
+        rtx.set_tuning(trace_loop=2, trace_lds_depth=8, grid_blocks=16)
         for name in ("rand20000", "straddle", "cornell"):
             v, t = suite[name]
             g = rtx.build(v, t)

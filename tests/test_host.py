@@ -192,3 +192,21 @@ def test_oracle_ibl_lookup():
     down = S.camera_lookat((0, 0, 0), (0, -1, 0.0001), 10.0)
     img = O.render(o, down, O.make_params(4, 4, 1, 0, 3, seed=1, env_mode=2, ibl=rows))
     assert (img == 3).all()                   # straight down: the bottom row
+
+
+def test_tuning_validation():
+    """rtx_set_tuning accepts only launch shapes the kernels support; a refused call
+    leaves the tuning unchanged. trace_loop 2 (persistent dynamic-fetch traversal) is
+    switched off: a launch of 4M rays on the 1M-triangle tree did not finish and no cause
+    has been found in the kernel, so the tuning call refuses it instead of running it."""
+    saved = rtx.get_tuning()
+    try:
+        for bad in (dict(trace_loop=2), dict(trace_loop=3), dict(trace_lds_depth=10), dict(shade_threads=96),
+                    dict(batch_paths=512), dict(grid_blocks=0), dict(ray_sort=2)):
+            with pytest.raises(rtx.RtxError):
+                rtx.set_tuning(**bad)
+            assert rtx.get_tuning() == saved, bad
+        rtx.set_tuning(ray_sort=1, trace_lds_depth=12, trace_loop=0)
+        assert rtx.get_tuning() == dict(saved, ray_sort=1, trace_lds_depth=12, trace_loop=0)
+    finally:
+        rtx.set_tuning(**saved)
