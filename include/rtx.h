@@ -186,8 +186,9 @@ void rtx_batch_free(rtx_batch* batch);
  * batch_paths: paths in flight per wavefront batch (>= 1024); grid_blocks: blocks of the
  * persistent trace/shade kernels; shade_threads: threads per shade block (multiple of 64);
  * trace_lds_depth: LDS stack entries per lane (8, 12 or 16); trace_loop: 0 = one
- * node/leaf loop, 1 = while-while (triangle tests run with the wave converged); the value
- * 2 (a persistent dynamic-fetch grid) is reserved and refused; ray_sort: 1 = the trace entry points
+ * node/leaf loop, 1 = while-while (triangle tests run with the wave converged), 2 = the
+ * while-while loop in a persistent grid of grid_blocks blocks where idle lanes fetch new
+ * rays (plain trace entry points; the renderer runs loop 1 for it); ray_sort: 1 = the trace entry points
  * first sort rays (>= 65536 of them) by a Morton key of origin and direction and trace in
  * that order, writing results back at each ray's index. The sort's scratch lives on the
  * BVH handle and grows to the largest ray count seen, so with ray_sort on, trace calls

@@ -1009,7 +1009,7 @@ struct Scratch {
 
 void free_products(rtx_bvh* b) {
     void* ps[] = {b->verts, b->nodes48, b->tris3, b->prims, b->gnodes, b->leaves, b->gtris, b->codes, b->sorted,
-                  b->bfs48, b->rs_keys[0], b->rs_keys[1], b->rs_vals[0], b->rs_vals[1], b->rs_hist,
+                  b->bfs48, b->work, b->rs_keys[0], b->rs_keys[1], b->rs_vals[0], b->rs_vals[1], b->rs_hist,
                   b->rs_part};
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -1145,7 +1145,8 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
     B->gnodes = dalloc<GNode>(std::max<uint32_t>(B->ninternal, 1));
     B->leaves = dalloc<Leaf>(B->nleaves);
     B->gtris = dalloc<GTri>(B->ntri);
-    if (!B->nodes48 || !B->bfs48 || !B->tris3 || !B->prims || !B->gnodes || !B->leaves || !B->gtris)
+    B->work = dalloc<uint32_t>(64);
+    if (!B->nodes48 || !B->bfs48 || !B->tris3 || !B->prims || !B->gnodes || !B->leaves || !B->gtris || !B->work)
         return fail(RTX_E_HIP, "device allocation failed on rtx_build()");
     Out O{B->nodes48, B->bfs48, B->tris3, B->prims, B->gnodes, B->leaves, B->gtris};
     k_emit<<<nblk(total, 128), 128, 0, st>>>(C, total, D, O);
@@ -1189,9 +1190,10 @@ int upload_reference_tree(rtx_bvh* B, const Node48* nodes, uint32_t nn, const ui
     B->gnodes = dalloc<GNode>(std::max<uint32_t>(ni, 1));
     B->leaves = dalloc<Leaf>(std::max<uint32_t>(nl, 1));
     B->gtris = dalloc<GTri>(std::max<uint32_t>(ntri, 1));
+    B->work = dalloc<uint32_t>(64);
     uint32_t* d_i = dalloc<uint32_t>(nn);
     uint32_t* d_l = dalloc<uint32_t>(nn);
-    if (!B->nodes48 || !B->tris3 || !B->prims || !B->verts || !B->gnodes || !B->leaves || !B->gtris ||
+    if (!B->nodes48 || !B->tris3 || !B->prims || !B->verts || !B->gnodes || !B->leaves || !B->gtris || !B->work ||
         !d_i || !d_l)
         return fail(RTX_E_HIP, "device allocation failed on rtx_load()");
     std::vector<uint32_t> prims(ntri);

@@ -44,6 +44,7 @@ struct rtx_bvh {
     rtx::GTri* gtris = nullptr;
     float root_box[6] = {0, 0, 0, 0, 0, 0};
     uint32_t root_ref = 0;
+    uint32_t* work = nullptr;         // ray counter of the persistent trace kernel
     // ray-sort scratch (grow-only)
     uint64_t* rs_keys[2] = {nullptr, nullptr};
     uint32_t* rs_vals[2] = {nullptr, nullptr};

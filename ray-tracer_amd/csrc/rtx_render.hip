@@ -555,8 +555,6 @@ extern "C" int rtx_set_tuning(const rtx_tuning* t) {
         t->shade_threads > 1024 || !(t->trace_lds_depth == 8 || t->trace_lds_depth == 12 || t->trace_lds_depth == 16) ||
         t->trace_loop > 2 || t->ray_sort > 1)
         return fail(RTX_E_ARG, "bad tuning on rtx_set_tuning()");
-    if (t->trace_loop == 2)
-        return fail(RTX_E_ARG, "trace_loop 2 (persistent dynamic-fetch traversal) is disabled on rtx_set_tuning()");
     g_tuning = *t;
     return RTX_OK;
 }

@@ -58,6 +58,78 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray3
     }
 }
 
+// Persistent variant (tuning trace_loop = 2): a fixed grid of waves; when at least
+// kRefill lanes of a wave are idle, one atomic on `work` hands them the next rays, so a
+// wave no longer waits for its slowest ray. Every wave-level operation (ballot, leader
+// election, shuffle, the exit test) sits in control flow that is uniform across the wave:
+// no lane leaves the loop body early with `continue`, because the compiler may then run
+// those lanes' next iteration on their own, and a refill seen by only part of the wave
+// would split `more` and could elect a leader that never issues the atomic.
+constexpr int kRefill = 16;
+
+template <bool ANY, bool COUNT, int DEPTH>
+__global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, const Ray32* __restrict__ rays,
+                                                                 uint32_t n, Hit16* __restrict__ hits,
+                                                                 float* __restrict__ normals,
+                                                                 uint8_t* __restrict__ occl,
+                                                                 uint32_t* __restrict__ counts, uint32_t* work,
+                                                                 const uint32_t* __restrict__ perm) {
+    __shared__ uint2 lst[DEPTH][kTraceThreads];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t lt = (1ull << __lane_id()) - 1ull;
+    Walker<DEPTH> w;
+    w.live = false;
+    w.best = kInvalid;
+    uint32_t id = kInvalid;
+    bool more = true;   // the same on every lane: set only from wave-wide values
+    for (;;) {
+        if (!w.live && id != kInvalid) {
+            if (ANY) {
+                occl[id] = w.best != kInvalid ? 1 : 0;
+            } else {
+                reinterpret_cast<float4*>(hits)[id] =
+                    make_float4(w.best != kInvalid ? w.bestT : 0.f, w.bu, w.bv, __uint_as_float(w.best));
+                if (COUNT) { counts[2 * id] = w.nI; counts[2 * id + 1] = w.nT; }
+                if (normals) {
+                    f3 nn{0.f, 0.f, 0.f};
+                    if (w.best != kInvalid) nn = hit_normal(A, w.best, w.bu, w.bv);
+                    normals[3 * id] = nn.x; normals[3 * id + 1] = nn.y; normals[3 * id + 2] = nn.z;
+                }
+            }
+            id = kInvalid;
+        }
+        const uint64_t idle = __ballot(!w.live);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (more && nidle >= (uint32_t)kRefill) {
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if (__lane_id() == leader) base = atomicAdd(work, nidle);
+            base = __shfl(base, leader);
+            more = base + nidle < n;
+            if (!w.live) {
+                const uint32_t k = base + (uint32_t)__popcll(idle & lt);
+                if (k < n) {
+                    id = perm ? perm[k] : k;
+                    const float4 r0 = reinterpret_cast<const float4*>(rays)[2 * id];
+                    const float4 r1 = reinterpret_cast<const float4*>(rays)[2 * id + 1];
+                    w.start(A, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w);
+                }
+            }
+        }
+        // uniform exit test: a started ray that missed the root is written at the top
+        if (!__any(w.live || id != kInvalid) && !more) break;
+        while (w.live && !(w.ref & kLeafBit)) w.template node_step<COUNT>(A, lst, tid);
+        if (w.live) {
+            if (w.template leaf_step<ANY, COUNT>(A)) w.live = false;
+            else w.pop(lst, tid);
+        }
+    }
+}
+
+__global__ void k_zero_work(uint32_t* w) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *w = 0u;
+}
+
 // Ray order key: 24-bit Morton code of the origin (8 bits per axis inside the scene box)
 // above a 24-bit Morton code of the direction ((d + 1) / 2, 8 bits per axis). Rays with
 // nearby origins and directions then share nodes and triangles in cache and in a wave.
@@ -129,6 +201,17 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
                                                     S.vals[0]);
         radix_sort_pairs(S, (uint32_t)n, 0x3Fu, s);
         perm = S.vals[S.cur];
+    }
+    if (g_tuning.trace_loop == 2) {
+        const uint32_t pb = std::min<uint32_t>(blocks, g_tuning.grid_blocks);
+        k_zero_work<<<1, 64, 0, s>>>(b->work);
+        with_trace_variant([&](auto D, auto) {
+            k_trace_persist<ANY, COUNT, decltype(D)::value><<<pb, kTraceThreads, 0, s>>>(
+                A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
+                counts, b->work, perm);
+        });
+        RTX_HIP(hipGetLastError());
+        return RTX_OK;
     }
     with_trace_variant([&](auto D, auto L) {
         k_trace<ANY, COUNT, decltype(D)::value, decltype(L)::value><<<blocks, kTraceThreads, 0, s>>>(

@@ -31,6 +31,8 @@ void with_trace_variant(F&& f) {
     using I16 = std::integral_constant<int, 16>;
     using L0 = std::integral_constant<int, 0>;
     using L1 = std::integral_constant<int, 1>;
+    // loop 2 (persistent, dynamic fetch) exists for the plain trace only; elsewhere it
+    // runs as the while-while loop it is built on
     const uint32_t d = g_tuning.trace_lds_depth, l = g_tuning.trace_loop;
     if (d == 8) { if (l) f(I8{}, L1{}); else f(I8{}, L0{}); }
     else if (d == 12) { if (l) f(I12{}, L1{}); else f(I12{}, L0{}); }

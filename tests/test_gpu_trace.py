@@ -117,7 +117,7 @@ def test_tuning_variants_give_identical_hits():
     try:
         for sort in (0, 1):
             for depth in (8, 12, 16):
-                for loop in (0, 1):
+                for loop in (0, 1, 2):
                     rtx.set_tuning(trace_lds_depth=depth, trace_loop=loop, ray_sort=sort)
                     hits.fill_(-1.0)
                     g.trace_closest_device(rays, hits)
@@ -153,26 +153,33 @@ def test_ray_sort_counts_any_hit_and_normals(suite):
         rtx.set_tuning(**saved)
 
 
-@pytest.mark.skip(reason="persistent dynamic-fetch traversal (trace_loop 2) is disabled: a 4M-ray launch on "
-                         "the 1M-triangle tree did not finish; cause not yet found")
 def test_persistent_variant_counts_and_any_hit(suite):
-    """The persistent dynamic-fetch kernel serves closest hit, counts and any-hit exactly
-    when lanes take second rays (30001 rays per 118 blocks is one ray per lane; the
-    grid here is forced smaller so lanes refill)."""
+    """The persistent dynamic-fetch kernel serves closest hit, counts, normals and any-hit
+    exactly while lanes take many rays each (grid forced to 16 blocks). The camera rays
+    mostly miss the root box, so whole groups of lanes go idle the moment they start a
+    ray; the kernel's refill and exit must stay wave-uniform for them."""
     saved = rtx.get_tuning()
     try:
-        rtx.set_tuning(trace_loop=2, trace_lds_depth=8, grid_blocks=16)
-        for name in ("rand20000", "straddle", "cornell"):
+        for name, sort in (("rand20000", 0), ("straddle", 0), ("cornell", 0), ("rand20000", 1)):
+            rtx.set_tuning(trace_loop=2, trace_lds_depth=8, grid_blocks=16, ray_sort=sort)
             v, t = suite[name]
             g = rtx.build(v, t)
             o = O.build(v, t)
-            rays = _rays_for(v, 30001, seed=11)
-            h, nrm, occ, cnt = _device_trace(g, rays)
-            ref = O.trace(o, rays, counts=True, normals=True, nthreads=8)
-            assert np.array_equal(h[:, 3].view(np.uint32), ref["tri"]), name
-            assert np.array_equal(h[:, 0].view(np.uint32), ref["t"].view(np.uint32)), name
-            assert np.array_equal(occ.astype(bool), ref["tri"] != 0xFFFFFFFF), name
-            assert np.array_equal(cnt, ref["counts"]), name
-            np.testing.assert_allclose(nrm, ref["normal"], atol=1e-6)
+            lo, hi = v[:, :3].min(0), v[:, :3].max(0)
+            c = (lo + hi) / 2
+            span = float((hi - lo).max())
+            cam = S.camera_lookat(tuple(c - np.array([0.0, 0.0, 6.0 * span])), tuple(c), 60.0)
+            camrays = S.pinhole_rays(cam, 256, 256)
+            for kind, rays in (("scattered", _rays_for(v, 30001, seed=11)), ("camera", camrays)):
+                print(f"[persistent] {name} sort={sort} {kind}", flush=True)
+                h, nrm, occ, cnt = _device_trace(g, rays)
+                ref = O.trace(o, rays, counts=True, normals=True, nthreads=8)
+                if kind == "camera":
+                    assert (ref["tri"] == 0xFFFFFFFF).mean() > 0.5
+                assert np.array_equal(h[:, 3].view(np.uint32), ref["tri"]), (name, kind)
+                assert np.array_equal(h[:, 0].view(np.uint32), ref["t"].view(np.uint32)), (name, kind)
+                assert np.array_equal(occ.astype(bool), ref["tri"] != 0xFFFFFFFF), (name, kind)
+                assert np.array_equal(cnt, ref["counts"]), (name, kind)
+                np.testing.assert_allclose(nrm, ref["normal"], atol=1e-6)
     finally:
         rtx.set_tuning(**saved)

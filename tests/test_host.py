@@ -196,12 +196,10 @@ def test_oracle_ibl_lookup():
 
 def test_tuning_validation():
     """rtx_set_tuning accepts only launch shapes the kernels support; a refused call
-    leaves the tuning unchanged. trace_loop 2 (persistent dynamic-fetch traversal) is
-    switched off: a launch of 4M rays on the 1M-triangle tree did not finish and no cause
-    has been found in the kernel, so the tuning call refuses it instead of running it."""
+    leaves the tuning unchanged."""
     saved = rtx.get_tuning()
     try:
-        for bad in (dict(trace_loop=2), dict(trace_loop=3), dict(trace_lds_depth=10), dict(shade_threads=96),
+        for bad in (dict(trace_loop=3), dict(trace_lds_depth=10), dict(shade_threads=96),
                     dict(batch_paths=512), dict(grid_blocks=0), dict(ray_sort=2)):
             with pytest.raises(rtx.RtxError):
                 rtx.set_tuning(**bad)
