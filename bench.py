@@ -50,7 +50,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (the reference point that scales HBM "
                          "counters under the profiler)")
-    ap.add_argument("--autotune", action="store_true", help="tune launch shapes on this GPU before timing")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time with the library's default launch shapes instead of tuning them here first")
     ap.add_argument("--skip-extras", action="store_true",
                     help="time only the headline launch (used under the profiler so k_trace's average is "
                          "the incoherent launch alone)")
@@ -64,6 +65,7 @@ def main():
     import torch
 
     import rtx
+    from rtx import autotune
     from rtx import scenes as S
 
     dev = torch.device("cuda:0")
@@ -103,20 +105,33 @@ def main():
         torch.cuda.synchronize()
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
         t_wall = time.perf_counter()
-        for a, b in evs:
-            a.record(stream)
-            fn()
-            b.record(stream)
+        with rtx.KernelTimer() as kt:
+            for a, b in evs:
+                a.record(stream)
+                fn()
+                b.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t_wall
         per = [a.elapsed_time(b) for a, b in evs]
-        return float(np.mean(per)), wall
+        kernel_ms = kt.ms / kt.launches if kt.launches else float("nan")
+        return float(np.mean(per)), wall, kernel_ms
 
-    ms_inc, wall = timed(lambda: bvh.trace_closest_device(inc, hits, stream=stream), args.steps, args.warmup)
+    # Launch shapes for the headline: the project autotuner (coordinate descent over LDS
+    # stack depth, loop structure, ray sorting and persistent grid size) on this GPU.
+    trace_tuning, trace_tune_s = rtx.get_tuning(), 0.0
+    if not args.no_autotune:
+        t_tune = time.perf_counter()
+        trace_tuning, _, _ = autotune.tune(lambda: (lambda: bvh.trace_closest_device(inc, hits, stream=stream)),
+                                           keys=["trace_loop", "trace_lds_depth", "ray_sort", "grid_blocks"], reps=3)
+        trace_tune_s = time.perf_counter() - t_tune
+    rtx.set_tuning(**trace_tuning)
+
+    ms_inc, wall, kms_inc = timed(lambda: bvh.trace_closest_device(inc, hits, stream=stream), args.steps,
+                                  args.warmup)
     ms_coh = ms_any = float("nan")
     if not args.skip_extras:
-        ms_coh, _ = timed(lambda: bvh.trace_closest_device(coh, hits, stream=stream), max(3, args.steps // 4), 1)
-        ms_any, _ = timed(lambda: bvh.trace_any_device(inc, occ, stream=stream), max(3, args.steps // 4), 1)
+        ms_coh, _, _ = timed(lambda: bvh.trace_closest_device(coh, hits, stream=stream), max(3, args.steps // 4), 1)
+        ms_any, _, _ = timed(lambda: bvh.trace_any_device(inc, occ, stream=stream), max(3, args.steps // 4), 1)
 
     # Algorithmic bytes of the incoherent launch: 32 B ray + 16 B hit + 64 B per internal
     # node visited + 48 B per triangle tested, counts from the instrumented kernel.
@@ -125,7 +140,9 @@ def main():
     c = counts.to(torch.int64).sum(0).cpu().numpy()
     visits, tests = int(c[0]), int(c[1])
     alg_bytes = 48 * n + 64 * visits + 48 * tests
-    achieved = alg_bytes / (ms_inc * 1e-3) / 1e9
+    # roofline of the dominant kernel: its own average duration (HIP events around the
+    # traversal kernel on the launch stream; the ray sort before it is not included)
+    achieved = alg_bytes / (kms_inc * 1e-3) / 1e9
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", "trace_closest_c2_traffic.json")
@@ -141,7 +158,8 @@ def main():
     if not (args.no_render or args.skip_extras):
         del inc, coh, hits, occ, counts
         torch.cuda.empty_cache()
-        render = {"differentiable": bench_differentiable(stream), "batched": bench_batched(stream, args.c3_scenes)}
+        batched = bench_batched(stream, args.c3_scenes, tune=not args.no_autotune)
+        render = {"differentiable": bench_differentiable(stream), "batched": batched}
 
     mrays = n / (ms_inc * 1e-3) / 1e6
     out = {
@@ -172,6 +190,10 @@ def main():
             "avg_internal_visits_per_ray": round(visits / n, 2),
             "avg_triangle_tests_per_ray": round(tests / n, 2),
             "algorithmic_bytes_per_launch": alg_bytes,
+            "trace_kernel_ms": round(kms_inc, 4),
+            "ray_sort_and_launch_ms": round(ms_inc - kms_inc, 4),
+            "tuning": trace_tuning,
+            "autotune": None if args.no_autotune else {"seconds": round(trace_tune_s, 2)},
             "timed_wall_s": round(wall, 3),
         },
         "render": render,
@@ -226,13 +248,16 @@ def bench_differentiable(stream, reps=3):
             "paths": 512 * 512 * 32}
 
 
-def bench_batched(stream, nscenes, reps=2):
+def bench_batched(stream, nscenes, reps=2, tune=True):
     """Batched wavefront render of fBm-displaced spheres of uneven size (seed 2): one
     launch sequence per scene (plain) vs all scenes in shared launches replayed from one
-    hipGraph (batched)."""
+    hipGraph (batched). The queue size (paths per wavefront batch), block sizes, grid and
+    stack depth are first chosen by the autotuner on the batched launch; both sides are
+    then timed with that tuning."""
     import torch
 
     import rtx
+    from rtx import autotune
     from rtx import scenes as S
     sc = S.batched_scenes(nscenes, seed=2)
     bvhs = [rtx.build(s["verts"], s["tris"]) for s in sc]
@@ -242,6 +267,18 @@ def bench_batched(stream, nscenes, reps=2):
     paths = sum(s["width"] * s["height"] * s["spp"] for s in sc)
     out = {"workload": f"{nscenes} scenes, {sum(len(s['tris']) for s in sc)} triangles, {paths} paths, "
                        "max_bounce 6", "paths": paths}
+    if tune:
+        cams = [s["cam"] for s in sc]
+
+        def make_step():
+            b = rtx.Batch(bvhs, cams, params, [None] * nscenes, [None] * nscenes, imgs, 1)
+            return lambda: b.launch(stream)
+        t0 = time.perf_counter()
+        best, _, hist = autotune.tune(make_step, keys=["batch_paths", "shade_threads", "grid_blocks",
+                                                       "trace_lds_depth"], reps=1)
+        rtx.set_tuning(**best)
+        out["autotune"] = {"seconds": round(time.perf_counter() - t0, 2), "candidates": len(hist)}
+    out["tuning"] = rtx.get_tuning()
     for mode, name in ((0, "plain"), (1, "batched_graph")):
         b = rtx.Batch(bvhs, [s["cam"] for s in sc], params, [None] * nscenes, [None] * nscenes, imgs, mode)
         ms = _cuda_ms(lambda: b.launch(stream), reps, stream)

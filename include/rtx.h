@@ -204,6 +204,15 @@ typedef struct rtx_tuning {
 int rtx_set_tuning(const rtx_tuning* t);
 int rtx_get_tuning(rtx_tuning* t);
 
+/* Kernel timing for measurement tools (no counterpart in the reference). Between
+ * rtx_profile_begin() and rtx_profile_end(), every trace entry point records a HIP event
+ * pair on the caller's stream around its traversal kernel alone (ray sorting excluded).
+ * rtx_profile_end waits for the last pair and returns the summed kernel milliseconds and
+ * the number of launches, then stops recording. Process-wide; calls from several threads
+ * in that window are serialized. */
+int rtx_profile_begin(void);
+int rtx_profile_end(double* kernel_ms, uint32_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -920,6 +921,17 @@ __device__ __forceinline__ Node48 to48(const BNode& n, uint32_t a, uint32_t b) {
     return o;
 }
 
+// Sets *flag when any node box coordinate is not finite (the traversal then keeps the
+// NaN-exact slab test for every ray).
+__global__ void k_boxes_nonfinite(const Node48* nodes, uint32_t nn, uint32_t* flag) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const Node48& n = nodes[i];
+    bool ok = true;
+    for (int k = 0; k < 3; ++k) ok = ok && __builtin_isfinite(n.bmin[k]) && __builtin_isfinite(n.bmax[k]);
+    if (!ok) *flag = 1u;
+}
+
 __global__ void k_emit(Ctx C, uint32_t total, Dfs D, Out O) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
@@ -1150,9 +1162,14 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
         return fail(RTX_E_HIP, "device allocation failed on rtx_build()");
     Out O{B->nodes48, B->bfs48, B->tris3, B->prims, B->gnodes, B->leaves, B->gtris};
     k_emit<<<nblk(total, 128), 128, 0, st>>>(C, total, D, O);
+    RTX_HIP(hipMemsetAsync(B->work + 1, 0, sizeof(uint32_t), st));
+    k_boxes_nonfinite<<<nblk(total, 256), 256, 0, st>>>(B->nodes48, total, B->work + 1);
     BNode root;
+    uint32_t nonfinite = 1;
     RTX_HIP(hipMemcpyAsync(&root, nodes, sizeof(root), hipMemcpyDeviceToHost, st));
+    RTX_HIP(hipMemcpyAsync(&nonfinite, B->work + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     RTX_HIP(hipStreamSynchronize(st));
+    B->boxes_finite = nonfinite == 0;
     B->root_box[0] = root.lo.x; B->root_box[1] = root.lo.y; B->root_box[2] = root.lo.z;
     B->root_box[3] = root.hi.x; B->root_box[4] = root.hi.y; B->root_box[5] = root.hi.z;
     B->root_ref = (root.flags & F_LEAF) ? kLeafBit : 0u;
@@ -1214,6 +1231,10 @@ int upload_reference_tree(rtx_bvh* B, const Node48* nodes, uint32_t nn, const ui
     if (!ok) return RTX_E_HIP;
     for (int k = 0; k < 3; ++k) { B->root_box[k] = nodes[0].bmin[k]; B->root_box[3 + k] = nodes[0].bmax[k]; }
     B->root_ref = nodes[0].is_leaf ? kLeafBit : 0u;
+    B->boxes_finite = true;
+    for (uint32_t i = 0; i < nn; ++i)
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(nodes[i].bmin[k]) || !std::isfinite(nodes[i].bmax[k])) B->boxes_finite = false;
     // Depth by a host walk; every node must be reached exactly once (a tree, no cycles).
     std::vector<uint8_t> seen(nn, 0);
     std::vector<std::pair<uint32_t, uint32_t>> st{{0u, 0u}};

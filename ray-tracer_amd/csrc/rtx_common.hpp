@@ -129,6 +129,21 @@ RTX_HD float slab(float ox, float oy, float oz, float ix, float iy, float iz, fl
     return tn <= tf ? tn : kInf;
 }
 
+// slab() with single-instruction min/max. The two agree whenever no t is NaN: every min
+// and max then returns the same value (a -0 against +0 tie may flip the zero's sign, which
+// no comparison on the result can see). NaN needs 0 * inf, so the caller takes this form
+// only for rays with finite, nonzero inverse directions, a finite origin, non-NaN
+// tmin/tmax, and a tree whose boxes are all finite.
+RTX_HD float slab_fast(float ox, float oy, float oz, float ix, float iy, float iz, float tmin,
+                       float x0, float y0, float z0, float x1, float y1, float z1, float tmax) {
+    float t0x = (x0 - ox) * ix, t1x = (x1 - ox) * ix;
+    float t0y = (y0 - oy) * iy, t1y = (y1 - oy) * iy;
+    float t0z = (z0 - oz) * iz, t1z = (z1 - oz) * iz;
+    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+    return tn <= tf ? tn : kInf;
+}
+
 // Moller-Trumbore with back-face culling and no epsilons. Returns t (kInf on a miss).
 RTX_HD float tri_test(f3 o, f3 d, float tmin, float tmax, f3 v0, f3 e1, f3 e2, float& u_out, float& v_out) {
     f3 h = cross(d, e2);

@@ -14,6 +14,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <utility>
+#include <vector>
 
 #include "rtx_sort.hpp"
 #include "rtx_trace_impl.hpp"
@@ -78,6 +81,8 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, co
     const uint32_t tid = threadIdx.x;
     const uint64_t lt = (1ull << __lane_id()) - 1ull;
     Walker<DEPTH> w;
+    uint2 spill[kSpillDepth];
+    w.spill = spill;
     w.live = false;
     w.best = kInvalid;
     uint32_t id = kInvalid;
@@ -160,6 +165,31 @@ __global__ void k_ray_keys(TraceArgs A, const Ray32* __restrict__ rays, uint32_t
 
 constexpr uint32_t kSortMinRays = 1u << 16;
 
+// Event pairs of rtx_profile_begin/end.
+struct Profile {
+    std::mutex mu;
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+Profile g_prof;
+
+// Records the start event of a traversal kernel when profiling; returns the pair index.
+int prof_start(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    if (!g_prof.on) return -1;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess) return -1;
+    if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return -1; }
+    (void)hipEventRecord(a, s);
+    g_prof.ev.emplace_back(a, b);
+    return (int)g_prof.ev.size() - 1;
+}
+void prof_stop(int k, hipStream_t s) {
+    if (k < 0) return;
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    (void)hipEventRecord(g_prof.ev[k].second, s);
+}
+
 // Grow-only scratch of the ray sort, kept on the handle.
 bool reserve_sort(rtx_bvh* b, uint32_t n) {
     if (b->rs_cap >= n) return true;
@@ -205,30 +235,63 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
     if (g_tuning.trace_loop == 2) {
         const uint32_t pb = std::min<uint32_t>(blocks, g_tuning.grid_blocks);
         k_zero_work<<<1, 64, 0, s>>>(b->work);
+        const int pk = prof_start(s);
         with_trace_variant([&](auto D, auto) {
             k_trace_persist<ANY, COUNT, decltype(D)::value><<<pb, kTraceThreads, 0, s>>>(
                 A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
                 counts, b->work, perm);
         });
+        prof_stop(pk, s);
         RTX_HIP(hipGetLastError());
         return RTX_OK;
     }
+    const int pk = prof_start(s);
     with_trace_variant([&](auto D, auto L) {
         k_trace<ANY, COUNT, decltype(D)::value, decltype(L)::value><<<blocks, kTraceThreads, 0, s>>>(
             A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
             counts, perm);
     });
+    prof_stop(pk, s);
     RTX_HIP(hipGetLastError());
     return RTX_OK;
 }
 
 }  // namespace
 
-rtx_tuning g_tuning = {1u << 22, 2048, 256, 8, 1, 0};
+rtx_tuning g_tuning = {1u << 22, 2048, 256, 8, 2, 1};
 
 }  // namespace rtx
 
 using namespace rtx;
+
+extern "C" int rtx_profile_begin(void) {
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    for (auto& e : g_prof.ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    g_prof.ev.clear();
+    g_prof.on = true;
+    return RTX_OK;
+}
+
+extern "C" int rtx_profile_end(double* kernel_ms, uint32_t* launches) {
+    if (!kernel_ms || !launches) return fail(RTX_E_ARG, "null argument on rtx_profile_end()");
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    g_prof.on = false;
+    double total = 0.0;
+    int rc = RTX_OK;
+    for (auto& e : g_prof.ev) {
+        float ms = 0.f;
+        if (rc == RTX_OK && !(hip_ok(hipEventSynchronize(e.second), "hipEventSynchronize") &&
+                              hip_ok(hipEventElapsedTime(&ms, e.first, e.second), "hipEventElapsedTime")))
+            rc = RTX_E_HIP;
+        total += ms;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    *kernel_ms = total;
+    *launches = (uint32_t)g_prof.ev.size();
+    g_prof.ev.clear();
+    return rc;
+}
 
 extern "C" int rtx_trace_closest(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float* normals,
                                  void* stream) {

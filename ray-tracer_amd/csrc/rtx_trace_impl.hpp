@@ -19,6 +19,7 @@ struct TraceArgs {
     const float* verts;
     float rb[6];
     uint32_t root_ref;
+    uint32_t finite;    // every node box coordinate is finite
 };
 
 extern rtx_tuning g_tuning;
@@ -51,6 +52,7 @@ inline TraceArgs make_trace_args(const rtx_bvh* b) {
     A.verts = b->verts;
     for (int k = 0; k < 6; ++k) A.rb[k] = b->root_box[k];
     A.root_ref = b->root_ref;
+    A.finite = b->boxes_finite ? 1u : 0u;
     return A;
 }
 
@@ -75,8 +77,9 @@ struct Walker {
     float ix, iy, iz, tmin, tmax;
     float bestT, bu, bv;
     uint32_t best, ref, top, lo, nI, nT;
-    bool live;
-    uint2 spill[kSpillDepth];
+    bool live, fast;
+    uint2* spill;   // kSpillDepth private entries; kept outside the struct so that the
+                    // fields above stay in registers
 
     __device__ __forceinline__ uint32_t slot(uint32_t e) const {
         return DEPTH == 16 || DEPTH == 8 ? (e & (DEPTH - 1)) : (e % DEPTH);
@@ -87,6 +90,9 @@ struct Walker {
         bestT = tmax; bu = 0.f; bv = 0.f;
         best = kInvalid; top = 0; lo = 0; nI = 0; nT = 0;
         ref = A.root_ref;
+        fast = A.finite && __builtin_isfinite(ix) && __builtin_isfinite(iy) && __builtin_isfinite(iz) &&
+               ix != 0.f && iy != 0.f && iz != 0.f && __builtin_isfinite(o.x) && __builtin_isfinite(o.y) &&
+               __builtin_isfinite(o.z) && !__builtin_isnan(tmin) && !__builtin_isnan(tmax);
         live = slab(o.x, o.y, o.z, ix, iy, iz, tmin, A.rb[0], A.rb[1], A.rb[2], A.rb[3], A.rb[4], A.rb[5], bestT) < bestT;
     }
     __device__ __forceinline__ void push(uint2 (*lst)[kTraceThreads], uint32_t tid, uint32_t r, float t) {
@@ -112,8 +118,14 @@ struct Walker {
         const GNode& g = A.nodes[ref];
         const float4 l0 = g.l0, l1 = g.l1, r1b = g.r1;
         const uint2 refs = *reinterpret_cast<const uint2*>(&g.lref);
-        const float tl = slab(o.x, o.y, o.z, ix, iy, iz, tmin, l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, bestT);
-        const float tr = slab(o.x, o.y, o.z, ix, iy, iz, tmin, l1.z, l1.w, r1b.x, r1b.y, r1b.z, r1b.w, bestT);
+        float tl, tr;
+        if (__all(fast)) {   // wave vote: a scalar branch, both forms give the same values
+            tl = slab_fast(o.x, o.y, o.z, ix, iy, iz, tmin, l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, bestT);
+            tr = slab_fast(o.x, o.y, o.z, ix, iy, iz, tmin, l1.z, l1.w, r1b.x, r1b.y, r1b.z, r1b.w, bestT);
+        } else {
+            tl = slab(o.x, o.y, o.z, ix, iy, iz, tmin, l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, bestT);
+            tr = slab(o.x, o.y, o.z, ix, iy, iz, tmin, l1.z, l1.w, r1b.x, r1b.y, r1b.z, r1b.w, bestT);
+        }
         uint32_t nearRef, farRef;
         float tn, tf;
         if (tl <= tr) { nearRef = refs.x; tn = tl; farRef = refs.y; tf = tr; }
@@ -151,6 +163,8 @@ template <bool ANY, bool COUNT, int DEPTH, int LOOP>
 __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, float tmin, float tmax,
                                               uint2 (*lst)[kTraceThreads], uint32_t tid) {
     Walker<DEPTH> w;
+    uint2 spill[kSpillDepth];
+    w.spill = spill;
     w.start(A, o, d, tmin, tmax);
     if (LOOP == 0) {
         while (w.live) {

@@ -24,7 +24,8 @@ EXPORTS = ["rtx_last_error", "rtx_device_count", "rtx_build", "rtx_build_device"
            "rtx_trace_any", "rtx_trace_counts", "rtx_trace_closest_host", "rtx_save", "rtx_load",
            "rtx_scene_load", "rtx_free_host", "rtx_bake_scene", "rtx_geo_sphere", "rtx_geo_box", "rtx_render",
            "rtx_render_adjoint", "rtx_render_batched", "rtx_batch_create", "rtx_batch_launch", "rtx_batch_free",
-           "rtx_set_tuning", "rtx_get_tuning", "rtx_hdr_write", "rtx_hdr_read"]
+           "rtx_set_tuning", "rtx_get_tuning", "rtx_hdr_write", "rtx_hdr_read", "rtx_profile_begin",
+           "rtx_profile_end"]
 
 
 class RtxError(RuntimeError):
@@ -105,6 +106,8 @@ def lib():
         L.rtx_hdr_read.argtypes = [C.c_char_p, C.POINTER(i32), C.POINTER(i32), C.POINTER(C.POINTER(C.c_float))]
         L.rtx_set_tuning.argtypes = [C.POINTER(Tuning)]
         L.rtx_get_tuning.argtypes = [C.POINTER(Tuning)]
+        L.rtx_profile_begin.argtypes = []
+        L.rtx_profile_end.argtypes = [C.POINTER(C.c_double), C.POINTER(u32)]
         _LIB = L
     return _LIB
 
@@ -315,6 +318,23 @@ def get_tuning():
     t = Tuning()
     _check(lib().rtx_get_tuning(C.byref(t)), "rtx_get_tuning")
     return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+
+class KernelTimer:
+    """Times the traversal kernels of every trace call made inside the `with` block (HIP
+    events on the caller's stream around the kernel alone, ray sorting excluded).
+    After the block: .ms (summed kernel milliseconds) and .launches."""
+
+    def __enter__(self):
+        _check(lib().rtx_profile_begin(), "rtx_profile_begin")
+        self.ms, self.launches = 0.0, 0
+        return self
+
+    def __exit__(self, *exc):
+        ms, k = C.c_double(), C.c_uint32()
+        _check(lib().rtx_profile_end(C.byref(ms), C.byref(k)), "rtx_profile_end")
+        self.ms, self.launches = ms.value, k.value
+        return False
 
 
 def set_tuning(**kw):

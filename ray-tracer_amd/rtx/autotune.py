@@ -16,7 +16,7 @@ CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 SPACE = {
     "trace_lds_depth": [8, 12, 16],
-    "trace_loop": [0, 1, 2],
+    "trace_loop": [0, 1, 2],   # the renderer runs 2 as 1
     "ray_sort": [0, 1],
     "grid_blocks": [1024, 2048, 4096, 8192],
     "shade_threads": [64, 128, 256, 512],

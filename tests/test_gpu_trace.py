@@ -183,3 +183,35 @@ def test_persistent_variant_counts_and_any_hit(suite):
                 np.testing.assert_allclose(nrm, ref["normal"], atol=1e-6)
     finally:
         rtx.set_tuning(**saved)
+
+
+def test_kernel_timer_counts_traversal_launches():
+    """rtx_profile_begin/end time exactly the traversal kernels of the trace calls made in
+    between (sorting excluded, so never more than the whole calls)."""
+    import torch
+    v, t = S.random_triangles(50_000, seed=3)
+    g = rtx.build(v, t)
+    rays = torch.from_numpy(S.incoherent_rays(1 << 17, seed=4)).cuda()
+    hits = torch.zeros((len(rays), 4), dtype=torch.float32, device="cuda")
+    occ = torch.zeros(len(rays), dtype=torch.uint8, device="cuda")
+    saved = rtx.get_tuning()
+    try:
+        for loop in (1, 2):
+            rtx.set_tuning(trace_loop=loop, ray_sort=1)
+            g.trace_closest_device(rays, hits)
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            with rtx.KernelTimer() as kt:
+                a.record()
+                for _ in range(3):
+                    g.trace_closest_device(rays, hits)
+                g.trace_any_device(rays, occ)
+                b.record()
+            torch.cuda.synchronize()
+            assert kt.launches == 4
+            assert 0.0 < kt.ms <= a.elapsed_time(b) * 1.01 + 0.05
+        with rtx.KernelTimer() as kt:
+            pass
+        assert kt.launches == 0 and kt.ms == 0.0
+    finally:
+        rtx.set_tuning(**saved)
