@@ -52,6 +52,9 @@ def main():
                          "counters under the profiler)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time with the library's default launch shapes instead of tuning them here first")
+    ap.add_argument("--tuning", default=None,
+                    help='JSON launch shapes applied to the headline after tuning, e.g. \'{"trace_lds_depth": 12}\' '
+                         "(with --no-autotune: exactly these on top of the defaults)")
     ap.add_argument("--skip-extras", action="store_true",
                     help="time only the headline launch (used under the profiler so k_trace's average is "
                          "the incoherent launch alone)")
@@ -124,6 +127,8 @@ def main():
         trace_tuning, _, _ = autotune.tune(lambda: (lambda: bvh.trace_closest_device(inc, hits, stream=stream)),
                                            keys=["trace_loop", "trace_lds_depth", "ray_sort", "grid_blocks"], reps=3)
         trace_tune_s = time.perf_counter() - t_tune
+    if args.tuning:
+        trace_tuning = dict(trace_tuning, **json.loads(args.tuning))
     rtx.set_tuning(**trace_tuning)
 
     ms_inc, wall, kms_inc = timed(lambda: bvh.trace_closest_device(inc, hits, stream=stream), args.steps,
