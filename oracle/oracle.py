@@ -56,6 +56,8 @@ def lib():
         L.orc_sample.argtypes = [u32, u32, u32, u32]
         L.orc_sample.restype = C.c_float
         L.orc_sincos.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.orc_atan2.argtypes = [C.c_float, C.c_float]
+        L.orc_atan2.restype = C.c_float
         L.orc_render.argtypes = [vp, u32, vp, vp, u32, vp, u32, vp, vp, C.POINTER(Camera),
                                  C.POINTER(RenderParams), vp, C.c_int]
         L.orc_render_loss_f64.argtypes = [vp, u32, vp, vp, u32, vp, u32, vp, vp, C.POINTER(Camera),
@@ -210,6 +212,10 @@ def render_loss_f64(tree, cam, params, albedo, emission, target, nthreads=1):
 
 def sample(seed, pixel, s, dim):
     return lib().orc_sample(seed, pixel, s, dim)
+
+
+def atan2(y, x):
+    return lib().orc_atan2(float(y), float(x))
 
 
 def sincos(x):

@@ -778,6 +778,8 @@ inline float atan2_spec(float y, float x) {
     return r;
 }
 
+extern "C" float orc_atan2(float y, float x) { return atan2_spec(y, x); }
+
 // IBL::sample (main.cpp:107-117) with the atan2/acos above.
 inline V3 ibl_sample(const orc_render_params* p, V3 d) {
     float phi = atan2_spec(d.z, d.x);

@@ -72,6 +72,8 @@ int orc_trace(const orc_node48* nodes, uint32_t nn, const uint32_t* tris, uint32
 /* Counter-based sample table value (seed, pixel, sample, dim) in [0,1). */
 float orc_sample(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dim);
 void  orc_sincos(float x, float* s, float* c);
+/* atan2 of the IBL lookup (octant reduction + degree-7 polynomial; shared with the GPU). */
+float orc_atan2(float y, float x);
 
 typedef struct orc_camera {
     float pos[3], fwd[3], right[3], up[3];

@@ -180,3 +180,21 @@ def test_sincos_and_sample_table():
     u = np.array([O.sample(3, p, s, d) for p in range(40) for s in range(10) for d in range(8)])
     assert u.min() >= 0 and u.max() < 1 and abs(u.mean() - 0.5) < 0.02
     assert O.sample(1, 2, 3, 4) != O.sample(1, 2, 3, 5)
+
+
+def test_atan2_spec_against_numpy():
+    """The IBL lookup's atan2 (octant reduction + polynomial) against numpy.arctan2 over
+    all directions of a dense angle grid and the axis/diagonal special cases: within
+    2e-6 rad, exact quadrant signs, and exact 0, +-pi/2, pi on the axes."""
+    ang = np.linspace(-np.pi, np.pi, 20001)
+    err = 0.0
+    for r in (1e-3, 1.0, 7.5):
+        for a in ang:
+            y, x = np.float32(r * np.sin(a)), np.float32(r * np.cos(a))
+            got = O.atan2(y, x)
+            err = max(err, abs(got - float(np.arctan2(np.float64(y), np.float64(x)))))
+    assert err < 2e-6, err
+    assert O.atan2(0.0, 1.0) == 0.0
+    assert abs(O.atan2(1.0, 0.0) - np.pi / 2) < 1e-7 and abs(O.atan2(-1.0, 0.0) + np.pi / 2) < 1e-7
+    assert abs(O.atan2(0.0, -1.0) - np.pi) < 1e-6
+    assert abs(O.atan2(1.0, 1.0) - np.pi / 4) < 1e-6 and abs(O.atan2(-1.0, -1.0) + 3 * np.pi / 4) < 1e-6
