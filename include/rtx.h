@@ -190,7 +190,9 @@ void rtx_batch_free(rtx_batch* batch);
  * while-while loop in a persistent grid of grid_blocks blocks where idle lanes fetch new
  * rays (plain trace entry points; the renderer runs loop 1 for it); ray_sort: 1 = the trace entry points
  * first sort rays (>= 65536 of them) by a Morton key of origin and direction and trace in
- * that order, writing results back at each ray's index. The sort's scratch lives on the
+ * that order, writing results back at each ray's index; wait_lanes (0..64): in the
+ * while-while loops, the leaf phase starts once at most this many lanes of a wave are
+ * still at internal nodes (0 = wait for all). The sort's scratch lives on the
  * BVH handle and grows to the largest ray count seen, so with ray_sort on, trace calls
  * that overlap on one handle (other streams or threads) race. */
 typedef struct rtx_tuning {
@@ -200,6 +202,7 @@ typedef struct rtx_tuning {
     uint32_t trace_lds_depth;
     uint32_t trace_loop;
     uint32_t ray_sort;
+    uint32_t wait_lanes;
 } rtx_tuning;
 int rtx_set_tuning(const rtx_tuning* t);
 int rtx_get_tuning(rtx_tuning* t);

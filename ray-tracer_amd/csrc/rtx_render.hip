@@ -553,7 +553,7 @@ struct rtx_batch {
 extern "C" int rtx_set_tuning(const rtx_tuning* t) {
     if (!t || t->batch_paths < 1024 || t->grid_blocks == 0 || t->shade_threads == 0 || t->shade_threads % 64 ||
         t->shade_threads > 1024 || !(t->trace_lds_depth == 8 || t->trace_lds_depth == 12 || t->trace_lds_depth == 16) ||
-        t->trace_loop > 2 || t->ray_sort > 1)
+        t->trace_loop > 2 || t->ray_sort > 1 || t->wait_lanes > 64)
         return fail(RTX_E_ARG, "bad tuning on rtx_set_tuning()");
     g_tuning = *t;
     return RTX_OK;

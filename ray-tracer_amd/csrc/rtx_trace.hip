@@ -123,8 +123,8 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, co
         }
         // uniform exit test: a started ray that missed the root is written at the top
         if (!__any(w.live || id != kInvalid) && !more) break;
-        while (w.live && !(w.ref & kLeafBit)) w.template node_step<COUNT>(A, lst, tid);
-        if (w.live) {
+        w.template node_phase<COUNT>(A, lst, tid);
+        if (w.live && (w.ref & kLeafBit)) {
             if (w.template leaf_step<ANY, COUNT>(A)) w.live = false;
             else w.pop(lst, tid);
         }
@@ -258,7 +258,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
 
 }  // namespace
 
-rtx_tuning g_tuning = {1u << 22, 2048, 256, 8, 2, 1};
+rtx_tuning g_tuning = {1u << 22, 2048, 256, 8, 2, 1, 8};
 
 }  // namespace rtx
 

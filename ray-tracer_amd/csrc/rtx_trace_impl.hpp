@@ -10,6 +10,10 @@ namespace rtx {
 constexpr int kTraceThreads = 256;
 constexpr int kLdsDepthMax = 16;       // LDS ring entries per lane: 8, 12 or 16 (a tuning choice)
 constexpr int kSpillDepth = 128;
+#ifndef RTX_TRI_BATCH
+#define RTX_TRI_BATCH 4
+#endif
+constexpr int kTriBatch = RTX_TRI_BATCH;   // triangles of a leaf loaded together
 
 struct TraceArgs {
     const GNode* nodes;
@@ -20,6 +24,7 @@ struct TraceArgs {
     float rb[6];
     uint32_t root_ref;
     uint32_t finite;    // every node box coordinate is finite
+    uint32_t wait;      // while-while: lanes still at internal nodes the leaf phase need not wait for
 };
 
 extern rtx_tuning g_tuning;
@@ -53,6 +58,7 @@ inline TraceArgs make_trace_args(const rtx_bvh* b) {
     for (int k = 0; k < 6; ++k) A.rb[k] = b->root_box[k];
     A.root_ref = b->root_ref;
     A.finite = b->boxes_finite ? 1u : 0u;
+    A.wait = g_tuning.wait_lanes;
     return A;
 }
 
@@ -137,18 +143,51 @@ struct Walker {
             pop(lst, tid);
         }
     }
-    // all triangles of the current leaf; true when an any-hit query is answered
+    // Node phase of the while-while loop: node steps for the lanes at internal nodes until
+    // none is left, or until at most A.wait are left while some lane already waits at a
+    // leaf (those lanes then continue in a later phase). Every vote is wave-uniform; each
+    // lane's own sequence of visits is unchanged.
+    template <bool COUNT>
+    __device__ __forceinline__ void node_phase(const TraceArgs& A, uint2 (*lst)[kTraceThreads], uint32_t tid) {
+        for (;;) {
+            const bool need = live && !(ref & kLeafBit);
+            const uint64_t nm = __ballot(need);
+            if (nm == 0) break;
+            if ((uint32_t)__popcll(nm) <= A.wait && __ballot(live && (ref & kLeafBit)) != 0) break;
+            if (need) node_step<COUNT>(A, lst, tid);
+        }
+    }
+    // All triangles of the current leaf, in order; true when an any-hit query is answered.
+    // Triangles are loaded kTriBatch at a time before any of them is tested, so a leaf
+    // costs ceil(count / kTriBatch) memory round trips instead of count. Loads past the
+    // leaf's end re-read its last triangle (always in bounds) and are not tested.
     template <bool ANY, bool COUNT>
     __device__ __forceinline__ bool leaf_step(const TraceArgs& A) {
         const Leaf lf = A.leaves[ref & ~kLeafBit];
-        for (uint32_t k = lf.begin; k < lf.begin + lf.count; ++k) {
-            const float4 a = A.tris[k].a, b = A.tris[k].b, c = A.tris[k].c;
-            if (COUNT) ++nT;
-            float u, v;
-            float t = tri_test(o, d, tmin, tmax, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y}, f3{b.z, b.w, c.x}, u, v);
-            if (t < bestT) {
-                bestT = t; bu = u; bv = v; best = k;
-                if (ANY) return true;
+        const uint32_t end = lf.begin + lf.count;
+        for (uint32_t k = lf.begin; k < end; k += kTriBatch) {
+            float4 ta[kTriBatch], tb[kTriBatch];
+            float tc[kTriBatch];
+#pragma unroll
+            for (int j = 0; j < kTriBatch; ++j) {
+                const uint32_t kk = min(k + (uint32_t)j, end - 1u);
+                ta[j] = A.tris[kk].a;
+                tb[j] = A.tris[kk].b;
+                tc[j] = A.tris[kk].c.x;
+            }
+#pragma unroll
+            for (int j = 0; j < kTriBatch; ++j) {
+                if (k + (uint32_t)j < end) {
+                    if (COUNT) ++nT;
+                    float u, v;
+                    const float4 a = ta[j], b = tb[j];
+                    const float t = tri_test(o, d, tmin, tmax, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y},
+                                             f3{b.z, b.w, tc[j]}, u, v);
+                    if (t < bestT) {
+                        bestT = t; bu = u; bv = v; best = k + (uint32_t)j;
+                        if (ANY) return true;
+                    }
+                }
             }
         }
         return false;
@@ -156,9 +195,9 @@ struct Walker {
 };
 
 // One ray against one tree. LOOP 0 visits nodes and leaves in one loop; LOOP 1
-// ("while-while") lets a lane that reaches a leaf wait until every lane of the wave is at
-// a leaf, so triangle tests run with the wave converged. Both visit the same nodes in the
-// same order per ray.
+// ("while-while") lets a lane that reaches a leaf wait until the other lanes of the wave
+// are at leaves too (all but A.wait of them), so triangle tests run with the wave mostly
+// converged. Both visit the same nodes in the same order per ray.
 template <bool ANY, bool COUNT, int DEPTH, int LOOP>
 __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, float tmin, float tmax,
                                               uint2 (*lst)[kTraceThreads], uint32_t tid) {
@@ -177,9 +216,9 @@ __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, fl
         }
     } else {
         for (;;) {
-            while (w.live && !(w.ref & kLeafBit)) w.template node_step<COUNT>(A, lst, tid);
+            w.template node_phase<COUNT>(A, lst, tid);
             if (!__any(w.live)) break;
-            if (w.live) {
+            if (w.live && (w.ref & kLeafBit)) {
                 if (w.template leaf_step<ANY, COUNT>(A)) w.live = false;
                 else w.pop(lst, tid);
             }

@@ -18,6 +18,7 @@ SPACE = {
     "trace_lds_depth": [8, 12, 16],
     "trace_loop": [0, 1, 2],   # the renderer runs 2 as 1
     "ray_sort": [0, 1],
+    "wait_lanes": [0, 4, 8, 16, 24, 32],
     "grid_blocks": [1024, 2048, 4096, 8192],
     "shade_threads": [64, 128, 256, 512],
     "batch_paths": [1 << 20, 1 << 21, 1 << 22, 1 << 23],

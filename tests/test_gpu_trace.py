@@ -105,8 +105,8 @@ def test_trace_1m_subset():
 
 
 def test_tuning_variants_give_identical_hits():
-    """Every launch-shape variant (LDS stack depth, loop structure, ray sorting) returns the
-    same hits at each ray's own index."""
+    """Every launch-shape variant (LDS stack depth, loop structure, ray sorting, how many
+    lanes the leaf phase leaves behind) returns the same hits at each ray's own index."""
     import torch
     v, t = S.random_triangles(200_000, seed=5)
     g = rtx.build(v, t)
@@ -115,17 +115,17 @@ def test_tuning_variants_give_identical_hits():
     saved = rtx.get_tuning()
     ref = None
     try:
-        for sort in (0, 1):
-            for depth in (8, 12, 16):
-                for loop in (0, 1, 2):
-                    rtx.set_tuning(trace_lds_depth=depth, trace_loop=loop, ray_sort=sort)
-                    hits.fill_(-1.0)
-                    g.trace_closest_device(rays, hits)
-                    torch.cuda.synchronize()
-                    h = hits.view(torch.int32).cpu().numpy().copy()
-                    if ref is None:
-                        ref = h
-                    assert np.array_equal(h, ref), f"variant depth={depth} loop={loop} sort={sort} differs"
+        combos = [(sort, depth, loop, 0) for sort in (0, 1) for depth in (8, 12, 16) for loop in (0, 1, 2)]
+        combos += [(sort, 8, loop, wait) for sort in (0, 1) for loop in (1, 2) for wait in (1, 8, 32, 64)]
+        for sort, depth, loop, wait in combos:
+            rtx.set_tuning(trace_lds_depth=depth, trace_loop=loop, ray_sort=sort, wait_lanes=wait)
+            hits.fill_(-1.0)
+            g.trace_closest_device(rays, hits)
+            torch.cuda.synchronize()
+            h = hits.view(torch.int32).cpu().numpy().copy()
+            if ref is None:
+                ref = h
+            assert np.array_equal(h, ref), f"variant depth={depth} loop={loop} sort={sort} wait={wait} differs"
     finally:
         rtx.set_tuning(**saved)
 
@@ -161,7 +161,7 @@ def test_persistent_variant_counts_and_any_hit(suite):
     saved = rtx.get_tuning()
     try:
         for name, sort in (("rand20000", 0), ("straddle", 0), ("cornell", 0), ("rand20000", 1)):
-            rtx.set_tuning(trace_loop=2, trace_lds_depth=8, grid_blocks=16, ray_sort=sort)
+            rtx.set_tuning(trace_loop=2, trace_lds_depth=8, grid_blocks=16, ray_sort=sort, wait_lanes=8 * sort)
             v, t = suite[name]
             g = rtx.build(v, t)
             o = O.build(v, t)

@@ -200,7 +200,7 @@ def test_tuning_validation():
     saved = rtx.get_tuning()
     try:
         for bad in (dict(trace_loop=3), dict(trace_lds_depth=10), dict(shade_threads=96),
-                    dict(batch_paths=512), dict(grid_blocks=0), dict(ray_sort=2)):
+                    dict(batch_paths=512), dict(grid_blocks=0), dict(ray_sort=2), dict(wait_lanes=65)):
             with pytest.raises(rtx.RtxError):
                 rtx.set_tuning(**bad)
             assert rtx.get_tuning() == saved, bad
