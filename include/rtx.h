@@ -189,7 +189,7 @@ void rtx_batch_free(rtx_batch* batch);
  * node/leaf loop, 1 = while-while (triangle tests run with the wave converged), 2 = the
  * while-while loop in a persistent grid of grid_blocks blocks where idle lanes fetch new
  * rays (plain trace entry points; the renderer runs loop 1 for it); ray_sort: 1 = the trace entry points
- * first sort rays (>= 65536 of them) by a Morton key of origin and direction and trace in
+ * first sort rays (>= 65536 of them) by a Morton key of their origins and trace in
  * that order, writing results back at each ray's index; wait_lanes (0..64): in the
  * while-while loops, the leaf phase starts once at most this many lanes of a wave are
  * still at internal nodes (0 = wait for all). The sort's scratch lives on the

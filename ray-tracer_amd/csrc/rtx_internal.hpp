@@ -47,7 +47,7 @@ struct rtx_bvh {
     uint32_t* work = nullptr;         // ray counter of the persistent trace kernel
     bool boxes_finite = false;        // every node box coordinate finite (fast slab test)
     // ray-sort scratch (grow-only)
-    uint64_t* rs_keys[2] = {nullptr, nullptr};
+    uint32_t* rs_keys[2] = {nullptr, nullptr};
     uint32_t* rs_vals[2] = {nullptr, nullptr};
     uint32_t* rs_hist = nullptr;
     uint32_t* rs_part = nullptr;

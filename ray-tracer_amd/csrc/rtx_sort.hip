@@ -14,7 +14,8 @@ namespace {
 constexpr int kRsThreads = 256, kRsItems = 16, kRsTile = kRsThreads * kRsItems;
 constexpr int kScanThreads = 1024, kScanItems = 4, kScanSeg = kScanThreads * kScanItems;
 
-__global__ void k_rs_hist(const uint64_t* keys, uint32_t n, int shift, uint32_t* hist, uint32_t ntiles) {
+template <class K>
+__global__ void k_rs_hist(const K* keys, uint32_t n, int shift, uint32_t* hist, uint32_t ntiles) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -86,7 +87,8 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_add(uint32_t* data, uint3
         if (b + k < m) data[b + k] += add;
 }
 
-__global__ void k_rs_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t* kout, uint32_t* vout, uint32_t n,
+template <class K>
+__global__ void k_rs_scatter(const K* kin, const uint32_t* vin, K* kout, uint32_t* vout, uint32_t n,
                              int shift, const uint32_t* hist, uint32_t ntiles) {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[kRsThreads / 64][256];
@@ -99,7 +101,7 @@ __global__ void k_rs_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t*
         __syncthreads();
         uint32_t i = base + r * kRsThreads + threadIdx.x;
         bool valid = i < n;
-        uint64_t k = valid ? kin[i] : 0ull;
+        K k = valid ? kin[i] : (K)0;
         uint32_t v = valid ? vin[i] : 0u;
         uint32_t d = (uint32_t)(k >> shift) & 255u;
         uint64_t mask = __ballot(valid);
@@ -129,24 +131,32 @@ __global__ void k_rs_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t*
 size_t sort_hist_entries(uint32_t n) { return 256ull * ((n + kRsTile - 1) / kRsTile); }
 size_t sort_part_entries(uint32_t n) { return (sort_hist_entries(n) + kScanSeg - 1) / kScanSeg + 1; }
 
-int radix_sort_pairs(SortBuffers& S, uint32_t n, uint32_t pass_mask, hipStream_t st) {
+template <class K>
+int sort_impl(SortBuffersT<K>& S, uint32_t n, uint32_t pass_mask, hipStream_t st) {
     S.cur = 0;
     if (n == 0) return 0;
     const uint32_t ntiles = (n + kRsTile - 1) / kRsTile;
     const uint32_t m = 256 * ntiles;
     const uint32_t nseg = (m + kScanSeg - 1) / kScanSeg;
-    for (int pass = 0; pass < 8; ++pass) {
+    for (int pass = 0; pass < (int)sizeof(K); ++pass) {
         if (!((pass_mask >> pass) & 1u)) continue;
         const int shift = 8 * pass;
-        k_rs_hist<<<ntiles, kRsThreads, 0, st>>>(S.keys[S.cur], n, shift, S.hist, ntiles);
+        k_rs_hist<K><<<ntiles, kRsThreads, 0, st>>>(S.keys[S.cur], n, shift, S.hist, ntiles);
         k_scan_seg<<<nseg, kScanThreads, 0, st>>>(S.hist, m, S.part);
         k_scan_parts<<<1, kScanThreads, 0, st>>>(S.part, nseg);
         k_scan_add<<<nseg, kScanThreads, 0, st>>>(S.hist, m, S.part);
-        k_rs_scatter<<<ntiles, kRsThreads, 0, st>>>(S.keys[S.cur], S.vals[S.cur], S.keys[S.cur ^ 1],
-                                                    S.vals[S.cur ^ 1], n, shift, S.hist, ntiles);
+        k_rs_scatter<K><<<ntiles, kRsThreads, 0, st>>>(S.keys[S.cur], S.vals[S.cur], S.keys[S.cur ^ 1],
+                                                       S.vals[S.cur ^ 1], n, shift, S.hist, ntiles);
         S.cur ^= 1;
     }
     return 0;
+}
+
+int radix_sort_pairs(SortBuffers& S, uint32_t n, uint32_t pass_mask, hipStream_t st) {
+    return sort_impl(S, n, pass_mask, st);
+}
+int radix_sort_pairs(SortBuffers32& S, uint32_t n, uint32_t pass_mask, hipStream_t st) {
+    return sort_impl(S, n, pass_mask, st);
 }
 
 }  // namespace rtx

@@ -7,14 +7,17 @@
 namespace rtx {
 
 // Ping-pong buffers. The input is keys[0]/vals[0]; after radix_sort_pairs the sorted
-// pairs are in keys[cur]/vals[cur].
-struct SortBuffers {
-    uint64_t* keys[2];
+// pairs are in keys[cur]/vals[cur]. K is uint64_t (BVH Morton codes) or uint32_t (rays).
+template <class K>
+struct SortBuffersT {
+    K* keys[2];
     uint32_t* vals[2];
     uint32_t* hist;   // sort_hist_entries(n) entries
     uint32_t* part;   // sort_part_entries(n) entries
     int cur;
 };
+using SortBuffers = SortBuffersT<uint64_t>;
+using SortBuffers32 = SortBuffersT<uint32_t>;
 
 size_t sort_hist_entries(uint32_t n);
 size_t sort_part_entries(uint32_t n);
@@ -22,5 +25,6 @@ size_t sort_part_entries(uint32_t n);
 // Runs the 8-bit passes whose bit is set in pass_mask (pass p sorts bits 8p..8p+7).
 // Skipping a pass whose digit is equal for every key leaves the result unchanged.
 int radix_sort_pairs(SortBuffers& S, uint32_t n, uint32_t pass_mask, hipStream_t st);
+int radix_sort_pairs(SortBuffers32& S, uint32_t n, uint32_t pass_mask, hipStream_t st);
 
 }  // namespace rtx

@@ -135,30 +135,33 @@ __global__ void k_zero_work(uint32_t* w) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *w = 0u;
 }
 
-// Ray order key: 24-bit Morton code of the origin (8 bits per axis inside the scene box)
-// above a 24-bit Morton code of the direction ((d + 1) / 2, 8 bits per axis). Rays with
-// nearby origins and directions then share nodes and triangles in cache and in a wave.
-__device__ __forceinline__ uint64_t spread8(uint32_t v) {
-    uint64_t r = 0;
-    for (int b = 0; b < 8; ++b) r |= (uint64_t)((v >> b) & 1u) << (3 * b);
-    return r;
+// Ray order key: 30-bit Morton code of the origin inside the scene box (10 bits per axis).
+// Rays with nearby keys share nodes and triangles in the L2 of the XCD that traces them
+// and follow similar paths within a wave. (Direction bits below the origin code, or a
+// 48-bit origin+direction key, measured slower: the sort costs more and L2 reuse comes
+// from the origins.)
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {   // bit b of x -> bit 3b
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
 }
-__device__ __forceinline__ uint32_t q8(float x) {
-    float f = x * 255.0f;
-    return f > 0.f ? (f < 255.f ? (uint32_t)f : 255u) : 0u;
+__device__ __forceinline__ uint32_t quant(float x, float levels) {   // x in [0,1] -> [0, levels-1]
+    const float f = x * levels;
+    return f > 0.f ? (f < levels - 1.f ? (uint32_t)f : (uint32_t)(levels - 1.f)) : 0u;
 }
-__global__ void k_ray_keys(TraceArgs A, const Ray32* __restrict__ rays, uint32_t n, uint64_t* keys, uint32_t* vals) {
+__global__ void k_ray_keys(TraceArgs A, const Ray32* __restrict__ rays, uint32_t n, uint32_t* keys, uint32_t* vals) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 r0 = reinterpret_cast<const float4*>(rays)[2 * i];
-    const float4 r1 = reinterpret_cast<const float4*>(rays)[2 * i + 1];
     const float sx = A.rb[3] - A.rb[0], sy = A.rb[4] - A.rb[1], sz = A.rb[5] - A.rb[2];
-    const uint64_t o = (spread8(q8(sx > 0.f ? (r0.x - A.rb[0]) / sx : 0.f)) << 2) |
-                       (spread8(q8(sy > 0.f ? (r0.y - A.rb[1]) / sy : 0.f)) << 1) |
-                       spread8(q8(sz > 0.f ? (r0.z - A.rb[2]) / sz : 0.f));
-    const uint64_t d = (spread8(q8((r1.x + 1.f) * 0.5f)) << 2) | (spread8(q8((r1.y + 1.f) * 0.5f)) << 1) |
-                       spread8(q8((r1.z + 1.f) * 0.5f));
-    keys[i] = (o << 24) | d;
+    const float ux = sx > 0.f ? (r0.x - A.rb[0]) / sx : 0.f;
+    const float uy = sy > 0.f ? (r0.y - A.rb[1]) / sy : 0.f;
+    const float uz = sz > 0.f ? (r0.z - A.rb[2]) / sz : 0.f;
+    const uint32_t key = (spread10(quant(ux, 1024.f)) << 2) | (spread10(quant(uy, 1024.f)) << 1) |
+                         spread10(quant(uz, 1024.f));
+    keys[i] = key;
     vals[i] = i;
 }
 
@@ -196,8 +199,8 @@ bool reserve_sort(rtx_bvh* b, uint32_t n) {
     void* ps[] = {b->rs_keys[0], b->rs_keys[1], b->rs_vals[0], b->rs_vals[1], b->rs_hist, b->rs_part};
     for (void* p : ps)
         if (p) (void)hipFree(p);
-    b->rs_keys[0] = dalloc<uint64_t>(n);
-    b->rs_keys[1] = dalloc<uint64_t>(n);
+    b->rs_keys[0] = dalloc<uint32_t>(n);
+    b->rs_keys[1] = dalloc<uint32_t>(n);
     b->rs_vals[0] = dalloc<uint32_t>(n);
     b->rs_vals[1] = dalloc<uint32_t>(n);
     b->rs_hist = dalloc<uint32_t>(sort_hist_entries(n));
@@ -226,10 +229,11 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
     if (g_tuning.ray_sort && n >= kSortMinRays) {
         rtx_bvh* mb = const_cast<rtx_bvh*>(b);
         if (!reserve_sort(mb, (uint32_t)n)) return fail(RTX_E_HIP, "device allocation failed for ray sorting");
-        SortBuffers S{{mb->rs_keys[0], mb->rs_keys[1]}, {mb->rs_vals[0], mb->rs_vals[1]}, mb->rs_hist, mb->rs_part, 0};
+        SortBuffers32 S{{mb->rs_keys[0], mb->rs_keys[1]}, {mb->rs_vals[0], mb->rs_vals[1]}, mb->rs_hist,
+                        mb->rs_part, 0};
         k_ray_keys<<<blocks, kTraceThreads, 0, s>>>(A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, S.keys[0],
                                                     S.vals[0]);
-        radix_sort_pairs(S, (uint32_t)n, 0x3Fu, s);
+        radix_sort_pairs(S, (uint32_t)n, 0xFu, s);
         perm = S.vals[S.cur];
     }
     if (g_tuning.trace_loop == 2) {
@@ -258,7 +262,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
 
 }  // namespace
 
-rtx_tuning g_tuning = {1u << 22, 2048, 256, 8, 2, 1, 8};
+rtx_tuning g_tuning = {1u << 22, 2048, 256, 12, 2, 1, 16};
 
 }  // namespace rtx
 
