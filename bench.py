@@ -157,14 +157,15 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(bvh, v, inc[: 1 << 21].cpu().numpy(), args.cpu_seconds)
+        cpu = cpu_baseline(bvh, v, t, inc[: 1 << 21].cpu().numpy(), coh[: 1 << 20].cpu().numpy(), args.cpu_seconds)
 
     render = None
     if not (args.no_render or args.skip_extras):
         del inc, coh, hits, occ, counts
         torch.cuda.empty_cache()
         batched = bench_batched(stream, args.c3_scenes, tune=not args.no_autotune)
-        render = {"differentiable": bench_differentiable(stream), "batched": batched}
+        render = {"differentiable": bench_differentiable(stream), "batched": batched,
+                  "cornell": bench_cornell(stream, cpu=not args.no_cpu)}
 
     mrays = n / (ms_inc * 1e-3) / 1e6
     out = {
@@ -253,6 +254,36 @@ def bench_differentiable(stream, reps=3):
             "paths": 512 * 512 * 32}
 
 
+def bench_cornell(stream, cpu=True, reps=3):
+    """C1: open Cornell box (20 wall + 2 light + 1280 icosphere triangles), 256x256,
+    16 spp, max_bounce 5, roulette from bounce 3; GPU render time, and the oracle's
+    render of the same image on the host at 1 thread and at all cores."""
+    import torch
+
+    import rtx
+    from rtx import scenes as S
+    v, t, albedo, emission, cam = S.cornell_box(seed=0)
+    g = rtx.build(v, t)
+    p = rtx.make_params(256, 256, 16, 5, 3, seed=1)
+    a_t, e_t = torch.from_numpy(albedo).cuda(), torch.from_numpy(emission).cuda()
+    img = torch.empty((256, 256, 3), device="cuda")
+    ms = _cuda_ms(lambda: rtx.render(g, cam, p, a_t, e_t, out=img, stream=stream), reps, stream)
+    out = {"workload": "256x256, 16 spp, max_bounce 5, roulette from bounce 3, 1302 triangles",
+           "paths": 256 * 256 * 16, "gpu_ms": round(ms, 3)}
+    if cpu:
+        from oracle import oracle as O
+        o = O.build(v, t)
+        op = O.make_params(256, 256, 16, 5, 3, seed=1)
+        threads = min(16, os.cpu_count() or 1)
+        for name, nt in (("cpu_1thread_s", 1), ("cpu_allcore_s", threads)):
+            t0 = time.perf_counter()
+            O.render(o, cam, op, albedo, emission, nthreads=nt)
+            out[name] = round(time.perf_counter() - t0, 3)
+        out["cpu_threads"] = threads
+        out["cpu_kind"] = "port"
+    return out
+
+
 def bench_batched(stream, nscenes, reps=2, tune=True):
     """Batched wavefront render of fBm-displaced spheres of uneven size (seed 2): one
     launch sequence per scene (plain) vs all scenes in shared launches replayed from one
@@ -294,39 +325,52 @@ def bench_batched(stream, nscenes, reps=2, tune=True):
     return out
 
 
-def cpu_baseline(bvh, verts, rays, seconds):
+def cpu_baseline(bvh, verts, tris, rays, coh_rays, seconds):
     """The compiled reference's trace() (single thread, as the reference runs) on a bounded
-    sample of the same incoherent rays over the same tree; falls back to the oracle port."""
+    sample of the same incoherent rays over the same tree; falls back to the oracle port.
+    Beside it: the all-core port, a coherent-ray sample, and the reference's serial build
+    of the same 1M triangles."""
     import numpy as np
     from oracle import oracle as O
 
-    nodes, tris, prims = bvh.export()
-    tree = O.Tree(nodes, tris, prims, verts)
+    nodes, rtris, prims = bvh.export()
+    tree = O.Tree(nodes, rtris, prims, verts)
     kind = "reference" if O.ref_available() else "port"
-    if kind == "reference":
-        O.ref_trace(tree, rays[:1])   # load the tree into the reference tracer once
-    done, t0 = 0, time.perf_counter()
-    chunk = 2048
-    while time.perf_counter() - t0 < seconds and done < len(rays):
-        r = rays[done: done + chunk]
+
+    def rate(rs, budget):
         if kind == "reference":
-            O.ref_trace(None, r)
-        else:
-            O.trace(tree, r, nthreads=1)
-        done += len(r)
-    el = time.perf_counter() - t0
+            O.ref_trace(tree, rs[:1])   # load the tree into the reference tracer once
+        done, t0 = 0, time.perf_counter()
+        chunk = 2048
+        while time.perf_counter() - t0 < budget and done < len(rs):
+            r = rs[done: done + chunk]
+            if kind == "reference":
+                O.ref_trace(None, r)
+            else:
+                O.trace(tree, r, nthreads=1)
+            done += len(r)
+        el = time.perf_counter() - t0
+        return done, el
+    done, el = rate(rays, seconds)
     value = done / el / 1e6
+    cdone, cel = rate(coh_rays, seconds / 4)
     cores = os.cpu_count() or 1
     threads = min(16, cores)
     t1 = time.perf_counter()
     m = min(len(rays), 1 << 18)
     O.trace(tree, rays[:m], nthreads=threads)
     allcore = m / (time.perf_counter() - t1) / 1e6
+    build_s = None
+    if kind == "reference":
+        t2 = time.perf_counter()
+        O.ref_build(verts, tris)
+        build_s = round(time.perf_counter() - t2, 3)
     return {"value": round(value, 4), "unit": "Mrays/s", "cores": 1, "kind": kind,
             "sample": f"{done} of the 4,194,304 incoherent rays, same 1M-triangle tree, single thread "
                       f"({el:.1f} s)",
+            "coherent_mrays_s": round(cdone / cel / 1e6, 4), "coherent_sample": f"{cdone} coherent rays ({cel:.1f} s)",
             "allcore_port_mrays_s": round(allcore, 4), "allcore_threads": threads, "host_cores": cores,
-            "cpu_model": cpu_model()}
+            "build_1m_serial_s": build_s, "cpu_model": cpu_model()}
 
 
 if __name__ == "__main__":
