@@ -120,12 +120,12 @@ def main():
         return float(np.mean(per)), wall, kernel_ms
 
     # Launch shapes for the headline: the project autotuner (coordinate descent over LDS
-    # stack depth, loop structure, ray sorting and persistent grid size) on this GPU.
+    # stack depth, loop structure, ray sorting, leaf-phase wait and persistent grid size).
     trace_tuning, trace_tune_s = rtx.get_tuning(), 0.0
     if not args.no_autotune:
         t_tune = time.perf_counter()
         trace_tuning, _, _ = autotune.tune(lambda: (lambda: bvh.trace_closest_device(inc, hits, stream=stream)),
-                                           keys=["trace_loop", "trace_lds_depth", "ray_sort", "grid_blocks"], reps=3)
+                                           keys=autotune.TRACE_KEYS, reps=3)
         trace_tune_s = time.perf_counter() - t_tune
     if args.tuning:
         trace_tuning = dict(trace_tuning, **json.loads(args.tuning))
@@ -310,8 +310,7 @@ def bench_batched(stream, nscenes, reps=2, tune=True):
             b = rtx.Batch(bvhs, cams, params, [None] * nscenes, [None] * nscenes, imgs, 1)
             return lambda: b.launch(stream)
         t0 = time.perf_counter()
-        best, _, hist = autotune.tune(make_step, keys=["batch_paths", "shade_threads", "grid_blocks",
-                                                       "trace_lds_depth"], reps=1)
+        best, _, hist = autotune.tune(make_step, keys=autotune.RENDER_KEYS, reps=1)
         rtx.set_tuning(**best)
         out["autotune"] = {"seconds": round(time.perf_counter() - t0, 2), "candidates": len(hist)}
     out["tuning"] = rtx.get_tuning()

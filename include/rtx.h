@@ -182,19 +182,28 @@ int rtx_batch_create(const rtx_bvh* const* scenes, int nscenes, const rtx_camera
 int rtx_batch_launch(rtx_batch* batch, void* stream);
 void rtx_batch_free(rtx_batch* batch);
 
-/* Launch shapes (process-wide; chosen by the project autotuner, none changes results).
- * batch_paths: paths in flight per wavefront batch (>= 1024); grid_blocks: blocks of the
- * persistent trace/shade kernels; shade_threads: threads per shade block (multiple of 64);
- * trace_lds_depth: LDS stack entries per lane (8, 12 or 16); trace_loop: 0 = one
- * node/leaf loop, 1 = while-while (triangle tests run with the wave converged), 2 = the
- * while-while loop in a persistent grid of grid_blocks blocks where idle lanes fetch new
- * rays (plain trace entry points; the renderer runs loop 1 for it); ray_sort: 1 = the trace entry points
- * first sort rays (>= 65536 of them) by a Morton key of their origins and trace in
- * that order, writing results back at each ray's index; wait_lanes (0..64): in the
- * while-while loops, the leaf phase starts once at most this many lanes of a wave are
- * still at internal nodes (0 = wait for all). The sort's scratch lives on the
- * BVH handle and grows to the largest ray count seen, so with ray_sort on, trace calls
- * that overlap on one handle (other streams or threads) race. */
+/* Launch shapes (process-wide; chosen by the project autotuner; none changes a result).
+ *
+ * Trace entry points (rtx_trace_closest / _any / _counts / _closest_host):
+ *   trace_lds_depth    LDS stack entries per lane: 8, 12 or 16;
+ *   trace_loop         0 = one node/leaf loop, 1 = while-while (triangle tests run with the
+ *                      wave mostly converged), 2 = while-while in a persistent grid of
+ *                      trace_grid_blocks blocks where idle lanes fetch new rays;
+ *   ray_sort           1 = sort rays (>= 65536 of them) by a Morton key of their origins
+ *                      and trace in that order, writing results back at each ray's index.
+ *                      The sort's scratch lives on the BVH handle and grows to the largest
+ *                      ray count seen, so with ray_sort on, trace calls that overlap on one
+ *                      handle (other streams or threads) race;
+ *   wait_lanes         0..64: the while-while leaf phase starts once at most this many lanes
+ *                      of a wave are still at internal nodes (0 = wait for all);
+ *   trace_grid_blocks  blocks of the persistent trace kernel.
+ * Renderer (rtx_render, rtx_render_adjoint, rtx_batch_*):
+ *   batch_paths        paths in flight per wavefront batch (>= 1024);
+ *   grid_blocks        blocks of the wavefront trace and shade kernels;
+ *   shade_threads      threads per shade block (multiple of 64, <= 1024);
+ *   render_lds_depth   LDS stack entries per lane: 8, 12 or 16;
+ *   render_loop        0 or 1 as trace_loop;
+ *   render_wait_lanes  as wait_lanes. */
 typedef struct rtx_tuning {
     uint32_t batch_paths;
     uint32_t grid_blocks;
@@ -203,6 +212,10 @@ typedef struct rtx_tuning {
     uint32_t trace_loop;
     uint32_t ray_sort;
     uint32_t wait_lanes;
+    uint32_t trace_grid_blocks;
+    uint32_t render_lds_depth;
+    uint32_t render_loop;
+    uint32_t render_wait_lanes;
 } rtx_tuning;
 int rtx_set_tuning(const rtx_tuning* t);
 int rtx_get_tuning(rtx_tuning* t);

@@ -136,7 +136,8 @@ __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* _
 }
 
 template <int DEPTH, int LOOP>
-__global__ void __launch_bounds__(kTraceThreads) k_wtrace(const SceneDesc* __restrict__ scenes, PathState P, int qi) {
+__global__ void __launch_bounds__(kTraceThreads) k_wtrace(const SceneDesc* __restrict__ scenes, PathState P, int qi,
+                                                          uint32_t wait) {
     __shared__ uint2 lst[DEPTH][kTraceThreads];
     const uint32_t n = P.cnt[qi];
     const uint32_t* Q = P.q[qi];
@@ -155,7 +156,7 @@ __global__ void __launch_bounds__(kTraceThreads) k_wtrace(const SceneDesc* __res
             S = scenes + P.sid[i];
         }
         const TraceOut h = trace_ray<false, false, DEPTH, LOOP>(S->T, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z},
-                                                                r0.w, r1.w, lst, threadIdx.x);
+                                                                r0.w, r1.w, lst, threadIdx.x, wait);
         if (in) P.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
     }
 }
@@ -481,7 +482,7 @@ int Batch::setup(uint32_t batch_paths, bool adjoint) {
 
 // Kernel sequence for the paths of scenes [s_first, s_last): batches of B paths.
 void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
-    const uint32_t grid = g_tuning.grid_blocks, sh = g_tuning.shade_threads;
+    const uint32_t grid = g_tuning.grid_blocks, sh = g_tuning.shade_threads, wait = g_tuning.render_wait_lanes;
     const bool adj = dimg != nullptr;
     int mb = 0;
     for (int s = s_first; s < s_last; ++s) {
@@ -502,8 +503,8 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
         // then the consumed queue becomes the next output and its length is cleared.
         int qi = 0;
         for (int b = 0; b <= mb; ++b) {
-            with_trace_variant([&](auto D, auto L) {
-                k_wtrace<decltype(D)::value, decltype(L)::value><<<grid, kTraceThreads, 0, st>>>(d_scenes, P, qi);
+            with_trace_variant(g_tuning.render_lds_depth, g_tuning.render_loop, [&](auto D, auto L) {
+                k_wtrace<decltype(D)::value, decltype(L)::value><<<grid, kTraceThreads, 0, st>>>(d_scenes, P, qi, wait);
             });
             if (adj) k_shade<true><<<grid, sh, 0, st>>>(d_scenes, P, qi, b);
             else k_shade<false><<<grid, sh, 0, st>>>(d_scenes, P, qi, b);
@@ -553,7 +554,9 @@ struct rtx_batch {
 extern "C" int rtx_set_tuning(const rtx_tuning* t) {
     if (!t || t->batch_paths < 1024 || t->grid_blocks == 0 || t->shade_threads == 0 || t->shade_threads % 64 ||
         t->shade_threads > 1024 || !(t->trace_lds_depth == 8 || t->trace_lds_depth == 12 || t->trace_lds_depth == 16) ||
-        t->trace_loop > 2 || t->ray_sort > 1 || t->wait_lanes > 64)
+        t->trace_loop > 2 || t->ray_sort > 1 || t->wait_lanes > 64 || t->trace_grid_blocks == 0 ||
+        !(t->render_lds_depth == 8 || t->render_lds_depth == 12 || t->render_lds_depth == 16) || t->render_loop > 1 ||
+        t->render_wait_lanes > 64)
         return fail(RTX_E_ARG, "bad tuning on rtx_set_tuning()");
     g_tuning = *t;
     return RTX_OK;

@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray3
         r1 = reinterpret_cast<const float4*>(rays)[2 * i + 1];
     }
     const TraceOut h = trace_ray<ANY, COUNT, DEPTH, LOOP>(A, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w,
-                                                          lst, tid);
+                                                          lst, tid, A.wait);
     if (!in) return;
     if (ANY) {
         occl[i] = h.tri != kInvalid ? 1 : 0;
@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, co
         }
         // uniform exit test: a started ray that missed the root is written at the top
         if (!__any(w.live || id != kInvalid) && !more) break;
-        w.template node_phase<COUNT>(A, lst, tid);
+        w.template node_phase<COUNT>(A, lst, tid, A.wait);
         if (w.live && (w.ref & kLeafBit)) {
             if (w.template leaf_step<ANY, COUNT>(A)) w.live = false;
             else w.pop(lst, tid);
@@ -237,10 +237,10 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
         perm = S.vals[S.cur];
     }
     if (g_tuning.trace_loop == 2) {
-        const uint32_t pb = std::min<uint32_t>(blocks, g_tuning.grid_blocks);
+        const uint32_t pb = std::min<uint32_t>(blocks, g_tuning.trace_grid_blocks);
         k_zero_work<<<1, 64, 0, s>>>(b->work);
         const int pk = prof_start(s);
-        with_trace_variant([&](auto D, auto) {
+        with_trace_variant(g_tuning.trace_lds_depth, 1, [&](auto D, auto) {
             k_trace_persist<ANY, COUNT, decltype(D)::value><<<pb, kTraceThreads, 0, s>>>(
                 A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
                 counts, b->work, perm);
@@ -250,7 +250,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
         return RTX_OK;
     }
     const int pk = prof_start(s);
-    with_trace_variant([&](auto D, auto L) {
+    with_trace_variant(g_tuning.trace_lds_depth, g_tuning.trace_loop, [&](auto D, auto L) {
         k_trace<ANY, COUNT, decltype(D)::value, decltype(L)::value><<<blocks, kTraceThreads, 0, s>>>(
             A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
             counts, perm);
@@ -262,7 +262,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
 
 }  // namespace
 
-rtx_tuning g_tuning = {1u << 22, 2048, 256, 12, 2, 1, 16};
+rtx_tuning g_tuning = {1u << 22, 2048, 256, 12, 2, 1, 16, 2048, 8, 1, 0};
 
 }  // namespace rtx
 

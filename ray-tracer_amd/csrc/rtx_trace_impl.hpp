@@ -29,17 +29,16 @@ struct TraceArgs {
 
 extern rtx_tuning g_tuning;
 
-// Calls f(integral_constant<DEPTH>, integral_constant<LOOP>) for the tuned variant.
+// Calls f(integral_constant<DEPTH>, integral_constant<LOOP>) for LDS depth d (8, 12, 16)
+// and loop shape l (0, or 1 for the while-while loop; the persistent trace kernel is
+// dispatched by its caller and reuses the while-while loop's depth instantiation).
 template <class F>
-void with_trace_variant(F&& f) {
+void with_trace_variant(uint32_t d, uint32_t l, F&& f) {
     using I8 = std::integral_constant<int, 8>;
     using I12 = std::integral_constant<int, 12>;
     using I16 = std::integral_constant<int, 16>;
     using L0 = std::integral_constant<int, 0>;
     using L1 = std::integral_constant<int, 1>;
-    // loop 2 (persistent, dynamic fetch) exists for the plain trace only; elsewhere it
-    // runs as the while-while loop it is built on
-    const uint32_t d = g_tuning.trace_lds_depth, l = g_tuning.trace_loop;
     if (d == 8) { if (l) f(I8{}, L1{}); else f(I8{}, L0{}); }
     else if (d == 12) { if (l) f(I12{}, L1{}); else f(I12{}, L0{}); }
     else { if (l) f(I16{}, L1{}); else f(I16{}, L0{}); }
@@ -144,16 +143,18 @@ struct Walker {
         }
     }
     // Node phase of the while-while loop: node steps for the lanes at internal nodes until
-    // none is left, or until at most A.wait are left while some lane already waits at a
+    // none is left, or until at most `wait` are left while some lane already waits at a
     // leaf (those lanes then continue in a later phase). Every vote is wave-uniform; each
     // lane's own sequence of visits is unchanged.
+    // `wait` must be the same on every lane (a kernel argument or a uniform field).
     template <bool COUNT>
-    __device__ __forceinline__ void node_phase(const TraceArgs& A, uint2 (*lst)[kTraceThreads], uint32_t tid) {
+    __device__ __forceinline__ void node_phase(const TraceArgs& A, uint2 (*lst)[kTraceThreads], uint32_t tid,
+                                               uint32_t wait) {
         for (;;) {
             const bool need = live && !(ref & kLeafBit);
             const uint64_t nm = __ballot(need);
             if (nm == 0) break;
-            if ((uint32_t)__popcll(nm) <= A.wait && __ballot(live && (ref & kLeafBit)) != 0) break;
+            if ((uint32_t)__popcll(nm) <= wait && __ballot(live && (ref & kLeafBit)) != 0) break;
             if (need) node_step<COUNT>(A, lst, tid);
         }
     }
@@ -200,7 +201,7 @@ struct Walker {
 // converged. Both visit the same nodes in the same order per ray.
 template <bool ANY, bool COUNT, int DEPTH, int LOOP>
 __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, float tmin, float tmax,
-                                              uint2 (*lst)[kTraceThreads], uint32_t tid) {
+                                              uint2 (*lst)[kTraceThreads], uint32_t tid, uint32_t wait) {
     Walker<DEPTH> w;
     uint2 spill[kSpillDepth];
     w.spill = spill;
@@ -216,7 +217,7 @@ __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, fl
         }
     } else {
         for (;;) {
-            w.template node_phase<COUNT>(A, lst, tid);
+            w.template node_phase<COUNT>(A, lst, tid, wait);
             if (!__any(w.live)) break;
             if (w.live && (w.ref & kLeafBit)) {
                 if (w.template leaf_step<ANY, COUNT>(A)) w.live = false;

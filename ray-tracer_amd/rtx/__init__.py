@@ -53,7 +53,8 @@ class RenderParams(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("batch_paths", C.c_uint32), ("grid_blocks", C.c_uint32), ("shade_threads", C.c_uint32),
                 ("trace_lds_depth", C.c_uint32), ("trace_loop", C.c_uint32), ("ray_sort", C.c_uint32),
-                ("wait_lanes", C.c_uint32)]
+                ("wait_lanes", C.c_uint32), ("trace_grid_blocks", C.c_uint32), ("render_lds_depth", C.c_uint32),
+                ("render_loop", C.c_uint32), ("render_wait_lanes", C.c_uint32)]
 
 
 _LIB = None

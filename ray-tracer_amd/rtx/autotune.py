@@ -15,14 +15,22 @@ import rtx
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 SPACE = {
+    # trace entry points
     "trace_lds_depth": [8, 12, 16],
-    "trace_loop": [0, 1, 2],   # the renderer runs 2 as 1
+    "trace_loop": [0, 1, 2],
     "ray_sort": [0, 1],
     "wait_lanes": [0, 4, 8, 16, 24, 32],
+    "trace_grid_blocks": [1024, 2048, 4096, 8192],
+    # renderer
+    "render_lds_depth": [8, 12, 16],
+    "render_loop": [0, 1],
+    "render_wait_lanes": [0, 4, 8, 16],
     "grid_blocks": [1024, 2048, 4096, 8192],
     "shade_threads": [64, 128, 256, 512],
     "batch_paths": [1 << 20, 1 << 21, 1 << 22, 1 << 23],
 }
+TRACE_KEYS = ["trace_loop", "trace_lds_depth", "ray_sort", "wait_lanes", "trace_grid_blocks"]
+RENDER_KEYS = ["batch_paths", "shade_threads", "grid_blocks", "render_lds_depth", "render_loop", "render_wait_lanes"]
 
 
 def _time(make_step, reps):

@@ -112,6 +112,35 @@ def test_batched_modes_match_single_and_oracle():
             assert np.array_equal(a.cpu().numpy(), s), f"mode {mode} differs from single-scene render"
 
 
+def test_render_trace_variants_bit_identical():
+    """The wavefront renderer's traversal variants (one loop, while-while with any
+    render_wait_lanes, stack depths, grid and batch sizes) give bit-identical images in a
+    shared multi-scene batch."""
+    import torch
+    sc = _small_batch_scenes()
+    bvhs = [rtx.build(v, t) for v, t, _, _, _ in sc]
+    saved = rtx.get_tuning()
+    try:
+        ref = None
+        for loop, depth, wait, grid, bp in ((1, 8, 0, 2048, 1 << 22), (1, 12, 16, 2048, 1 << 22),
+                                            (1, 8, 8, 8, 4096), (1, 16, 32, 64, 8192), (0, 8, 0, 512, 1 << 20)):
+            rtx.set_tuning(render_loop=loop, render_lds_depth=depth, render_wait_lanes=wait, grid_blocks=grid,
+                           batch_paths=bp)
+            print(f"[render variant] loop={loop} depth={depth} wait={wait} grid={grid}", flush=True)
+            imgs = [torch.zeros((p.height, p.width, 3), device="cuda") for _, _, _, p, _ in sc]
+            b = rtx.Batch(bvhs, [c for _, _, c, _, _ in sc], [p for _, _, _, p, _ in sc], [None] * 3, [None] * 3,
+                          imgs, 2)
+            b.launch()
+            torch.cuda.synchronize()
+            out = [a.cpu().numpy() for a in imgs]
+            if ref is None:
+                ref = out
+            for a, r in zip(out, ref):
+                assert np.array_equal(a, r), (loop, depth, wait, grid)
+    finally:
+        rtx.set_tuning(**saved)
+
+
 def test_graph_replays_match_eager():
     """The captured batch is replayed several times with eager work in between; every
     replay must reproduce the eager render."""

@@ -161,7 +161,7 @@ def test_persistent_variant_counts_and_any_hit(suite):
     saved = rtx.get_tuning()
     try:
         for name, sort in (("rand20000", 0), ("straddle", 0), ("cornell", 0), ("rand20000", 1)):
-            rtx.set_tuning(trace_loop=2, trace_lds_depth=8, grid_blocks=16, ray_sort=sort, wait_lanes=8 * sort)
+            rtx.set_tuning(trace_loop=2, trace_lds_depth=8, trace_grid_blocks=16, ray_sort=sort, wait_lanes=8 * sort)
             v, t = suite[name]
             g = rtx.build(v, t)
             o = O.build(v, t)
