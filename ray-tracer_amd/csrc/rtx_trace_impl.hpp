@@ -10,10 +10,7 @@ namespace rtx {
 constexpr int kTraceThreads = 256;
 constexpr int kLdsDepthMax = 16;       // LDS ring entries per lane: 8, 12 or 16 (a tuning choice)
 constexpr int kSpillDepth = 128;
-#ifndef RTX_TRI_BATCH
-#define RTX_TRI_BATCH 4
-#endif
-constexpr int kTriBatch = RTX_TRI_BATCH;   // triangles of a leaf loaded together
+constexpr int kTraceTriBatch = 4;   // triangles of a leaf loaded together by the trace kernels
 
 struct TraceArgs {
     const GNode* nodes;
@@ -75,8 +72,9 @@ struct TraceOut {
 };
 
 // Traversal state of one lane. lst is the block's LDS stack, DEPTH entries per lane in
-// column tid; entries beyond that spill (oldest first) to private memory.
-template <int DEPTH>
+// column tid; entries beyond that spill (oldest first) to private memory. TB triangles of
+// a leaf are loaded together.
+template <int DEPTH, int TB = kTraceTriBatch>
 struct Walker {
     f3 o, d;
     float ix, iy, iz, tmin, tmax;
@@ -159,25 +157,25 @@ struct Walker {
         }
     }
     // All triangles of the current leaf, in order; true when an any-hit query is answered.
-    // Triangles are loaded kTriBatch at a time before any of them is tested, so a leaf
-    // costs ceil(count / kTriBatch) memory round trips instead of count. Loads past the
+    // Triangles are loaded TB at a time before any of them is tested, so a leaf costs
+    // ceil(count / TB) memory round trips instead of count. Loads past the
     // leaf's end re-read its last triangle (always in bounds) and are not tested.
     template <bool ANY, bool COUNT>
     __device__ __forceinline__ bool leaf_step(const TraceArgs& A) {
         const Leaf lf = A.leaves[ref & ~kLeafBit];
         const uint32_t end = lf.begin + lf.count;
-        for (uint32_t k = lf.begin; k < end; k += kTriBatch) {
-            float4 ta[kTriBatch], tb[kTriBatch];
-            float tc[kTriBatch];
+        for (uint32_t k = lf.begin; k < end; k += TB) {
+            float4 ta[TB], tb[TB];
+            float tc[TB];
 #pragma unroll
-            for (int j = 0; j < kTriBatch; ++j) {
+            for (int j = 0; j < TB; ++j) {
                 const uint32_t kk = min(k + (uint32_t)j, end - 1u);
                 ta[j] = A.tris[kk].a;
                 tb[j] = A.tris[kk].b;
                 tc[j] = A.tris[kk].c.x;
             }
 #pragma unroll
-            for (int j = 0; j < kTriBatch; ++j) {
+            for (int j = 0; j < TB; ++j) {
                 if (k + (uint32_t)j < end) {
                     if (COUNT) ++nT;
                     float u, v;
@@ -199,10 +197,10 @@ struct Walker {
 // ("while-while") lets a lane that reaches a leaf wait until the other lanes of the wave
 // are at leaves too (all but A.wait of them), so triangle tests run with the wave mostly
 // converged. Both visit the same nodes in the same order per ray.
-template <bool ANY, bool COUNT, int DEPTH, int LOOP>
+template <bool ANY, bool COUNT, int DEPTH, int LOOP, int TB = kTraceTriBatch>
 __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, float tmin, float tmax,
                                               uint2 (*lst)[kTraceThreads], uint32_t tid, uint32_t wait) {
-    Walker<DEPTH> w;
+    Walker<DEPTH, TB> w;
     uint2 spill[kSpillDepth];
     w.spill = spill;
     w.start(A, o, d, tmin, tmax);
