@@ -100,32 +100,45 @@ __device__ Box range_box(const Ctx& C, uint32_t b, uint32_t e) {
 }
 
 // Sutherland-Hodgman clip of a triangle to a box, then its AABB (AABB.cpp:86-171).
+// Same vertex order and float operations as the reference. The polygon (at most 9
+// vertices after 6 planes) is kept as three small coordinate arrays that the compiler
+// keeps in registers: an array of 3-float structs went to scratch memory, and a chain of
+// dependent scratch accesses per clip dominated the SAH kernels.
 __device__ Box clip_tri(f3 v0, f3 v1, f3 v2, const Box& r) {
-    f3 poly[12], tmp[12];
+    constexpr int kMaxV = 10;
+    float px[kMaxV], py[kMaxV], pz[kMaxV], qx[kMaxV], qy[kMaxV], qz[kMaxV];
+    px[0] = v0.x; py[0] = v0.y; pz[0] = v0.z;
+    px[1] = v1.x; py[1] = v1.y; pz[1] = v1.z;
+    px[2] = v2.x; py[2] = v2.y; pz[2] = v2.z;
     int n = 3;
-    poly[0] = v0; poly[1] = v1; poly[2] = v2;
+#pragma unroll
     for (int axis = 0; axis < 3; ++axis) {
+#pragma unroll
         for (int side = 0; side < 2; ++side) {
-            float plane = side == 0 ? comp(r.lo, axis) : comp(r.hi, axis);
+            const float plane = side == 0 ? comp(r.lo, axis) : comp(r.hi, axis);
+            const float* pa = axis == 0 ? px : (axis == 1 ? py : pz);
             int m = 0;
             for (int i = 0; i < n; ++i) {
-                int j = (i + 1 == n) ? 0 : i + 1;
-                float pi = comp(poly[i], axis), pj = comp(poly[j], axis);
-                bool ci = side == 0 ? (pi >= plane) : (pi <= plane);
-                bool cj = side == 0 ? (pj >= plane) : (pj <= plane);
-                if (ci) tmp[m++] = poly[i];
+                const int j = (i + 1 == n) ? 0 : i + 1;
+                const float pi = pa[i], pj = pa[j];
+                const bool ci = side == 0 ? (pi >= plane) : (pi <= plane);
+                const bool cj = side == 0 ? (pj >= plane) : (pj <= plane);
+                if (ci) { qx[m] = px[i]; qy[m] = py[i]; qz[m] = pz[i]; ++m; }
                 if (ci != cj) {
-                    float t = (plane - pi) / (pj - pi);
-                    tmp[m++] = add(poly[i], scl(t, sub(poly[j], poly[i])));
+                    const float t = (plane - pi) / (pj - pi);
+                    qx[m] = px[i] + t * (px[j] - px[i]);
+                    qy[m] = py[i] + t * (py[j] - py[i]);
+                    qz[m] = pz[i] + t * (pz[j] - pz[i]);
+                    ++m;
                 }
             }
-            for (int i = 0; i < m; ++i) poly[i] = tmp[i];
+            for (int i = 0; i < m; ++i) { px[i] = qx[i]; py[i] = qy[i]; pz[i] = qz[i]; }
             n = m;
             if (n == 0) return Box::empty();
         }
     }
     Box b = Box::empty();
-    for (int i = 0; i < n; ++i) b.extend(poly[i]);
+    for (int i = 0; i < n; ++i) b.extend(f3{px[i], py[i], pz[i]});
     return b;
 }
 
@@ -462,6 +475,7 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
     __shared__ uint32_t tsi[64];
     __shared__ float bbox[3 * 16][6];
     __shared__ uint32_t bc0[3 * 16], bc1[3 * 16];
+    __shared__ uint32_t tsi_slot[64];   // lane 0's partition order
     if (blockIdx.x >= C.ctr->nwave) return;
     const uint32_t k = C.wlist[blockIdx.x];
     const uint32_t ni = ls + k;
@@ -599,7 +613,7 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
     d.oAxis = oAxis; d.oBin = oBin; d.oCMin = oCMin; d.oScale = oScale; d.nb = nb;
     d.plane = 0.f; d.sAxis = -1;
     d.type = D_LEAF;
-    uint32_t slot[64];
+    uint32_t* slot = tsi_slot;
     for (uint32_t i = 0; i < n; ++i) slot[i] = i;
     auto opred = [&](uint32_t s_) { return min(f2i_x86((tc[s_][oAxis] - oCMin) * oScale), nb - 1) < oBin; };
     auto part = [&](uint32_t first, uint32_t last, auto pred) {
