@@ -779,81 +779,184 @@ __device__ void object_children(const Ctx& C, uint32_t ni, uint32_t begin, uint3
     p.b = c + 1;
 }
 
-__global__ void k_level_apply(Ctx C, uint32_t ls, uint32_t cnt) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+// Ordered combine of the 64 lanes' boxes (lane order): lane 0 receives
+// combine(combine(b0, b1), combine(b2, b3)) ... . The extension keeps the last of equal
+// extrema, so this tree equals the serial fold over the same order. Empty boxes are
+// identities.
+__device__ __forceinline__ Box wave_box(Box b) {
+    const int lane = __lane_id();
+    for (int off = 1; off < 64; off <<= 1) {
+        Box o;
+        o.lo.x = __shfl_down(b.lo.x, off); o.lo.y = __shfl_down(b.lo.y, off); o.lo.z = __shfl_down(b.lo.z, off);
+        o.hi.x = __shfl_down(b.hi.x, off); o.hi.y = __shfl_down(b.hi.y, off); o.hi.z = __shfl_down(b.hi.z, off);
+        if ((lane & (2 * off - 1)) == 0 && lane + off < 64) b = combine(b, o);
+    }
+    return b;   // valid in lane 0
+}
+
+// Ordered box of triangles at s_triIndex positions [b, e), one wave, 64 at a time
+// (computeRangeAABB, bvh_bfs.cpp:14-28). Result in every lane.
+__device__ Box wave_range_box(const Ctx& C, uint32_t b, uint32_t e) {
+    Box acc = Box::empty();
+    for (uint32_t c0 = b; c0 < e; c0 += 64) {
+        const uint32_t i = c0 + __lane_id();
+        Box x = Box::empty();
+        if (i < e) x = load_tbox(C.tbox, C.sorted[C.tindex[i]]);
+        acc = combine(acc, wave_box(x));
+    }
+    Box r;
+    r.lo.x = __shfl(acc.lo.x, 0); r.lo.y = __shfl(acc.lo.y, 0); r.lo.z = __shfl(acc.lo.z, 0);
+    r.hi.x = __shfl(acc.hi.x, 0); r.hi.y = __shfl(acc.hi.y, 0); r.hi.z = __shfl(acc.hi.z, 0);
+    return r;
+}
+
+// libstdc++'s two-ended std::partition (stl_algo.h:1487-1510) of s_triIndex[first, first+n)
+// for n <= 64, one element per lane. With M elements satisfying pred, the algorithm swaps
+// the k-th failing element of [0, M) (from the left) with the k-th satisfying element of
+// [M, n) (from the right); every other element stays. Returns first + M.
+template <class P>
+__device__ uint32_t wave_partition(uint32_t* a, uint32_t first, uint32_t n, P pred, uint32_t* lds_lane) {
+    const uint32_t lane = __lane_id();
+    const bool in = lane < n;
+    const uint32_t v = in ? a[first + lane] : 0u;
+    const bool p = in && pred(v);
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t M = (uint32_t)__popcll(__ballot(p));
+    const uint64_t fl = __ballot(in && !p && lane < M);     // failing, left of M
+    const uint64_t tr = __ballot(in && p && lane >= M);     // satisfying, right of M
+    const bool isF = (fl >> lane) & 1ull, isT = (tr >> lane) & 1ull;
+    if (isT) lds_lane[__popcll(tr & ~below & ~(1ull << lane))] = lane;   // rank from the right
+    __syncthreads();
+    uint32_t partner = lane;
+    if (isF) partner = lds_lane[__popcll(fl & below)];                   // rank from the left
+    __syncthreads();
+    if (isF) lds_lane[__popcll(fl & below)] = lane;
+    __syncthreads();
+    if (isT) partner = lds_lane[__popcll(tr & ~below & ~(1ull << lane))];
+    const uint32_t nv = __shfl(v, (int)partner);
+    if (in && partner != lane) a[first + lane] = nv;
+    return first + M;
+}
+
+// Children of every frontier node that splits, one 64-lane wave per node: Morton splits
+// (child boxes of ranges that will not be split by Morton bits again), object splits,
+// and spatial splits (bvh_sah.cpp:247-460): right-only references appended,
+// straddlers duplicated, both children clipped to the split plane.
+__global__ void __launch_bounds__(64) k_level_apply(Ctx C, uint32_t ls, uint32_t cnt) {
+    __shared__ uint32_t lds_lane[64];
+    const uint32_t k = blockIdx.x;
     if (k >= cnt) return;
+    const uint32_t lane = __lane_id();
     const uint32_t ni = ls + k;
     const Dec d = C.dec[k];
     const BNode nd = C.nodes[ni];
     const uint32_t begin = nd.a, n = nd.b, end = begin + n, c = C.child[k];
     if (d.type == D_LEAF || c == kInvalid) return;
     if (c + 1 >= C.ncap) {  // cannot happen for a valid build (nodes <= 2 * cap - 1)
-        atomicOr(&C.ctr->err, 1u);
+        if (lane == 0) atomicOr(&C.ctr->err, 1u);
         return;
     }
-    if (d.type == D_MORTON) {
-        // Children of a Morton split (bvh_bfs.cpp:109-130). A child that will itself be
-        // split by Morton bits gets its box from the bottom-up refit; the others now.
-        const uint32_t cb[2] = {begin, d.mid}, ce[2] = {d.mid, end};
-        for (int s = 0; s < 2; ++s) {
-            uint32_t sz = ce[s] - cb[s];
-            uint32_t fl = F_LEAF | (sz <= (uint32_t)kLbvhThreshold ? F_SAH : 0u);
-            bool direct = sz <= (uint32_t)kLbvhThreshold || C.scodes[cb[s]] == C.scodes[ce[s] - 1];
-            Box b = Box::empty();
-            if (direct) b = range_box(C, cb[s], ce[s]);
-            else fl |= F_PENDING;
-            set_child(C, c + s, b, cb[s], sz, fl);
-        }
+    auto link = [&]() {
+        if (lane != 0) return;
         BNode& p = C.nodes[ni];
         p.flags &= ~F_LEAF;
         p.a = c;
         p.b = c + 1;
+    };
+    auto object_split = [&](uint32_t mid) {
+        const Box lb = wave_range_box(C, begin, mid), rb = wave_range_box(C, mid, end);
+        if (lane == 0) {
+            set_child(C, c, lb, begin, mid - begin, F_LEAF | F_SAH);
+            set_child(C, c + 1, rb, mid, end - mid, F_LEAF | F_SAH);
+        }
+        link();
+    };
+    if (d.type == D_MORTON) {
+        // Children of a Morton split (bvh_bfs.cpp:109-130). A child that will itself be
+        // split by Morton bits gets its box from the bottom-up refit; the others now.
+        const uint32_t cb[2] = {begin, d.mid}, ce[2] = {d.mid, end};
+        for (int s2 = 0; s2 < 2; ++s2) {
+            const uint32_t sz = ce[s2] - cb[s2];
+            uint32_t fl = F_LEAF | (sz <= (uint32_t)kLbvhThreshold ? F_SAH : 0u);
+            const bool direct = sz <= (uint32_t)kLbvhThreshold || C.scodes[cb[s2]] == C.scodes[ce[s2] - 1];
+            Box b = Box::empty();
+            if (direct) b = wave_range_box(C, cb[s2], ce[s2]);
+            else fl |= F_PENDING;
+            if (lane == 0) set_child(C, c + s2, b, cb[s2], sz, fl);
+        }
+        link();
         return;
     }
-    if (d.type == D_OBJECT) { object_children(C, ni, begin, d.mid, end, c); return; }
+    if (d.type == D_OBJECT) { object_split(d.mid); return; }
     // D_SPATIAL
     if (!C.acc[k]) {
         // Budget exhausted: object-split fallback on the spatially partitioned range.
-        uint32_t mid = partition_idx(C.tindex, begin, end, [&](uint32_t si) {
-            return obj_pred(C, si, d.oAxis, d.oBin, d.oCMin, d.oScale, d.nb);
-        });
-        object_children(C, ni, begin, mid, end, c);
+        auto pred = [&](uint32_t si) { return obj_pred(C, si, d.oAxis, d.oBin, d.oCMin, d.oScale, d.nb); };
+        uint32_t mid;
+        if (n <= 64) {
+            mid = wave_partition(C.tindex, begin, n, pred, lds_lane);
+        } else {
+            if (lane == 0) lds_lane[0] = partition_idx(C.tindex, begin, end, pred);
+            __syncthreads();
+            mid = lds_lane[0];
+        }
+        __syncthreads();   // the partitioned s_triIndex is read by other lanes below
+        object_split(mid);
         return;
     }
     // Accepted spatial split: append right-only references, duplicate straddlers
     // (bvh_sah.cpp:396-414), and clip children to the split plane (:427-445).
     const uint32_t tOff = C.toff[k], sOff = C.soff[k];
     const uint32_t leftEnd = begin + d.nl, strEnd = leftEnd + d.ns;
-    for (uint32_t q = 0; q < d.nr; ++q) C.tindex[tOff + q] = C.tindex[strEnd + q];
-    for (uint32_t q = 0; q < d.ns; ++q) {
-        uint32_t old = C.tindex[leftEnd + q];
-        C.sorted[sOff + q] = C.sorted[old];
-        C.tindex[tOff + d.nr + q] = sOff + q;
-    }
     const uint32_t lsize = d.nl + d.ns, rsize = d.nr + d.ns;
-    Box pbox{nd.lo, nd.hi};
+    const Box pbox{nd.lo, nd.hi};
     Box lclip = pbox, rclip = pbox;
     set_comp(lclip.hi, d.sAxis, d.plane);
     set_comp(rclip.lo, d.sAxis, d.plane);
-    Box lb = Box::empty(), rb = Box::empty();
-    for (uint32_t q = begin; q < begin + lsize; ++q) {
-        f3 p0, p1, p2;
-        tri_verts(C, C.sorted[C.tindex[q]], p0, p1, p2);
-        Box cb = clip_tri(p0, p1, p2, lclip);
-        if (cb.valid()) lb.extend(cb);
+    // Left child: positions [begin, begin + lsize), unchanged.
+    Box lacc = Box::empty();
+    for (uint32_t c0 = 0; c0 < lsize; c0 += 64) {
+        const uint32_t q = c0 + lane;
+        Box x = Box::empty();
+        if (q < lsize) {
+            f3 p0, p1, p2;
+            tri_verts(C, C.sorted[C.tindex[begin + q]], p0, p1, p2);
+            const Box cb = clip_tri(p0, p1, p2, lclip);
+            if (cb.valid()) x = cb;
+        }
+        lacc = combine(lacc, wave_box(x));
     }
-    for (uint32_t q = tOff; q < tOff + rsize; ++q) {
-        f3 p0, p1, p2;
-        tri_verts(C, C.sorted[C.tindex[q]], p0, p1, p2);
-        Box cb = clip_tri(p0, p1, p2, rclip);
-        if (cb.valid()) rb.extend(cb);
+    // Right child, in its new order: the nr right-only references, then the ns straddlers
+    // (their triangles read from the source positions, not from the copies written here).
+    Box racc = Box::empty();
+    for (uint32_t c0 = 0; c0 < rsize; c0 += 64) {
+        const uint32_t q = c0 + lane;
+        Box x = Box::empty();
+        if (q < rsize) {
+            uint32_t t;
+            if (q < d.nr) {
+                const uint32_t si = C.tindex[strEnd + q];
+                C.tindex[tOff + q] = si;
+                t = C.sorted[si];
+            } else {
+                const uint32_t qs = q - d.nr;
+                const uint32_t old = C.tindex[leftEnd + qs];
+                t = C.sorted[old];
+                C.sorted[sOff + qs] = t;
+                C.tindex[tOff + q] = sOff + qs;
+            }
+            f3 p0, p1, p2;
+            tri_verts(C, t, p0, p1, p2);
+            const Box cb = clip_tri(p0, p1, p2, rclip);
+            if (cb.valid()) x = cb;
+        }
+        racc = combine(racc, wave_box(x));
     }
-    set_child(C, c, lb, begin, lsize, F_LEAF | F_SAH);
-    set_child(C, c + 1, rb, tOff, rsize, F_LEAF | F_SAH);
-    BNode& p = C.nodes[ni];
-    p.flags &= ~F_LEAF;
-    p.a = c;
-    p.b = c + 1;
+    if (lane == 0) {
+        set_child(C, c, lacc, begin, lsize, F_LEAF | F_SAH);
+        set_child(C, c + 1, racc, tOff, rsize, F_LEAF | F_SAH);
+    }
+    link();
 }
 
 // Bottom-up boxes for Morton-phase nodes: box(parent range) = ordered combine of the
@@ -1141,7 +1244,7 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
         k_level_decide<<<nblk(cnt, 64), 64, 0, st>>>(C, ls, cnt);
         k_sah_wave<<<cnt, 64, 0, st>>>(C, ls);
         k_level_budget<<<1, 1024, 0, st>>>(C, cnt);
-        k_level_apply<<<nblk(cnt, 64), 64, 0, st>>>(C, ls, cnt);
+        k_level_apply<<<cnt, 64, 0, st>>>(C, ls, cnt);
         RTX_HIP(hipMemcpyAsync(&hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
         RTX_HIP(hipStreamSynchronize(st));
         if (hc.err || hc.total > ncap) return fail(RTX_E_LIMIT, "node capacity exceeded on rtx_build()");
