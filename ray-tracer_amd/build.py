@@ -22,14 +22,9 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=o
          "-Wno-unused-command-line-argument", "-I", os.path.join(HERE, "..", "include")]
 
 
-# Per-source flags. The SAH kernels clip triangles with small dynamically indexed polygon
-# arrays; a higher promote-alloca limit keeps them in registers instead of scratch memory.
-SOURCE_FLAGS = {"rtx_build.hip": ["-mllvm", "-amdgpu-promote-alloca-to-vector-limit=512"]}
-
-
 def _compile(src):
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, *SOURCE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src), "-o", out]
+    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")

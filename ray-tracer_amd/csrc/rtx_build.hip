@@ -99,47 +99,66 @@ __device__ Box range_box(const Ctx& C, uint32_t b, uint32_t e) {
     return r;
 }
 
-// Sutherland-Hodgman clip of a triangle to a box, then its AABB (AABB.cpp:86-171).
-// Same vertex order and float operations as the reference. The polygon (at most 9
-// vertices after 6 planes) is kept as three small coordinate arrays that the compiler
-// keeps in registers: an array of 3-float structs went to scratch memory, and a chain of
-// dependent scratch accesses per clip dominated the SAH kernels.
-__device__ Box clip_tri(f3 v0, f3 v1, f3 v2, const Box& r) {
-    constexpr int kMaxV = 10;
-    float px[kMaxV], py[kMaxV], pz[kMaxV], qx[kMaxV], qy[kMaxV], qz[kMaxV];
-    px[0] = v0.x; py[0] = v0.y; pz[0] = v0.z;
-    px[1] = v1.x; py[1] = v1.y; pz[1] = v1.z;
-    px[2] = v2.x; py[2] = v2.y; pz[2] = v2.z;
-    int n = 3;
-#pragma unroll
+// Sutherland-Hodgman clip of a triangle to a box, then its AABB (AABB.cpp:86-171), with
+// the reference's vertex order and float operations. The polygon (at most 9 vertices
+// after 6 planes) lives in a PolyStore:
+//  - PolyRegs: small per-lane arrays (the serial SAH path for nodes over 64 triangles);
+//  - PolyLds: a per-lane column of LDS (the wave kernels). Output positions differ
+//    between lanes, and indexing registers with a lane-varying index costs a loop over
+//    the distinct indices in the wave, while LDS addresses are per lane.
+constexpr int kPolyMax = 10;
+
+struct PolyRegs {
+    float v[2][3][kPolyMax];
+    __device__ __forceinline__ float& at(int buf, int c, int i) { return v[buf][c][i]; }
+};
+
+struct PolyLds {
+    float* base;   // kPolyLdsFloats per block, lane-minor: [buf][coord][vertex][64]
+    __device__ __forceinline__ float& at(int buf, int c, int i) { return base[((buf * 3 + c) * kPolyMax + i) * 64]; }
+};
+constexpr int kPolyLdsFloats = 2 * 3 * kPolyMax * 64;
+
+template <class PS>
+__device__ Box clip_tri_in(f3 v0, f3 v1, f3 v2, const Box& r, PS& P) {
+    P.at(0, 0, 0) = v0.x; P.at(0, 1, 0) = v0.y; P.at(0, 2, 0) = v0.z;
+    P.at(0, 0, 1) = v1.x; P.at(0, 1, 1) = v1.y; P.at(0, 2, 1) = v1.z;
+    P.at(0, 0, 2) = v2.x; P.at(0, 1, 2) = v2.y; P.at(0, 2, 2) = v2.z;
+    int n = 3, cur = 0;
     for (int axis = 0; axis < 3; ++axis) {
-#pragma unroll
         for (int side = 0; side < 2; ++side) {
             const float plane = side == 0 ? comp(r.lo, axis) : comp(r.hi, axis);
-            const float* pa = axis == 0 ? px : (axis == 1 ? py : pz);
+            const int nx = cur ^ 1;
             int m = 0;
             for (int i = 0; i < n; ++i) {
                 const int j = (i + 1 == n) ? 0 : i + 1;
-                const float pi = pa[i], pj = pa[j];
+                const float pi = P.at(cur, axis, i), pj = P.at(cur, axis, j);
                 const bool ci = side == 0 ? (pi >= plane) : (pi <= plane);
                 const bool cj = side == 0 ? (pj >= plane) : (pj <= plane);
-                if (ci) { qx[m] = px[i]; qy[m] = py[i]; qz[m] = pz[i]; ++m; }
+                const float xi = P.at(cur, 0, i), yi = P.at(cur, 1, i), zi = P.at(cur, 2, i);
+                if (ci) { P.at(nx, 0, m) = xi; P.at(nx, 1, m) = yi; P.at(nx, 2, m) = zi; ++m; }
                 if (ci != cj) {
                     const float t = (plane - pi) / (pj - pi);
-                    qx[m] = px[i] + t * (px[j] - px[i]);
-                    qy[m] = py[i] + t * (py[j] - py[i]);
-                    qz[m] = pz[i] + t * (pz[j] - pz[i]);
+                    const float xj = P.at(cur, 0, j), yj = P.at(cur, 1, j), zj = P.at(cur, 2, j);
+                    P.at(nx, 0, m) = xi + t * (xj - xi);
+                    P.at(nx, 1, m) = yi + t * (yj - yi);
+                    P.at(nx, 2, m) = zi + t * (zj - zi);
                     ++m;
                 }
             }
-            for (int i = 0; i < m; ++i) { px[i] = qx[i]; py[i] = qy[i]; pz[i] = qz[i]; }
+            cur = nx;
             n = m;
             if (n == 0) return Box::empty();
         }
     }
     Box b = Box::empty();
-    for (int i = 0; i < n; ++i) b.extend(f3{px[i], py[i], pz[i]});
+    for (int i = 0; i < n; ++i) b.extend(f3{P.at(cur, 0, i), P.at(cur, 1, i), P.at(cur, 2, i)});
     return b;
+}
+
+__device__ Box clip_tri(f3 v0, f3 v1, f3 v2, const Box& r) {
+    PolyRegs P;
+    return clip_tri_in(v0, v1, v2, r, P);
 }
 
 __device__ __forceinline__ void tri_verts(const Ctx& C, uint32_t t, f3& a, f3& b, f3& c) {
@@ -476,6 +495,7 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
     __shared__ float bbox[3 * 16][6];
     __shared__ uint32_t bc0[3 * 16], bc1[3 * 16];
     __shared__ uint32_t tsi_slot[64];   // lane 0's partition order
+    __shared__ float poly_lds[kPolyLdsFloats];
     if (blockIdx.x >= C.ctr->nwave) return;
     const uint32_t k = C.wlist[blockIdx.x];
     const uint32_t ni = ls + k;
@@ -589,8 +609,9 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
                 cin += (b0 == bin);
                 cout += (b1 == bin);
                 if (b0 <= bin && bin <= b1) {
-                    Box cb = clip_tri(f3{tv[i][0], tv[i][1], tv[i][2]}, f3{tv[i][3], tv[i][4], tv[i][5]},
-                                      f3{tv[i][6], tv[i][7], tv[i][8]}, reg);
+                    PolyLds P{poly_lds + lane};
+                    Box cb = clip_tri_in(f3{tv[i][0], tv[i][1], tv[i][2]}, f3{tv[i][3], tv[i][4], tv[i][5]},
+                                         f3{tv[i][6], tv[i][7], tv[i][8]}, reg, P);
                     if (cb.valid()) bx.extend(cb);
                 }
             }
@@ -844,6 +865,7 @@ __device__ uint32_t wave_partition(uint32_t* a, uint32_t first, uint32_t n, P pr
 // straddlers duplicated, both children clipped to the split plane.
 __global__ void __launch_bounds__(64) k_level_apply(Ctx C, uint32_t ls, uint32_t cnt) {
     __shared__ uint32_t lds_lane[64];
+    __shared__ float poly_lds[kPolyLdsFloats];
     const uint32_t k = blockIdx.x;
     if (k >= cnt) return;
     const uint32_t lane = __lane_id();
@@ -921,7 +943,8 @@ __global__ void __launch_bounds__(64) k_level_apply(Ctx C, uint32_t ls, uint32_t
         if (q < lsize) {
             f3 p0, p1, p2;
             tri_verts(C, C.sorted[C.tindex[begin + q]], p0, p1, p2);
-            const Box cb = clip_tri(p0, p1, p2, lclip);
+            PolyLds P{poly_lds + lane};
+            const Box cb = clip_tri_in(p0, p1, p2, lclip, P);
             if (cb.valid()) x = cb;
         }
         lacc = combine(lacc, wave_box(x));
@@ -947,7 +970,8 @@ __global__ void __launch_bounds__(64) k_level_apply(Ctx C, uint32_t ls, uint32_t
             }
             f3 p0, p1, p2;
             tri_verts(C, t, p0, p1, p2);
-            const Box cb = clip_tri(p0, p1, p2, rclip);
+            PolyLds P{poly_lds + lane};
+            const Box cb = clip_tri_in(p0, p1, p2, rclip, P);
             if (cb.valid()) x = cb;
         }
         racc = combine(racc, wave_box(x));
