@@ -233,7 +233,9 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
                         mb->rs_part, 0};
         k_ray_keys<<<blocks, kTraceThreads, 0, s>>>(A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, S.keys[0],
                                                     S.vals[0]);
-        radix_sort_pairs(S, (uint32_t)n, 0xFu, s);
+        // stable sort on the top 24 key bits (8 per axis): the lowest pass bought no
+        // locality the traversal could measure, sorting on 16 bits lost some
+        radix_sort_pairs(S, (uint32_t)n, 0xEu, s);
         perm = S.vals[S.cur];
     }
     if (g_tuning.trace_loop == 2) {
