@@ -192,8 +192,9 @@ void rtx_batch_free(rtx_batch* batch);
  *   ray_sort           1 = sort rays (>= 65536 of them) by a Morton key of their origins
  *                      and trace in that order, writing results back at each ray's index.
  *                      The sort's scratch lives on the BVH handle and grows to the largest
- *                      ray count seen, so with ray_sort on, trace calls that overlap on one
- *                      handle (other streams or threads) race;
+ *                      ray count seen. Trace calls that use the handle's scratch (sorting
+ *                      or trace_loop 2) are ordered on the GPU: each call's stream waits for
+ *                      the previous such call on that handle, from any stream or thread;
  *   wait_lanes         0..64: the while-while leaf phase starts once at most this many lanes
  *                      of a wave are still at internal nodes (0 = wait for all);
  *   trace_grid_blocks  blocks of the persistent trace kernel.

@@ -1166,6 +1166,8 @@ void free_products(rtx_bvh* b) {
                   b->rs_part};
     for (void* p : ps)
         if (p) (void)hipFree(p);
+    if (b->scratch_ev) (void)hipEventDestroy(b->scratch_ev);
+    b->scratch_ev = nullptr;
 }
 
 }  // namespace

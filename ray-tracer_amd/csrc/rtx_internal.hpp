@@ -1,5 +1,6 @@
 // Internal state of an rtx_bvh handle and helpers shared by the librtx sources.
 #pragma once
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -52,6 +53,11 @@ struct rtx_bvh {
     uint32_t* rs_hist = nullptr;
     uint32_t* rs_part = nullptr;
     uint32_t rs_cap = 0;
+    // Trace calls that use the handle's scratch (ray counter, sort buffers) are ordered on
+    // the GPU: each waits for the previous user's event on its own stream.
+    std::mutex scratch_mu;
+    hipEvent_t scratch_ev = nullptr;
+    bool scratch_used = false;
     // diagnostics
     uint64_t* codes = nullptr;        // Morton code per input triangle
     uint32_t* sorted = nullptr;       // s_sortedTris[0..nt)

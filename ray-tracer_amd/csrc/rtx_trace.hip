@@ -226,8 +226,24 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
     uint32_t blocks = (uint32_t)((n + kTraceThreads - 1) / kTraceThreads);
     const TraceArgs A = make_trace_args(b);
     const uint32_t* perm = nullptr;
-    if (g_tuning.ray_sort && n >= kSortMinRays) {
-        rtx_bvh* mb = const_cast<rtx_bvh*>(b);
+    rtx_bvh* mb = const_cast<rtx_bvh*>(b);
+    const bool sort = g_tuning.ray_sort && n >= kSortMinRays;
+    // The ray counter and the sort buffers belong to the handle: a call that uses them
+    // waits (on its own stream) for the previous such call, whatever stream that used.
+    std::unique_lock<std::mutex> lk(mb->scratch_mu, std::defer_lock);
+    const bool scratch = sort || g_tuning.trace_loop == 2;
+    if (scratch) {
+        lk.lock();
+        if (!mb->scratch_ev) RTX_HIP(hipEventCreateWithFlags(&mb->scratch_ev, hipEventDisableTiming));
+        if (mb->scratch_used) RTX_HIP(hipStreamWaitEvent(s, mb->scratch_ev, 0));
+    }
+    auto release = [&]() -> int {
+        if (!scratch) return RTX_OK;
+        RTX_HIP(hipEventRecord(mb->scratch_ev, s));
+        mb->scratch_used = true;
+        return RTX_OK;
+    };
+    if (sort) {
         if (!reserve_sort(mb, (uint32_t)n)) return fail(RTX_E_HIP, "device allocation failed for ray sorting");
         SortBuffers32 S{{mb->rs_keys[0], mb->rs_keys[1]}, {mb->rs_vals[0], mb->rs_vals[1]}, mb->rs_hist,
                         mb->rs_part, 0};
@@ -249,7 +265,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
         });
         prof_stop(pk, s);
         RTX_HIP(hipGetLastError());
-        return RTX_OK;
+        return release();
     }
     const int pk = prof_start(s);
     with_trace_variant(g_tuning.trace_lds_depth, g_tuning.trace_loop, [&](auto D, auto L) {
@@ -259,7 +275,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
     });
     prof_stop(pk, s);
     RTX_HIP(hipGetLastError());
-    return RTX_OK;
+    return release();
 }
 
 }  // namespace

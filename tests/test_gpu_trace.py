@@ -215,3 +215,38 @@ def test_kernel_timer_counts_traversal_launches():
         assert kt.launches == 0 and kt.ms == 0.0
     finally:
         rtx.set_tuning(**saved)
+
+
+def test_concurrent_streams_share_one_handle():
+    """Trace calls on two streams against one handle (persistent kernel + ray sorting, both
+    of which use the handle's scratch) give the same hits as the calls made one at a time.
+    Both ray sets have the same length."""
+    import torch
+    v, t = S.random_triangles(100_000, seed=8)
+    g = rtx.build(v, t)
+    n = 1 << 17
+    ra = torch.from_numpy(S.incoherent_rays(n, seed=21)).cuda()
+    rb = torch.from_numpy(S.incoherent_rays(n, seed=22)).cuda()
+    saved = rtx.get_tuning()
+    try:
+        rtx.set_tuning(trace_loop=2, ray_sort=1)
+        ha = torch.zeros((n, 4), device="cuda")
+        hb = torch.zeros((n, 4), device="cuda")
+        g.trace_closest_device(ra, ha)
+        g.trace_closest_device(rb, hb)
+        torch.cuda.synchronize()
+        ref_a, ref_b = ha.view(torch.int32).cpu().clone(), hb.view(torch.int32).cpu().clone()
+        s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+        for rep in range(4):
+            ha.fill_(-1.0)
+            hb.fill_(-1.0)
+            torch.cuda.synchronize()
+            with torch.cuda.stream(s1):
+                g.trace_closest_device(ra, ha, stream=s1)
+            with torch.cuda.stream(s2):
+                g.trace_closest_device(rb, hb, stream=s2)
+            torch.cuda.synchronize()
+            assert torch.equal(ha.view(torch.int32).cpu(), ref_a), rep
+            assert torch.equal(hb.view(torch.int32).cpu(), ref_b), rep
+    finally:
+        rtx.set_tuning(**saved)
