@@ -45,8 +45,9 @@ def main():
     ap.add_argument("--rays", type=int, default=1 << 22)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-render", action="store_true", help="skip the batched (C3) and differentiable (C4) legs")
-    ap.add_argument("--c3-scenes", type=int, default=64)
+    ap.add_argument("--no-render", action="store_true",
+                    help="skip the render legs (batched 64-scene, differentiable, Cornell box)")
+    ap.add_argument("--batched-scenes", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (the reference point that scales HBM "
                          "counters under the profiler)")
@@ -163,7 +164,7 @@ def main():
     if not (args.no_render or args.skip_extras):
         del inc, coh, hits, occ, counts
         torch.cuda.empty_cache()
-        batched = bench_batched(stream, args.c3_scenes, tune=not args.no_autotune)
+        batched = bench_batched(stream, args.batched_scenes, tune=not args.no_autotune)
         render = {"differentiable": bench_differentiable(stream), "batched": batched,
                   "cornell": bench_cornell(stream, cpu=not args.no_cpu)}
 
@@ -255,7 +256,7 @@ def bench_differentiable(stream, reps=3):
 
 
 def bench_cornell(stream, cpu=True, reps=3):
-    """C1: open Cornell box (20 wall + 2 light + 1280 icosphere triangles), 256x256,
+    """Cornell box: open box (20 wall + 2 light + 1280 icosphere triangles), 256x256,
     16 spp, max_bounce 5, roulette from bounce 3; GPU render time, and the oracle's
     render of the same image on the host at 1 thread and at all cores."""
     import torch

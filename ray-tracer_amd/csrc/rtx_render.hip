@@ -136,7 +136,8 @@ __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* _
 }
 
 // The wavefront trace loads leaf triangles two at a time and is held to 64 VGPRs so that
-// 8 waves fit per SIMD (measured faster than 4-triangle batches at 5 waves for C3 and C4).
+// 8 waves fit per SIMD (measured faster than 4-triangle batches at 5 waves, both for the
+// 64-scene batched render and for the 4118-triangle differentiable render).
 constexpr int kRenderTriBatch = 2;
 
 template <int DEPTH, int LOOP>
