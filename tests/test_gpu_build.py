@@ -37,7 +37,7 @@ def _check(name, v, t):
 
 
 @pytest.mark.parametrize("name", ["rand1", "rand5", "rand17", "rand65", "rand100", "rand1000", "rand20000",
-                                  "cornell", "test_traverse", "straddle", "dupcodes"])
+                                  "cornell", "test_traverse", "straddle", "dupcodes", "degenerate", "grid"])
 def test_build_matches_oracle(suite, name):
     v, t = suite[name]
     _check(name, v, t)

@@ -40,7 +40,7 @@ def _device_trace(g, rays):
 
 
 @pytest.mark.parametrize("name", ["rand1", "rand17", "rand1000", "rand20000", "cornell", "test_traverse",
-                                  "straddle", "dupcodes"])
+                                  "straddle", "dupcodes", "degenerate", "grid"])
 def test_trace_matches_oracle(suite, name):
     v, t = suite[name]
     print(f"[trace parity] {name}", flush=True)
@@ -58,6 +58,35 @@ def test_trace_matches_oracle(suite, name):
     np.testing.assert_allclose(nrm, ref["normal"], rtol=0, atol=1e-6)
     if name not in ("dupcodes", "rand1"):
         assert (tri != 0xFFFFFFFF).any()
+
+
+@pytest.mark.parametrize("name", ["grid", "degenerate", "cornell"])
+def test_edge_rays_all_variants(suite, name):
+    """Zero, -0 and denormal direction components and origins on box planes (the exact
+    NaN-aware slab test) mixed with ordinary rays (the fast slab test) in the same waves:
+    every loop shape, with and without ray sorting, equals the oracle bit for bit, and so
+    do the counts."""
+    import torch
+    from scenes_util import edge_rays
+    v, t = suite[name]
+    g = rtx.build(v, t)
+    o = O.build(v, t)
+    rays = edge_rays(v, 80000, seed=9)
+    ref = O.trace(o, rays, counts=True, normals=True, nthreads=8)
+    saved = rtx.get_tuning()
+    try:
+        for loop, sort, wait in ((0, 0, 0), (1, 0, 8), (2, 1, 16), (2, 0, 0)):
+            rtx.set_tuning(trace_loop=loop, ray_sort=sort, wait_lanes=wait)
+            print(f"[edge rays] {name} loop={loop} sort={sort}", flush=True)
+            h, nrm, occ, cnt = _device_trace(g, rays)
+            assert np.array_equal(h[:, 3].view(np.uint32), ref["tri"]), (loop, sort)
+            for k, c in (("t", 0), ("u", 1), ("v", 2)):
+                hit = ref["tri"] != 0xFFFFFFFF
+                assert np.array_equal(h[hit, c].view(np.uint32), ref[k][hit].view(np.uint32)), (k, loop, sort)
+            assert np.array_equal(occ.astype(bool), ref["tri"] != 0xFFFFFFFF), (loop, sort)
+            assert np.array_equal(cnt, ref["counts"]), (loop, sort)
+    finally:
+        rtx.set_tuning(**saved)
 
 
 def test_trace_host_wrapper_and_reference_rays(suite):

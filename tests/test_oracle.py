@@ -22,7 +22,7 @@ def suite():
 
 @needs_ref
 @pytest.mark.parametrize("name", ["rand1", "rand5", "rand17", "rand65", "rand100", "rand1000", "rand20000",
-                                  "cornell", "test_traverse", "straddle", "dupcodes"])
+                                  "cornell", "test_traverse", "straddle", "dupcodes", "degenerate", "grid"])
 def test_build_matches_compiled_reference(suite, name):
     v, t = suite[name]
     o = O.build(v, t)
@@ -136,7 +136,8 @@ def test_trace_matches_brute_force(suite, name):
 
 
 @needs_ref
-@pytest.mark.parametrize("name", ["rand20000", "cornell", "test_traverse", "straddle", "dupcodes"])
+@pytest.mark.parametrize("name", ["rand20000", "cornell", "test_traverse", "straddle", "dupcodes", "degenerate",
+                                  "grid"])
 def test_trace_matches_compiled_reference(suite, name):
     v, t = suite[name]
     r = O.ref_build(v, t)
@@ -151,6 +152,25 @@ def test_trace_matches_compiled_reference(suite, name):
     assert np.array_equal(a["normal"].view(np.uint32), b["normal"].view(np.uint32))
     anyh = O.trace(r, rays, any_hit=True)
     assert np.array_equal(anyh["tri"] != 0xFFFFFFFF, b["tri"] != 0xFFFFFFFF)
+
+
+@needs_ref
+@pytest.mark.parametrize("name", ["grid", "degenerate", "cornell", "rand1000"])
+def test_edge_rays_match_compiled_reference(suite, name):
+    """Zero, -0 and denormal direction components, origins on box planes (0 * inf in the
+    slab test) and tmin/tmax windows: the oracle traversal equals the reference bit for bit."""
+    from scenes_util import edge_rays
+    v, t = suite[name]
+    r = O.ref_build(v, t)
+    rays = edge_rays(v, 8000, seed=5)
+    a = O.trace(r, rays, normals=True)
+    b = O.ref_trace(r, rays, normals=True)
+    for k in ("t", "u", "v"):
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
+    assert np.array_equal(a["tri"], b["tri"])
+    assert np.array_equal(a["normal"].view(np.uint32), b["normal"].view(np.uint32))
+    if name == "grid":
+        assert (a["tri"] != 0xFFFFFFFF).mean() > 0.2
 
 
 def test_reference_traverse_known_answers(suite):
