@@ -140,10 +140,21 @@ typedef struct rtx_render_params {
     int rr_bounce;    /* Russian roulette when bounce >= rr_bounce (reference: 3, main.cpp:188) */
     uint32_t seed;    /* sample-table seed */
     int env_mode;     /* 0 = sky gradient (main.cpp:169-172), 1 = constant env_color,
-                         2 = equirectangular IBL map ibl_d (IBL::sample, main.cpp:107-117) */
+                         2 = equirectangular IBL map ibl_d (IBL::sample, main.cpp:107-117),
+                         3 = env_color plus sun_radiance inside the cone dot(d, sun_dir) >= sun_cos */
     float env_color[3];
     const float* ibl_d;   /* env_mode 2: ibl_width * ibl_height * 3 floats in HBM, top row first */
     int ibl_width, ibl_height;
+    /* Texture albedo (optional, an extension): uv_d holds 2 floats per input vertex, tex_d a
+     * tex_width x tex_height RGB texture (texel (x, y) at 3 * (y * tex_width + x)). When tex_d
+     * is set, the albedo at a hit is the nearest texel at the uv interpolated like the shading
+     * normal; the albedo argument is unused and rtx_render_adjoint's grad_albedo is per texel. */
+    const float* uv_d;
+    const float* tex_d;
+    int tex_width, tex_height;
+    float sun_dir[3];       /* env_mode 3: a distant sun seen by the rays that reach its cone */
+    float sun_cos;
+    float sun_radiance[3];
 } rtx_render_params;
 
 /* Wavefront path tracer: replaces the raster loop + pathTrace (src/main.cpp:145-204,

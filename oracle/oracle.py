@@ -26,7 +26,9 @@ class Camera(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("max_bounce", C.c_int),
                 ("rr_bounce", C.c_int), ("seed", C.c_uint32), ("env_mode", C.c_int),
-                ("env_color", C.c_float * 3), ("ibl", C.c_void_p), ("ibl_w", C.c_int), ("ibl_h", C.c_int)]
+                ("env_color", C.c_float * 3), ("ibl", C.c_void_p), ("ibl_w", C.c_int), ("ibl_h", C.c_int),
+                ("uv", C.c_void_p), ("tex", C.c_void_p), ("tex_w", C.c_int), ("tex_h", C.c_int),
+                ("sun_dir", C.c_float * 3), ("sun_cos", C.c_float), ("sun_rad", C.c_float * 3)]
 
 
 def _p(a):
@@ -175,18 +177,38 @@ def make_camera(cam):
     return c
 
 
-def make_params(width, height, spp, max_bounce, rr_bounce=3, seed=0, env_mode=0, env_color=(0, 0, 0), ibl=None):
-    """ibl: (h, w, 3) float32 array, top row first (env_mode 2); kept alive on the params."""
+def make_params(width, height, spp, max_bounce, rr_bounce=3, seed=0, env_mode=0, env_color=(0, 0, 0), ibl=None,
+                uv=None, tex=None, sun=None):
+    """ibl: (h, w, 3) float32 array, top row first (env_mode 2); uv: (nverts, 2) and tex:
+    (h, w, 3) for texture albedo; sun: (direction, cos of the cone, radiance) for env_mode 3.
+    Arrays are kept alive on the params."""
     p = RenderParams()
     p.width, p.height, p.spp, p.max_bounce, p.rr_bounce, p.seed = width, height, spp, max_bounce, rr_bounce, seed
     p.env_mode = env_mode
     p.env_color[:] = [float(x) for x in env_color]
+    p._keep = []
     if ibl is not None:
         arr = np.ascontiguousarray(ibl, np.float32)
-        p._ibl_keep = arr
+        p._keep.append(arr)
         p.ibl = arr.ctypes.data
         p.ibl_h, p.ibl_w = arr.shape[:2]
+    if tex is not None:
+        set_texture(p, uv, tex)
+    if sun is not None:
+        d, c, rad = sun
+        p.sun_dir[:] = [float(x) for x in d]
+        p.sun_cos = float(c)
+        p.sun_rad[:] = [float(x) for x in rad]
     return p
+
+
+def set_texture(p, uv, tex):
+    """Point the params at a (possibly new) texture; uv (nverts, 2), tex (h, w, 3)."""
+    u = np.ascontiguousarray(uv, np.float32)
+    t = np.ascontiguousarray(tex, np.float32)
+    p._keep += [u, t]
+    p.uv, p.tex = u.ctypes.data, t.ctypes.data
+    p.tex_h, p.tex_w = t.shape[:2]
 
 
 def render(tree, cam, params, albedo=None, emission=None, nthreads=1):
@@ -202,7 +224,7 @@ def render(tree, cam, params, albedo=None, emission=None, nthreads=1):
 
 
 def render_loss_f64(tree, cam, params, albedo, emission, target, nthreads=1):
-    a = np.ascontiguousarray(albedo, np.float32)
+    a = None if albedo is None else np.ascontiguousarray(albedo, np.float32)
     e = None if emission is None else np.ascontiguousarray(emission, np.float32)
     tg = np.ascontiguousarray(target, np.float32)
     return lib().orc_render_loss_f64(_p(tree.nodes), len(tree.nodes), _p(tree.tris), _p(tree.prims),

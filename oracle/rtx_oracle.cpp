@@ -798,10 +798,33 @@ inline V3 ibl_sample(const orc_render_params* p, V3 d) {
 template <class R>
 inline Vec3T<R> env_color(const orc_render_params* p, V3 d) {
     if (p->env_mode == 1) return {(R)p->env_color[0], (R)p->env_color[1], (R)p->env_color[2]};
+    if (p->env_mode == 3) {
+        // constant sky plus a sun disk (a distant light seen by the rays that reach it)
+        const float c = dot(d, V3{p->sun_dir[0], p->sun_dir[1], p->sun_dir[2]});
+        float r = p->env_color[0], g = p->env_color[1], b = p->env_color[2];
+        if (c >= p->sun_cos) { r = r + p->sun_rad[0]; g = g + p->sun_rad[1]; b = b + p->sun_rad[2]; }
+        return {(R)r, (R)g, (R)b};
+    }
     if (p->env_mode == 2) { V3 e = ibl_sample(p, d); return {(R)e.x, (R)e.y, (R)e.z}; }
     float t = d.y * 0.5f + 0.5f;
     float omt = 1.f - t;
     return {(R)(1.0f * omt + 0.5f * t), (R)(1.0f * omt + 0.7f * t), (R)(1.0f * omt + 1.0f * t)};
+}
+
+// Texel of a hit for texture albedo: uv interpolated like the shading normal, then the
+// nearest texel (x86 truncation, clamped to the texture).
+inline uint32_t texel_at(const orc_render_params* p, const uint32_t* t3, float u, float v) {
+    const float w = 1.f - u - v;
+    const float* a = p->uv + 2 * (size_t)t3[0];
+    const float* b = p->uv + 2 * (size_t)t3[1];
+    const float* c = p->uv + 2 * (size_t)t3[2];
+    const float U = (w * a[0] + u * b[0]) + v * c[0];
+    const float V = (w * a[1] + u * b[1]) + v * c[1];
+    int px = std::min(f2i(U * (float)p->tex_w), p->tex_w - 1);
+    int py = std::min(f2i(V * (float)p->tex_h), p->tex_h - 1);
+    px = std::max(px, 0);
+    py = std::max(py, 0);
+    return (uint32_t)py * (uint32_t)p->tex_w + (uint32_t)px;
 }
 
 // One path. R = float reproduces the renderer; R = double is the finite-difference
@@ -822,7 +845,13 @@ Vec3T<R> path(const SceneRef& S, const orc_render_params* p, RayF ray, uint32_t 
             L = {L.x + tp.x * (R)e[0], L.y + tp.y * (R)e[1], L.z + tp.z * (R)e[2]};
         }
         R ax = (R)0.8f, ay = (R)0.8f, az = (R)0.8f;
-        if (S.albedo) { ax = S.albedo[3 * prim]; ay = S.albedo[3 * prim + 1]; az = S.albedo[3 * prim + 2]; }
+        if (p->tex) {
+            const uint32_t* t3 = S.T.tris + 3 * (size_t)h.tri;
+            const float* a = p->tex + 3 * (size_t)texel_at(p, t3, h.u, h.v);
+            ax = a[0]; ay = a[1]; az = a[2];
+        } else if (S.albedo) {
+            ax = S.albedo[3 * prim]; ay = S.albedo[3 * prim + 1]; az = S.albedo[3 * prim + 2];
+        }
         float u1 = orc_sample(p->seed, pixel, s, 2 + 3 * bounce);
         float u2 = orc_sample(p->seed, pixel, s, 3 + 3 * bounce);
         V3 wd = to_world(cos_hemisphere(u1, u2), h.n);

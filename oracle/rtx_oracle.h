@@ -84,10 +84,18 @@ typedef struct orc_render_params {
     int width, height, spp, max_bounce;
     int rr_bounce;      /* Russian roulette applies when bounce >= rr_bounce (reference: 3) */
     uint32_t seed;
-    int env_mode;       /* 0 = reference sky gradient, 1 = constant env_color, 2 = IBL map */
+    int env_mode;       /* 0 = reference sky gradient, 1 = constant env_color, 2 = IBL map,
+                           3 = env_color plus sun_rad inside the cone dot(d, sun_dir) >= sun_cos */
     float env_color[3];
     const float* ibl;   /* env_mode 2: ibl_w * ibl_h * 3 floats, top row first */
     int ibl_w, ibl_h;
+    const float* uv;    /* texture albedo: 2 floats per vertex */
+    const float* tex;   /* tex_w * tex_h * 3 floats; when set, albedo is the nearest texel at the
+                           interpolated uv and the albedo argument is unused */
+    int tex_w, tex_h;
+    float sun_dir[3];
+    float sun_cos;
+    float sun_rad[3];
 } orc_render_params;
 
 /* Path-trace loop of main.cpp:145-204 / 339-365 driven by the sample table.

@@ -37,6 +37,10 @@ struct SceneDesc {
     uint32_t nprim;      // original triangle count (gradient array length / 3)
     const float* ibl;
     int ibl_w, ibl_h;
+    const float* uv;     // texture albedo: 2 floats per input vertex
+    const float* tex;    // tex_w * tex_h * 3; null = per-triangle albedo
+    int tex_w, tex_h;
+    float sun_dir[3], sun_cos, sun_rad[3];
 };
 
 struct PathState {
@@ -90,13 +94,37 @@ __device__ __forceinline__ f3 bounce_dir(float u1, float u2, f3 n) {
     return add(add(scl(l.x, t), scl(l.y, n)), scl(l.z, b));
 }
 
-// Miss shading: the sky gradient of main.cpp:169-172, a constant, or the IBL map.
+// Miss shading: the sky gradient of main.cpp:169-172, a constant, the IBL map, or a
+// constant sky plus a sun disk (a distant light seen by the rays that reach its cone).
 __device__ __forceinline__ f3 env_color(const SceneDesc& S, f3 d) {
     if (S.env_mode == 1) return f3{S.env[0], S.env[1], S.env[2]};
     if (S.env_mode == 2) return ibl_lookup(S.ibl, S.ibl_w, S.ibl_h, d);
+    if (S.env_mode == 3) {
+        const float c = dot(d, f3{S.sun_dir[0], S.sun_dir[1], S.sun_dir[2]});
+        if (c >= S.sun_cos) return f3{S.env[0] + S.sun_rad[0], S.env[1] + S.sun_rad[1], S.env[2] + S.sun_rad[2]};
+        return f3{S.env[0], S.env[1], S.env[2]};
+    }
     float t = d.y * 0.5f + 0.5f;
     float omt = 1.f - t;
     return f3{1.0f * omt + 0.5f * t, 1.0f * omt + 0.7f * t, 1.0f * omt + 1.0f * t};
+}
+
+// Texel of a hit for texture albedo: uv interpolated like the shading normal, then the
+// nearest texel (x86 truncation, clamped to the texture).
+__device__ __forceinline__ uint32_t texel_at(const SceneDesc& S, uint32_t tri, float u, float v) {
+    const uint32_t* t3 = S.T.tris3 + 3 * (size_t)tri;
+    const float w = 1.f - u - v;
+    const float2 a = reinterpret_cast<const float2*>(S.uv)[t3[0]];
+    const float2 b = reinterpret_cast<const float2*>(S.uv)[t3[1]];
+    const float2 c = reinterpret_cast<const float2*>(S.uv)[t3[2]];
+    const float U = (w * a.x + u * b.x) + v * c.x;
+    const float V = (w * a.y + u * b.y) + v * c.y;
+    int px = f2i_x86(U * (float)S.tex_w), py = f2i_x86(V * (float)S.tex_h);
+    px = px < S.tex_w - 1 ? px : S.tex_w - 1;
+    py = py < S.tex_h - 1 ? py : S.tex_h - 1;
+    px = px < 0 ? 0 : px;
+    py = py < 0 ? 0 : py;
+    return (uint32_t)py * (uint32_t)S.tex_w + (uint32_t)px;
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint64_t n) {
@@ -194,18 +222,21 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
                 L = make_float4(L.x + tp.x * e.x, L.y + tp.y * e.y, L.z + tp.z * e.z, 0.f);
             } else {
                 const uint32_t prim = S.prims[tri];
+                // albedo entry: the hit's texel (texture albedo) or its original triangle
+                const float* alb = S.tex ? S.tex : S.albedo;
+                const uint32_t aidx = S.tex ? texel_at(S, tri, h.y, h.z) : prim;
                 if (ADJ) P.vtp[(size_t)bounce * P.B + i] = tp;
                 if (S.emission) {
                     const float* e = S.emission + 3 * prim;
                     L = make_float4(L.x + tp.x * e[0], L.y + tp.y * e[1], L.z + tp.z * e[2], 0.f);
                 }
                 if (ADJ) {
-                    P.vlup[(size_t)bounce * P.B + i] = make_float4(L.x, L.y, L.z, __uint_as_float(prim));
+                    P.vlup[(size_t)bounce * P.B + i] = make_float4(L.x, L.y, L.z, __uint_as_float(aidx));
                     P.nv[i] = (uint32_t)bounce + 1;
                 }
                 if (bounce < S.max_bounce) {
                     float ax = 0.8f, ay = 0.8f, az = 0.8f;
-                    if (S.albedo) { ax = S.albedo[3 * prim]; ay = S.albedo[3 * prim + 1]; az = S.albedo[3 * prim + 2]; }
+                    if (alb) { ax = alb[3 * aidx]; ay = alb[3 * aidx + 1]; az = alb[3 * aidx + 2]; }
                     const uint32_t px = P.pix[i], sm = P.smp[i];
                     const float u1 = sample01(S.seed, px, sm, 2 + 3 * bounce);
                     const float u2 = sample01(S.seed, px, sm, 3 + 3 * bounce);
@@ -347,8 +378,8 @@ __global__ void __launch_bounds__(kGradThreads) k_grad(const SceneDesc* __restri
             if (has) {
                 const float4 lu = P.vlup[(size_t)k * P.B + i];
                 const float4 tp = P.vtp[(size_t)k * P.B + i];
-                prim = __float_as_uint(lu.w);
-                const float* a = S.albedo + 3 * prim;
+                prim = __float_as_uint(lu.w);   // albedo entry: texel or original triangle
+                const float* a = (S.tex ? S.tex : S.albedo) + 3 * prim;
                 gA = f3{delta.x * (L.x - lu.x) / a[0], delta.y * (L.y - lu.y) / a[1], delta.z * (L.z - lu.z) / a[2]};
                 gE = f3{delta.x * tp.x, delta.y * tp.y, delta.z * tp.z};
             }
@@ -397,6 +428,13 @@ SceneDesc make_desc(const rtx_bvh* b, const rtx_camera* c, const rtx_render_para
     S.ibl = p->ibl_d;
     S.ibl_w = p->ibl_width;
     S.ibl_h = p->ibl_height;
+    S.uv = p->uv_d;
+    S.tex = p->tex_d;
+    S.tex_w = p->tex_width;
+    S.tex_h = p->tex_height;
+    if (S.tex) S.nprim = (uint32_t)(p->tex_width * p->tex_height);   // gradient entries: texels
+    for (int k = 0; k < 3; ++k) { S.sun_dir[k] = p->sun_dir[k]; S.sun_rad[k] = p->sun_radiance[k]; }
+    S.sun_cos = p->sun_cos;
     return S;
 }
 
@@ -404,8 +442,11 @@ int check_scene(const rtx_bvh* b, const rtx_camera* c, const rtx_render_params* 
                 const char* fn) {
     if (!b || !c || !p || !image) return fail(RTX_E_ARG, std::string("null argument on ") + fn);
     if (!b->gnodes) return fail(RTX_E_STATE, std::string("BVH not loaded on ") + fn);
-    if (p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->max_bounce < 0 || p->env_mode < 0 || p->env_mode > 2)
+    if (p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->max_bounce < 0 || p->env_mode < 0 || p->env_mode > 3)
         return fail(RTX_E_ARG, std::string("bad render parameters on ") + fn);
+    if (p->tex_d && (!p->uv_d || p->tex_width <= 0 || p->tex_height <= 0 ||
+                     (uint64_t)p->tex_width * p->tex_height > 0x3FFFFFFFull))
+        return fail(RTX_E_ARG, std::string("texture albedo needs uv_d and a valid texture size on ") + fn);
     if (p->env_mode == 2 && (!p->ibl_d || p->ibl_width <= 0 || p->ibl_height <= 0))
         return fail(RTX_E_ARG, std::string("env_mode 2 needs an IBL map on ") + fn);
     if ((uint64_t)p->width * p->height > 0xFFFFFFFFull / 3)
@@ -639,7 +680,9 @@ extern "C" int rtx_render_adjoint(const rtx_bvh* b, const rtx_camera* cam, const
                                   float* grad_albedo_d, float* grad_emission_d, void* stream) {
     int rc = check_scene(b, cam, p, d_image_d, "rtx_render_adjoint()");
     if (rc) return rc;
-    if (!albedo_d) return fail(RTX_E_ARG, "albedo is required on rtx_render_adjoint()");
+    if (!albedo_d && !p->tex_d) return fail(RTX_E_ARG, "albedo is required on rtx_render_adjoint()");
+    if (p->tex_d && grad_emission_d)
+        return fail(RTX_E_ARG, "emission gradients are per triangle: not with texture albedo on rtx_render_adjoint()");
     if (p->rr_bounce <= p->max_bounce)
         return fail(RTX_E_ARG, "the adjoint needs rr_bounce > max_bounce (no Russian roulette)");
     if (p->max_bounce > 64) return fail(RTX_E_LIMIT, "max_bounce above 64 on rtx_render_adjoint()");

@@ -47,7 +47,9 @@ class Camera(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("max_bounce", C.c_int),
                 ("rr_bounce", C.c_int), ("seed", C.c_uint32), ("env_mode", C.c_int),
-                ("env_color", C.c_float * 3), ("ibl_d", C.c_void_p), ("ibl_width", C.c_int), ("ibl_height", C.c_int)]
+                ("env_color", C.c_float * 3), ("ibl_d", C.c_void_p), ("ibl_width", C.c_int), ("ibl_height", C.c_int),
+                ("uv_d", C.c_void_p), ("tex_d", C.c_void_p), ("tex_width", C.c_int), ("tex_height", C.c_int),
+                ("sun_dir", C.c_float * 3), ("sun_cos", C.c_float), ("sun_radiance", C.c_float * 3)]
 
 
 class Tuning(C.Structure):
@@ -284,18 +286,38 @@ def make_camera(cam):
     return c
 
 
-def make_params(width, height, spp, max_bounce, rr_bounce=3, seed=0, env_mode=0, env_color=(0, 0, 0), ibl=None):
-    """ibl: (h, w, 3) float32 CUDA tensor, top row first, for env_mode 2 (kept alive on the
-    returned params)."""
+def make_params(width, height, spp, max_bounce, rr_bounce=3, seed=0, env_mode=0, env_color=(0, 0, 0), ibl=None,
+                uv=None, tex=None, sun=None):
+    """ibl: (h, w, 3) float32 CUDA tensor, top row first, for env_mode 2. uv: (nverts, 2)
+    and tex: (h, w, 3) float32 CUDA tensors for texture albedo. sun: (direction, cos of the
+    cone, radiance) for env_mode 3. Tensors are kept alive on the returned params."""
     p = RenderParams()
     p.width, p.height, p.spp, p.max_bounce, p.rr_bounce, p.seed = width, height, spp, max_bounce, rr_bounce, seed
     p.env_mode = env_mode
     p.env_color[:] = [float(x) for x in env_color]
+    p._keep = []
     if ibl is not None:
-        p._ibl_keep = ibl
+        p._keep.append(ibl)
         p.ibl_d = ibl.data_ptr()
         p.ibl_height, p.ibl_width = int(ibl.shape[0]), int(ibl.shape[1])
+    if tex is not None:
+        set_texture(p, uv, tex)
+    if sun is not None:
+        d, c, rad = sun
+        p.sun_dir[:] = [float(x) for x in d]
+        p.sun_cos = float(c)
+        p.sun_radiance[:] = [float(x) for x in rad]
     return p
+
+
+def set_texture(p, uv, tex):
+    """Point render params at texture albedo: uv (nverts, 2), tex (h, w, 3), contiguous
+    float32 CUDA tensors (kept alive on the params)."""
+    if not (uv.is_contiguous() and tex.is_contiguous()):
+        raise ValueError("uv and tex must be contiguous")
+    p._tex_keep = (uv, tex)
+    p.uv_d, p.tex_d = uv.data_ptr(), tex.data_ptr()
+    p.tex_height, p.tex_width = int(tex.shape[0]), int(tex.shape[1])
 
 
 def write_hdr(path, img, flip_y=True):
@@ -362,11 +384,14 @@ def render(bvh, cam, params, albedo=None, emission=None, out=None, stream=None):
 
 
 def render_adjoint(bvh, cam, params, albedo, emission, d_image, grad_albedo=None, grad_emission=None, stream=None):
-    """Accumulate dLoss/dalbedo and dLoss/demission given dLoss/dimage (path replay)."""
+    """Accumulate dLoss/dalbedo and dLoss/demission given dLoss/dimage (path replay). With
+    texture albedo (params.tex_d set) grad_albedo is per texel and there is no emission
+    gradient."""
     import torch
+    textured = bool(params.tex_d)
     if grad_albedo is None:
-        grad_albedo = torch.zeros_like(albedo)
-    if grad_emission is None and emission is not None:
+        grad_albedo = torch.zeros_like(params._tex_keep[1] if textured else albedo)
+    if grad_emission is None and emission is not None and not textured:
         grad_emission = torch.zeros_like(emission)
     _check(lib().rtx_render_adjoint(bvh.handle, C.byref(make_camera(cam)), C.byref(params), _dptr(albedo),
                                     _dptr(emission), _dptr(d_image.contiguous()), _dptr(grad_albedo),
@@ -380,7 +405,7 @@ class Batch:
 
     def __init__(self, bvhs, cams, params, albedos, emissions, images, mode):
         n = len(bvhs)
-        self._keep = (bvhs, albedos, emissions, images)
+        self._keep = (bvhs, albedos, emissions, images, params)   # params hold IBL/texture tensors
         hs = (C.c_void_p * n)(*[b.handle for b in bvhs])
         cs = (Camera * n)(*[make_camera(c) for c in cams])
         ps = (RenderParams * n)(*params)
@@ -423,7 +448,44 @@ def _autograd_fn():
     return RenderFunction
 
 
+def _texture_autograd_fn():
+    import torch
+
+    class TextureRenderFunction(torch.autograd.Function):
+        """image = render with albedo from a texture; backward re-traces the paths and
+        accumulates the gradient per texel."""
+
+        @staticmethod
+        def forward(ctx, tex, uv, emission, bvh, cam, params):
+            tex_c = tex.detach().contiguous()
+            set_texture(params, uv, tex_c)
+            img = render(bvh, cam, params, None, emission)
+            ctx.save_for_backward(tex_c)
+            ctx.scene = (uv, emission, bvh, cam, params)
+            return img
+
+        @staticmethod
+        def backward(ctx, grad_img):
+            (tex_c,) = ctx.saved_tensors
+            uv, emission, bvh, cam, params = ctx.scene
+            set_texture(params, uv, tex_c)
+            gtex, _ = render_adjoint(bvh, cam, params, None, emission, grad_img.contiguous())
+            return gtex, None, None, None, None, None
+
+    return TextureRenderFunction
+
+
 _RF = None
+_TF = None
+
+
+def differentiable_render_texture(tex, uv, bvh, cam, params, emission=None):
+    """torch.autograd entry for texture albedo: gradients flow to tex ((h, w, 3) CUDA
+    tensor; uv (nverts, 2)). Requires params.rr_bounce > params.max_bounce."""
+    global _TF
+    if _TF is None:
+        _TF = _texture_autograd_fn()
+    return _TF.apply(tex, uv, emission, bvh, cam, params)
 
 
 def differentiable_render(albedo, emission, bvh, cam, params):

@@ -274,3 +274,72 @@ def batched_scenes(n=64, seed=2, lo=1000, hi=500_000, displace=0.05):
         cam = camera_lookat((0.3 * math.sin(i), 0.4, -3.2), (0, 0, 0), 45.0)
         out.append(dict(verts=v, tris=t, cam=cam, width=res, height=res, spp=spp, max_bounce=6))
     return out
+
+
+def _value_noise(rng, res, x, z):
+    """Bilinear value noise on a res x res lattice over [0,1]^2 with smoothstep weights."""
+    g = rng.random((res + 1, res + 1))
+    fx, fz = x * res, z * res
+    ix = np.minimum(fx.astype(np.int64), res - 1)
+    iz = np.minimum(fz.astype(np.int64), res - 1)
+    tx, tz = fx - ix, fz - iz
+    sx, sz = tx * tx * (3 - 2 * tx), tz * tz * (3 - 2 * tz)
+    a = g[ix, iz] * (1 - sx) + g[ix + 1, iz] * sx
+    b = g[ix, iz + 1] * (1 - sx) + g[ix + 1, iz + 1] * sx
+    return a * (1 - sz) + b * sz
+
+
+def fbm2(n, octaves=6, seed=4, base_res=4):
+    """n x n fBm height samples over [0,1]^2 (lacunarity 2, gain 0.5), values in about [0,1]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x, z = np.meshgrid(np.linspace(0.0, 1.0, n), np.linspace(0.0, 1.0, n), indexing="ij")
+    h = np.zeros((n, n))
+    amp, tot = 1.0, 0.0
+    for o in range(octaves):
+        h += amp * _value_noise(rng, base_res << o, x, z)
+        tot += amp
+        amp *= 0.5
+    return h / tot
+
+
+def terrain(n=2048, octaves=6, seed=4, height=0.15):
+    """Heightfield terrain over [0,1] x [0,1] in x/z (y up): n x n vertices from fBm, two
+    triangles per cell (2 (n-1)^2 triangles) wound for the reference's back-face test so
+    they are seen from above, vertex normals from central differences, uv = (x, z).
+    Returns verts6 (n*n, 6), tris (2 (n-1)^2, 3), uv (n*n, 2)."""
+    h = fbm2(n, octaves, seed) * height
+    xs = np.linspace(0.0, 1.0, n)
+    x, z = np.meshgrid(xs, xs, indexing="ij")
+    dx = np.gradient(h, xs, axis=0)
+    dz = np.gradient(h, xs, axis=1)
+    nrm = np.stack([-dx, np.ones_like(h), -dz], -1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    verts = np.empty((n, n, 6), F32)
+    verts[..., 0], verts[..., 1], verts[..., 2] = x, h, z
+    verts[..., 3:] = nrm
+    idx = np.arange(n * n, dtype=np.uint32).reshape(n, n)
+    a, b = idx[:-1, :-1], idx[1:, :-1]          # (i, j), (i+1, j)
+    c, d = idx[:-1, 1:], idx[1:, 1:]            # (i, j+1), (i+1, j+1)
+    t1 = np.stack([a, b, c], -1).reshape(-1, 3)
+    t2 = np.stack([b, d, c], -1).reshape(-1, 3)
+    tris = np.concatenate([t1, t2], 0).astype(np.uint32)
+    uv = np.stack([x, z], -1).reshape(-1, 2).astype(F32)
+    return verts.reshape(-1, 6), tris, uv
+
+
+def albedo_texture(res=512, seed=4):
+    """Ground-truth RGB albedo texture (res x res x 3, values in [0.1, 0.9]): per-channel
+    fBm patterns."""
+    ch = [fbm2(res, 4, seed * 7 + k, base_res=3) for k in range(3)]
+    t = np.stack(ch, -1)
+    t = (t - t.min()) / max(t.max() - t.min(), 1e-6)
+    return (0.1 + 0.8 * t).astype(F32)
+
+
+def terrain_setup(n=2048, res=(1920, 1080), seed=4):
+    """Camera over the terrain and a sun + sky environment (env_mode 3):
+    (camera, sky colour, (sun direction, cos of the cone, sun radiance))."""
+    cam = camera_lookat((0.5, 0.7, 0.15), (0.5, 0.0, 0.55), 35.0)
+    sun_dir = _normalize(np.array([[0.35, 1.0, 0.25]]))[0]
+    sun = (tuple(float(x) for x in sun_dir), float(np.cos(np.radians(6.0))), (25.0, 22.0, 18.0))
+    return cam, (0.35, 0.45, 0.65), sun
