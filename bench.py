@@ -48,6 +48,7 @@ def main():
     ap.add_argument("--no-render", action="store_true",
                     help="skip the render legs (batched 64-scene, differentiable, Cornell box)")
     ap.add_argument("--batched-scenes", type=int, default=64)
+    ap.add_argument("--no-inverse", action="store_true", help="skip the terrain inverse-render leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (the reference point that scales HBM "
                          "counters under the profiler)")
@@ -167,6 +168,8 @@ def main():
         batched = bench_batched(stream, args.batched_scenes, tune=not args.no_autotune)
         render = {"differentiable": bench_differentiable(stream), "batched": batched,
                   "cornell": bench_cornell(stream, cpu=not args.no_cpu)}
+        if not args.no_inverse:
+            render["inverse_terrain"] = bench_inverse(stream)
 
     mrays = n / (ms_inc * 1e-3) / 1e6
     out = {
@@ -253,6 +256,58 @@ def bench_differentiable(stream, reps=3):
     return {"workload": "512x512, 32 spp, max_bounce 4, 4118 triangles", "forward_ms": round(ms_f, 3),
             "forward_adjoint_ms_per_iter": round(ms_fa, 3), "ratio": round(ms_fa / ms_f, 3),
             "paths": 512 * 512 * 32}
+
+
+def bench_inverse(stream, steps=3):
+    """Terrain inverse render: 2048x2048-vertex fBm heightfield (8,380,418 triangles, seed 4),
+    sun + constant sky, 1920x1080, 64 spp, max_bounce 8, no roulette; the 512x512 albedo
+    texture starts at 0.5 and Adam (lr 1e-2) fits it to the image of the seed-4 texture.
+    Reports the GPU build, one forward render, and the time per Adam iteration (forward +
+    path-replay adjoint + update), with the losses of the first iterations."""
+    import numpy as np
+    import torch
+
+    import rtx
+    from rtx import scenes as S
+    v, t, uv = S.terrain(2048, seed=4)
+    vd = torch.from_numpy(v).cuda()
+    td = torch.from_numpy(t.view(np.int32)).cuda()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g = rtx.build_device(vd, td, stream=stream)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    del vd, td
+    cam, sky, sun = S.terrain_setup()
+    uv_t = torch.from_numpy(uv).cuda()
+    tex_true = torch.from_numpy(S.albedo_texture(512, seed=4)).cuda()
+    p = rtx.make_params(1920, 1080, 64, 8, 99, seed=4, env_mode=3, env_color=sky, uv=uv_t, tex=tex_true, sun=sun)
+    target = rtx.render(g, cam, p, stream=stream).clone()
+    x = torch.full_like(tex_true, 0.5).requires_grad_(True)
+    opt = torch.optim.Adam([x], lr=1e-2)
+    img = torch.empty_like(target)
+
+    def forward():
+        rtx.set_texture(p, uv_t, x.detach())
+        rtx.render(g, cam, p, out=img, stream=stream)
+    fwd_ms = _cuda_ms(forward, 1, stream)
+    losses, step_ms = [], []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        opt.zero_grad()
+        loss = ((rtx.differentiable_render_texture(x, uv_t, g, cam, p) - target) ** 2).sum()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            x.clamp_(0.0, 1.0)
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - t1) * 1e3)
+        losses.append(round(float(loss.detach()), 2))
+    return {"workload": "2048^2 fBm terrain (8,380,418 triangles), 1920x1080, 64 spp, max_bounce 8, "
+                        "512x512 texture, Adam lr 1e-2",
+            "paths": 1920 * 1080 * 64, "build_ms": round(build_ms, 1), "forward_ms": round(fwd_ms, 1),
+            "adam_iteration_ms": round(min(step_ms[1:] or step_ms), 1), "losses": losses}
 
 
 def bench_cornell(stream, cpu=True, reps=3):
