@@ -136,7 +136,22 @@ def main():
     ms_inc, wall, kms_inc = timed(lambda: bvh.trace_closest_device(inc, hits, stream=stream), args.steps,
                                   args.warmup)
     ms_coh = ms_any = float("nan")
+    host = None
     if not args.skip_extras:
+        # host buffers through the C ABI (PCIe-inclusive): rays in, hits out
+        inc_h = inc.cpu().numpy()
+        bvh.trace(inc_h)
+        t_h = time.perf_counter()
+        bvh.trace(inc_h)
+        host_s = time.perf_counter() - t_h
+        one = inc_h[:1].copy()
+        for _ in range(20):
+            bvh.trace(one)
+        t_h = time.perf_counter()
+        for _ in range(200):
+            bvh.trace(one)
+        host = {"pcie_inclusive_mrays_s": round(n / host_s / 1e6, 2),
+                "single_ray_call_us": round((time.perf_counter() - t_h) / 200 * 1e6, 1)}
         ms_coh, _, _ = timed(lambda: bvh.trace_closest_device(coh, hits, stream=stream), max(3, args.steps // 4), 1)
         ms_any, _, _ = timed(lambda: bvh.trace_any_device(inc, occ, stream=stream), max(3, args.steps // 4), 1)
 
@@ -193,6 +208,7 @@ def main():
         "cpu_baseline": cpu,
         "extra": {
             "coherent_closest_mrays_s": None if args.skip_extras else round(n / (ms_coh * 1e-3) / 1e6, 2),
+            "host_buffers": host,
             "incoherent_any_hit_mrays_s": None if args.skip_extras else round(n / (ms_any * 1e-3) / 1e6, 2),
             "build_ms": round(build_ms, 2),
             "build_mtris_s": round(args.tris / build_ms / 1e3, 3),

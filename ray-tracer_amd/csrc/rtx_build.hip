@@ -1163,7 +1163,7 @@ struct Scratch {
 void free_products(rtx_bvh* b) {
     void* ps[] = {b->verts, b->nodes48, b->tris3, b->prims, b->gnodes, b->leaves, b->gtris, b->codes, b->sorted,
                   b->bfs48, b->work, b->rs_keys[0], b->rs_keys[1], b->rs_vals[0], b->rs_vals[1], b->rs_hist,
-                  b->rs_part};
+                  b->rs_part, b->host_rays, b->host_hits, b->host_nrm};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (b->scratch_ev) (void)hipEventDestroy(b->scratch_ev);

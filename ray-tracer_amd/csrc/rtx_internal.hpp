@@ -58,6 +58,12 @@ struct rtx_bvh {
     std::mutex scratch_mu;
     hipEvent_t scratch_ev = nullptr;
     bool scratch_used = false;
+    // device staging of rtx_trace_closest_host (grow-only, one caller at a time)
+    std::mutex host_mu;
+    void* host_rays = nullptr;
+    void* host_hits = nullptr;
+    float* host_nrm = nullptr;
+    size_t host_cap = 0;
     // diagnostics
     uint64_t* codes = nullptr;        // Morton code per input triangle
     uint32_t* sorted = nullptr;       // s_sortedTris[0..nt)

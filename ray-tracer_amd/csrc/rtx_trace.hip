@@ -167,6 +167,7 @@ __global__ void k_ray_keys(TraceArgs A, const Ray32* __restrict__ rays, uint32_t
 
 
 constexpr uint32_t kSortMinRays = 1u << 16;
+constexpr uint32_t kPersistMinRays = 1u << 12;
 
 // Event pairs of rtx_profile_begin/end.
 struct Profile {
@@ -231,7 +232,10 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
     // The ray counter and the sort buffers belong to the handle: a call that uses them
     // waits (on its own stream) for the previous such call, whatever stream that used.
     std::unique_lock<std::mutex> lk(mb->scratch_mu, std::defer_lock);
-    const bool scratch = sort || g_tuning.trace_loop == 2;
+    // the persistent grid pays off only with rays for many waves; small calls (the per-ray
+    // bvh::trace shim) take the while-while kernel, which gives the same results
+    const bool persist = g_tuning.trace_loop == 2 && n >= kPersistMinRays;
+    const bool scratch = sort || persist;
     if (scratch) {
         lk.lock();
         if (!mb->scratch_ev) RTX_HIP(hipEventCreateWithFlags(&mb->scratch_ev, hipEventDisableTiming));
@@ -254,7 +258,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
         radix_sort_pairs(S, (uint32_t)n, 0xEu, s);
         perm = S.vals[S.cur];
     }
-    if (g_tuning.trace_loop == 2) {
+    if (persist) {
         const uint32_t pb = std::min<uint32_t>(blocks, g_tuning.trace_grid_blocks);
         k_zero_work<<<1, 64, 0, s>>>(b->work);
         const int pk = prof_start(s);
@@ -268,7 +272,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
         return release();
     }
     const int pk = prof_start(s);
-    with_trace_variant(g_tuning.trace_lds_depth, g_tuning.trace_loop, [&](auto D, auto L) {
+    with_trace_variant(g_tuning.trace_lds_depth, g_tuning.trace_loop == 0 ? 0u : 1u, [&](auto D, auto L) {
         k_trace<ANY, COUNT, decltype(D)::value, decltype(L)::value><<<blocks, kTraceThreads, 0, s>>>(
             A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
             counts, perm);
@@ -344,17 +348,28 @@ extern "C" int rtx_trace_closest_host(const rtx_bvh* b, const rtx_ray* rays, siz
     int rc = check(b, rays, n, hits, "rtx_trace_closest_host()");
     if (rc) return rc;
     if (n == 0) return RTX_OK;
-    rtx_ray* dr = dalloc<rtx_ray>(n);
-    rtx_hit* dh = dalloc<rtx_hit>(n);
-    float* dn = normals ? dalloc<float>(3 * n) : nullptr;
-    rc = (!dr || !dh || (normals && !dn)) ? fail(RTX_E_HIP, "device allocation failed on rtx_trace_closest_host()")
-                                          : RTX_OK;
-    if (rc == RTX_OK && !hip_ok(hipMemcpy(dr, rays, sizeof(rtx_ray) * n, hipMemcpyHostToDevice), "copy")) rc = RTX_E_HIP;
-    if (rc == RTX_OK) rc = launch<false, false>(b, dr, n, dh, dn, nullptr, nullptr, nullptr);
-    if (rc == RTX_OK && !hip_ok(hipMemcpy(hits, dh, sizeof(rtx_hit) * n, hipMemcpyDeviceToHost), "copy")) rc = RTX_E_HIP;
-    if (rc == RTX_OK && normals && !hip_ok(hipMemcpy(normals, dn, 12 * n, hipMemcpyDeviceToHost), "copy")) rc = RTX_E_HIP;
-    if (dr) (void)hipFree(dr);
-    if (dh) (void)hipFree(dh);
-    if (dn) (void)hipFree(dn);
-    return rc;
+    // Device staging stays on the handle (grow-only), so per-ray callers such as the
+    // bvh::trace shim pay two small copies and a launch, not three allocations.
+    rtx_bvh* mb = const_cast<rtx_bvh*>(b);
+    std::lock_guard<std::mutex> lk(mb->host_mu);
+    if (mb->host_cap < n) {
+        (void)hipFree(mb->host_rays);
+        (void)hipFree(mb->host_hits);
+        (void)hipFree(mb->host_nrm);
+        const size_t cap = std::max<size_t>(n, 1024);
+        mb->host_rays = dalloc<rtx_ray>(cap);
+        mb->host_hits = dalloc<rtx_hit>(cap);
+        mb->host_nrm = dalloc<float>(3 * cap);
+        mb->host_cap = (mb->host_rays && mb->host_hits && mb->host_nrm) ? cap : 0;
+        if (!mb->host_cap) return fail(RTX_E_HIP, "device allocation failed on rtx_trace_closest_host()");
+    }
+    rtx_ray* dr = static_cast<rtx_ray*>(mb->host_rays);
+    rtx_hit* dh = static_cast<rtx_hit*>(mb->host_hits);
+    float* dn = normals ? mb->host_nrm : nullptr;
+    RTX_HIP(hipMemcpy(dr, rays, sizeof(rtx_ray) * n, hipMemcpyHostToDevice));
+    rc = launch<false, false>(b, dr, n, dh, dn, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    RTX_HIP(hipMemcpy(hits, dh, sizeof(rtx_hit) * n, hipMemcpyDeviceToHost));
+    if (normals) RTX_HIP(hipMemcpy(normals, dn, 12 * n, hipMemcpyDeviceToHost));
+    return RTX_OK;
 }
