@@ -1,0 +1,60 @@
+"""Writes tests/golden/reference_*.npz: outputs of the compiled reference (oracle/_ref,
+built from the reference's own sources by oracle/Makefile) on small seeded scenes, so
+the oracle stays pinned where the reference checkout is absent.
+
+Each file holds the inputs (vertices, triangles, rays) and the reference's outputs:
+Morton codes and scene box, sort order, builder-order and DFS-order nodes (raw 48-byte
+BVHNode records), reordered triangles and original ids, s_triIndex / s_sortedTris sizes,
+and closest-hit t/u/v/triIndex/normal for a random ray set and an edge-case ray set.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "ray-tracer_amd"), os.path.dirname(HERE)]
+
+import numpy as np  # noqa: E402
+
+from oracle import oracle as O  # noqa: E402
+from scenes_util import edge_rays, scene_suite  # noqa: E402
+from rtx import scenes as S  # noqa: E402
+
+SCENES = ["rand17", "rand1000", "cornell", "test_traverse", "straddle", "dupcodes", "degenerate", "grid"]
+
+
+def rays_for(v, n, seed):
+    lo, hi = v[:, :3].min(0) - 0.1, v[:, :3].max(0) + 0.1
+    r = S.incoherent_rays(n, seed=seed)
+    r[:, :3] = lo + (hi - lo) * r[:, :3]
+    return r
+
+
+def main():
+    if not O.ref_available():
+        raise SystemExit("oracle/_ref/libref.so is not built (run make -C oracle where the reference exists)")
+    suite = scene_suite(O.ref_geo_sphere, O.ref_geo_box)
+    for name in SCENES:
+        v, t = suite[name]
+        r = O.ref_build(v, t)
+        rays = rays_for(v, 1000, seed=len(t))
+        edge = edge_rays(v, 1000, seed=3)
+        out = {"verts": np.ascontiguousarray(v, np.float32), "tris": np.ascontiguousarray(t, np.uint32),
+               "codes": r.codes, "scene_box": r.scene_box, "sorted": r.sorted,
+               "nodes": r.nodes.view(np.uint8).reshape(-1, 48), "bfs_nodes": r.bfs_nodes.view(np.uint8).reshape(-1, 48),
+               "rtris": r.tris, "prims": r.prims,
+               "sizes": np.array([r.info["tindex_size"], r.info["sorted_size"]], np.uint32)}
+        for key, rs in (("rays", rays), ("edge", edge)):
+            h = O.ref_trace(r, rs, normals=True)
+            out[key] = rs
+            out[key + "_hit"] = np.stack([h["t"], h["u"], h["v"], h["tri"].view(np.float32)], 1)
+            out[key + "_normal"] = h["normal"]
+        path = os.path.join(HERE, f"reference_{name}.npz")
+        np.savez_compressed(path, **out)
+        print(f"{path}: {len(t)} triangles, {len(r.nodes)} nodes, {os.path.getsize(path)} bytes")
+
+
+if __name__ == "__main__":
+    main()
