@@ -169,7 +169,9 @@ int rtx_render(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_param
  * dLoss/dalbedo and dLoss/demission (3 floats per original triangle, added to the
  * buffers) given d_image_d = dLoss/dI.  Paths are re-traced, not stored.  Requires
  * rr_bounce > max_bounce (no Russian roulette) so the path set does not depend on
- * albedo.  No reference counterpart. */
+ * albedo.  The albedo gradient is exact for albedo > 0; a path vertex on a zero albedo
+ * channel adds 0 to that channel (the replay holds no radiance past it), so an optimiser
+ * should keep albedo in (0, 1].  No reference counterpart. */
 int rtx_render_adjoint(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_params* p,
                        const float* albedo_d, const float* emission_d, const float* d_image_d,
                        float* grad_albedo_d, float* grad_emission_d, void* stream);
@@ -216,7 +218,13 @@ void rtx_batch_free(rtx_batch* batch);
  *   shade_threads      threads per shade block (multiple of 64, <= 1024);
  *   render_lds_depth   LDS stack entries per lane: 8, 12 or 16;
  *   render_loop        0 or 1 as trace_loop;
- *   render_wait_lanes  as wait_lanes. */
+ *   render_wait_lanes  as wait_lanes.
+ * Both:
+ *   leaf_coop          1 = the while-while closest-hit loops of the trace entry points
+ *                      (trace_loop 1 and 2) run cooperative leaf phases: the triangle tests
+ *                      of all lanes waiting at leaves are spread over the whole wave, 64 per
+ *                      round, instead of each lane testing its own leaf;
+ *   render_leaf_coop   the same for the renderer's while-while loop (render_loop 1). */
 typedef struct rtx_tuning {
     uint32_t batch_paths;
     uint32_t grid_blocks;
@@ -229,6 +237,8 @@ typedef struct rtx_tuning {
     uint32_t render_lds_depth;
     uint32_t render_loop;
     uint32_t render_wait_lanes;
+    uint32_t leaf_coop;
+    uint32_t render_leaf_coop;
 } rtx_tuning;
 int rtx_set_tuning(const rtx_tuning* t);
 int rtx_get_tuning(rtx_tuning* t);

@@ -172,6 +172,7 @@ template <int DEPTH, int LOOP>
 __global__ void __launch_bounds__(kTraceThreads, 8) k_wtrace(const SceneDesc* __restrict__ scenes, PathState P, int qi,
                                                              uint32_t wait) {
     __shared__ uint2 lst[DEPTH][kTraceThreads];
+    __shared__ CoopLds coop[LOOP == kLoopCoop ? kTraceThreads / 64 : 1];
     const uint32_t n = P.cnt[qi];
     const uint32_t* Q = P.q[qi];
     // wave-uniform trip count (the while-while loop votes across the wave)
@@ -188,8 +189,9 @@ __global__ void __launch_bounds__(kTraceThreads, 8) k_wtrace(const SceneDesc* __
             r1 = P.ray1[i];
             S = scenes + P.sid[i];
         }
-        const TraceOut h = trace_ray<false, false, DEPTH, LOOP, kRenderTriBatch>(S->T, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z},
-                                                                r0.w, r1.w, lst, threadIdx.x, wait);
+        const TraceOut h = trace_ray<false, false, DEPTH, LOOP, kRenderTriBatch, true>(
+            S->T, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w, lst, threadIdx.x, wait,
+            coop + (LOOP == kLoopCoop ? threadIdx.x / 64 : 0));
         if (in) P.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
     }
 }
@@ -380,7 +382,12 @@ __global__ void __launch_bounds__(kGradThreads) k_grad(const SceneDesc* __restri
                 const float4 tp = P.vtp[(size_t)k * P.B + i];
                 prim = __float_as_uint(lu.w);   // albedo entry: texel or original triangle
                 const float* a = (S.tex ? S.tex : S.albedo) + 3 * prim;
-                gA = f3{delta.x * (L.x - lu.x) / a[0], delta.y * (L.y - lu.y) / a[1], delta.z * (L.z - lu.z) / a[2]};
+                // (L - lu) / a is the radiance past this vertex per unit albedo. At a zero
+                // albedo the path carried nothing past the vertex, so the replay has no
+                // record of it: that vertex contributes 0 (not 0/0).
+                gA = f3{a[0] != 0.f ? delta.x * (L.x - lu.x) / a[0] : 0.f,
+                        a[1] != 0.f ? delta.y * (L.y - lu.y) / a[1] : 0.f,
+                        a[2] != 0.f ? delta.z * (L.z - lu.z) / a[2] : 0.f};
                 gE = f3{delta.x * tp.x, delta.y * tp.y, delta.z * tp.z};
             }
             if (LDS) {
@@ -549,7 +556,9 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
         // then the consumed queue becomes the next output and its length is cleared.
         int qi = 0;
         for (int b = 0; b <= mb; ++b) {
-            with_trace_variant(g_tuning.render_lds_depth, g_tuning.render_loop, [&](auto D, auto L) {
+            const uint32_t loop = g_tuning.render_loop == 0 ? (uint32_t)kLoopSingle
+                                  : g_tuning.render_leaf_coop ? (uint32_t)kLoopCoop : (uint32_t)kLoopWhileWhile;
+            with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
                 k_wtrace<decltype(D)::value, decltype(L)::value><<<grid, kTraceThreads, 0, st>>>(d_scenes, P, qi, wait);
             });
             if (adj) k_shade<true><<<grid, sh, 0, st>>>(d_scenes, P, qi, b);
@@ -602,7 +611,7 @@ extern "C" int rtx_set_tuning(const rtx_tuning* t) {
         t->shade_threads > 1024 || !(t->trace_lds_depth == 8 || t->trace_lds_depth == 12 || t->trace_lds_depth == 16) ||
         t->trace_loop > 2 || t->ray_sort > 1 || t->wait_lanes > 64 || t->trace_grid_blocks == 0 ||
         !(t->render_lds_depth == 8 || t->render_lds_depth == 12 || t->render_lds_depth == 16) || t->render_loop > 1 ||
-        t->render_wait_lanes > 64)
+        t->render_wait_lanes > 64 || t->leaf_coop > 1 || t->render_leaf_coop > 1)
         return fail(RTX_E_ARG, "bad tuning on rtx_set_tuning()");
     g_tuning = *t;
     return RTX_OK;

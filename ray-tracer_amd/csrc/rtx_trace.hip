@@ -30,6 +30,7 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray3
                                                          uint8_t* __restrict__ occl, uint32_t* __restrict__ counts,
                                                          const uint32_t* __restrict__ perm) {
     __shared__ uint2 lst[DEPTH][kTraceThreads];
+    __shared__ CoopLds coop[LOOP == kLoopCoop ? kTraceThreads / 64 : 1];
     const uint32_t tid = threadIdx.x;
     const uint32_t k = xcd_block(blockIdx.x) * kTraceThreads + tid;
     // out-of-range lanes still run (with a ray that misses) so wave-wide votes see them
@@ -41,7 +42,7 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray3
         r1 = reinterpret_cast<const float4*>(rays)[2 * i + 1];
     }
     const TraceOut h = trace_ray<ANY, COUNT, DEPTH, LOOP>(A, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w,
-                                                          lst, tid, A.wait);
+                                                          lst, tid, A.wait, coop + (LOOP == kLoopCoop ? tid / 64 : 0));
     if (!in) return;
     if (ANY) {
         occl[i] = h.tri != kInvalid ? 1 : 0;
@@ -70,7 +71,7 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray3
 // would split `more` and could elect a leader that never issues the atomic.
 constexpr int kRefill = 24;   // measured: 4 and 8 much slower, 16 about 1% slower, 32 level with 24
 
-template <bool ANY, bool COUNT, int DEPTH>
+template <bool ANY, bool COUNT, int DEPTH, int LOOP>
 __global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, const Ray32* __restrict__ rays,
                                                                  uint32_t n, Hit16* __restrict__ hits,
                                                                  float* __restrict__ normals,
@@ -78,6 +79,7 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, co
                                                                  uint32_t* __restrict__ counts, uint32_t* work,
                                                                  const uint32_t* __restrict__ perm) {
     __shared__ uint2 lst[DEPTH][kTraceThreads];
+    __shared__ CoopLds coop[LOOP == kLoopCoop ? kTraceThreads / 64 : 1];
     const uint32_t tid = threadIdx.x;
     const uint64_t lt = (1ull << __lane_id()) - 1ull;
     Walker<DEPTH> w;
@@ -124,7 +126,11 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, co
         // uniform exit test: a started ray that missed the root is written at the top
         if (!__any(w.live || id != kInvalid) && !more) break;
         w.template node_phase<COUNT>(A, lst, tid, A.wait);
-        if (w.live && (w.ref & kLeafBit)) {
+        if (LOOP == kLoopCoop && !ANY) {   // all lanes active: the loop is wave-uniform
+            const bool at = w.live && (w.ref & kLeafBit);
+            w.template leaf_coop<COUNT, false>(A, coop + tid / 64);
+            if (at) w.pop(lst, tid);
+        } else if (w.live && (w.ref & kLeafBit)) {
             if (w.template leaf_step<ANY, COUNT>(A)) w.live = false;
             else w.pop(lst, tid);
         }
@@ -220,6 +226,10 @@ int check(const rtx_bvh* b, const void* rays, size_t n, const void* out, const c
     return RTX_OK;
 }
 
+// While-while loop shape of the trace kernels: cooperative leaf phases when tuned on
+// (closest hit; any-hit kernels fall back to per-lane leaf steps inside the same shape).
+uint32_t loop_shape() { return g_tuning.leaf_coop ? (uint32_t)kLoopCoop : (uint32_t)kLoopWhileWhile; }
+
 template <bool ANY, bool COUNT>
 int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float* normals, uint8_t* occl,
            uint32_t* counts, hipStream_t s) {
@@ -262,8 +272,8 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
         const uint32_t pb = std::min<uint32_t>(blocks, g_tuning.trace_grid_blocks);
         k_zero_work<<<1, 64, 0, s>>>(b->work);
         const int pk = prof_start(s);
-        with_trace_variant(g_tuning.trace_lds_depth, 1, [&](auto D, auto) {
-            k_trace_persist<ANY, COUNT, decltype(D)::value><<<pb, kTraceThreads, 0, s>>>(
+        with_trace_variant(g_tuning.trace_lds_depth, loop_shape(), [&](auto D, auto L) {
+            k_trace_persist<ANY, COUNT, decltype(D)::value, decltype(L)::value><<<pb, kTraceThreads, 0, s>>>(
                 A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
                 counts, b->work, perm);
         });
@@ -272,7 +282,8 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
         return release();
     }
     const int pk = prof_start(s);
-    with_trace_variant(g_tuning.trace_lds_depth, g_tuning.trace_loop == 0 ? 0u : 1u, [&](auto D, auto L) {
+    with_trace_variant(g_tuning.trace_lds_depth, g_tuning.trace_loop == 0 ? (uint32_t)kLoopSingle : loop_shape(),
+                       [&](auto D, auto L) {
         k_trace<ANY, COUNT, decltype(D)::value, decltype(L)::value><<<blocks, kTraceThreads, 0, s>>>(
             A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
             counts, perm);
@@ -284,7 +295,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
 
 }  // namespace
 
-rtx_tuning g_tuning = {1u << 22, 2048, 256, 12, 2, 1, 16, 2048, 8, 1, 0};
+rtx_tuning g_tuning = {1u << 22, 2048, 256, 12, 2, 1, 16, 2048, 8, 1, 0, 0, 0};
 
 }  // namespace rtx
 

@@ -26,19 +26,23 @@ struct TraceArgs {
 
 extern rtx_tuning g_tuning;
 
+// Loop shapes of trace_ray: one loop over nodes and leaves; the while-while loop; the
+// while-while loop with cooperative leaf phases (closest hit only; any-hit queries take
+// the while-while leaf steps).
+constexpr int kLoopSingle = 0, kLoopWhileWhile = 1, kLoopCoop = 2;
+
 // Calls f(integral_constant<DEPTH>, integral_constant<LOOP>) for LDS depth d (8, 12, 16)
-// and loop shape l (0, or 1 for the while-while loop; the persistent trace kernel is
-// dispatched by its caller and reuses the while-while loop's depth instantiation).
+// and loop shape l (kLoopSingle, kLoopWhileWhile or kLoopCoop).
 template <class F>
 void with_trace_variant(uint32_t d, uint32_t l, F&& f) {
-    using I8 = std::integral_constant<int, 8>;
-    using I12 = std::integral_constant<int, 12>;
-    using I16 = std::integral_constant<int, 16>;
-    using L0 = std::integral_constant<int, 0>;
-    using L1 = std::integral_constant<int, 1>;
-    if (d == 8) { if (l) f(I8{}, L1{}); else f(I8{}, L0{}); }
-    else if (d == 12) { if (l) f(I12{}, L1{}); else f(I12{}, L0{}); }
-    else { if (l) f(I16{}, L1{}); else f(I16{}, L0{}); }
+    auto by_loop = [&](auto D) {
+        if (l == (uint32_t)kLoopCoop) f(D, std::integral_constant<int, kLoopCoop>{});
+        else if (l == (uint32_t)kLoopWhileWhile) f(D, std::integral_constant<int, kLoopWhileWhile>{});
+        else f(D, std::integral_constant<int, kLoopSingle>{});
+    };
+    if (d == 8) by_loop(std::integral_constant<int, 8>{});
+    else if (d == 12) by_loop(std::integral_constant<int, 12>{});
+    else by_loop(std::integral_constant<int, 16>{});
 }
 
 // Deepest tree every variant can traverse (smallest LDS ring + spill).
@@ -63,6 +67,22 @@ inline TraceArgs make_trace_args(const rtx_bvh* b) {
 __device__ __forceinline__ uint32_t xcd_block(uint32_t g) {
     const uint32_t G = gridDim.x, per = G / 8, rem = G % 8, x = g % 8, j = g / 8;
     return x < rem ? x * (per + 1) + j : rem * (per + 1) + (x - rem) * per + j;
+}
+
+// LDS of the cooperative leaf phase, per wave of 64 lanes.
+struct CoopLds {
+    uint32_t mark[64];            // lane whose pairs start at this position of the round
+    unsigned long long mins[64];  // per owner lane: smallest (t, triangle index) key
+};
+constexpr uint32_t kNoLane = 0xFFFFFFFFu;
+
+// Orders (t, triangle index) as the in-order strict-less-than scan of a leaf does: smaller
+// t first, then the earlier triangle. -0 and +0 compare equal, as they do as floats. t is
+// never NaN here (a NaN t fails t < best).
+__device__ __forceinline__ unsigned long long coop_key(float t, uint32_t k) {
+    uint32_t b = t == 0.f ? 0u : __float_as_uint(t);
+    b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    return ((unsigned long long)b << 32) | k;
 }
 
 struct TraceOut {
@@ -191,15 +211,98 @@ struct Walker {
         }
         return false;
     }
+    // Cooperative leaf phase (closest hit). The (ray, triangle) pairs of every lane waiting
+    // at a leaf are dealt over the whole wave, 64 per round: lane r's pairs take positions
+    // [start_r, start_r + count_r) of the wave's exclusive scan of leaf sizes, so each round
+    // tests 64 pairs whatever the leaf sizes, and neighbouring lanes load neighbouring
+    // triangles. A pair lane finds its owner from the round's segment starts (one LDS mark
+    // per owner and a ballot), takes the owner's ray by shuffle and tests its triangle; an
+    // LDS atomic min of coop_key(t, k) per owner keeps, among t < the owner's best t, the
+    // smallest t and then the earliest triangle: what testing the leaf in order with a
+    // strict t < best update keeps. The owner re-tests that one triangle for u and v (the
+    // same inputs give the same bits). The caller keeps all 64 lanes active and the
+    // control flow uniform. MIXED: lanes may trace different trees (the batched renderer),
+    // so the triangle array is taken from the owner too.
+    template <bool COUNT, bool MIXED>
+    __device__ __forceinline__ void leaf_coop(const TraceArgs& A, CoopLds* C) {
+        const uint32_t lane = __lane_id();
+        const bool at = live && (ref & kLeafBit);
+        uint32_t c = 0, kb = 0;
+        if (at) {
+            const Leaf lf = A.leaves[ref & ~kLeafBit];
+            c = lf.count;
+            kb = lf.begin;
+        }
+        uint32_t inc = c;
+#pragma unroll
+        for (int s = 1; s < 64; s <<= 1) {
+            const uint32_t y = __shfl_up(inc, s);
+            if ((int)lane >= s) inc += y;
+        }
+        const uint32_t total = __shfl(inc, 63);
+        const uint32_t start = inc - c;
+        const uint32_t kbase = kb - start;   // triangle of position p is kbase + p (mod 2^32)
+        if (COUNT) nT += c;
+        volatile uint32_t* mark = C->mark;
+        volatile unsigned long long* vmins = C->mins;
+        if (c) vmins[lane] = ~0ull;
+        const uint64_t upto = (2ull << lane) - 1ull;   // lanes 0..lane
+        for (uint32_t W = 0; W < total; W += 64) {
+            mark[lane] = kNoLane;
+            __builtin_amdgcn_wave_barrier();
+            // segments do not overlap, so at most one owner marks each position; the
+            // segment holding position W marks position 0
+            if (c && start < W + 64u && start + c > W) mark[(start > W ? start : W) - W] = lane;
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t m = mark[lane];
+            const uint64_t starts = __ballot(m != kNoLane);
+            const int src = 63 - __clzll((long long)(starts & upto));
+            const uint32_t r = __shfl(m, src);
+            const float pox = __shfl(o.x, r), poy = __shfl(o.y, r), poz = __shfl(o.z, r);
+            const float pdx = __shfl(d.x, r), pdy = __shfl(d.y, r), pdz = __shfl(d.z, r);
+            const float ptmin = __shfl(tmin, r), ptmax = __shfl(tmax, r), pbest = __shfl(bestT, r);
+            const uint32_t k = __shfl(kbase, r) + W + lane;
+            const GTri* tris = A.tris;
+            if (MIXED) {
+                const uint64_t tp = reinterpret_cast<uint64_t>(A.tris);
+                const uint32_t lo32 = __shfl((uint32_t)tp, r), hi32 = __shfl((uint32_t)(tp >> 32), r);
+                tris = reinterpret_cast<const GTri*>(((uint64_t)hi32 << 32) | lo32);
+            }
+            if (W + lane < total) {
+                const float4 a = tris[k].a, b = tris[k].b;
+                const float cz = tris[k].c.x;
+                float u, v;
+                const float t = tri_test(f3{pox, poy, poz}, f3{pdx, pdy, pdz}, ptmin, ptmax, f3{a.x, a.y, a.z},
+                                         f3{a.w, b.x, b.y}, f3{b.z, b.w, cz}, u, v);
+                if (t < pbest) atomicMin(&C->mins[r], coop_key(t, k));
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (c) {
+            const unsigned long long mk = vmins[lane];
+            if (mk != ~0ull) {
+                const uint32_t k = (uint32_t)mk;
+                const float4 a = A.tris[k].a, b = A.tris[k].b;
+                const float cz = A.tris[k].c.x;
+                float u, v;
+                bestT = tri_test(o, d, tmin, tmax, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y}, f3{b.z, b.w, cz}, u, v);
+                bu = u;
+                bv = v;
+                best = k;
+            }
+        }
+    }
 };
 
 // One ray against one tree. LOOP 0 visits nodes and leaves in one loop; LOOP 1
 // ("while-while") lets a lane that reaches a leaf wait until the other lanes of the wave
 // are at leaves too (all but A.wait of them), so triangle tests run with the wave mostly
-// converged. Both visit the same nodes in the same order per ray.
-template <bool ANY, bool COUNT, int DEPTH, int LOOP, int TB = kTraceTriBatch>
+// converged. LOOP 2 is LOOP 1 with cooperative leaf phases (leaf_coop) for closest-hit
+// queries. All visit the same nodes in the same order per ray and return the same hit.
+template <bool ANY, bool COUNT, int DEPTH, int LOOP, int TB = kTraceTriBatch, bool MIXED = false>
 __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, float tmin, float tmax,
-                                              uint2 (*lst)[kTraceThreads], uint32_t tid, uint32_t wait) {
+                                              uint2 (*lst)[kTraceThreads], uint32_t tid, uint32_t wait,
+                                              CoopLds* coop) {
     Walker<DEPTH, TB> w;
     uint2 spill[kSpillDepth];
     w.spill = spill;
@@ -217,7 +320,11 @@ __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, fl
         for (;;) {
             w.template node_phase<COUNT>(A, lst, tid, wait);
             if (!__any(w.live)) break;
-            if (w.live && (w.ref & kLeafBit)) {
+            if (LOOP == kLoopCoop && !ANY && __builtin_amdgcn_read_exec() == ~0ull) {
+                const bool at = w.live && (w.ref & kLeafBit);
+                w.template leaf_coop<COUNT, MIXED>(A, coop);
+                if (at) w.pop(lst, tid);
+            } else if (w.live && (w.ref & kLeafBit)) {
                 if (w.template leaf_step<ANY, COUNT>(A)) w.live = false;
                 else w.pop(lst, tid);
             }

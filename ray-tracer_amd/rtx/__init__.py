@@ -56,7 +56,8 @@ class Tuning(C.Structure):
     _fields_ = [("batch_paths", C.c_uint32), ("grid_blocks", C.c_uint32), ("shade_threads", C.c_uint32),
                 ("trace_lds_depth", C.c_uint32), ("trace_loop", C.c_uint32), ("ray_sort", C.c_uint32),
                 ("wait_lanes", C.c_uint32), ("trace_grid_blocks", C.c_uint32), ("render_lds_depth", C.c_uint32),
-                ("render_loop", C.c_uint32), ("render_wait_lanes", C.c_uint32)]
+                ("render_loop", C.c_uint32), ("render_wait_lanes", C.c_uint32),
+                ("leaf_coop", C.c_uint32), ("render_leaf_coop", C.c_uint32)]
 
 
 _LIB = None
@@ -481,7 +482,9 @@ _TF = None
 
 def differentiable_render_texture(tex, uv, bvh, cam, params, emission=None):
     """torch.autograd entry for texture albedo: gradients flow to tex ((h, w, 3) CUDA
-    tensor; uv (nverts, 2)). Requires params.rr_bounce > params.max_bounce."""
+    tensor; uv (nverts, 2)). Requires params.rr_bounce > params.max_bounce. Gradients are
+    exact for texels > 0; a zero texel channel gets no gradient from the path vertices on
+    it, so keep the fitted texture in (0, 1]."""
     global _TF
     if _TF is None:
         _TF = _texture_autograd_fn()
@@ -490,7 +493,8 @@ def differentiable_render_texture(tex, uv, bvh, cam, params, emission=None):
 
 def differentiable_render(albedo, emission, bvh, cam, params):
     """torch.autograd entry: gradients flow to albedo and emission ((ntri, 3) CUDA tensors).
-    Requires params.rr_bounce > params.max_bounce."""
+    Requires params.rr_bounce > params.max_bounce. Albedo gradients are exact for albedo > 0
+    (a zero albedo channel gets none from the vertices on it)."""
     global _RF
     if _RF is None:
         _RF = _autograd_fn()

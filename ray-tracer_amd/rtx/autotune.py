@@ -21,16 +21,19 @@ SPACE = {
     "ray_sort": [0, 1],
     "wait_lanes": [0, 4, 8, 16, 24, 32],
     "trace_grid_blocks": [1024, 2048, 4096, 8192],
+    "leaf_coop": [0, 1],
     # renderer
     "render_lds_depth": [8, 12, 16],
     "render_loop": [0, 1],
     "render_wait_lanes": [0, 4, 8, 16],
+    "render_leaf_coop": [0, 1],
     "grid_blocks": [1024, 2048, 4096, 8192],
     "shade_threads": [64, 128, 256, 512],
     "batch_paths": [1 << 20, 1 << 21, 1 << 22, 1 << 23],
 }
-TRACE_KEYS = ["trace_loop", "trace_lds_depth", "ray_sort", "wait_lanes", "trace_grid_blocks"]
-RENDER_KEYS = ["batch_paths", "shade_threads", "grid_blocks", "render_lds_depth", "render_loop", "render_wait_lanes"]
+TRACE_KEYS = ["trace_loop", "leaf_coop", "trace_lds_depth", "ray_sort", "wait_lanes", "trace_grid_blocks"]
+RENDER_KEYS = ["batch_paths", "shade_threads", "grid_blocks", "render_lds_depth", "render_loop", "render_leaf_coop",
+               "render_wait_lanes"]
 
 
 def _time(make_step, reps):

@@ -122,11 +122,13 @@ def test_render_trace_variants_bit_identical():
     saved = rtx.get_tuning()
     try:
         ref = None
-        for loop, depth, wait, grid, bp in ((1, 8, 0, 2048, 1 << 22), (1, 12, 16, 2048, 1 << 22),
-                                            (1, 8, 8, 8, 4096), (1, 16, 32, 64, 8192), (0, 8, 0, 512, 1 << 20)):
+        for loop, depth, wait, grid, bp, coop in ((1, 8, 0, 2048, 1 << 22, 0), (1, 12, 16, 2048, 1 << 22, 0),
+                                                  (1, 8, 8, 8, 4096, 0), (1, 16, 32, 64, 8192, 0),
+                                                  (0, 8, 0, 512, 1 << 20, 0), (1, 8, 0, 2048, 1 << 22, 1),
+                                                  (1, 12, 32, 8, 4096, 1), (0, 8, 0, 512, 1 << 20, 1)):
             rtx.set_tuning(render_loop=loop, render_lds_depth=depth, render_wait_lanes=wait, grid_blocks=grid,
-                           batch_paths=bp)
-            print(f"[render variant] loop={loop} depth={depth} wait={wait} grid={grid}", flush=True)
+                           batch_paths=bp, render_leaf_coop=coop)
+            print(f"[render variant] loop={loop} depth={depth} wait={wait} grid={grid} coop={coop}", flush=True)
             imgs = [torch.zeros((p.height, p.width, 3), device="cuda") for _, _, _, p, _ in sc]
             b = rtx.Batch(bvhs, [c for _, _, c, _, _ in sc], [p for _, _, _, p, _ in sc], [None] * 3, [None] * 3,
                           imgs, 2)
@@ -136,7 +138,7 @@ def test_render_trace_variants_bit_identical():
             if ref is None:
                 ref = out
             for a, r in zip(out, ref):
-                assert np.array_equal(a, r), (loop, depth, wait, grid)
+                assert np.array_equal(a, r), (loop, depth, wait, grid, coop)
     finally:
         rtx.set_tuning(**saved)
 

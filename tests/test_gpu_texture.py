@@ -76,6 +76,28 @@ def test_texture_gradient_matches_finite_differences(tex_res):
     print(f"[texture adjoint] {tex_res}^2 texels: worst relative error {worst:.2e}", flush=True)
 
 
+def test_zero_texel_gradient_is_finite():
+    """A texel channel at exactly 0 (where a clamped optimiser can put it) gets gradient 0
+    from the vertices on it instead of 0/0, so the whole gradient stays finite; the other
+    channels still follow the path-replay formula."""
+    import torch
+    v, t, uv, tex_true, cam, sky, sun = _setup(tex_res=8)
+    g = rtx.build(v, t)
+    uv_t = _t(uv)
+    p = rtx.make_params(32, 24, 4, 3, 99, seed=3, env_mode=3, env_color=sky, uv=uv_t, tex=_t(tex_true), sun=sun)
+    target = rtx.render(g, cam, p)
+    tex = np.full_like(tex_true, 0.5)
+    tex[::2, :, 0] = 0.0
+    tex[:, ::3, 2] = 0.0
+    x = _t(tex).requires_grad_(True)
+    loss = ((rtx.differentiable_render_texture(x, uv_t, g, cam, p) - target) ** 2).sum()
+    loss.backward()
+    gr = x.grad.cpu().numpy()
+    assert np.isfinite(gr).all()
+    assert np.all(gr[tex == 0.0] == 0.0)
+    assert np.abs(gr[tex != 0.0]).max() > 0.0
+
+
 def test_adam_on_texture_lowers_loss():
     import torch
     v, t, uv, tex_true, cam, sky, sun = _setup(tex_res=16)
@@ -92,7 +114,7 @@ def test_adam_on_texture_lowers_loss():
         loss.backward()
         opt.step()
         with torch.no_grad():
-            x.clamp_(0.0, 1.0)
+            x.clamp_(1e-3, 1.0)
         losses.append(float(loss.detach()))
     print(f"[texture adam] loss {losses[0]:.4g} -> {losses[-1]:.4g}", flush=True)
     assert losses[-1] < 0.5 * losses[0]

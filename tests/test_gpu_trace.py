@@ -75,9 +75,10 @@ def test_edge_rays_all_variants(suite, name):
     ref = O.trace(o, rays, counts=True, normals=True, nthreads=8)
     saved = rtx.get_tuning()
     try:
-        for loop, sort, wait in ((0, 0, 0), (1, 0, 8), (2, 1, 16), (2, 0, 0)):
-            rtx.set_tuning(trace_loop=loop, ray_sort=sort, wait_lanes=wait)
-            print(f"[edge rays] {name} loop={loop} sort={sort}", flush=True)
+        for loop, sort, wait, coop in ((0, 0, 0, 0), (1, 0, 8, 0), (2, 1, 16, 0), (2, 0, 0, 0), (1, 0, 32, 1),
+                                       (2, 1, 16, 1)):
+            rtx.set_tuning(trace_loop=loop, ray_sort=sort, wait_lanes=wait, leaf_coop=coop)
+            print(f"[edge rays] {name} loop={loop} sort={sort} coop={coop}", flush=True)
             h, nrm, occ, cnt = _device_trace(g, rays)
             assert np.array_equal(h[:, 3].view(np.uint32), ref["tri"]), (loop, sort)
             for k, c in (("t", 0), ("u", 1), ("v", 2)):
@@ -133,6 +134,33 @@ def test_trace_1m_subset():
         assert np.array_equal(cnt, ref["counts"]), name
 
 
+@pytest.mark.parametrize("name", ["rand1", "rand1000", "rand20000", "cornell", "straddle", "dupcodes", "degenerate",
+                                  "grid"])
+def test_cooperative_leaves_match_oracle(suite, name):
+    """Cooperative leaf phases (the pairs of all lanes at leaves dealt over the wave, a
+    segment running over several 64-pair rounds when a leaf is large) give the oracle's
+    hits, t/u/v bits and work counts, in the while-while and the persistent kernel."""
+    v, t = suite[name]
+    g = rtx.build(v, t)
+    o = O.build(v, t)
+    rays = _rays_for(v, 20000, seed=len(t) + 1)
+    ref = O.trace(o, rays, counts=True, normals=True, nthreads=8)
+    saved = rtx.get_tuning()
+    try:
+        for loop, wait in ((1, 0), (1, 32), (2, 16), (2, 64)):
+            rtx.set_tuning(trace_loop=loop, wait_lanes=wait, leaf_coop=1)
+            print(f"[coop parity] {name} loop={loop} wait={wait}", flush=True)
+            h, nrm, occ, cnt = _device_trace(g, rays)
+            assert np.array_equal(h[:, 3].view(np.uint32), ref["tri"]), (loop, wait)
+            for k, col in (("t", 0), ("u", 1), ("v", 2)):
+                assert np.array_equal(h[:, col].view(np.uint32), ref[k].view(np.uint32)), (k, loop, wait)
+            assert np.array_equal(occ.astype(bool), ref["tri"] != 0xFFFFFFFF), (loop, wait)
+            assert np.array_equal(cnt, ref["counts"]), (loop, wait)
+            np.testing.assert_allclose(nrm, ref["normal"], rtol=0, atol=1e-6)
+    finally:
+        rtx.set_tuning(**saved)
+
+
 def test_tuning_variants_give_identical_hits():
     """Every launch-shape variant (LDS stack depth, loop structure, ray sorting, how many
     lanes the leaf phase leaves behind) returns the same hits at each ray's own index."""
@@ -144,17 +172,19 @@ def test_tuning_variants_give_identical_hits():
     saved = rtx.get_tuning()
     ref = None
     try:
-        combos = [(sort, depth, loop, 0) for sort in (0, 1) for depth in (8, 12, 16) for loop in (0, 1, 2)]
-        combos += [(sort, 8, loop, wait) for sort in (0, 1) for loop in (1, 2) for wait in (1, 8, 32, 64)]
-        for sort, depth, loop, wait in combos:
-            rtx.set_tuning(trace_lds_depth=depth, trace_loop=loop, ray_sort=sort, wait_lanes=wait)
+        combos = [(sort, depth, loop, 0, 0) for sort in (0, 1) for depth in (8, 12, 16) for loop in (0, 1, 2)]
+        combos += [(sort, 8, loop, wait, 0) for sort in (0, 1) for loop in (1, 2) for wait in (1, 8, 32, 64)]
+        combos += [(sort, depth, loop, wait, 1) for sort in (0, 1) for depth in (8, 16) for loop in (1, 2)
+                   for wait in (0, 16, 64)]
+        for sort, depth, loop, wait, coop in combos:
+            rtx.set_tuning(trace_lds_depth=depth, trace_loop=loop, ray_sort=sort, wait_lanes=wait, leaf_coop=coop)
             hits.fill_(-1.0)
             g.trace_closest_device(rays, hits)
             torch.cuda.synchronize()
             h = hits.view(torch.int32).cpu().numpy().copy()
             if ref is None:
                 ref = h
-            assert np.array_equal(h, ref), f"variant depth={depth} loop={loop} sort={sort} wait={wait} differs"
+            assert np.array_equal(h, ref), f"variant depth={depth} loop={loop} sort={sort} wait={wait} coop={coop} differs"
     finally:
         rtx.set_tuning(**saved)
 
@@ -189,8 +219,10 @@ def test_persistent_variant_counts_and_any_hit(suite):
     ray; the kernel's refill and exit must stay wave-uniform for them."""
     saved = rtx.get_tuning()
     try:
-        for name, sort in (("rand20000", 0), ("straddle", 0), ("cornell", 0), ("rand20000", 1)):
-            rtx.set_tuning(trace_loop=2, trace_lds_depth=8, trace_grid_blocks=16, ray_sort=sort, wait_lanes=8 * sort)
+        for name, sort, coop in (("rand20000", 0, 0), ("straddle", 0, 0), ("cornell", 0, 0), ("rand20000", 1, 0),
+                                 ("rand20000", 0, 1), ("straddle", 1, 1), ("cornell", 0, 1)):
+            rtx.set_tuning(trace_loop=2, trace_lds_depth=8, trace_grid_blocks=16, ray_sort=sort, wait_lanes=8 * sort,
+                           leaf_coop=coop)
             v, t = suite[name]
             g = rtx.build(v, t)
             o = O.build(v, t)
