@@ -76,6 +76,29 @@ struct CoopLds {
 };
 constexpr uint32_t kNoLane = 0xFFFFFFFFu;
 
+// LDS words shared by the lanes of one wave: relaxed wave-scope atomics, so the compiler
+// keeps every access (a lane reads what another lane wrote) and still emits plain ds_*
+// instructions (a volatile pointer would turn them into flat accesses). A wave's LDS
+// operations complete in issue order.
+template <class T>
+__device__ __forceinline__ void wave_st(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+template <class T>
+__device__ __forceinline__ T wave_ld(T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+// The 36 bytes of a traversal triangle a test reads, loaded before the test starts: the
+// empty asm takes the values as operands, so the compiler cannot sink the v0 load behind
+// the back-face test (a second dependent memory round trip for the lanes that pass it).
+__device__ __forceinline__ void load_tri(const GTri* p, float4& a, float4& b, float& c) {
+    a = p->a;
+    b = p->b;
+    c = p->c.x;
+    __asm__ volatile("" : : "v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(b.x), "v"(b.y), "v"(b.z), "v"(b.w), "v"(c));
+}
+
 // Orders (t, triangle index) as the in-order strict-less-than scan of a leaf does: smaller
 // t first, then the earlier triangle. -0 and +0 compare equal, as they do as floats. t is
 // never NaN here (a NaN t fails t < best).
@@ -129,8 +152,15 @@ struct Walker {
         while (top > 0) {
             --top;
             uint2 e;
-            if (top < lo) { --lo; e = spill[top]; }
-            else e = lst[slot(top)][tid];
+            if (top < lo) {
+                --lo;
+                e = spill[top];
+            } else {
+                // an atomic read: the compiler cannot merge it with the private read above
+                // into one flat load through a selected pointer
+                const uint64_t x = wave_ld(reinterpret_cast<uint64_t*>(&lst[slot(top)][tid]));
+                e = make_uint2((uint32_t)x, (uint32_t)(x >> 32));
+            }
             if (__uint_as_float(e.y) < bestT) { ref = e.x; live = true; break; }
         }
     }
@@ -243,18 +273,16 @@ struct Walker {
         const uint32_t start = inc - c;
         const uint32_t kbase = kb - start;   // triangle of position p is kbase + p (mod 2^32)
         if (COUNT) nT += c;
-        volatile uint32_t* mark = C->mark;
-        volatile unsigned long long* vmins = C->mins;
-        if (c) vmins[lane] = ~0ull;
+        if (c) wave_st(&C->mins[lane], ~0ull);
         const uint64_t upto = (2ull << lane) - 1ull;   // lanes 0..lane
         for (uint32_t W = 0; W < total; W += 64) {
-            mark[lane] = kNoLane;
+            wave_st(&C->mark[lane], kNoLane);
             __builtin_amdgcn_wave_barrier();
             // segments do not overlap, so at most one owner marks each position; the
             // segment holding position W marks position 0
-            if (c && start < W + 64u && start + c > W) mark[(start > W ? start : W) - W] = lane;
+            if (c && start < W + 64u && start + c > W) wave_st(&C->mark[(start > W ? start : W) - W], lane);
             __builtin_amdgcn_wave_barrier();
-            const uint32_t m = mark[lane];
+            const uint32_t m = wave_ld(&C->mark[lane]);
             const uint64_t starts = __ballot(m != kNoLane);
             const int src = 63 - __clzll((long long)(starts & upto));
             const uint32_t r = __shfl(m, src);
@@ -269,8 +297,9 @@ struct Walker {
                 tris = reinterpret_cast<const GTri*>(((uint64_t)hi32 << 32) | lo32);
             }
             if (W + lane < total) {
-                const float4 a = tris[k].a, b = tris[k].b;
-                const float cz = tris[k].c.x;
+                float4 a, b;
+                float cz;
+                load_tri(tris + k, a, b, cz);
                 float u, v;
                 const float t = tri_test(f3{pox, poy, poz}, f3{pdx, pdy, pdz}, ptmin, ptmax, f3{a.x, a.y, a.z},
                                          f3{a.w, b.x, b.y}, f3{b.z, b.w, cz}, u, v);
@@ -279,11 +308,12 @@ struct Walker {
             __builtin_amdgcn_wave_barrier();
         }
         if (c) {
-            const unsigned long long mk = vmins[lane];
+            const unsigned long long mk = wave_ld(&C->mins[lane]);
             if (mk != ~0ull) {
                 const uint32_t k = (uint32_t)mk;
-                const float4 a = A.tris[k].a, b = A.tris[k].b;
-                const float cz = A.tris[k].c.x;
+                float4 a, b;
+                float cz;
+                load_tri(A.tris + k, a, b, cz);
                 float u, v;
                 bestT = tri_test(o, d, tmin, tmax, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y}, f3{b.z, b.w, cz}, u, v);
                 bu = u;
