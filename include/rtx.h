@@ -220,8 +220,8 @@ void rtx_batch_free(rtx_batch* batch);
  *   render_loop        0 or 1 as trace_loop;
  *   render_wait_lanes  as wait_lanes.
  * Both:
- *   leaf_coop          1 = the while-while closest-hit loops of the trace entry points
- *                      (trace_loop 1 and 2) run cooperative leaf phases: the triangle tests
+ *   leaf_coop          1 = the while-while loops of the trace entry points (trace_loop
+ *                      1 and 2) run cooperative leaf phases: the triangle tests
  *                      of all lanes waiting at leaves are spread over the whole wave, 64 per
  *                      round, instead of each lane testing its own leaf;
  *   render_leaf_coop   the same for the renderer's while-while loop (render_loop 1). */

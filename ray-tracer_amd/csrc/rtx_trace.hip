@@ -126,10 +126,10 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, co
         // uniform exit test: a started ray that missed the root is written at the top
         if (!__any(w.live || id != kInvalid) && !more) break;
         w.template node_phase<COUNT>(A, lst, tid, A.wait);
-        if (LOOP == kLoopCoop && !ANY) {   // all lanes active: the loop is wave-uniform
+        if (LOOP == kLoopCoop) {   // all lanes active: the loop is wave-uniform
             const bool at = w.live && (w.ref & kLeafBit);
-            w.template leaf_coop<COUNT, false>(A, coop + tid / 64);
-            if (at) w.pop(lst, tid);
+            if (w.template leaf_coop<ANY, COUNT, false>(A, coop + tid / 64)) w.live = false;
+            else if (at) w.pop(lst, tid);
         } else if (w.live && (w.ref & kLeafBit)) {
             if (w.template leaf_step<ANY, COUNT>(A)) w.live = false;
             else w.pop(lst, tid);
@@ -226,8 +226,7 @@ int check(const rtx_bvh* b, const void* rays, size_t n, const void* out, const c
     return RTX_OK;
 }
 
-// While-while loop shape of the trace kernels: cooperative leaf phases when tuned on
-// (closest hit; any-hit kernels fall back to per-lane leaf steps inside the same shape).
+// While-while loop shape of the trace kernels: cooperative leaf phases when tuned on.
 uint32_t loop_shape() { return g_tuning.leaf_coop ? (uint32_t)kLoopCoop : (uint32_t)kLoopWhileWhile; }
 
 template <bool ANY, bool COUNT>
@@ -295,7 +294,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
 
 }  // namespace
 
-rtx_tuning g_tuning = {1u << 22, 2048, 256, 12, 2, 1, 16, 2048, 8, 1, 0, 0, 0};
+rtx_tuning g_tuning = {1u << 22, 2048, 256, 8, 2, 1, 24, 2048, 8, 1, 0, 1, 0};
 
 }  // namespace rtx
 

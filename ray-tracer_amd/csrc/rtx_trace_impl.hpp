@@ -27,8 +27,7 @@ struct TraceArgs {
 extern rtx_tuning g_tuning;
 
 // Loop shapes of trace_ray: one loop over nodes and leaves; the while-while loop; the
-// while-while loop with cooperative leaf phases (closest hit only; any-hit queries take
-// the while-while leaf steps).
+// while-while loop with cooperative leaf phases.
 constexpr int kLoopSingle = 0, kLoopWhileWhile = 1, kLoopCoop = 2;
 
 // Calls f(integral_constant<DEPTH>, integral_constant<LOOP>) for LDS depth d (8, 12, 16)
@@ -250,11 +249,13 @@ struct Walker {
     // LDS atomic min of coop_key(t, k) per owner keeps, among t < the owner's best t, the
     // smallest t and then the earliest triangle: what testing the leaf in order with a
     // strict t < best update keeps. The owner re-tests that one triangle for u and v (the
-    // same inputs give the same bits). The caller keeps all 64 lanes active and the
-    // control flow uniform. MIXED: lanes may trace different trees (the batched renderer),
-    // so the triangle array is taken from the owner too.
-    template <bool COUNT, bool MIXED>
-    __device__ __forceinline__ void leaf_coop(const TraceArgs& A, CoopLds* C) {
+    // same inputs give the same bits). Any-hit (ANY) keeps the smallest accepted triangle
+    // index instead: the first triangle in leaf order that the in-order scan would stop at.
+    // Returns true on the owner lanes whose any-hit query is answered. The caller keeps all
+    // 64 lanes active and the control flow uniform. MIXED: lanes may trace different trees
+    // (the batched renderer), so the triangle array is taken from the owner too.
+    template <bool ANY, bool COUNT, bool MIXED>
+    __device__ __forceinline__ bool leaf_coop(const TraceArgs& A, CoopLds* C) {
         const uint32_t lane = __lane_id();
         const bool at = live && (ref & kLeafBit);
         uint32_t c = 0, kb = 0;
@@ -272,7 +273,7 @@ struct Walker {
         const uint32_t total = __shfl(inc, 63);
         const uint32_t start = inc - c;
         const uint32_t kbase = kb - start;   // triangle of position p is kbase + p (mod 2^32)
-        if (COUNT) nT += c;
+        if (COUNT && !ANY) nT += c;
         if (c) wave_st(&C->mins[lane], ~0ull);
         const uint64_t upto = (2ull << lane) - 1ull;   // lanes 0..lane
         for (uint32_t W = 0; W < total; W += 64) {
@@ -303,9 +304,19 @@ struct Walker {
                 float u, v;
                 const float t = tri_test(f3{pox, poy, poz}, f3{pdx, pdy, pdz}, ptmin, ptmax, f3{a.x, a.y, a.z},
                                          f3{a.w, b.x, b.y}, f3{b.z, b.w, cz}, u, v);
-                if (t < pbest) atomicMin(&C->mins[r], coop_key(t, k));
+                if (t < pbest) atomicMin(&C->mins[r], ANY ? (unsigned long long)k : coop_key(t, k));
             }
             __builtin_amdgcn_wave_barrier();
+        }
+        if (ANY) {
+            bool hit = false;
+            if (c) {
+                const unsigned long long mk = wave_ld(&C->mins[lane]);
+                hit = mk != ~0ull;
+                if (hit) best = (uint32_t)mk;
+                if (COUNT) nT += hit ? (uint32_t)mk - kb + 1u : c;   // tests up to the first accepted
+            }
+            return hit;
         }
         if (c) {
             const unsigned long long mk = wave_ld(&C->mins[lane]);
@@ -321,14 +332,15 @@ struct Walker {
                 best = k;
             }
         }
+        return false;
     }
 };
 
 // One ray against one tree. LOOP 0 visits nodes and leaves in one loop; LOOP 1
 // ("while-while") lets a lane that reaches a leaf wait until the other lanes of the wave
 // are at leaves too (all but A.wait of them), so triangle tests run with the wave mostly
-// converged. LOOP 2 is LOOP 1 with cooperative leaf phases (leaf_coop) for closest-hit
-// queries. All visit the same nodes in the same order per ray and return the same hit.
+// converged. LOOP 2 is LOOP 1 with cooperative leaf phases (leaf_coop). All visit the
+// same nodes in the same order per ray and return the same hit.
 template <bool ANY, bool COUNT, int DEPTH, int LOOP, int TB = kTraceTriBatch, bool MIXED = false>
 __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, float tmin, float tmax,
                                               uint2 (*lst)[kTraceThreads], uint32_t tid, uint32_t wait,
@@ -350,10 +362,10 @@ __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, fl
         for (;;) {
             w.template node_phase<COUNT>(A, lst, tid, wait);
             if (!__any(w.live)) break;
-            if (LOOP == kLoopCoop && !ANY && __builtin_amdgcn_read_exec() == ~0ull) {
+            if (LOOP == kLoopCoop && __builtin_amdgcn_read_exec() == ~0ull) {
                 const bool at = w.live && (w.ref & kLeafBit);
-                w.template leaf_coop<COUNT, MIXED>(A, coop);
-                if (at) w.pop(lst, tid);
+                if (w.template leaf_coop<ANY, COUNT, MIXED>(A, coop)) w.live = false;
+                else if (at) w.pop(lst, tid);
             } else if (w.live && (w.ref & kLeafBit)) {
                 if (w.template leaf_step<ANY, COUNT>(A)) w.live = false;
                 else w.pop(lst, tid);
