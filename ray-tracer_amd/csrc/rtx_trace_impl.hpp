@@ -175,8 +175,13 @@ struct Walker {
             tl = slab_fast(o.x, o.y, o.z, ix, iy, iz, tmin, l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, bestT);
             tr = slab_fast(o.x, o.y, o.z, ix, iy, iz, tmin, l1.z, l1.w, r1b.x, r1b.y, r1b.z, r1b.w, bestT);
         } else {
-            tl = slab(o.x, o.y, o.z, ix, iy, iz, tmin, l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, bestT);
-            tr = slab(o.x, o.y, o.z, ix, iy, iz, tmin, l1.z, l1.w, r1b.x, r1b.y, r1b.z, r1b.w, bestT);
+            // Opaque copies of the origin: the compiler cannot share this branch's products
+            // with the fast branch, so the fast branch computes them in its own block, where
+            // they are known canonical and its min/max need no canonicalizing moves.
+            float ox = o.x, oy = o.y, oz = o.z;
+            __asm__ volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
+            tl = slab(ox, oy, oz, ix, iy, iz, tmin, l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, bestT);
+            tr = slab(ox, oy, oz, ix, iy, iz, tmin, l1.z, l1.w, r1b.x, r1b.y, r1b.z, r1b.w, bestT);
         }
         uint32_t nearRef, farRef;
         float tn, tf;
