@@ -114,9 +114,10 @@ __device__ __forceinline__ f3 env_color(const SceneDesc& S, f3 d) {
 __device__ __forceinline__ uint32_t texel_at(const SceneDesc& S, uint32_t tri, float u, float v) {
     const uint32_t* t3 = S.T.tris3 + 3 * (size_t)tri;
     const float w = 1.f - u - v;
-    const float2 a = reinterpret_cast<const float2*>(S.uv)[t3[0]];
-    const float2 b = reinterpret_cast<const float2*>(S.uv)[t3[1]];
-    const float2 c = reinterpret_cast<const float2*>(S.uv)[t3[2]];
+    const uint32_t i0 = ld_g(t3), i1 = ld_g(t3 + 1), i2 = ld_g(t3 + 2);
+    const float2 a{ld_g(S.uv + 2 * i0), ld_g(S.uv + 2 * i0 + 1)};
+    const float2 b{ld_g(S.uv + 2 * i1), ld_g(S.uv + 2 * i1 + 1)};
+    const float2 c{ld_g(S.uv + 2 * i2), ld_g(S.uv + 2 * i2 + 1)};
     const float U = (w * a.x + u * b.x) + v * c.x;
     const float V = (w * a.y + u * b.y) + v * c.y;
     int px = f2i_x86(U * (float)S.tex_w), py = f2i_x86(V * (float)S.tex_h);
@@ -165,11 +166,14 @@ __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* _
 
 // The wavefront trace loads leaf triangles two at a time and is held to 64 VGPRs so that
 // 8 waves fit per SIMD (measured faster than 4-triangle batches at 5 waves, both for the
-// 64-scene batched render and for the 4118-triangle differentiable render).
+// 64-scene batched render and for the 4118-triangle differentiable render). The
+// cooperative-leaf loop needs about 86 VGPRs and is held to 5 waves instead (at 64 it
+// spilled to scratch).
 constexpr int kRenderTriBatch = 2;
 
 template <int DEPTH, int LOOP>
-__global__ void __launch_bounds__(kTraceThreads, 8) k_wtrace(const SceneDesc* __restrict__ scenes, PathState P, int qi,
+__global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 5 : 8)
+    k_wtrace(const SceneDesc* __restrict__ scenes, PathState P, int qi,
                                                              uint32_t wait) {
     __shared__ uint2 lst[DEPTH][kTraceThreads];
     __shared__ CoopLds coop[LOOP == kLoopCoop ? kTraceThreads / 64 : 1];
@@ -223,14 +227,14 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
                 f3 e = env_color(S, d);
                 L = make_float4(L.x + tp.x * e.x, L.y + tp.y * e.y, L.z + tp.z * e.z, 0.f);
             } else {
-                const uint32_t prim = S.prims[tri];
+                const uint32_t prim = ld_g(S.prims + tri);
                 // albedo entry: the hit's texel (texture albedo) or its original triangle
                 const float* alb = S.tex ? S.tex : S.albedo;
                 const uint32_t aidx = S.tex ? texel_at(S, tri, h.y, h.z) : prim;
                 if (ADJ) P.vtp[(size_t)bounce * P.B + i] = tp;
                 if (S.emission) {
                     const float* e = S.emission + 3 * prim;
-                    L = make_float4(L.x + tp.x * e[0], L.y + tp.y * e[1], L.z + tp.z * e[2], 0.f);
+                    L = make_float4(L.x + tp.x * ld_g(e), L.y + tp.y * ld_g(e + 1), L.z + tp.z * ld_g(e + 2), 0.f);
                 }
                 if (ADJ) {
                     P.vlup[(size_t)bounce * P.B + i] = make_float4(L.x, L.y, L.z, __uint_as_float(aidx));
@@ -238,7 +242,7 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
                 }
                 if (bounce < S.max_bounce) {
                     float ax = 0.8f, ay = 0.8f, az = 0.8f;
-                    if (alb) { ax = alb[3 * aidx]; ay = alb[3 * aidx + 1]; az = alb[3 * aidx + 2]; }
+                    if (alb) { ax = ld_g(alb + 3 * aidx); ay = ld_g(alb + 3 * aidx + 1); az = ld_g(alb + 3 * aidx + 2); }
                     const uint32_t px = P.pix[i], sm = P.smp[i];
                     const float u1 = sample01(S.seed, px, sm, 2 + 3 * bounce);
                     const float u2 = sample01(S.seed, px, sm, 3 + 3 * bounce);

@@ -69,11 +69,32 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t g) {
 }
 
 // LDS of the cooperative leaf phase, per wave of 64 lanes.
+// 576 bytes per wave: with a 12-entry stack ring a 256-thread block needs 26,880 bytes of
+// LDS, so six blocks fit in a CU's 160 KiB (32-bit marks made it 27,648 and five blocks).
 struct CoopLds {
-    uint32_t mark[64];            // lane whose pairs start at this position of the round
     unsigned long long mins[64];  // per owner lane: smallest (t, triangle index) key
+    uint8_t mark[64];             // lane whose pairs start at this position of the round
 };
-constexpr uint32_t kNoLane = 0xFFFFFFFFu;
+constexpr uint8_t kNoLane = 0xFF;
+
+// Loads through global-memory pointers that the compiler cannot prove global (pointers
+// read from memory, such as the renderer's per-scene descriptors) would otherwise be flat
+// loads, which wait on both the vector-memory and the LDS counters. These load through an
+// explicit global address-space pointer.
+typedef float gfloat4 __attribute__((ext_vector_type(4)));
+typedef uint32_t guint2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 ld_g(const float4* p) {
+    const gfloat4 v = *(__attribute__((address_space(1))) const gfloat4*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 ld_g(const uint2* p) {
+    const guint2 v = *(__attribute__((address_space(1))) const guint2*)p;
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ float ld_g(const float* p) { return *(__attribute__((address_space(1))) const float*)p; }
+__device__ __forceinline__ uint32_t ld_g(const uint32_t* p) {
+    return *(__attribute__((address_space(1))) const uint32_t*)p;
+}
 
 // LDS words shared by the lanes of one wave: relaxed wave-scope atomics, so the compiler
 // keeps every access (a lane reads what another lane wrote) and still emits plain ds_*
@@ -92,9 +113,9 @@ __device__ __forceinline__ T wave_ld(T* p) {
 // empty asm takes the values as operands, so the compiler cannot sink the v0 load behind
 // the back-face test (a second dependent memory round trip for the lanes that pass it).
 __device__ __forceinline__ void load_tri(const GTri* p, float4& a, float4& b, float& c) {
-    a = p->a;
-    b = p->b;
-    c = p->c.x;
+    a = ld_g(&p->a);
+    b = ld_g(&p->b);
+    c = ld_g(&p->c.x);
     __asm__ volatile("" : : "v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(b.x), "v"(b.y), "v"(b.z), "v"(b.w), "v"(c));
 }
 
@@ -167,9 +188,9 @@ struct Walker {
     template <bool COUNT>
     __device__ __forceinline__ void node_step(const TraceArgs& A, uint2 (*lst)[kTraceThreads], uint32_t tid) {
         if (COUNT) ++nI;
-        const GNode& g = A.nodes[ref];
-        const float4 l0 = g.l0, l1 = g.l1, r1b = g.r1;
-        const uint2 refs = *reinterpret_cast<const uint2*>(&g.lref);
+        const GNode* g = A.nodes + ref;
+        const float4 l0 = ld_g(&g->l0), l1 = ld_g(&g->l1), r1b = ld_g(&g->r1);
+        const uint2 refs = ld_g(reinterpret_cast<const uint2*>(&g->lref));
         float tl, tr;
         if (__all(fast)) {   // wave vote: a scalar branch, both forms give the same values
             tl = slab_fast(o.x, o.y, o.z, ix, iy, iz, tmin, l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, bestT);
@@ -216,17 +237,17 @@ struct Walker {
     // leaf's end re-read its last triangle (always in bounds) and are not tested.
     template <bool ANY, bool COUNT>
     __device__ __forceinline__ bool leaf_step(const TraceArgs& A) {
-        const Leaf lf = A.leaves[ref & ~kLeafBit];
-        const uint32_t end = lf.begin + lf.count;
-        for (uint32_t k = lf.begin; k < end; k += TB) {
+        const uint2 lf = ld_g(reinterpret_cast<const uint2*>(A.leaves + (ref & ~kLeafBit)));   // begin, count
+        const uint32_t end = lf.x + lf.y;
+        for (uint32_t k = lf.x; k < end; k += TB) {
             float4 ta[TB], tb[TB];
             float tc[TB];
 #pragma unroll
             for (int j = 0; j < TB; ++j) {
                 const uint32_t kk = min(k + (uint32_t)j, end - 1u);
-                ta[j] = A.tris[kk].a;
-                tb[j] = A.tris[kk].b;
-                tc[j] = A.tris[kk].c.x;
+                ta[j] = ld_g(&A.tris[kk].a);
+                tb[j] = ld_g(&A.tris[kk].b);
+                tc[j] = ld_g(&A.tris[kk].c.x);
             }
 #pragma unroll
             for (int j = 0; j < TB; ++j) {
@@ -265,9 +286,9 @@ struct Walker {
         const bool at = live && (ref & kLeafBit);
         uint32_t c = 0, kb = 0;
         if (at) {
-            const Leaf lf = A.leaves[ref & ~kLeafBit];
-            c = lf.count;
-            kb = lf.begin;
+            const uint2 lf = ld_g(reinterpret_cast<const uint2*>(A.leaves + (ref & ~kLeafBit)));
+            c = lf.y;
+            kb = lf.x;
         }
         uint32_t inc = c;
 #pragma unroll
@@ -286,7 +307,7 @@ struct Walker {
             __builtin_amdgcn_wave_barrier();
             // segments do not overlap, so at most one owner marks each position; the
             // segment holding position W marks position 0
-            if (c && start < W + 64u && start + c > W) wave_st(&C->mark[(start > W ? start : W) - W], lane);
+            if (c && start < W + 64u && start + c > W) wave_st(&C->mark[(start > W ? start : W) - W], (uint8_t)lane);
             __builtin_amdgcn_wave_barrier();
             const uint32_t m = wave_ld(&C->mark[lane]);
             const uint64_t starts = __ballot(m != kNoLane);
@@ -390,10 +411,10 @@ __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, fl
 // Interpolated shading normal of a hit (bvh_traverse.cpp:135-139).
 __device__ __forceinline__ f3 hit_normal(const TraceArgs& A, uint32_t tri, float u, float v) {
     const float* V = A.verts;
-    uint32_t i0 = A.tris3[3 * tri], i1 = A.tris3[3 * tri + 1], i2 = A.tris3[3 * tri + 2];
-    f3 n0{V[6 * i0 + 3], V[6 * i0 + 4], V[6 * i0 + 5]};
-    f3 n1{V[6 * i1 + 3], V[6 * i1 + 4], V[6 * i1 + 5]};
-    f3 n2{V[6 * i2 + 3], V[6 * i2 + 4], V[6 * i2 + 5]};
+    uint32_t i0 = ld_g(A.tris3 + 3 * tri), i1 = ld_g(A.tris3 + 3 * tri + 1), i2 = ld_g(A.tris3 + 3 * tri + 2);
+    f3 n0{ld_g(V + 6 * i0 + 3), ld_g(V + 6 * i0 + 4), ld_g(V + 6 * i0 + 5)};
+    f3 n1{ld_g(V + 6 * i1 + 3), ld_g(V + 6 * i1 + 4), ld_g(V + 6 * i1 + 5)};
+    f3 n2{ld_g(V + 6 * i2 + 3), ld_g(V + 6 * i2 + 4), ld_g(V + 6 * i2 + 5)};
     float w = 1.f - u - v;
     return normalize(add(add(scl(w, n0), scl(u, n1)), scl(v, n2)));
 }
