@@ -113,11 +113,17 @@ struct PolyRegs {
     __device__ __forceinline__ float& at(int buf, int c, int i) { return v[buf][c][i]; }
 };
 
-struct PolyLds {
-    float* base;   // kPolyLdsFloats per block, lane-minor: [buf][coord][vertex][64]
-    __device__ __forceinline__ float& at(int buf, int c, int i) { return base[((buf * 3 + c) * kPolyMax + i) * 64]; }
+template <int COLS>
+struct PolyLdsT {
+    float* base;   // poly_floats(COLS) per block, lane-minor: [buf][coord][vertex][COLS]
+    __device__ __forceinline__ float& at(int buf, int c, int i) { return base[((buf * 3 + c) * kPolyMax + i) * COLS]; }
 };
-constexpr int kPolyLdsFloats = 2 * 3 * kPolyMax * 64;
+using PolyLds = PolyLdsT<64>;
+constexpr int poly_floats(int cols) { return 2 * 3 * kPolyMax * cols; }
+constexpr int kPolyLdsFloats = poly_floats(64);
+// k_sah_wave clips in its (axis, bin) lanes only: 3 axes x at most 16 bins (nodes of at
+// most 64 triangles), so 48 columns; its LDS then fits nine blocks per CU instead of seven.
+constexpr int kSahPolyCols = 48;
 
 template <class PS>
 __device__ Box clip_tri_in(f3 v0, f3 v1, f3 v2, const Box& r, PS& P) {
@@ -495,7 +501,7 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
     __shared__ float bbox[3 * 16][6];
     __shared__ uint32_t bc0[3 * 16], bc1[3 * 16];
     __shared__ uint32_t tsi_slot[64];   // lane 0's partition order
-    __shared__ float poly_lds[kPolyLdsFloats];
+    __shared__ float poly_lds[poly_floats(kSahPolyCols)];
     if (blockIdx.x >= C.ctr->nwave) return;
     const uint32_t k = C.wlist[blockIdx.x];
     const uint32_t ni = ls + k;
@@ -609,7 +615,7 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
                 cin += (b0 == bin);
                 cout += (b1 == bin);
                 if (b0 <= bin && bin <= b1) {
-                    PolyLds P{poly_lds + lane};
+                    PolyLdsT<kSahPolyCols> P{poly_lds + lane};   // act: lane < 3 * nb <= 48
                     Box cb = clip_tri_in(f3{tv[i][0], tv[i][1], tv[i][2]}, f3{tv[i][3], tv[i][4], tv[i][5]},
                                          f3{tv[i][6], tv[i][7], tv[i][8]}, reg, P);
                     if (cb.valid()) bx.extend(cb);
