@@ -85,6 +85,11 @@ int rtx_bvh_export(const rtx_bvh* bvh, void* nodes48, uint32_t* tris3, uint32_t*
 int rtx_bvh_morton(const rtx_bvh* bvh, uint64_t* codes, uint32_t* sorted);
 /* Node array before DFS renumbering (the s_nodes order of bfsLoop, bvh_bfs.cpp:72-134). */
 int rtx_bvh_export_bfs(const rtx_bvh* bvh, void* nodes48, uint32_t* count);
+/* Host run of the builder's register-only clip (clipTriangleToAABB, src/bvh/AABB.cpp:86-171,
+ * when only the two planes of `axis` cut the triangle), for the parity tests: returns 1 and
+ * the box in out6 (min xyz, max xyz) when that path applies, 0 when the builder would run
+ * the full clip instead.  No GPU work. */
+int rtx_debug_clip_slab(const float* v9, const float* region6, int axis, float* out6);
 
 /* ---- trace ------------------------------------------------------------------
  * Closest hit for n rays: replaces bvh::trace (src/bvh/bvh_traverse.cpp:237-246,

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <vector>
 
+#include "rtx_clip.hpp"
 #include "rtx_internal.hpp"
 #include "rtx_sort.hpp"
 
@@ -121,9 +122,8 @@ struct PolyLdsT {
 using PolyLds = PolyLdsT<64>;
 constexpr int poly_floats(int cols) { return 2 * 3 * kPolyMax * cols; }
 constexpr int kPolyLdsFloats = poly_floats(64);
-// k_sah_wave clips in its (axis, bin) lanes only: 3 axes x at most 16 bins (nodes of at
-// most 64 triangles), so 48 columns; its LDS then fits nine blocks per CU instead of seven.
-constexpr int kSahPolyCols = 48;
+// Clip pairs per chunk of k_sah_wave's spatial pass (two rounds of 64 lanes).
+constexpr int kClipChunk = 128;
 
 template <class PS>
 __device__ Box clip_tri_in(f3 v0, f3 v1, f3 v2, const Box& r, PS& P) {
@@ -501,7 +501,11 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
     __shared__ float bbox[3 * 16][6];
     __shared__ uint32_t bc0[3 * 16], bc1[3 * 16];
     __shared__ uint32_t tsi_slot[64];   // lane 0's partition order
-    __shared__ float poly_lds[poly_floats(kSahPolyCols)];
+    __shared__ float poly_lds[kPolyLdsFloats];
+    __shared__ uint8_t sb0[64][3], sb1[64][3];   // spatial entry / exit bin per triangle and axis
+    __shared__ uint32_t tax[64][3];              // first clip pair of triangle i on axis a
+    __shared__ float pbuf[kClipChunk][6];        // clipped boxes of one chunk of pairs
+    __shared__ uint8_t powner[kClipChunk];       // triangle of each pair of the chunk
     if (blockIdx.x >= C.ctr->nwave) return;
     const uint32_t k = C.wlist[blockIdx.x];
     const uint32_t ni = ls + k;
@@ -599,33 +603,97 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
     __syncthreads();
 
     // ---- spatial split: node-extent bins with per-bin clipping (bvh_sah.cpp:157-243)
+    // Each triangle is clipped to every bin it spans on each axis. Those (triangle, axis,
+    // bin) pairs are listed in triangle order and dealt over all 64 lanes, kClipChunk at a
+    // time, each lane clipping one pair into an LDS box; then each (axis, bin) lane folds
+    // its boxes in triangle order, as the reference's loop over the node's triangles does.
+    // (A lane per bin walking all triangles ran the clip for the whole wave at every
+    // triangle while a few lanes needed it.)
     const float nmin = act ? comp(pbox.lo, axis) : 0.f, nmax = act ? comp(pbox.hi, axis) : 0.f;
     const bool sOk = act && !(nmax - nmin < kSpatialEps);
     const float sscale = sOk ? (float)nb / (nmax - nmin) : 0.f;
-    if (act) {
-        Box bx = Box::empty();
-        uint32_t cin = 0, cout = 0;
+    float nminA[3], sscaleA[3];
+    bool sOkA[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {   // the same values the bin lanes of axis a compute
+        nminA[a] = comp(pbox.lo, a);
+        const float nmaxA = comp(pbox.hi, a);
+        sOkA[a] = !(nmaxA - nminA[a] < kSpatialEps);
+        sscaleA[a] = sOkA[a] ? (float)nb / (nmaxA - nminA[a]) : 0.f;
+    }
+    if (lane < 48) { bc0[lane] = 0; bc1[lane] = 0; }
+    __syncthreads();
+    uint32_t span[3] = {0, 0, 0};
+    if ((uint32_t)lane < n) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (!sOkA[a]) continue;
+            const int b0 = clampi(f2i_x86((tb[lane][a] - nminA[a]) * sscaleA[a]), 0, nb - 1);
+            const int b1 = clampi(f2i_x86((tb[lane][3 + a] - nminA[a]) * sscaleA[a]), 0, nb - 1);
+            sb0[lane][a] = (uint8_t)b0;
+            sb1[lane][a] = (uint8_t)b1;
+            atomicAdd(&bc0[a * nb + b0], 1u);   // entry / exit counts: integer, any order
+            atomicAdd(&bc1[a * nb + b1], 1u);
+            span[a] = b0 <= b1 ? (uint32_t)(b1 - b0 + 1) : 0u;
+        }
+    }
+    const uint32_t cnt = span[0] + span[1] + span[2];
+    uint32_t inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    const uint32_t total = __shfl(inc, 63), tpre = inc - cnt;
+    if ((uint32_t)lane < n) {
+        tax[lane][0] = tpre;
+        tax[lane][1] = tpre + span[0];
+        tax[lane][2] = tpre + span[0] + span[1];
+    }
+    Box bx = Box::empty();
+    for (uint32_t c0 = 0; c0 < total; c0 += kClipChunk) {
+        if ((uint32_t)lane < n) {
+            const uint32_t lo = max(tpre, c0), hi = min(tpre + cnt, c0 + (uint32_t)kClipChunk);
+            for (uint32_t p = lo; p < hi; ++p) powner[p - c0] = (uint8_t)lane;
+        }
+        __syncthreads();
+        for (int r = 0; r < kClipChunk; r += 64) {
+            const uint32_t p = c0 + r + lane;
+            if (p < total) {
+                const uint32_t i = powner[r + lane];
+                const int a = p >= tax[i][2] ? 2 : (p >= tax[i][1] ? 1 : 0);
+                const int b = (int)sb0[i][a] + (int)(p - tax[i][a]);
+                Box reg = pbox;
+                set_comp(reg.lo, a, nminA[a] + (float)b / sscaleA[a]);
+                set_comp(reg.hi, a, nminA[a] + (float)(b + 1) / sscaleA[a]);
+                const f3 p0{tv[i][0], tv[i][1], tv[i][2]}, p1{tv[i][3], tv[i][4], tv[i][5]},
+                    p2{tv[i][6], tv[i][7], tv[i][8]};
+                Box cb;
+                if (!clip_tri_slab(p0, p1, p2, reg, a, cb)) {
+                    PolyLds P{poly_lds + lane};
+                    cb = clip_tri_in(p0, p1, p2, reg, P);
+                }
+                pbuf[r + lane][0] = cb.lo.x; pbuf[r + lane][1] = cb.lo.y; pbuf[r + lane][2] = cb.lo.z;
+                pbuf[r + lane][3] = cb.hi.x; pbuf[r + lane][4] = cb.hi.y; pbuf[r + lane][5] = cb.hi.z;
+            }
+        }
+        __syncthreads();
         if (sOk) {
-            Box reg = pbox;
-            set_comp(reg.lo, axis, nmin + (float)bin / sscale);
-            set_comp(reg.hi, axis, nmin + (float)(bin + 1) / sscale);
             for (uint32_t i = 0; i < n; ++i) {
-                int b0 = clampi(f2i_x86((tb[i][axis] - nmin) * sscale), 0, nb - 1);
-                int b1 = clampi(f2i_x86((tb[i][3 + axis] - nmin) * sscale), 0, nb - 1);
-                cin += (b0 == bin);
-                cout += (b1 == bin);
+                const int b0 = sb0[i][axis], b1 = sb1[i][axis];
                 if (b0 <= bin && bin <= b1) {
-                    PolyLdsT<kSahPolyCols> P{poly_lds + lane};   // act: lane < 3 * nb <= 48
-                    Box cb = clip_tri_in(f3{tv[i][0], tv[i][1], tv[i][2]}, f3{tv[i][3], tv[i][4], tv[i][5]},
-                                         f3{tv[i][6], tv[i][7], tv[i][8]}, reg, P);
-                    if (cb.valid()) bx.extend(cb);
+                    const uint32_t idx = tax[i][axis] + (uint32_t)(bin - b0);
+                    if (idx >= c0 && idx < c0 + (uint32_t)kClipChunk) {
+                        const uint32_t q = idx - c0;
+                        const Box cb{{pbuf[q][0], pbuf[q][1], pbuf[q][2]}, {pbuf[q][3], pbuf[q][4], pbuf[q][5]}};
+                        if (cb.valid()) bx.extend(cb);
+                    }
                 }
             }
         }
-        save_bin(lane, bx);
-        bc0[lane] = cin;
-        bc1[lane] = cout;
+        __syncthreads();
     }
+    if (act) save_bin(lane, bx);
     __syncthreads();
     int sAxis, sBin;
     float sCost = argmin(sOk ? price(true) : kInf, sAxis, sBin);

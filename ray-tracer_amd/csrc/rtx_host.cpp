@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "rtx_clip.hpp"
 #include "rtx_internal.hpp"
 
 namespace rtx {
@@ -314,6 +315,16 @@ extern "C" int rtx_bvh_export(const rtx_bvh* b, void* nodes48, uint32_t* tris3, 
     if (tris3) RTX_HIP(hipMemcpy(tris3, b->tris3, 12ull * b->ntri, hipMemcpyDeviceToHost));
     if (prims) RTX_HIP(hipMemcpy(prims, b->prims, 4ull * b->ntri, hipMemcpyDeviceToHost));
     return RTX_OK;
+}
+
+extern "C" int rtx_debug_clip_slab(const float* v9, const float* region6, int axis, float* out6) {
+    if (!v9 || !region6 || !out6 || axis < 0 || axis > 2) return fail(RTX_E_ARG, "bad argument on rtx_debug_clip_slab()");
+    const Box r{{region6[0], region6[1], region6[2]}, {region6[3], region6[4], region6[5]}};
+    Box b;
+    if (!clip_tri_slab(f3{v9[0], v9[1], v9[2]}, f3{v9[3], v9[4], v9[5]}, f3{v9[6], v9[7], v9[8]}, r, axis, b)) return 0;
+    out6[0] = b.lo.x; out6[1] = b.lo.y; out6[2] = b.lo.z;
+    out6[3] = b.hi.x; out6[4] = b.hi.y; out6[5] = b.hi.z;
+    return 1;
 }
 
 extern "C" int rtx_bvh_morton(const rtx_bvh* b, uint64_t* codes, uint32_t* sorted) {

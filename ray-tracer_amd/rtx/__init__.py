@@ -20,7 +20,7 @@ NODE48 = np.dtype([("bmin", "<f4", 4), ("bmax", "<f4", 4), ("is_leaf", "u1"), ("
 
 # Every symbol include/rtx.h declares.
 EXPORTS = ["rtx_last_error", "rtx_device_count", "rtx_build", "rtx_build_device", "rtx_bvh_free",
-           "rtx_bvh_get_info", "rtx_bvh_export", "rtx_bvh_morton", "rtx_bvh_export_bfs", "rtx_trace_closest",
+           "rtx_bvh_get_info", "rtx_bvh_export", "rtx_bvh_morton", "rtx_bvh_export_bfs", "rtx_debug_clip_slab", "rtx_trace_closest",
            "rtx_trace_any", "rtx_trace_counts", "rtx_trace_closest_host", "rtx_save", "rtx_load",
            "rtx_scene_load", "rtx_free_host", "rtx_bake_scene", "rtx_geo_sphere", "rtx_geo_box", "rtx_render",
            "rtx_render_adjoint", "rtx_render_batched", "rtx_batch_create", "rtx_batch_launch", "rtx_batch_free",
@@ -86,6 +86,7 @@ def lib():
         L.rtx_bvh_get_info.argtypes = [vp, C.POINTER(BvhInfo)]
         L.rtx_bvh_export.argtypes = [vp, vp, vp, vp]
         L.rtx_bvh_morton.argtypes = [vp, vp, vp]
+        L.rtx_debug_clip_slab.argtypes = [vp, vp, C.c_int, vp]
         L.rtx_bvh_export_bfs.argtypes = [vp, vp, C.POINTER(u32)]
         L.rtx_trace_closest.argtypes = [vp, vp, sz, vp, vp, vp]
         L.rtx_trace_any.argtypes = [vp, vp, sz, vp, vp]
@@ -214,6 +215,18 @@ class BVH:
     def trace_counts_device(self, rays_d, counts_d, stream=None):
         _check(lib().rtx_trace_counts(self._h, _dptr(rays_d), rays_d.shape[0], _dptr(counts_d),
                                       _stream_ptr(stream)), "rtx_trace_counts")
+
+
+def debug_clip_slab(v9, region6, axis):
+    """Host run of the builder's register-only triangle clip: (True, box6) when that path
+    applies, (False, None) when the builder would run the full clip."""
+    v = np.ascontiguousarray(v9, np.float32).reshape(9)
+    r = np.ascontiguousarray(region6, np.float32).reshape(6)
+    out = np.zeros(6, np.float32)
+    rc = lib().rtx_debug_clip_slab(_p(v), _p(r), int(axis), _p(out))
+    if rc < 0:
+        _check(rc, "rtx_debug_clip_slab")
+    return (True, out) if rc == 1 else (False, None)
 
 
 def build(verts6, tris):

@@ -2,7 +2,9 @@
 built from the reference's own sources by oracle/Makefile) on small seeded scenes, so
 the oracle stays pinned where the reference checkout is absent.
 
-Each file holds the inputs (vertices, triangles, rays) and the reference's outputs:
+reference_clip.npz holds triangle/box pairs and clipTriangleToAABB's boxes (AABB.cpp:86-171).
+
+Each scene file holds the inputs (vertices, triangles, rays) and the reference's outputs:
 Morton codes and scene box, sort order, builder-order and DFS-order nodes (raw 48-byte
 BVHNode records), reordered triangles and original ids, s_triIndex / s_sortedTris sizes,
 and closest-hit t/u/v/triIndex/normal for a random ray set and an edge-case ray set.
@@ -56,5 +58,44 @@ def main():
         print(f"{path}: {len(t)} triangles, {len(r.nodes)} nodes, {os.path.getsize(path)} bytes")
 
 
+def clip_cases(n=6000, seed=11):
+    """Triangle/box pairs for the builder's clip: random and grid-valued vertices (ties on
+    planes), signed zeros, degenerate triangles; the box is the triangle's own extent on two
+    axes (sometimes cut) and a random slab on the third."""
+    rng = np.random.default_rng(seed)
+    v = rng.uniform(-1, 1, (n, 9)).astype(np.float32)
+    grid = (rng.integers(0, 10, (n, 9)) * 0.125 - 0.5).astype(np.float32)
+    v[1::4] = grid[1::4]
+    z = rng.integers(0, 10, (n, 9)) == 0
+    zm = np.zeros((n, 9), bool)
+    zm[2::4] = z[2::4]
+    v[zm] = np.where(rng.integers(0, 2, zm.sum()) == 1, np.float32(0.0), np.float32(-0.0))
+    v[3::4, 3:6] = v[3::4, 0:3]
+    axis = np.arange(n) % 3
+    box = np.zeros((n, 6), np.float32)
+    for i in range(n):
+        p = v[i].reshape(3, 3)
+        lo, hi = p.min(0), p.max(0)
+        for b in range(3):
+            if b == axis[i]:
+                s, e = np.sort(lo[b] + (hi[b] - lo[b]) * rng.uniform(0, 1, 2).astype(np.float32))
+                if i % 4 == 1:
+                    s = np.float32(rng.integers(0, 10) * 0.125 - 0.5)
+                    e = np.float32(s + 0.125 * (1 + rng.integers(0, 3)))
+                box[i, b], box[i, 3 + b] = s, e
+            elif rng.integers(0, 10) == 0:
+                box[i, b], box[i, 3 + b] = lo[b] + np.float32(0.25) * (hi[b] - lo[b]), hi[b]
+            else:
+                box[i, b], box[i, 3 + b] = lo[b], hi[b]
+    out = np.zeros((n, 6), np.float32)
+    valid = np.zeros(n, bool)
+    for i in range(n):
+        valid[i], out[i] = O.ref_clip(v[i], box[i])
+    path = os.path.join(HERE, "reference_clip.npz")
+    np.savez_compressed(path, verts=v, region=box, axis=axis.astype(np.int32), box=out, valid=valid)
+    print(f"{path}: {n} clip cases, {os.path.getsize(path)} bytes")
+
+
 if __name__ == "__main__":
     main()
+    clip_cases()

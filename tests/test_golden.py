@@ -12,7 +12,8 @@ import pytest
 from oracle import oracle as O
 from scenes_util import assert_same_nodes
 
-GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_*.npz")))
+GOLDEN = sorted(p for p in glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_*.npz"))
+                if not p.endswith("reference_clip.npz"))
 
 
 def test_golden_files_present():
@@ -41,3 +42,29 @@ def test_oracle_matches_reference_golden(path):
         got = np.stack([h["t"], h["u"], h["v"], h["tri"].view(np.float32)], 1)
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), key
         assert np.array_equal(h["normal"].view(np.uint32), g[key + "_normal"].view(np.uint32)), key
+
+
+def test_builder_clip_fast_path_matches_reference_golden():
+    """The builder's register-only clip (only the two planes of the split axis cut) against
+    the compiled reference's clipTriangleToAABB on the committed cases: bit-identical boxes
+    wherever it applies, the empty box where the reference's polygon vanishes, and it
+    declines the cases where a plane of another axis cuts."""
+    import rtx
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_clip.npz"))
+    v, reg, axis, box, valid = g["verts"], g["region"], g["axis"], g["box"], g["valid"]
+    taken = 0
+    for i in range(len(v)):
+        ok, b = rtx.debug_clip_slab(v[i], reg[i], int(axis[i]))
+        p = v[i].reshape(3, 3)
+        others = [a for a in range(3) if a != axis[i]]
+        cut_elsewhere = any(p[:, a].min() < reg[i, a] or p[:, a].max() > reg[i, 3 + a] for a in others)
+        if cut_elsewhere and not any(axis[i] < a for a in others if p[:, a].min() < reg[i, a] or p[:, a].max() > reg[i, 3 + a]):
+            assert not ok, i   # a plane before the split axis cuts the triangle itself
+        if not ok:
+            continue
+        taken += 1
+        if valid[i]:
+            assert np.array_equal(b.view(np.uint32), box[i].view(np.uint32)), (i, b, box[i])
+        else:
+            assert not (b[0] <= b[3] and b[1] <= b[4] and b[2] <= b[5]), (i, b)
+    assert taken > len(v) // 2
