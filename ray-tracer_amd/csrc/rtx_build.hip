@@ -500,7 +500,7 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
     __shared__ uint32_t tsi[64];
     __shared__ float bbox[3 * 16][6];
     __shared__ uint32_t bc0[3 * 16], bc1[3 * 16];
-    __shared__ uint32_t tsi_slot[64];   // lane 0's partition order
+    __shared__ uint32_t tsi_slot[64];   // rank exchange of the partitions
     __shared__ float poly_lds[kPolyLdsFloats];
     __shared__ uint8_t sb0[64][3], sb1[64][3];   // spatial entry / exit bin per triangle and axis
     __shared__ uint32_t tax[64][3];              // first clip pair of triangle i on axis a
@@ -679,7 +679,10 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
         }
         __syncthreads();
         if (sOk) {
-            for (uint32_t i = 0; i < n; ++i) {
+            // pairs are listed in triangle order: the chunk holds pairs of triangles
+            // powner[0] .. powner[last] only
+            const uint32_t i0 = powner[0], i1 = powner[min(total - c0, (uint32_t)kClipChunk) - 1];
+            for (uint32_t i = i0; i <= i1; ++i) {
                 const int b0 = sb0[i][axis], b1 = sb1[i][axis];
                 if (b0 <= bin && bin <= b1) {
                     const uint32_t idx = tax[i][axis] + (uint32_t)(bin - b0);
@@ -701,33 +704,43 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
     const int ws = sAxis >= 0 ? sAxis * nb + 1 : 0;
     const float sMin = __shfl(nmin, ws), sScale = __shfl(sscale, ws);
 
-    if (lane != 0) return;
-    // ---- apply the partitions serially, as std::partition orders them (bvh_sah.cpp:247-388)
+    // ---- apply the partitions as std::partition orders them (bvh_sah.cpp:247-388). Lane l
+    // holds the slot at position l; each partition is libstdc++'s two-ended algorithm
+    // (stl_algo.h:1487-1510) over positions [first, last): with M elements satisfying pred,
+    // the k-th failing element of [first, first+M) from the left swaps with the k-th
+    // satisfying element of [first+M, last) from the right, found with ballots. Every value
+    // below is the same on all lanes, so the control flow stays wave-uniform.
     Dec d;
     d.mid = 0; d.nl = d.ns = d.nr = 0; d.fb_split = 0;
     d.oAxis = oAxis; d.oBin = oBin; d.oCMin = oCMin; d.oScale = oScale; d.nb = nb;
     d.plane = 0.f; d.sAxis = -1;
     d.type = D_LEAF;
-    uint32_t* slot = tsi_slot;
-    for (uint32_t i = 0; i < n; ++i) slot[i] = i;
+    uint32_t sv = (uint32_t)lane;
     auto opred = [&](uint32_t s_) { return min(f2i_x86((tc[s_][oAxis] - oCMin) * oScale), nb - 1) < oBin; };
-    auto part = [&](uint32_t first, uint32_t last, auto pred) {
-        for (;;) {
-            for (;;) {
-                if (first == last) return first;
-                if (pred(slot[first])) ++first; else break;
-            }
-            --last;
-            for (;;) {
-                if (first == last) return first;
-                if (!pred(slot[last])) --last; else break;
-            }
-            uint32_t t_ = slot[first]; slot[first] = slot[last]; slot[last] = t_;
-            ++first;
-        }
+    auto part = [&](uint32_t first, uint32_t last, auto pred) -> uint32_t {
+        const bool inr = (uint32_t)lane >= first && (uint32_t)lane < last;
+        const bool p = inr && pred(sv);
+        const uint64_t below = (1ull << lane) - 1ull;
+        const uint32_t M = (uint32_t)__popcll(__ballot(p));
+        const uint64_t fl = __ballot(inr && !p && (uint32_t)lane < first + M);   // failing, left part
+        const uint64_t tr = __ballot(inr && p && (uint32_t)lane >= first + M);   // satisfying, right part
+        const bool isF = (fl >> lane) & 1ull, isT = (tr >> lane) & 1ull;
+        const uint32_t rF = (uint32_t)__popcll(fl & below);                      // rank from the left
+        const uint32_t rT = (uint32_t)__popcll(tr & ~below & ~(1ull << lane));   // rank from the right
+        if (isT) tsi_slot[rT] = (uint32_t)lane;
+        __syncthreads();
+        uint32_t partner = (uint32_t)lane;
+        if (isF) partner = tsi_slot[rF];
+        __syncthreads();
+        if (isF) tsi_slot[rF] = (uint32_t)lane;
+        __syncthreads();
+        if (isT) partner = tsi_slot[rT];
+        __syncthreads();
+        sv = __shfl(sv, (int)partner);
+        return first + M;
     };
     if (oAxis == -1 && !spatial) {
-        C.dec[k] = d;
+        if (lane == 0) C.dec[k] = d;
         return;
     }
     if (!spatial) {
@@ -747,14 +760,13 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
             }
         } else {
             uint32_t c = 0;
-            if (oAxis != -1)
-                for (uint32_t i = 0; i < n; ++i) c += opred(i) ? 1u : 0u;
+            if (oAxis != -1) c = (uint32_t)__popcll(__ballot((uint32_t)lane < n && opred((uint32_t)lane)));
             d.fb_split = (oAxis != -1 && c > 0 && c < n) ? 1u : 0u;
             d.type = D_SPATIAL;
         }
     }
-    for (uint32_t i = 0; i < n; ++i) C.tindex[begin + i] = tsi[slot[i]];
-    C.dec[k] = d;
+    if ((uint32_t)lane < n) C.tindex[begin + lane] = tsi[sv];
+    if (lane == 0) C.dec[k] = d;
 }
 
 // Block-wide exclusive scan for 1024 threads (16 waves). Returns the exclusive prefix of
