@@ -502,6 +502,7 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
     __shared__ uint32_t bc0[3 * 16], bc1[3 * 16];
     __shared__ uint32_t tsi_slot[64];   // rank exchange of the partitions
     __shared__ float poly_lds[kPolyLdsFloats];
+    __shared__ uint8_t obin[64][3];              // centroid bin per triangle and axis
     __shared__ uint8_t sb0[64][3], sb1[64][3];   // spatial entry / exit bin per triangle and axis
     __shared__ uint32_t tax[64][3];              // first clip pair of triangle i on axis a
     __shared__ float pbuf[kClipChunk][6];        // clipped boxes of one chunk of pairs
@@ -573,22 +574,48 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
     };
 
     // ---- object split: centroid bins (bvh_sah.cpp:58-126)
-    float cmin = 3.40282347e+38f, cmax = -3.40282347e+38f;
-    if (act) {
-        for (uint32_t i = 0; i < n; ++i) {
-            float c = tc[i][axis];
-            cmin = std_min(cmin, c);
-            cmax = std_max(cmax, c);
+    // Centroid range per axis: the reference folds std::min / std::max over the triangles
+    // from +-FLT_MAX. Here lane i holds triangle i and an ordered tree reduction combines
+    // (earlier, later) keeping the earlier value on ties, which is the fold's first
+    // minimum; a NaN centroid never enters the fold, so it enters the tree as the identity.
+    float cminA[3], cmaxA[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float c = (uint32_t)lane < n ? tc[lane][a] : 0.f;
+        const bool use = (uint32_t)lane < n && !__builtin_isnan(c);
+        float lo = use ? c : kInf, hi = use ? c : -kInf;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const float olo = __shfl_down(lo, off), ohi = __shfl_down(hi, off);
+            if ((lane & (2 * off - 1)) == 0) {
+                lo = std_min(lo, olo);
+                hi = std_max(hi, ohi);
+            }
+        }
+        cminA[a] = std_min(3.40282347e+38f, __shfl(lo, 0));
+        cmaxA[a] = std_max(-3.40282347e+38f, __shfl(hi, 0));
+    }
+    // centroid bin of each triangle on each axis (255: none, for a NaN centroid)
+    if ((uint32_t)lane < n) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const bool ok = !(cmaxA[a] - cminA[a] < kCentroidEps);
+            const float sc = ok ? (float)nb / (cmaxA[a] - cminA[a]) : 0.f;
+            const int b = min(f2i_x86((tc[lane][a] - cminA[a]) * sc), nb - 1);
+            obin[lane][a] = b >= 0 ? (uint8_t)b : (uint8_t)255;
         }
     }
+    const float cmin = act ? (axis == 0 ? cminA[0] : (axis == 1 ? cminA[1] : cminA[2])) : 3.40282347e+38f;
+    const float cmax = act ? (axis == 0 ? cmaxA[0] : (axis == 1 ? cmaxA[1] : cmaxA[2])) : -3.40282347e+38f;
     const bool axisOk = act && !(cmax - cmin < kCentroidEps);
     const float oscale = axisOk ? (float)nb / (cmax - cmin) : 0.f;
+    __syncthreads();
     if (act) {
         Box bx = Box::empty();
         uint32_t cnt = 0;
         if (axisOk) {
             for (uint32_t i = 0; i < n; ++i) {
-                if (min(f2i_x86((tc[i][axis] - cmin) * oscale), nb - 1) == bin) { ++cnt; bx.extend(tbox_i(i)); }
+                if (obin[i][axis] == bin) { ++cnt; bx.extend(tbox_i(i)); }
             }
         }
         save_bin(lane, bx);
