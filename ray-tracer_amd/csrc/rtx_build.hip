@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "rtx_clip.hpp"
@@ -57,6 +58,9 @@ struct Counters {
     uint32_t ssize;      // s_sortedTrisSize
     uint32_t err;
     uint32_t nwave;      // SAH nodes of this level queued for the wave evaluator
+    uint32_t lvl_ls;     // the level being built: its first node and node count
+    uint32_t lvl_cnt;
+    uint32_t lvl_le;     // one past its last node (the next level's first)
 };
 
 struct Ctx {
@@ -435,12 +439,24 @@ __global__ void k_init_root(BNode* nodes, const float* box6, uint32_t nt, uint32
         ctr->ssize = nt;
         ctr->err = 0;
         ctr->nwave = 0;
+        ctr->lvl_ls = 0;
+        ctr->lvl_cnt = 0;
+        ctr->lvl_le = 0;   // the first k_level_start makes the level [0, 1): the root
     }
     (void)tindex;
 }
 
-__global__ void k_clear_nwave(Counters* c) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) c->nwave = 0;
+// Starts the next level on the device: its nodes are those the previous level appended,
+// [previous end, total). The level kernels read this range, so the host can queue a level
+// before it knows the range (its grids are sized by an upper bound and surplus blocks exit).
+__global__ void k_level_start(Counters* c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const uint32_t le = c->total;
+        c->lvl_ls = c->lvl_le;
+        c->lvl_cnt = le - c->lvl_le;
+        c->lvl_le = le;
+        c->nwave = 0;
+    }
 }
 
 __global__ void k_iota(uint32_t* a, uint32_t n) {
@@ -448,7 +464,8 @@ __global__ void k_iota(uint32_t* a, uint32_t n) {
     if (i < n) a[i] = i;
 }
 
-__global__ void k_level_decide(Ctx C, uint32_t ls, uint32_t cnt) {
+__global__ void k_level_decide(Ctx C) {
+    const uint32_t ls = C.ctr->lvl_ls, cnt = C.ctr->lvl_cnt;
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= cnt) return;
     const uint32_t ni = ls + k;
@@ -493,7 +510,7 @@ __global__ void k_level_decide(Ctx C, uint32_t ls, uint32_t cnt) {
 // clipping (AABB.cpp:86-171) -- so every bin box is the reference's serial fold. Each
 // lane then prices split candidate `bin` of its axis; the winner is the minimum cost, and
 // on equal cost the first one the reference's loops meet (lower axis, higher bin).
-__global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
+__global__ void __launch_bounds__(64) k_sah_wave(Ctx C) {
     __shared__ float tv[64][9];
     __shared__ float tb[64][6];
     __shared__ float tc[64][3];
@@ -508,6 +525,7 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C, uint32_t ls) {
     __shared__ float pbuf[kClipChunk][6];        // clipped boxes of one chunk of pairs
     __shared__ uint8_t powner[kClipChunk];       // triangle of each pair of the chunk
     if (blockIdx.x >= C.ctr->nwave) return;
+    const uint32_t ls = C.ctr->lvl_ls;
     const uint32_t k = C.wlist[blockIdx.x];
     const uint32_t ni = ls + k;
     const BNode nd = C.nodes[ni];
@@ -823,7 +841,8 @@ __device__ uint32_t block_scan(uint32_t v, uint32_t* ws, uint32_t& total) {
 // numbering (s_totalNodeCount++ in FIFO order). One block of 1024 threads walks the
 // level in chunks; a chunk whose spatial candidates all fit is accepted with a scan,
 // otherwise lane 0 runs the greedy in order (a rejected candidate costs nothing).
-__global__ void __launch_bounds__(1024) k_level_budget(Ctx C, uint32_t cnt) {
+__global__ void __launch_bounds__(1024) k_level_budget(Ctx C) {
+    const uint32_t cnt = C.ctr->lvl_cnt;
     __shared__ uint32_t ws[17];
     __shared__ uint32_t base[3];     // total nodes, s_triIndexSize, s_sortedTrisSize
     __shared__ uint32_t slow;
@@ -976,9 +995,10 @@ __device__ uint32_t wave_partition(uint32_t* a, uint32_t first, uint32_t n, P pr
 // (child boxes of ranges that will not be split by Morton bits again), object splits,
 // and spatial splits (bvh_sah.cpp:247-460): right-only references appended,
 // straddlers duplicated, both children clipped to the split plane.
-__global__ void __launch_bounds__(64) k_level_apply(Ctx C, uint32_t ls, uint32_t cnt) {
+__global__ void __launch_bounds__(64) k_level_apply(Ctx C) {
     __shared__ uint32_t lds_lane[64];
     __shared__ float poly_lds[kPolyLdsFloats];
+    const uint32_t ls = C.ctr->lvl_ls, cnt = C.ctr->lvl_cnt;
     const uint32_t k = blockIdx.x;
     if (k >= cnt) return;
     const uint32_t lane = __lane_id();
@@ -1259,17 +1279,94 @@ __global__ void k_gtris_from_ref(const uint32_t* tris3, const uint32_t* prims, u
 
 inline uint32_t nblk(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
-// Scratch buffers of one build; freed on destruction.
+// Two pinned Counters slots and their events: the host reads one level's counters while
+// the next level runs.
+struct PinnedCounters {
+    Counters* h = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    PinnedCounters() {
+        if (hipHostMalloc((void**)&h, 2 * sizeof(Counters), hipHostMallocDefault) != hipSuccess) h = nullptr;
+        for (auto& e : ev)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
+    }
+    bool ok() const { return h && ev[0] && ev[1]; }
+    ~PinnedCounters() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (h) (void)hipHostFree(h);
+    }
+};
+
+// Process-wide cache of build scratch blocks, per device. A build allocates about 380
+// bytes of scratch per triangle in some twenty blocks; hipFree synchronizes the device and
+// unmaps, and freeing them cost more than the build's kernels. Blocks go back to the cache
+// once their build's stream is idle, and the next build takes the smallest cached block of
+// at least the size it needs (at most twice that). At most kMaxCached bytes stay cached;
+// beyond that blocks are freed.
+struct ScratchPool {
+    static constexpr size_t kMaxCached = size_t(4) << 30;
+    struct Block {
+        int dev;
+        size_t bytes;
+        void* p;
+    };
+    std::mutex mu;
+    std::vector<Block> blocks;
+    size_t cached = 0;
+    void* take(int dev, size_t bytes, size_t& got) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            int bi = -1;
+            for (int i = 0; i < (int)blocks.size(); ++i) {
+                const Block& b = blocks[i];
+                if (b.dev == dev && b.bytes >= bytes && b.bytes <= 2 * bytes + 4096 &&
+                    (bi < 0 || b.bytes < blocks[bi].bytes))
+                    bi = i;
+            }
+            if (bi >= 0) {
+                void* p = blocks[bi].p;
+                got = blocks[bi].bytes;
+                cached -= got;
+                blocks.erase(blocks.begin() + bi);
+                return p;
+            }
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        got = bytes;
+        return p;
+    }
+    void give(int dev, size_t bytes, void* p) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (cached + bytes <= kMaxCached) {
+                blocks.push_back({dev, bytes, p});
+                cached += bytes;
+                return;
+            }
+        }
+        (void)hipFree(p);
+    }
+};
+ScratchPool g_scratch_pool;
+
+// Scratch buffers of one build, taken from the cache; returned when the build's stream is
+// idle.
 struct Scratch {
-    std::vector<void*> ptrs;
+    hipStream_t st;
+    int dev = 0;
+    std::vector<std::pair<void*, size_t>> blocks;
+    explicit Scratch(hipStream_t s) : st(s) { (void)hipGetDevice(&dev); }
     template <class T>
     T* get(size_t n) {
-        T* p = dalloc<T>(n);
-        if (p) ptrs.push_back(p);
+        size_t got = 0;
+        T* p = (T*)g_scratch_pool.take(dev, std::max<size_t>(n, 1) * sizeof(T), got);
+        if (p) blocks.push_back({p, got});
         return p;
     }
     ~Scratch() {
-        for (void* p : ptrs) (void)hipFree(p);
+        (void)hipStreamSynchronize(st);   // no kernel of this build still uses the blocks
+        for (auto& b : blocks) g_scratch_pool.give(dev, b.second, b.first);
     }
 };
 
@@ -1292,7 +1389,7 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
                       rtx_bvh* B) {
     if (nt == 0 || nv == 0) return fail(RTX_E_ARG, "no triangles or vertices on rtx_build()");
     if (nt > (1u << 27)) return fail(RTX_E_LIMIT, "more than 2^27 triangles on rtx_build()");
-    Scratch S;
+    Scratch S(st);
     const uint32_t N = nt;
     const uint32_t cap = std::max<uint32_t>((uint32_t)((float)N * 1.5f), N);
     const uint32_t ncap = 2 * cap + 1;
@@ -1373,24 +1470,42 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
     k_init_root<<<1, 64, 0, st>>>(nodes, box6, N, tindex, ctr);
 
     Ctx C{V, tris_d, tbox, cent, scodes, sorted, tindex, nodes, dec, child, toff, soff, acc, cap, ncap, ctr, wlist};
+    // Levels are queued one ahead of the host: level L+1 is launched (grids sized for at
+    // most two children per node of level L) before the host reads level L's counters, so
+    // the GPU does not idle between levels while the host waits on a copy.
     std::vector<std::pair<uint32_t, uint32_t>> levels;
-    uint32_t ls = 0, le = 1;
-    Counters hc{};
-    while (ls < le) {
-        uint32_t cnt = le - ls;
+    PinnedCounters pc;
+    if (!pc.ok()) return fail(RTX_E_HIP, "pinned host allocation failed on rtx_build()");
+    auto launch_level = [&](int slot, uint32_t bound) -> int {
+        k_level_start<<<1, 64, 0, st>>>(ctr);
+        k_level_decide<<<nblk(bound, 64), 64, 0, st>>>(C);
+        k_sah_wave<<<bound, 64, 0, st>>>(C);
+        k_level_budget<<<1, 1024, 0, st>>>(C);
+        k_level_apply<<<bound, 64, 0, st>>>(C);
+        RTX_HIP(hipMemcpyAsync(pc.h + slot, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));
+        RTX_HIP(hipEventRecord(pc.ev[slot], st));
+        return RTX_OK;
+    };
+    uint32_t ls = 0, cnt = 1, total = 1;
+    Counters hlast{};
+    int rc = launch_level(0, 1);
+    for (int L = 0; rc == RTX_OK; ++L) {
         levels.push_back({ls, cnt});
-        k_clear_nwave<<<1, 64, 0, st>>>(ctr);
-        k_level_decide<<<nblk(cnt, 64), 64, 0, st>>>(C, ls, cnt);
-        k_sah_wave<<<cnt, 64, 0, st>>>(C, ls);
-        k_level_budget<<<1, 1024, 0, st>>>(C, cnt);
-        k_level_apply<<<cnt, 64, 0, st>>>(C, ls, cnt);
-        RTX_HIP(hipMemcpyAsync(&hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, st));
-        RTX_HIP(hipStreamSynchronize(st));
-        if (hc.err || hc.total > ncap) return fail(RTX_E_LIMIT, "node capacity exceeded on rtx_build()");
-        ls = le;
-        le = hc.total;
+        rc = launch_level((L + 1) & 1, 2 * cnt);
+        if (rc != RTX_OK) break;
+        if (!hip_ok(hipEventSynchronize(pc.ev[L & 1]), "hipEventSynchronize")) { rc = RTX_E_HIP; break; }
+        const Counters hc = pc.h[L & 1];
+        if (hc.err || hc.total > ncap) { rc = fail(RTX_E_LIMIT, "node capacity exceeded on rtx_build()"); break; }
+        hlast = hc;
+        total = hc.total;
+        ls += cnt;
+        cnt = total - ls;
+        if (cnt == 0) break;   // the level queued last has no nodes: its blocks all exit
     }
-    const uint32_t total = hc.total;
+    if (rc != RTX_OK) {
+        (void)hipStreamSynchronize(st);   // no kernel may still use the scratch freed on return
+        return rc;
+    }
     for (size_t l = levels.size(); l-- > 0;)
         k_refit<<<nblk(levels[l].second, 256), 256, 0, st>>>(nodes, levels[l].first, levels[l].second);
     for (size_t l = levels.size(); l-- > 0;)
@@ -1433,8 +1548,8 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
     B->nt = nt;
     B->levels = (uint32_t)levels.size();
     B->max_depth = B->levels - 1;
-    B->tindex_size = hc.tsize;
-    B->sorted_size = hc.ssize;
+    B->tindex_size = hlast.tsize;
+    B->sorted_size = hlast.ssize;
     return RTX_OK;
 }
 
