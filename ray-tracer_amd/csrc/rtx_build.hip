@@ -1354,21 +1354,35 @@ ScratchPool g_scratch_pool;
 // idle.
 struct Scratch {
     hipStream_t st;
-    int dev = 0;
     std::vector<std::pair<void*, size_t>> blocks;
-    explicit Scratch(hipStream_t s) : st(s) { (void)hipGetDevice(&dev); }
+    explicit Scratch(hipStream_t s) : st(s) {}
     template <class T>
     T* get(size_t n) {
         size_t got = 0;
-        T* p = (T*)g_scratch_pool.take(dev, std::max<size_t>(n, 1) * sizeof(T), got);
+        T* p = (T*)scratch_take(std::max<size_t>(n, 1) * sizeof(T), got);
         if (p) blocks.push_back({p, got});
         return p;
     }
     ~Scratch() {
         (void)hipStreamSynchronize(st);   // no kernel of this build still uses the blocks
-        for (auto& b : blocks) g_scratch_pool.give(dev, b.second, b.first);
+        for (auto& b : blocks) scratch_give(b.first, b.second);
     }
 };
+
+}  // namespace
+
+void* scratch_take(size_t bytes, size_t& got) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    return g_scratch_pool.take(dev, bytes, got);
+}
+void scratch_give(void* p, size_t bytes) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    g_scratch_pool.give(dev, bytes, p);
+}
+
+namespace {
 
 void free_products(rtx_bvh* b) {
     void* ps[] = {b->verts, b->nodes48, b->tris3, b->prims, b->gnodes, b->leaves, b->gtris, b->codes, b->sorted,

@@ -18,6 +18,13 @@ bool hip_ok(hipError_t e, const char* what);
         if (!::rtx::hip_ok((call), #call)) return RTX_E_HIP; \
     } while (0)
 
+// Process-wide cache of device scratch blocks (rtx_build.hip): scratch_take returns a
+// cached block of at least `bytes` (its size in `got`) or a new allocation; scratch_give
+// returns a block once no queued work uses it. Saves hipMalloc/hipFree (which synchronizes
+// the device) on every build and render.
+void* scratch_take(size_t bytes, size_t& got);
+void scratch_give(void* p, size_t bytes);
+
 template <class T>
 T* dalloc(size_t n) {
     void* p = nullptr;

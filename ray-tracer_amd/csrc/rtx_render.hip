@@ -477,7 +477,8 @@ struct Batch {
     SceneDesc* d_scenes = nullptr;
     uint64_t* d_off = nullptr;
     PathState P{};
-    std::vector<void*> allocs;
+    std::vector<std::pair<void*, size_t>> allocs;   // blocks of the scratch cache
+    hipEvent_t done = nullptr;                      // recorded after every launch
     uint32_t B = 0;
     int mode = 0;
     hipGraph_t graph = nullptr;
@@ -487,13 +488,25 @@ struct Batch {
     ~Batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (void* p : allocs) (void)hipFree(p);
+        // blocks go back to the cache only once no launch of this batch can still use them
+        // (launches of one batch are ordered by its caller: they share the path state)
+        if (done) {
+            (void)hipEventSynchronize(done);
+            (void)hipEventDestroy(done);
+        }
+        for (auto& a : allocs) scratch_give(a.first, a.second);
     }
     template <class T>
     T* alloc(size_t n) {
-        T* p = dalloc<T>(n);
-        if (p) allocs.push_back(p);
+        size_t got = 0;
+        T* p = (T*)scratch_take(std::max<size_t>(n, 1) * sizeof(T), got);
+        if (p) allocs.push_back({p, got});
         return p;
+    }
+    int note_launch(hipStream_t s) {
+        if (!done) RTX_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        RTX_HIP(hipEventRecord(done, s));
+        return RTX_OK;
     }
     // adjoint: dLoss/dimage in, gradients out (scene 0 only)
     const float* dimg = nullptr;
@@ -591,7 +604,7 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
 int Batch::launch(hipStream_t st) {
     if (mode == 1) {
         RTX_HIP(hipGraphLaunch(exec, st));
-        return RTX_OK;
+        return note_launch(st);
     }
     if (mode == 2) {
         enqueue_range(0, (int)descs.size(), st);
@@ -599,7 +612,7 @@ int Batch::launch(hipStream_t st) {
         for (int s = 0; s < (int)descs.size(); ++s) enqueue_range(s, s + 1, st);
     }
     RTX_HIP(hipGetLastError());
-    return RTX_OK;
+    return note_launch(st);
 }
 
 }  // namespace rtx
@@ -712,6 +725,8 @@ extern "C" int rtx_render_adjoint(const rtx_bvh* b, const rtx_camera* cam, const
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     B.enqueue_range(0, 1, st);
+    rc = B.note_launch(st);
+    if (rc) return rc;
     if (!hip_ok(hipGetLastError(), "adjoint launch") || !hip_ok(hipStreamSynchronize(st), "sync")) return RTX_E_HIP;
     return RTX_OK;
 }
