@@ -208,11 +208,13 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
     const uint32_t* Q = P.q[qi];
     uint32_t* Qo = P.q[qi ^ 1];
     const uint64_t lt = (1ull << __lane_id()) - 1ull;
-    // uniform trip count per wave so the ballot below sees every lane
+    __shared__ uint32_t wcount[16];   // surviving paths per wave of the block (<= 1024 threads)
+    __shared__ uint32_t bbase;
+    const uint32_t wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    // trip count uniform per block: the queue append below is one atomic per block
     const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t base0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-    for (uint32_t wb = base0; wb < n; wb += stride) {
-        const uint32_t k = wb + __lane_id();
+    for (uint32_t bb = blockIdx.x * blockDim.x; bb < n; bb += stride) {
+        const uint32_t k = bb + threadIdx.x;
         bool cont = false;
         uint32_t i = 0;
         if (k < n) {
@@ -269,12 +271,16 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
             P.L[i] = L;
         }
         const uint64_t m = __ballot(cont);
-        if (m) {
-            uint32_t base = 0;
-            if (__lane_id() == 0) base = atomicAdd(&P.cnt[qi ^ 1], (uint32_t)__popcll(m));
-            base = __shfl(base, 0);
-            if (cont) Qo[base + __popcll(m & lt)] = i;
+        if (__lane_id() == 0) wcount[wid] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t run = 0;
+            for (uint32_t w = 0; w < nw; ++w) { const uint32_t c = wcount[w]; wcount[w] = run; run += c; }
+            bbase = run ? atomicAdd(&P.cnt[qi ^ 1], run) : 0u;
         }
+        __syncthreads();
+        if (cont) Qo[bbase + wcount[wid] + __popcll(m & lt)] = i;
+        __syncthreads();
     }
 }
 
