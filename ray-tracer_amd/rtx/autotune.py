@@ -28,7 +28,7 @@ SPACE = {
     "render_wait_lanes": [0, 4, 8, 16],
     "render_leaf_coop": [0, 1],
     "grid_blocks": [1024, 2048, 4096, 8192],
-    "shade_threads": [64, 128, 256, 512],
+    "shade_threads": [64, 128, 256, 512, 1024],
     "batch_paths": [1 << 20, 1 << 21, 1 << 22, 1 << 23],
 }
 TRACE_KEYS = ["trace_loop", "leaf_coop", "trace_lds_depth", "ray_sort", "wait_lanes", "trace_grid_blocks"]
