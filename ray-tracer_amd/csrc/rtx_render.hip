@@ -167,12 +167,13 @@ __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* _
 // The wavefront trace loads leaf triangles two at a time and is held to 64 VGPRs so that
 // 8 waves fit per SIMD (measured faster than 4-triangle batches at 5 waves, both for the
 // 64-scene batched render and for the 4118-triangle differentiable render). The
-// cooperative-leaf loop needs about 86 VGPRs and is held to 5 waves instead (at 64 it
-// spilled to scratch).
+// cooperative-leaf loop is held to 7 waves (72 VGPRs, a few spilled): measured against 5
+// (86 VGPRs), 6 and 8 waves, it was fastest on all three render workloads (C5 215 -> 184
+// ms), and at 8 waves the spills cost more than the wave gained.
 constexpr int kRenderTriBatch = 2;
 
 template <int DEPTH, int LOOP>
-__global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 5 : 8)
+__global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
     k_wtrace(const SceneDesc* __restrict__ scenes, PathState P, int qi,
                                                              uint32_t wait) {
     __shared__ uint2 lst[DEPTH][kTraceThreads];
