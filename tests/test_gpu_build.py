@@ -74,3 +74,21 @@ def test_build_device_inputs():
     o = O.build(v, t)
     assert_same_nodes(nodes, o.nodes, "device inputs")
     assert np.array_equal(prims, o.prims)
+
+
+def test_builds_reusing_cached_scratch(suite):
+    """Build scratch is cached across builds, so later builds run on blocks that hold an
+    earlier build's data: interleaved builds of different sizes (the larger one first, so
+    smaller builds take its blocks) still equal the oracle bit for bit, and a handle
+    traced after other builds gives the same hits as when it was new."""
+    big = suite["rand20000"]
+    g_big, _ = _check("rand20000 (first)", *big)
+    rays = S.incoherent_rays(4096, seed=21)
+    lo, hi = big[0][:, :3].min(0), big[0][:, :3].max(0)
+    rays[:, :3] = lo + (hi - lo) * rays[:, :3]
+    h0 = g_big.trace(rays)
+    for name in ("rand17", "straddle", "rand1000", "dupcodes", "rand20000", "grid", "rand65"):
+        _check(name + " (after others)", *suite[name])
+    h1 = g_big.trace(rays)
+    for k in ("t", "u", "v", "tri"):
+        assert np.array_equal(np.asarray(h0[k]).view(np.uint32), np.asarray(h1[k]).view(np.uint32)), k
