@@ -222,14 +222,16 @@ void rtx_batch_free(rtx_batch* batch);
  *   grid_blocks        blocks of the wavefront trace and shade kernels;
  *   shade_threads      threads per shade block (multiple of 64, <= 1024);
  *   render_lds_depth   LDS stack entries per lane: 8, 12 or 16;
- *   render_loop        0 or 1 as trace_loop;
+ *   render_loop        0 or 1 as trace_loop; 2 = while-while in a persistent grid of
+ *                      trace_grid_blocks blocks whose waves refill idle lanes from the
+ *                      bounce queue;
  *   render_wait_lanes  as wait_lanes.
  * Both:
  *   leaf_coop          1 = the while-while loops of the trace entry points (trace_loop
  *                      1 and 2) run cooperative leaf phases: the triangle tests
  *                      of all lanes waiting at leaves are spread over the whole wave, 64 per
  *                      round, instead of each lane testing its own leaf;
- *   render_leaf_coop   the same for the renderer's while-while loop (render_loop 1). */
+ *   render_leaf_coop   the same for the renderer's while-while loops (render_loop 1 and 2). */
 typedef struct rtx_tuning {
     uint32_t batch_paths;
     uint32_t grid_blocks;

@@ -201,6 +201,80 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
     }
 }
 
+// Persistent variant (render_loop 2): a grid of waves that refill idle lanes (at least
+// kRenderRefill at a time) from the queue, so a wave does not wait for the slowest path of
+// a 64-path chunk. Each wave takes kRenderChunk queue positions per atomic on `work` and
+// serves its refills from them: with one atomic per refill the single counter serialized
+// the launch (C3 162 ms against 89 ms for the chunked kernel; 128-path chunks: 96 ms, and
+// C4 10.4 -> 8.7 ms, C5 184 -> 152 ms). The control flow follows k_trace_persist: every
+// ballot, the leader's atomic, the shuffle and the exit test are wave-uniform, and no lane
+// leaves the loop body early.
+constexpr int kRenderRefill = 24;
+constexpr int kRenderChunk = 128;   // queue positions per atomic on the path counter
+
+template <int DEPTH, int LOOP>
+__global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
+    k_wtrace_persist(const SceneDesc* __restrict__ scenes, PathState P, int qi, uint32_t wait, uint32_t* work) {
+    __shared__ uint2 lst[DEPTH][kTraceThreads];
+    __shared__ CoopLds coop[LOOP == kLoopCoop ? kTraceThreads / 64 : 1];
+    const uint32_t n = P.cnt[qi];
+    const uint32_t* Q = P.q[qi];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t lt = (1ull << __lane_id()) - 1ull;
+    Walker<DEPTH, kRenderTriBatch> w;
+    uint2 spill[kSpillDepth];
+    w.spill = spill;
+    w.live = false;
+    w.best = kInvalid;
+    uint32_t id = kInvalid;            // path being traced
+    const SceneDesc* S = scenes;       // its scene
+    uint32_t rbase = 0, rend = 0;      // wave-uniform reservoir of queue positions [rbase, rend)
+    bool gdone = false;                // `work` has passed n
+    bool more = true;                  // the same on every lane: set only from wave-wide values
+    for (;;) {
+        if (!w.live && id != kInvalid) {
+            P.hit[id] = make_float4(w.best != kInvalid ? w.bestT : 0.f, w.bu, w.bv, __uint_as_float(w.best));
+            id = kInvalid;
+        }
+        const uint64_t idle = __ballot(!w.live);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (more && nidle >= (uint32_t)kRenderRefill) {
+            if (rbase == rend) {   // one atomic takes kRenderChunk queue positions
+                const int leader = __ffsll((unsigned long long)idle) - 1;
+                uint32_t b = 0;
+                if (__lane_id() == leader) b = atomicAdd(work, (uint32_t)kRenderChunk);
+                b = __shfl(b, leader);
+                rbase = min(b, n);
+                rend = min(b + (uint32_t)kRenderChunk, n);
+                gdone = b + (uint32_t)kRenderChunk >= n;
+            }
+            const uint32_t base = rbase;
+            rbase = min(rbase + nidle, rend);
+            more = !gdone || rbase < rend;
+            if (!w.live) {
+                const uint32_t k = base + (uint32_t)__popcll(idle & lt);
+                if (k < rend) {
+                    id = Q[k];
+                    S = scenes + P.sid[id];
+                    const float4 r0 = P.ray0[id], r1 = P.ray1[id];
+                    w.start(S->T, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w);
+                }
+            }
+        }
+        // uniform exit test: a started path that missed the root is written at the top
+        if (!__any(w.live || id != kInvalid) && !more) break;
+        w.template node_phase<false>(S->T, lst, tid, wait);
+        if (LOOP == kLoopCoop) {   // all lanes active: the loop is wave-uniform
+            const bool at = w.live && (w.ref & kLeafBit);
+            w.template leaf_coop<false, false, true>(S->T, coop + tid / 64);
+            if (at) w.pop(lst, tid);
+        } else if (w.live && (w.ref & kLeafBit)) {
+            w.template leaf_step<false, false>(S->T);
+            w.pop(lst, tid);
+        }
+    }
+}
+
 // One bounce of pathTrace (main.cpp:158-200) for every queued path. ADJ additionally
 // records each surface hit for the adjoint (triangle, radiance so far, throughput).
 template <bool ADJ>
@@ -560,6 +634,7 @@ int Batch::setup(uint32_t batch_paths, bool adjoint) {
 // Kernel sequence for the paths of scenes [s_first, s_last): batches of B paths.
 void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
     const uint32_t grid = g_tuning.grid_blocks, sh = g_tuning.shade_threads, wait = g_tuning.render_wait_lanes;
+    const uint32_t pgrid = g_tuning.trace_grid_blocks;   // blocks of the persistent trace kernels
     const bool adj = dimg != nullptr;
     int mb = 0;
     for (int s = s_first; s < s_last; ++s) {
@@ -582,9 +657,18 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
         for (int b = 0; b <= mb; ++b) {
             const uint32_t loop = g_tuning.render_loop == 0 ? (uint32_t)kLoopSingle
                                   : g_tuning.render_leaf_coop ? (uint32_t)kLoopCoop : (uint32_t)kLoopWhileWhile;
-            with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
-                k_wtrace<decltype(D)::value, decltype(L)::value><<<grid, kTraceThreads, 0, st>>>(d_scenes, P, qi, wait);
-            });
+            if (g_tuning.render_loop == 2) {
+                k_clear1<<<1, 64, 0, st>>>(P.cnt + 2);   // the persistent kernel's path counter
+                with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
+                    k_wtrace_persist<decltype(D)::value, decltype(L)::value><<<pgrid, kTraceThreads, 0, st>>>(
+                        d_scenes, P, qi, wait, P.cnt + 2);
+                });
+            } else {
+                with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
+                    k_wtrace<decltype(D)::value, decltype(L)::value><<<grid, kTraceThreads, 0, st>>>(d_scenes, P, qi,
+                                                                                                   wait);
+                });
+            }
             if (adj) k_shade<true><<<grid, sh, 0, st>>>(d_scenes, P, qi, b);
             else k_shade<false><<<grid, sh, 0, st>>>(d_scenes, P, qi, b);
             k_clear1<<<1, 64, 0, st>>>(P.cnt + qi);
@@ -634,7 +718,7 @@ extern "C" int rtx_set_tuning(const rtx_tuning* t) {
     if (!t || t->batch_paths < 1024 || t->grid_blocks == 0 || t->shade_threads == 0 || t->shade_threads % 64 ||
         t->shade_threads > 1024 || !(t->trace_lds_depth == 8 || t->trace_lds_depth == 12 || t->trace_lds_depth == 16) ||
         t->trace_loop > 2 || t->ray_sort > 1 || t->wait_lanes > 64 || t->trace_grid_blocks == 0 ||
-        !(t->render_lds_depth == 8 || t->render_lds_depth == 12 || t->render_lds_depth == 16) || t->render_loop > 1 ||
+        !(t->render_lds_depth == 8 || t->render_lds_depth == 12 || t->render_lds_depth == 16) || t->render_loop > 2 ||
         t->render_wait_lanes > 64 || t->leaf_coop > 1 || t->render_leaf_coop > 1)
         return fail(RTX_E_ARG, "bad tuning on rtx_set_tuning()");
     g_tuning = *t;

@@ -24,7 +24,7 @@ SPACE = {
     "leaf_coop": [0, 1],
     # renderer
     "render_lds_depth": [8, 12, 16],
-    "render_loop": [0, 1],
+    "render_loop": [0, 1, 2],
     "render_wait_lanes": [0, 4, 8, 16],
     "render_leaf_coop": [0, 1],
     "grid_blocks": [1024, 2048, 4096, 8192],

@@ -114,8 +114,8 @@ def test_batched_modes_match_single_and_oracle():
 
 def test_render_trace_variants_bit_identical():
     """The wavefront renderer's traversal variants (one loop, while-while with any
-    render_wait_lanes, stack depths, grid and batch sizes) give bit-identical images in a
-    shared multi-scene batch."""
+    render_wait_lanes, cooperative leaves, the persistent kernel, stack depths, grid and
+    batch sizes) give bit-identical images in a shared multi-scene batch."""
     import torch
     sc = _small_batch_scenes()
     bvhs = [rtx.build(v, t) for v, t, _, _, _ in sc]
@@ -125,9 +125,13 @@ def test_render_trace_variants_bit_identical():
         for loop, depth, wait, grid, bp, coop in ((1, 8, 0, 2048, 1 << 22, 0), (1, 12, 16, 2048, 1 << 22, 0),
                                                   (1, 8, 8, 8, 4096, 0), (1, 16, 32, 64, 8192, 0),
                                                   (0, 8, 0, 512, 1 << 20, 0), (1, 8, 0, 2048, 1 << 22, 1),
-                                                  (1, 12, 32, 8, 4096, 1), (0, 8, 0, 512, 1 << 20, 1)):
+                                                  (1, 12, 32, 8, 4096, 1), (0, 8, 0, 512, 1 << 20, 1),
+                                                  (2, 8, 8, 16, 1 << 22, 1), (2, 12, 0, 16, 4096, 0),
+                                                  (2, 8, 16, 2048, 1 << 20, 1)):
+            # render_loop 2 is the persistent kernel, with trace_grid_blocks = grid here (16
+            # blocks: every lane takes many paths)
             rtx.set_tuning(render_loop=loop, render_lds_depth=depth, render_wait_lanes=wait, grid_blocks=grid,
-                           batch_paths=bp, render_leaf_coop=coop)
+                           batch_paths=bp, render_leaf_coop=coop, trace_grid_blocks=grid)
             print(f"[render variant] loop={loop} depth={depth} wait={wait} grid={grid} coop={coop}", flush=True)
             imgs = [torch.zeros((p.height, p.width, 3), device="cuda") for _, _, _, p, _ in sc]
             b = rtx.Batch(bvhs, [c for _, _, c, _, _ in sc], [p for _, _, _, p, _ in sc], [None] * 3, [None] * 3,
