@@ -3,7 +3,8 @@
 //
 // Per pass: k_rs_hist counts digits per 4096-key tile in LDS; a three-kernel scan turns
 // the digit-major histogram into global offsets; k_rs_scatter ranks keys stably inside a
-// tile with wave ballots (lanes holding the same digit) and per-wave LDS counts.
+// tile with wave ballots (lanes holding the same digit) and per-wave LDS counts, then
+// writes the tile out in digit order from LDS.
 #include <hip/hip_runtime.h>
 
 #include "rtx_sort.hpp"
@@ -87,42 +88,88 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_add(uint32_t* data, uint3
         if (b + k < m) data[b + k] += add;
 }
 
+// Scatter of one pass. Ranks are stable within the tile: by item round, then wave, then
+// lane (ballots give each lane its rank among the wave's lanes holding the same digit).
+// The tile is first placed in LDS in digit order and then written out slot by slot, so
+// consecutive lanes store consecutive elements of a digit's run (4-byte stores to unrelated
+// addresses cost a memory transaction each).
 template <class K>
-__global__ void k_rs_scatter(const K* kin, const uint32_t* vin, K* kout, uint32_t* vout, uint32_t n,
-                             int shift, const uint32_t* hist, uint32_t ntiles) {
-    __shared__ uint32_t run[256];
+__global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const K* kin, const uint32_t* vin, K* kout,
+                                                           uint32_t* vout, uint32_t n, int shift,
+                                                           const uint32_t* hist, uint32_t ntiles) {
+    __shared__ uint32_t run[256];      // elements of digit d seen so far in this tile
+    __shared__ uint32_t gbase[256];    // global position of this tile's first digit-d element
+    __shared__ uint32_t lstart[256];   // tile-local position of the first digit-d element
     __shared__ uint32_t wcnt[kRsThreads / 64][256];
+    __shared__ uint32_t wsum[kRsThreads / 64];
+    __shared__ K skey[kRsTile];
+    __shared__ uint32_t sval[kRsTile];
     const int lane = __lane_id(), w = threadIdx.x >> 6;
-    run[threadIdx.x] = hist[threadIdx.x * ntiles + blockIdx.x];
+    gbase[threadIdx.x] = hist[threadIdx.x * ntiles + blockIdx.x];
+    run[threadIdx.x] = 0;
     const uint64_t lt = (1ull << lane) - 1ull;
-    uint32_t base = blockIdx.x * kRsTile;
+    const uint32_t base = blockIdx.x * kRsTile;
+    K keys[kRsItems];
+    uint32_t vals[kRsItems], lrank[kRsItems];
+#pragma unroll
     for (int r = 0; r < kRsItems; ++r) {
         for (int ww = 0; ww < kRsThreads / 64; ++ww) wcnt[ww][threadIdx.x] = 0;
         __syncthreads();
-        uint32_t i = base + r * kRsThreads + threadIdx.x;
-        bool valid = i < n;
-        K k = valid ? kin[i] : (K)0;
-        uint32_t v = valid ? vin[i] : 0u;
-        uint32_t d = (uint32_t)(k >> shift) & 255u;
+        const uint32_t i = base + r * kRsThreads + threadIdx.x;
+        const bool valid = i < n;
+        const K k = valid ? kin[i] : (K)0;
+        const uint32_t v = valid ? vin[i] : 0u;
+        const uint32_t d = (uint32_t)(k >> shift) & 255u;
         uint64_t mask = __ballot(valid);
         for (int bit = 0; bit < 8; ++bit) {
-            uint64_t bb = __ballot(valid && ((d >> bit) & 1u));
+            const uint64_t bb = __ballot(valid && ((d >> bit) & 1u));
             mask &= ((d >> bit) & 1u) ? bb : ~bb;
         }
-        uint32_t rank = __popcll(mask & lt);
+        const uint32_t rank = __popcll(mask & lt);
         if (valid && rank == 0) wcnt[w][d] = __popcll(mask);
         __syncthreads();
-        if (valid) {
-            uint32_t pos = run[d] + rank;
-            for (int ww = 0; ww < w; ++ww) pos += wcnt[ww][d];
-            kout[pos] = k;
-            vout[pos] = v;
-        }
+        uint32_t lr = run[d] + rank;
+        for (int ww = 0; ww < w; ++ww) lr += wcnt[ww][d];
+        keys[r] = k;
+        vals[r] = v;
+        lrank[r] = lr;
         __syncthreads();
         uint32_t add_ = 0;
         for (int ww = 0; ww < kRsThreads / 64; ++ww) add_ += wcnt[ww][threadIdx.x];
         run[threadIdx.x] += add_;
-        __syncthreads();
+    }
+    __syncthreads();
+    // tile-local digit starts: exclusive scan of the tile's digit counts (one digit per thread)
+    const uint32_t c = run[threadIdx.x];
+    uint32_t inc = c;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t off = 0;
+    for (int ww = 0; ww < w; ++ww) off += wsum[ww];
+    lstart[threadIdx.x] = off + inc - c;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRsItems; ++r) {
+        const uint32_t i = base + r * kRsThreads + threadIdx.x;
+        if (i < n) {
+            const uint32_t d = (uint32_t)(keys[r] >> shift) & 255u;
+            const uint32_t slot = lstart[d] + lrank[r];
+            skey[slot] = keys[r];
+            sval[slot] = vals[r];
+        }
+    }
+    __syncthreads();
+    const uint32_t m = n - base < (uint32_t)kRsTile ? n - base : (uint32_t)kRsTile;
+    for (uint32_t j = threadIdx.x; j < m; j += kRsThreads) {
+        const K k = skey[j];
+        const uint32_t d = (uint32_t)(k >> shift) & 255u;
+        const uint32_t pos = gbase[d] + (j - lstart[d]);
+        kout[pos] = k;
+        vout[pos] = sval[j];
     }
 }
 
