@@ -393,29 +393,38 @@ __device__ __forceinline__ uint32_t quant(float c, float lo, float size) {
     return q > kMortonMax ? kMortonMax : q;
 }
 
-__global__ void k_morton(const float4* cent, uint32_t nt, const float* box6, uint64_t* codes, uint32_t* idx,
-                         unsigned long long* orand) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t code = 0;
-    bool valid = t < nt;
-    if (valid) {
-        float sx = box6[3] - box6[0], sy = box6[4] - box6[1], sz = box6[5] - box6[2];
-        if (sx == 0.f) sx = 1.f;
-        if (sy == 0.f) sy = 1.f;
-        if (sz == 0.f) sz = 1.f;
-        float4 c = cent[t];
-        code = (spread21(quant(c.x, box6[0], sx)) << 2) | (spread21(quant(c.y, box6[1], sy)) << 1) |
-               spread21(quant(c.z, box6[2], sz));
+// Grid-stride over the triangles (launched with at most kMortonBlocks blocks). The OR and
+// AND of all codes (to skip radix passes whose digit never varies) are reduced per block
+// and taken with one atomic pair per block: an atomic pair per wave on the same two words
+// serialized the kernel.
+constexpr uint32_t kMortonBlocks = 1024;
+
+__global__ void __launch_bounds__(256) k_morton(const float4* cent, uint32_t nt, const float* box6, uint64_t* codes,
+                                                uint32_t* idx, unsigned long long* orand) {
+    __shared__ unsigned long long so[4], sa[4];
+    float sx = box6[3] - box6[0], sy = box6[4] - box6[1], sz = box6[5] - box6[2];
+    if (sx == 0.f) sx = 1.f;
+    if (sy == 0.f) sy = 1.f;
+    if (sz == 0.f) sz = 1.f;
+    unsigned long long o = 0ull, a = ~0ull;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+        const float4 c = cent[t];
+        const uint64_t code = (spread21(quant(c.x, box6[0], sx)) << 2) | (spread21(quant(c.y, box6[1], sy)) << 1) |
+                              spread21(quant(c.z, box6[2], sz));
         codes[t] = code;
         idx[t] = t;
+        o |= code;
+        a &= code;
     }
-    // OR and AND of all codes, to skip radix passes whose digit never varies.
-    unsigned long long o = valid ? code : 0ull, a = valid ? code : ~0ull;
     for (int d = 32; d >= 1; d >>= 1) {
         o |= __shfl_xor(o, d);
         a &= __shfl_xor(a, d);
     }
-    if (__lane_id() == 0) {
+    const int w = threadIdx.x >> 6;
+    if (__lane_id() == 0) { so[w] = o; sa[w] = a; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) { o |= so[i]; a &= sa[i]; }
         atomicOr(&orand[0], o);
         atomicAnd(&orand[1], a);
     }
@@ -1459,7 +1468,7 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
     k_scene_box_final<<<1, 64, 0, st>>>(part, nparts, box6);
     unsigned long long orand_init[2] = {0ull, ~0ull};
     RTX_HIP(hipMemcpyAsync(orand, orand_init, sizeof(orand_init), hipMemcpyHostToDevice, st));
-    k_morton<<<nblk(N, 256), 256, 0, st>>>(cent, N, box6, B->codes, valsA, orand);
+    k_morton<<<std::min(nblk(N, 256), kMortonBlocks), 256, 0, st>>>(cent, N, box6, B->codes, valsA, orand);
     unsigned long long hor[2];
     RTX_HIP(hipMemcpyAsync(hor, orand, sizeof(hor), hipMemcpyDeviceToHost, st));
     RTX_HIP(hipMemcpyAsync(B->scene, box6, 6 * sizeof(float), hipMemcpyDeviceToHost, st));
