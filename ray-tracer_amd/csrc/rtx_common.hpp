@@ -36,7 +36,24 @@ RTX_HD f3 scl(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
 // vec3 dot sums lanes as (x+y)+(z+w) with a +0 w lane (src/math/vec3.hpp:41-49).
 RTX_HD float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + (a.z * b.z + 0.0f); }
 RTX_HD f3 cross(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-RTX_HD f3 normalize(f3 v) { float inv = 1.0f / sqrtf(dot(v, v)); return scl(inv, v); }
+
+// 1.0f / a, bit for bit. On the GPU, v_rcp_f32 followed by one FMA Newton step
+// y + y * (1 - a * y) is the correctly rounded reciprocal for every 2^-126 <= |a| < 2^126
+// (all 2^32 inputs were compared with the IEEE division on gfx950): 3 instructions instead
+// of the 11 of the scaled division sequence. Zero, subnormal, huge, infinite and NaN a take
+// the division.
+RTX_HD float recip(float a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float m = __builtin_fabsf(a);
+    if (m >= 0x1p-126f && m < 0x1p126f) {
+        const float y = __builtin_amdgcn_rcpf(a);
+        return __builtin_fmaf(__builtin_fmaf(-a, y, 1.0f), y, y);
+    }
+#endif
+    return 1.0f / a;
+}
+
+RTX_HD f3 normalize(f3 v) { float inv = recip(sqrtf(dot(v, v))); return scl(inv, v); }
 
 // SSE minps/maxps (vec3 min/max, vec3.hpp:71-72) and libstdc++ std::min/std::max.
 RTX_HD float sse_min(float a, float b) { return a < b ? a : b; }
@@ -149,7 +166,7 @@ RTX_HD float tri_test(f3 o, f3 d, float tmin, float tmax, f3 v0, f3 e1, f3 e2, f
     f3 h = cross(d, e2);
     float a = dot(e1, h);
     if (a >= 0) return kInf;
-    float f = 1.f / a;
+    float f = recip(a);
     f3 s = sub(o, v0);
     float u = f * dot(s, h);
     if (u < 0.f || u > 1.f) return kInf;

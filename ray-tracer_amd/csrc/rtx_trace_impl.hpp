@@ -128,6 +128,28 @@ __device__ __forceinline__ unsigned long long coop_key(float t, uint32_t k) {
     return ((unsigned long long)b << 32) | k;
 }
 
+// tri_test's accept/reject decision and its t up to the sign of a zero t, for the dealt
+// tests of the cooperative leaf phase (the owner re-tests the winner with tri_test for the
+// exact t, u and v). The dots leave out the +0 w lane: x + 0 differs from x only for
+// x = -0, so a dot, and every product of it, changes at most the sign of a zero, which no
+// comparison below, nor coop_key, sees.
+__device__ __forceinline__ float tri_test_t(f3 o, f3 d, float tmin, float tmax, f3 v0, f3 e1, f3 e2) {
+    auto dot3 = [](f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; };
+    const f3 h = cross(d, e2);
+    const float a = dot3(e1, h);
+    if (a >= 0) return kInf;
+    const float f = recip(a);
+    const f3 s = sub(o, v0);
+    const float u = f * dot3(s, h);
+    if (u < 0.f || u > 1.f) return kInf;
+    const f3 q = cross(s, e1);
+    const float v = f * dot3(d, q);
+    if (v < 0.f || u + v > 1.f) return kInf;
+    const float t = f * dot3(e2, q);
+    if (t < tmin || t > tmax) return kInf;
+    return t;
+}
+
 struct TraceOut {
     float t, u, v;
     uint32_t tri;       // kInvalid on a miss
@@ -152,7 +174,7 @@ struct Walker {
     }
     __device__ __forceinline__ void start(const TraceArgs& A, f3 o_, f3 d_, float tmin_, float tmax_) {
         o = o_; d = d_; tmin = tmin_; tmax = tmax_;
-        ix = 1.f / d.x; iy = 1.f / d.y; iz = 1.f / d.z;
+        ix = recip(d.x); iy = recip(d.y); iz = recip(d.z);
         bestT = tmax; bu = 0.f; bv = 0.f;
         best = kInvalid; top = 0; lo = 0; nI = 0; nT = 0;
         ref = A.root_ref;
@@ -327,9 +349,8 @@ struct Walker {
                 float4 a, b;
                 float cz;
                 load_tri(tris + k, a, b, cz);
-                float u, v;
-                const float t = tri_test(f3{pox, poy, poz}, f3{pdx, pdy, pdz}, ptmin, ptmax, f3{a.x, a.y, a.z},
-                                         f3{a.w, b.x, b.y}, f3{b.z, b.w, cz}, u, v);
+                const float t = tri_test_t(f3{pox, poy, poz}, f3{pdx, pdy, pdz}, ptmin, ptmax, f3{a.x, a.y, a.z},
+                                           f3{a.w, b.x, b.y}, f3{b.z, b.w, cz});
                 if (t < pbest) atomicMin(&C->mins[r], ANY ? (unsigned long long)k : coop_key(t, k));
             }
             __builtin_amdgcn_wave_barrier();
