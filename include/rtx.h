@@ -90,6 +90,13 @@ int rtx_bvh_export_bfs(const rtx_bvh* bvh, void* nodes48, uint32_t* count);
  * the box in out6 (min xyz, max xyz) when that path applies, 0 when the builder would run
  * the full clip instead.  No GPU work. */
 int rtx_debug_clip_slab(const float* v9, const float* region6, int axis, float* out6);
+/* The traversal's reciprocal (v_rcp_f32 and one FMA Newton step where 2^-126 <= |x| < 2^126,
+ * the IEEE division elsewhere) against the IEEE quotient 1.0f / x, on the GPU, for the float
+ * bit patterns [first, first + count) (first + count <= 2^32): the number of inputs whose
+ * result bits differ, NaN results counted as equal, and the lowest such pattern
+ * (0xFFFFFFFF when none).  Pins the Moller-Trumbore f = 1/a of bvh_traverse.cpp:62 and
+ * the 1/dir of rayAABB (:26-28) to the reference's division.  Synchronous. */
+int rtx_debug_recip_check(uint64_t first, uint64_t count, uint64_t* mismatches, uint32_t* first_bad);
 
 /* ---- trace ------------------------------------------------------------------
  * Closest hit for n rays: replaces bvh::trace (src/bvh/bvh_traverse.cpp:237-246,

@@ -383,3 +383,55 @@ extern "C" int rtx_trace_closest_host(const rtx_bvh* b, const rtx_ray* rays, siz
     if (normals) RTX_HIP(hipMemcpy(normals, dn, 12 * n, hipMemcpyDeviceToHost));
     return RTX_OK;
 }
+
+namespace rtx {
+namespace {
+// Compares recip(x) with the IEEE quotient 1.0f / x for the float bit patterns
+// [first, first + count); mismatches are counted per wave (NaN inputs match any NaN).
+__global__ void __launch_bounds__(256) k_recip_check(uint64_t first, uint64_t count, unsigned long long* out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long bad = 0, lowest = ~0ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const uint32_t bits = (uint32_t)(first + i);
+        const float x = __uint_as_float(bits);
+        const float q = 1.0f / x, r = recip(x);
+        const bool both_nan = q != q && r != r;
+        if (!both_nan && __float_as_uint(q) != __float_as_uint(r)) {
+            ++bad;
+            if (bits < lowest) lowest = bits;
+        }
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        bad += __shfl_xor(bad, o);
+        const unsigned long long w = __shfl_xor(lowest, o);
+        lowest = w < lowest ? w : lowest;
+    }
+    if (__lane_id() == 0 && bad) {
+        atomicAdd(&out[0], bad);
+        atomicMin(&out[1], lowest);
+    }
+}
+}  // namespace
+}  // namespace rtx
+
+extern "C" int rtx_debug_recip_check(uint64_t first, uint64_t count, uint64_t* mismatches, uint32_t* first_bad) {
+    if (!mismatches || !first_bad || first > 0xFFFFFFFFull || count > (1ull << 32) - first)
+        return fail(RTX_E_ARG, "bad argument on rtx_debug_recip_check()");
+    unsigned long long* d = dalloc<unsigned long long>(2);
+    if (!d) return fail(RTX_E_HIP, "device allocation failed on rtx_debug_recip_check()");
+    const unsigned long long init[2] = {0ull, ~0ull};
+    int rc = RTX_OK;
+    unsigned long long h[2] = {0ull, ~0ull};
+    if (!hip_ok(hipMemcpy(d, init, sizeof(init), hipMemcpyHostToDevice), "hipMemcpy")) rc = RTX_E_HIP;
+    if (rc == RTX_OK && count) {
+        k_recip_check<<<8192, 256>>>(first, count, d);
+        if (!hip_ok(hipGetLastError(), "k_recip_check") ||
+            !hip_ok(hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost), "hipMemcpy"))
+            rc = RTX_E_HIP;
+    }
+    (void)hipFree(d);
+    if (rc != RTX_OK) return rc;
+    *mismatches = h[0];
+    *first_bad = h[1] == ~0ull ? 0xFFFFFFFFu : (uint32_t)h[1];
+    return RTX_OK;
+}

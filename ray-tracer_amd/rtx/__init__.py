@@ -20,7 +20,7 @@ NODE48 = np.dtype([("bmin", "<f4", 4), ("bmax", "<f4", 4), ("is_leaf", "u1"), ("
 
 # Every symbol include/rtx.h declares.
 EXPORTS = ["rtx_last_error", "rtx_device_count", "rtx_build", "rtx_build_device", "rtx_bvh_free",
-           "rtx_bvh_get_info", "rtx_bvh_export", "rtx_bvh_morton", "rtx_bvh_export_bfs", "rtx_debug_clip_slab", "rtx_trace_closest",
+           "rtx_bvh_get_info", "rtx_bvh_export", "rtx_bvh_morton", "rtx_bvh_export_bfs", "rtx_debug_clip_slab", "rtx_debug_recip_check", "rtx_trace_closest",
            "rtx_trace_any", "rtx_trace_counts", "rtx_trace_closest_host", "rtx_save", "rtx_load",
            "rtx_scene_load", "rtx_free_host", "rtx_bake_scene", "rtx_geo_sphere", "rtx_geo_box", "rtx_render",
            "rtx_render_adjoint", "rtx_render_batched", "rtx_batch_create", "rtx_batch_launch", "rtx_batch_free",
@@ -87,6 +87,7 @@ def lib():
         L.rtx_bvh_export.argtypes = [vp, vp, vp, vp]
         L.rtx_bvh_morton.argtypes = [vp, vp, vp]
         L.rtx_debug_clip_slab.argtypes = [vp, vp, C.c_int, vp]
+        L.rtx_debug_recip_check.argtypes = [C.c_uint64, C.c_uint64, vp, vp]
         L.rtx_bvh_export_bfs.argtypes = [vp, vp, C.POINTER(u32)]
         L.rtx_trace_closest.argtypes = [vp, vp, sz, vp, vp, vp]
         L.rtx_trace_any.argtypes = [vp, vp, sz, vp, vp]
@@ -227,6 +228,16 @@ def debug_clip_slab(v9, region6, axis):
     if rc < 0:
         _check(rc, "rtx_debug_clip_slab")
     return (True, out) if rc == 1 else (False, None)
+
+
+def debug_recip_check(first=0, count=1 << 32):
+    """The traversal's reciprocal against the IEEE quotient 1.0f / x on the GPU, over the
+    float bit patterns [first, first + count): (mismatch count, lowest mismatching pattern
+    or None)."""
+    n = C.c_uint64(0)
+    fb = C.c_uint32(0)
+    _check(lib().rtx_debug_recip_check(int(first), int(count), C.byref(n), C.byref(fb)), "rtx_debug_recip_check")
+    return n.value, (None if n.value == 0 else fb.value)
 
 
 def build(verts6, tris):

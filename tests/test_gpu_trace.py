@@ -311,3 +311,16 @@ def test_concurrent_streams_share_one_handle():
             assert torch.equal(hb.view(torch.int32).cpu(), ref_b), rep
     finally:
         rtx.set_tuning(**saved)
+
+
+def test_reciprocal_matches_ieee_division_on_all_floats():
+    """The traversal computes 1/a (Moller-Trumbore, bvh_traverse.cpp:62) and 1/dir
+    (rayAABB, :26-28) as v_rcp_f32 plus one FMA Newton step where 2^-126 <= |x| < 2^126.
+    Every one of the 2^32 float inputs must give the bits of the IEEE quotient 1.0f / x."""
+    bad, first = rtx.debug_recip_check(0, 1 << 32)
+    assert bad == 0, f"{bad} inputs differ from 1.0f / x, first 0x{first:08x}"
+    # sub-ranges and the argument checks
+    assert rtx.debug_recip_check(0x3F800000, 1 << 23) == (0, None)
+    assert rtx.debug_recip_check(0xFFFFFFFF, 1) == (0, None)
+    with pytest.raises(rtx.RtxError):
+        rtx.debug_recip_check(0xFFFFFFFF, 2)
