@@ -138,6 +138,7 @@ def main():
     ms_inc, wall, kms_inc = timed(lambda: bvh.trace_closest_device(inc, hits, stream=stream), args.steps,
                                   args.warmup)
     ms_coh = ms_any = float("nan")
+    coh_tuning = any_tuning = trace_tuning
     host = None
     if not args.skip_extras:
         # host buffers through the C ABI (PCIe-inclusive): rays in, hits out
@@ -154,8 +155,19 @@ def main():
             bvh.trace(one)
         host = {"pcie_inclusive_mrays_s": round(n / host_s / 1e6, 2),
                 "single_ray_call_us": round((time.perf_counter() - t_h) / 200 * 1e6, 1)}
+        # the coherent and any-hit launches get launch shapes of their own (sorting rays
+        # that are already in camera order only costs the sort, for instance)
+        if not args.no_autotune:
+            coh_tuning, _, _ = autotune.tune(lambda: (lambda: bvh.trace_closest_device(coh, hits, stream=stream)),
+                                             keys=autotune.TRACE_KEYS, reps=3)
+            rtx.set_tuning(**trace_tuning)
+            any_tuning, _, _ = autotune.tune(lambda: (lambda: bvh.trace_any_device(inc, occ, stream=stream)),
+                                             keys=autotune.TRACE_KEYS, reps=3)
+        rtx.set_tuning(**coh_tuning)
         ms_coh, _, _ = timed(lambda: bvh.trace_closest_device(coh, hits, stream=stream), max(3, args.steps // 4), 1)
+        rtx.set_tuning(**any_tuning)
         ms_any, _, _ = timed(lambda: bvh.trace_any_device(inc, occ, stream=stream), max(3, args.steps // 4), 1)
+        rtx.set_tuning(**trace_tuning)
 
     # Algorithmic bytes of the incoherent launch: 32 B ray + 16 B hit + 64 B per internal
     # node visited + 48 B per triangle tested, counts from the instrumented kernel.
@@ -221,6 +233,8 @@ def main():
             "trace_kernel_ms": round(kms_inc, 4),
             "ray_sort_and_launch_ms": round(ms_inc - kms_inc, 4),
             "tuning": trace_tuning,
+            "coherent_tuning": {k: v for k, v in coh_tuning.items() if k in autotune.TRACE_KEYS},
+            "any_hit_tuning": {k: v for k, v in any_tuning.items() if k in autotune.TRACE_KEYS},
             "autotune": None if args.no_autotune else {"seconds": round(trace_tune_s, 2)},
             "timed_wall_s": round(wall, 3),
         },
