@@ -101,7 +101,8 @@ int rtx_debug_recip_check(uint64_t first, uint64_t count, uint64_t* mismatches, 
 /* ---- trace ------------------------------------------------------------------
  * Closest hit for n rays: replaces bvh::trace (src/bvh/bvh_traverse.cpp:237-246,
  * traverse :91-178).  rays_d/hits_d device pointers; normals_d optional (3 floats per
- * ray, the interpolated shading normal of bvh_traverse.cpp:135-139). */
+ * ray, the interpolated shading normal of bvh_traverse.cpp:135-139).  On every trace
+ * entry point n = 0 does nothing and the buffers may then be NULL. */
 int rtx_trace_closest(const rtx_bvh* bvh, const rtx_ray* rays_d, size_t n, rtx_hit* hits_d,
                       float* normals_d, void* stream);
 /* Any hit (occlusion): occluded_d[i] = 1 iff trace(rays[i]).hit.  No reference

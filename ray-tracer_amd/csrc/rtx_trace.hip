@@ -218,7 +218,7 @@ bool reserve_sort(rtx_bvh* b, uint32_t n) {
 }
 
 int check(const rtx_bvh* b, const void* rays, size_t n, const void* out, const char* fn) {
-    if (!b || !out || (!rays && n)) return fail(RTX_E_ARG, std::string("null argument on ") + fn);
+    if (!b || ((!out || !rays) && n)) return fail(RTX_E_ARG, std::string("null argument on ") + fn);
     if (!b->gnodes || !b->leaves || !b->gtris) return fail(RTX_E_STATE, std::string("BVH not loaded on ") + fn);
     if (n > (1ull << 31)) return fail(RTX_E_LIMIT, std::string("more than 2^31 rays on ") + fn);
     if (b->max_depth > kMaxTraceDepth)

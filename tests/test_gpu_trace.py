@@ -324,3 +324,20 @@ def test_reciprocal_matches_ieee_division_on_all_floats():
     assert rtx.debug_recip_check(0xFFFFFFFF, 1) == (0, None)
     with pytest.raises(rtx.RtxError):
         rtx.debug_recip_check(0xFFFFFFFF, 2)
+
+
+def test_empty_inputs():
+    """No triangles is an error (the reference's bake has no defined result for it); zero
+    rays is a no-op on every trace entry point, null buffers included."""
+    import torch
+    with pytest.raises(rtx.RtxError, match="no triangles"):
+        rtx.build(np.zeros((0, 6), np.float32), np.zeros((0, 3), np.uint32))
+    v, t = S.random_triangles(100, seed=2)
+    g = rtx.build(v, t)
+    out = g.trace(np.zeros((0, 8), np.float32), normals=True)
+    assert len(out["t"]) == 0 and out["normal"].shape == (0, 3)
+    rd = torch.zeros((0, 8), device="cuda")
+    g.trace_closest_device(rd, torch.zeros((0, 4), device="cuda"))
+    g.trace_any_device(rd, torch.zeros(0, dtype=torch.uint8, device="cuda"))
+    g.trace_counts_device(rd, torch.zeros((0, 2), dtype=torch.int32, device="cuda"))
+    torch.cuda.synchronize()
