@@ -1243,12 +1243,7 @@ __global__ void k_emit(Ctx C, uint32_t total, Dfs D, Out O) {
         O.tris3[3 * o] = i0; O.tris3[3 * o + 1] = i1; O.tris3[3 * o + 2] = i2;
         O.prims[o] = t;
         f3 v0 = vpos(C.verts, i0), v1 = vpos(C.verts, i1), v2 = vpos(C.verts, i2);
-        f3 e1 = sub(v1, v0), e2 = sub(v2, v0);
-        GTri g;
-        g.a = make_float4(v0.x, v0.y, v0.z, e1.x);
-        g.b = make_float4(e1.y, e1.z, e2.x, e2.y);
-        g.c = make_float4(e2.z, __uint_as_float(t), 0.f, 0.f);
-        O.gtris[o] = g;
+        store_gtri(O.gtris + o, v0, sub(v1, v0), sub(v2, v0));
     }
 }
 
@@ -1278,12 +1273,7 @@ __global__ void k_gtris_from_ref(const uint32_t* tris3, const uint32_t* prims, u
     uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= ntri) return;
     f3 v0 = vpos(verts, tris3[3 * o]), v1 = vpos(verts, tris3[3 * o + 1]), v2 = vpos(verts, tris3[3 * o + 2]);
-    f3 e1 = sub(v1, v0), e2 = sub(v2, v0);
-    GTri g;
-    g.a = make_float4(v0.x, v0.y, v0.z, e1.x);
-    g.b = make_float4(e1.y, e1.z, e2.x, e2.y);
-    g.c = make_float4(e2.z, __uint_as_float(prims ? prims[o] : o), 0.f, 0.f);
-    gtris[o] = g;
+    store_gtri(gtris + o, v0, sub(v1, v0), sub(v2, v0));
 }
 
 inline uint32_t nblk(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }

@@ -119,14 +119,27 @@ struct alignas(64) GNode {
 };
 static_assert(sizeof(GNode) == 64, "GNode layout");
 
-// Traversal triangle: 48 bytes, v0 and the two edges precomputed (bit-identical to
-// computing v1-v0, v2-v0 at trace time), plus the original triangle id.
-struct alignas(16) GTri {
-    float4 a;    // v0.x v0.y v0.z e1.x
-    float4 b;    // e1.y e1.z e2.x e2.y
-    float4 c;    // e2.z prim(bits) 0 0
+// Traversal triangle: 36 bytes, v0 and the two edges precomputed (bit-identical to
+// computing v1-v0, v2-v0 at trace time): v0.xyz e1.xyz e2.xyz. Packed at a 36-byte
+// stride (4-byte aligned): the cooperative leaf phase loads a leaf's triangles with
+// neighbouring lanes, so a round of 64 reads 18 cache lines instead of 24 at 48 bytes.
+struct GTri {
+    float v[9];
 };
-static_assert(sizeof(GTri) == 48, "GTri layout");
+static_assert(sizeof(GTri) == 36 && alignof(GTri) == 4, "GTri layout");
+
+// 16-byte vectors at 4-byte alignment: gfx950 loads and stores dwordx4 at any dword
+// address, which the packed triangles need.
+typedef float gtri_f4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef __attribute__((address_space(1))) gtri_f4 gtri_f4_global;
+typedef __attribute__((address_space(1))) float float_global;
+
+__device__ __forceinline__ void store_gtri(GTri* p, f3 v0, f3 e1, f3 e2) {
+    gtri_f4_global* q = (gtri_f4_global*)p->v;
+    q[0] = gtri_f4{v0.x, v0.y, v0.z, e1.x};
+    q[1] = gtri_f4{e1.y, e1.z, e2.x, e2.y};
+    ((float_global*)p->v)[8] = e2.z;
+}
 
 struct Leaf { uint32_t begin, count; };
 

@@ -112,10 +112,14 @@ __device__ __forceinline__ T wave_ld(T* p) {
 // The 36 bytes of a traversal triangle a test reads, loaded before the test starts: the
 // empty asm takes the values as operands, so the compiler cannot sink the v0 load behind
 // the back-face test (a second dependent memory round trip for the lanes that pass it).
+__device__ __forceinline__ void ld_tri(const GTri* p, float4& a, float4& b, float& c) {
+    const gtri_f4 x = ((const gtri_f4_global*)p->v)[0], y = ((const gtri_f4_global*)p->v)[1];
+    a = make_float4(x.x, x.y, x.z, x.w);
+    b = make_float4(y.x, y.y, y.z, y.w);
+    c = ((const float_global*)p->v)[8];
+}
 __device__ __forceinline__ void load_tri(const GTri* p, float4& a, float4& b, float& c) {
-    a = ld_g(&p->a);
-    b = ld_g(&p->b);
-    c = ld_g(&p->c.x);
+    ld_tri(p, a, b, c);
     __asm__ volatile("" : : "v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w), "v"(b.x), "v"(b.y), "v"(b.z), "v"(b.w), "v"(c));
 }
 
@@ -267,9 +271,7 @@ struct Walker {
 #pragma unroll
             for (int j = 0; j < TB; ++j) {
                 const uint32_t kk = min(k + (uint32_t)j, end - 1u);
-                ta[j] = ld_g(&A.tris[kk].a);
-                tb[j] = ld_g(&A.tris[kk].b);
-                tc[j] = ld_g(&A.tris[kk].c.x);
+                ld_tri(A.tris + kk, ta[j], tb[j], tc[j]);
             }
 #pragma unroll
             for (int j = 0; j < TB; ++j) {
