@@ -1228,8 +1228,8 @@ __global__ void k_emit(Ctx C, uint32_t total, Dfs D, Out O) {
         g.l0 = make_float4(l.lo.x, l.lo.y, l.lo.z, l.hi.x);
         g.l1 = make_float4(l.hi.y, l.hi.z, r.lo.x, r.lo.y);
         g.r1 = make_float4(r.lo.z, r.hi.x, r.hi.y, r.hi.z);
-        g.lref = (l.flags & F_LEAF) ? (kLeafBit | D.loff[n.a]) : D.ioff[n.a];
-        g.rref = (r.flags & F_LEAF) ? (kLeafBit | D.loff[n.b]) : D.ioff[n.b];
+        g.lref = (l.flags & F_LEAF) ? leaf_ref(D.loff[n.a], D.toff[n.a], l.b) : D.ioff[n.a];
+        g.rref = (r.flags & F_LEAF) ? leaf_ref(D.loff[n.b], D.toff[n.b], r.b) : D.ioff[n.b];
         g.pad0 = g.pad1 = 0;
         O.gnodes[D.ioff[i]] = g;
         return;
@@ -1262,8 +1262,8 @@ __global__ void k_from_ref(const Node48* nodes, uint32_t nn, const uint32_t* iid
     g.l0 = make_float4(l.bmin[0], l.bmin[1], l.bmin[2], l.bmax[0]);
     g.l1 = make_float4(l.bmax[1], l.bmax[2], r.bmin[0], r.bmin[1]);
     g.r1 = make_float4(r.bmin[2], r.bmax[0], r.bmax[1], r.bmax[2]);
-    g.lref = l.is_leaf ? (kLeafBit | lidx[n.a]) : iidx[n.a];
-    g.rref = r.is_leaf ? (kLeafBit | lidx[n.b]) : iidx[n.b];
+    g.lref = l.is_leaf ? leaf_ref(lidx[n.a], l.a, l.b) : iidx[n.a];
+    g.rref = r.is_leaf ? leaf_ref(lidx[n.b], r.a, r.b) : iidx[n.b];
     g.pad0 = g.pad1 = 0;
     gnodes[iidx[i]] = g;
 }
@@ -1583,6 +1583,7 @@ int upload_reference_tree(rtx_bvh* B, const Node48* nodes, uint32_t nn, const ui
     for (uint64_t i = 0; i < 3ull * ntri; ++i)
         if (tris3[i] >= nv) return fail(RTX_E_ARG, "triangle vertex index out of bounds");
     B->nn = nn; B->ntri = ntri; B->nv = nv; B->nt = ntri;
+    if (nl >= (1u << 30)) return fail(RTX_E_LIMIT, "more than 2^30 leaves on rtx_load()");
     B->ninternal = ni; B->nleaves = nl;
     B->nodes48 = dalloc<Node48>(nn);
     B->tris3 = dalloc<uint32_t>(3 * (size_t)std::max<uint32_t>(ntri, 1));

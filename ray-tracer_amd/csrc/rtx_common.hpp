@@ -110,7 +110,10 @@ struct Node48 {
 static_assert(sizeof(Node48) == 48, "Node48 layout");
 
 // Traversal node: one 64-byte line per internal node holding both children's boxes.
-// ref = internal node index, or kLeafBit | leaf index into the leaf table.
+// A child reference is an internal node index, or for a leaf (kLeafBit set) either
+//  - kInlineLeaf | count << 25 | begin, when count < 32 and begin < 2^25: the leaf's
+//    triangle range itself, so reaching the leaf needs no further load; or
+//  - the leaf's index into the leaf table (bit 30 clear, index < 2^30).
 struct alignas(64) GNode {
     float4 l0;   // lmin.x lmin.y lmin.z lmax.x
     float4 l1;   // lmax.y lmax.z rmin.x rmin.y
@@ -142,6 +145,12 @@ __device__ __forceinline__ void store_gtri(GTri* p, f3 v0, f3 e1, f3 e2) {
 }
 
 struct Leaf { uint32_t begin, count; };
+
+constexpr uint32_t kInlineLeaf = 0x40000000u;
+RTX_HD uint32_t leaf_ref(uint32_t leaf_index, uint32_t begin, uint32_t count) {
+    return (count < 32u && begin < (1u << 25)) ? (kLeafBit | kInlineLeaf | (count << 25) | begin)
+                                               : (kLeafBit | leaf_index);
+}
 
 struct Ray32 { float ox, oy, oz, tmin, dx, dy, dz, tmax; };
 struct Hit16 { float t, u, v; uint32_t tri; };

@@ -96,6 +96,13 @@ __device__ __forceinline__ uint32_t ld_g(const uint32_t* p) {
     return *(__attribute__((address_space(1))) const uint32_t*)p;
 }
 
+// Triangle range (begin, count) of a leaf reference: packed in the reference itself, or
+// read from the leaf table.
+__device__ __forceinline__ uint2 leaf_range(const Leaf* leaves, uint32_t ref) {
+    if (ref & kInlineLeaf) return make_uint2(ref & ((1u << 25) - 1u), (ref >> 25) & 31u);
+    return ld_g(reinterpret_cast<const uint2*>(leaves + (ref & ~kLeafBit)));
+}
+
 // LDS words shared by the lanes of one wave: relaxed wave-scope atomics, so the compiler
 // keeps every access (a lane reads what another lane wrote) and still emits plain ds_*
 // instructions (a volatile pointer would turn them into flat accesses). A wave's LDS
@@ -263,7 +270,7 @@ struct Walker {
     // leaf's end re-read its last triangle (always in bounds) and are not tested.
     template <bool ANY, bool COUNT>
     __device__ __forceinline__ bool leaf_step(const TraceArgs& A) {
-        const uint2 lf = ld_g(reinterpret_cast<const uint2*>(A.leaves + (ref & ~kLeafBit)));   // begin, count
+        const uint2 lf = leaf_range(A.leaves, ref);   // begin, count
         const uint32_t end = lf.x + lf.y;
         for (uint32_t k = lf.x; k < end; k += TB) {
             float4 ta[TB], tb[TB];
@@ -310,7 +317,7 @@ struct Walker {
         const bool at = live && (ref & kLeafBit);
         uint32_t c = 0, kb = 0;
         if (at) {
-            const uint2 lf = ld_g(reinterpret_cast<const uint2*>(A.leaves + (ref & ~kLeafBit)));
+            const uint2 lf = leaf_range(A.leaves, ref);
             c = lf.y;
             kb = lf.x;
         }
