@@ -142,8 +142,9 @@ __global__ void k_zero_work(uint32_t* w) {
 }
 
 // Ray order key: 30-bit Morton code of the origin inside the scene box (10 bits per axis).
-// Rays with nearby keys share nodes and triangles in the L2 of the XCD that traces them
-// and follow similar paths within a wave. (Direction bits below the origin code, or a
+// Rays with nearby keys share nodes and triangles in the caches and follow similar paths
+// within a wave: the persistent kernel deals rays in key order from one counter, so at any
+// time all XCDs work in one compact window of space. (Direction bits below the origin code, or a
 // 48-bit origin+direction key, measured slower: the sort costs more and L2 reuse comes
 // from the origins.)
 __device__ __forceinline__ uint32_t spread10(uint32_t x) {   // bit b of x -> bit 3b
