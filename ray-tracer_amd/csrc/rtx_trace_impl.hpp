@@ -143,8 +143,9 @@ __device__ __forceinline__ unsigned long long coop_key(float t, uint32_t k) {
 // tests of the cooperative leaf phase (the owner re-tests the winner with tri_test for the
 // exact t, u and v). The dots leave out the +0 w lane: x + 0 differs from x only for
 // x = -0, so a dot, and every product of it, changes at most the sign of a zero, which no
-// comparison below, nor coop_key, sees.
-__device__ __forceinline__ float tri_test_t(f3 o, f3 d, float tmin, float tmax, f3 v0, f3 e1, f3 e2) {
+// comparison below, nor coop_key, sees. The t > tmax rejection is left to the caller's
+// t < best test: best starts at tmax and only decreases, so it implies t < tmax.
+__device__ __forceinline__ float tri_test_t(f3 o, f3 d, float tmin, f3 v0, f3 e1, f3 e2) {
     auto dot3 = [](f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; };
     const f3 h = cross(d, e2);
     const float a = dot3(e1, h);
@@ -157,8 +158,7 @@ __device__ __forceinline__ float tri_test_t(f3 o, f3 d, float tmin, float tmax, 
     const float v = f * dot3(d, q);
     if (v < 0.f || u + v > 1.f) return kInf;
     const float t = f * dot3(e2, q);
-    if (t < tmin || t > tmax) return kInf;
-    return t;
+    return t < tmin ? kInf : t;
 }
 
 struct TraceOut {
@@ -346,7 +346,7 @@ struct Walker {
             const uint32_t r = __shfl(m, src);
             const float pox = __shfl(o.x, r), poy = __shfl(o.y, r), poz = __shfl(o.z, r);
             const float pdx = __shfl(d.x, r), pdy = __shfl(d.y, r), pdz = __shfl(d.z, r);
-            const float ptmin = __shfl(tmin, r), ptmax = __shfl(tmax, r), pbest = __shfl(bestT, r);
+            const float ptmin = __shfl(tmin, r), pbest = __shfl(bestT, r);
             const uint32_t k = __shfl(kbase, r) + W + lane;
             const GTri* tris = A.tris;
             if (MIXED) {
@@ -358,7 +358,7 @@ struct Walker {
                 float4 a, b;
                 float cz;
                 load_tri(tris + k, a, b, cz);
-                const float t = tri_test_t(f3{pox, poy, poz}, f3{pdx, pdy, pdz}, ptmin, ptmax, f3{a.x, a.y, a.z},
+                const float t = tri_test_t(f3{pox, poy, poz}, f3{pdx, pdy, pdz}, ptmin, f3{a.x, a.y, a.z},
                                            f3{a.w, b.x, b.y}, f3{b.z, b.w, cz});
                 if (t < pbest) atomicMin(&C->mins[r], ANY ? (unsigned long long)k : coop_key(t, k));
             }
