@@ -209,10 +209,14 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
 // C4 10.4 -> 8.7 ms, C5 184 -> 152 ms). The control flow follows k_trace_persist: every
 // ballot, the leader's atomic, the shuffle and the exit test are wave-uniform, and no lane
 // leaves the loop body early.
+// UNI: every queued path belongs to scenes[0] (a launch sequence for one scene). The
+// scene's tree pointers are then wave-uniform kernel data that stay in scalar registers;
+// with paths of several scenes in one queue each lane reads them through its own scene
+// descriptor, a dependent load before every node and leaf access.
 constexpr int kRenderRefill = 24;
 constexpr int kRenderChunk = 128;   // queue positions per atomic on the path counter
 
-template <int DEPTH, int LOOP>
+template <int DEPTH, int LOOP, bool UNI>
 __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
     k_wtrace_persist(const SceneDesc* __restrict__ scenes, PathState P, int qi, uint32_t wait, uint32_t* work) {
     __shared__ uint2 lst[DEPTH][kTraceThreads];
@@ -228,6 +232,7 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
     w.best = kInvalid;
     uint32_t id = kInvalid;            // path being traced
     const SceneDesc* S = scenes;       // its scene
+    const TraceArgs TU = scenes->T;    // UNI: the one scene's tree, loaded once into scalar registers
     uint32_t rbase = 0, rend = 0;      // wave-uniform reservoir of queue positions [rbase, rend)
     bool gdone = false;                // `work` has passed n
     bool more = true;                  // the same on every lane: set only from wave-wide values
@@ -255,21 +260,21 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
                 const uint32_t k = base + (uint32_t)__popcll(idle & lt);
                 if (k < rend) {
                     id = Q[k];
-                    S = scenes + P.sid[id];
+                    if (!UNI) S = scenes + P.sid[id];
                     const float4 r0 = P.ray0[id], r1 = P.ray1[id];
-                    w.start(S->T, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w);
+                    w.start(UNI ? TU : S->T, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w);
                 }
             }
         }
         // uniform exit test: a started path that missed the root is written at the top
         if (!__any(w.live || id != kInvalid) && !more) break;
-        w.template node_phase<false>(S->T, lst, tid, wait);
+        w.template node_phase<false>(UNI ? TU : S->T, lst, tid, wait);
         if (LOOP == kLoopCoop) {   // all lanes active: the loop is wave-uniform
             const bool at = w.live && (w.ref & kLeafBit);
-            w.template leaf_coop<false, false, true>(S->T, coop + tid / 64);
+            w.template leaf_coop<false, false, !UNI>(UNI ? TU : S->T, coop + tid / 64);
             if (at) w.pop(lst, tid);
         } else if (w.live && (w.ref & kLeafBit)) {
-            w.template leaf_step<false, false>(S->T);
+            w.template leaf_step<false, false>(UNI ? TU : S->T);
             w.pop(lst, tid);
         }
     }
@@ -660,8 +665,12 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
             if (g_tuning.render_loop == 2) {
                 k_clear1<<<1, 64, 0, st>>>(P.cnt + 2);   // the persistent kernel's path counter
                 with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
-                    k_wtrace_persist<decltype(D)::value, decltype(L)::value><<<pgrid, kTraceThreads, 0, st>>>(
-                        d_scenes, P, qi, wait, P.cnt + 2);
+                    if (s_last - s_first == 1)
+                        k_wtrace_persist<decltype(D)::value, decltype(L)::value, true>
+                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes + s_first, P, qi, wait, P.cnt + 2);
+                    else
+                        k_wtrace_persist<decltype(D)::value, decltype(L)::value, false>
+                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes, P, qi, wait, P.cnt + 2);
                 });
             } else {
                 with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {

@@ -173,7 +173,7 @@ struct TraceOut {
 template <int DEPTH, int TB = kTraceTriBatch>
 struct Walker {
     f3 o, d;
-    float ix, iy, iz, tmin, tmax;
+    float ix, iy, iz, tmin;   // tmax lives on as the starting best t (bestT <= tmax always)
     float bestT, bu, bv;
     uint32_t best, ref, top, lo, nI, nT;
     bool live, fast;
@@ -184,14 +184,14 @@ struct Walker {
         return DEPTH == 16 || DEPTH == 8 ? (e & (DEPTH - 1)) : (e % DEPTH);
     }
     __device__ __forceinline__ void start(const TraceArgs& A, f3 o_, f3 d_, float tmin_, float tmax_) {
-        o = o_; d = d_; tmin = tmin_; tmax = tmax_;
+        o = o_; d = d_; tmin = tmin_;
         ix = recip(d.x); iy = recip(d.y); iz = recip(d.z);
-        bestT = tmax; bu = 0.f; bv = 0.f;
+        bestT = tmax_; bu = 0.f; bv = 0.f;
         best = kInvalid; top = 0; lo = 0; nI = 0; nT = 0;
         ref = A.root_ref;
         fast = A.finite && __builtin_isfinite(ix) && __builtin_isfinite(iy) && __builtin_isfinite(iz) &&
                ix != 0.f && iy != 0.f && iz != 0.f && __builtin_isfinite(o.x) && __builtin_isfinite(o.y) &&
-               __builtin_isfinite(o.z) && !__builtin_isnan(tmin) && !__builtin_isnan(tmax);
+               __builtin_isfinite(o.z) && !__builtin_isnan(tmin) && !__builtin_isnan(tmax_);
         live = slab(o.x, o.y, o.z, ix, iy, iz, tmin, A.rb[0], A.rb[1], A.rb[2], A.rb[3], A.rb[4], A.rb[5], bestT) < bestT;
     }
     __device__ __forceinline__ void push(uint2 (*lst)[kTraceThreads], uint32_t tid, uint32_t r, float t) {
@@ -286,7 +286,8 @@ struct Walker {
                     if (COUNT) ++nT;
                     float u, v;
                     const float4 a = ta[j], b = tb[j];
-                    const float t = tri_test(o, d, tmin, tmax, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y},
+                    // t < bestT below implies t <= tmax (bestT starts at tmax and only decreases)
+                    const float t = tri_test(o, d, tmin, kInf, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y},
                                              f3{b.z, b.w, tc[j]}, u, v);
                     if (t < bestT) {
                         bestT = t; bu = u; bv = v; best = k + (uint32_t)j;
@@ -382,7 +383,8 @@ struct Walker {
                 float cz;
                 load_tri(A.tris + k, a, b, cz);
                 float u, v;
-                bestT = tri_test(o, d, tmin, tmax, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y}, f3{b.z, b.w, cz}, u, v);
+                // the winner passed t < best <= tmax in its dealt test
+                bestT = tri_test(o, d, tmin, kInf, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y}, f3{b.z, b.w, cz}, u, v);
                 bu = u;
                 bv = v;
                 best = k;
