@@ -209,18 +209,37 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
 // C4 10.4 -> 8.7 ms, C5 184 -> 152 ms). The control flow follows k_trace_persist: every
 // ballot, the leader's atomic, the shuffle and the exit test are wave-uniform, and no lane
 // leaves the loop body early.
-// UNI: every queued path belongs to scenes[0] (a launch sequence for one scene). The
-// scene's tree pointers are then wave-uniform kernel data that stay in scalar registers;
-// with paths of several scenes in one queue each lane reads them through its own scene
-// descriptor, a dependent load before every node and leaf access.
+// Where a lane finds its scene's tree (SC):
+//  - kSceneUni: every queued path belongs to scenes[0] (a launch sequence for one scene).
+//    The tree pointers are wave-uniform kernel data that stay in scalar registers.
+//  - kSceneLds: paths of up to kLdsScenes scenes share the queue. The block copies each
+//    scene's node, leaf and triangle pointers into an LDS table; a lane keeps its scene
+//    index and reads the pointers from LDS before a node or leaf access.
+//  - kSceneGlobal: more scenes: the lane reads them through its scene descriptor in global
+//    memory (a dependent load before every node and leaf access).
+constexpr int kSceneUni = 0, kSceneLds = 1, kSceneGlobal = 2;
+constexpr uint32_t kLdsScenes = 64;
+struct SceneTrees {
+    const GNode* nodes;
+    const Leaf* leaves;
+    const GTri* tris;
+};
+
 constexpr int kRenderRefill = 24;
 constexpr int kRenderChunk = 128;   // queue positions per atomic on the path counter
 
-template <int DEPTH, int LOOP, bool UNI>
+template <int DEPTH, int LOOP, int SC>
 __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
-    k_wtrace_persist(const SceneDesc* __restrict__ scenes, PathState P, int qi, uint32_t wait, uint32_t* work) {
+    k_wtrace_persist(const SceneDesc* __restrict__ scenes, uint32_t nscenes, PathState P, int qi, uint32_t wait,
+                     uint32_t* work) {
     __shared__ uint2 lst[DEPTH][kTraceThreads];
     __shared__ CoopLds coop[LOOP == kLoopCoop ? kTraceThreads / 64 : 1];
+    __shared__ SceneTrees tab[SC == kSceneLds ? kLdsScenes : 1];
+    if (SC == kSceneLds) {
+        for (uint32_t i = threadIdx.x; i < nscenes; i += kTraceThreads)
+            tab[i] = SceneTrees{scenes[i].T.nodes, scenes[i].T.leaves, scenes[i].T.tris};
+        __syncthreads();
+    }
     const uint32_t n = P.cnt[qi];
     const uint32_t* Q = P.q[qi];
     const uint32_t tid = threadIdx.x;
@@ -232,7 +251,18 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
     w.best = kInvalid;
     uint32_t id = kInvalid;            // path being traced
     const SceneDesc* S = scenes;       // its scene
-    const TraceArgs TU = scenes->T;    // UNI: the one scene's tree, loaded once into scalar registers
+    uint32_t sid = 0;                  // its scene index
+    const TraceArgs TU = scenes->T;    // kSceneUni: the one scene's tree, loaded once into scalar registers
+    // the lane's tree for node and leaf accesses (the other fields are only read at start)
+    auto args = [&]() -> TraceArgs {
+        if (SC == kSceneUni) return TU;
+        if (SC == kSceneGlobal) return S->T;
+        TraceArgs A = TU;
+        A.nodes = tab[sid].nodes;
+        A.leaves = tab[sid].leaves;
+        A.tris = tab[sid].tris;
+        return A;
+    };
     uint32_t rbase = 0, rend = 0;      // wave-uniform reservoir of queue positions [rbase, rend)
     bool gdone = false;                // `work` has passed n
     bool more = true;                  // the same on every lane: set only from wave-wide values
@@ -260,21 +290,24 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
                 const uint32_t k = base + (uint32_t)__popcll(idle & lt);
                 if (k < rend) {
                     id = Q[k];
-                    if (!UNI) S = scenes + P.sid[id];
+                    if (SC != kSceneUni) {
+                        sid = P.sid[id];
+                        S = scenes + sid;
+                    }
                     const float4 r0 = P.ray0[id], r1 = P.ray1[id];
-                    w.start(UNI ? TU : S->T, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w);
+                    w.start(SC == kSceneUni ? TU : S->T, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w);
                 }
             }
         }
         // uniform exit test: a started path that missed the root is written at the top
         if (!__any(w.live || id != kInvalid) && !more) break;
-        w.template node_phase<false>(UNI ? TU : S->T, lst, tid, wait);
+        w.template node_phase<false>(args(), lst, tid, wait);
         if (LOOP == kLoopCoop) {   // all lanes active: the loop is wave-uniform
             const bool at = w.live && (w.ref & kLeafBit);
-            w.template leaf_coop<false, false, !UNI>(UNI ? TU : S->T, coop + tid / 64);
+            w.template leaf_coop<false, false, SC != kSceneUni>(args(), coop + tid / 64);
             if (at) w.pop(lst, tid);
         } else if (w.live && (w.ref & kLeafBit)) {
-            w.template leaf_step<false, false>(UNI ? TU : S->T);
+            w.template leaf_step<false, false>(args());
             w.pop(lst, tid);
         }
     }
@@ -665,12 +698,16 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
             if (g_tuning.render_loop == 2) {
                 k_clear1<<<1, 64, 0, st>>>(P.cnt + 2);   // the persistent kernel's path counter
                 with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
+                    constexpr int DD = decltype(D)::value, LL = decltype(L)::value;
                     if (s_last - s_first == 1)
-                        k_wtrace_persist<decltype(D)::value, decltype(L)::value, true>
-                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes + s_first, P, qi, wait, P.cnt + 2);
+                        k_wtrace_persist<DD, LL, kSceneUni>
+                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes + s_first, 1u, P, qi, wait, P.cnt + 2);
+                    else if ((uint32_t)ns <= kLdsScenes)
+                        k_wtrace_persist<DD, LL, kSceneLds>
+                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, wait, P.cnt + 2);
                     else
-                        k_wtrace_persist<decltype(D)::value, decltype(L)::value, false>
-                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes, P, qi, wait, P.cnt + 2);
+                        k_wtrace_persist<DD, LL, kSceneGlobal>
+                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, wait, P.cnt + 2);
                 });
             } else {
                 with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
