@@ -314,8 +314,9 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
 }
 
 // One bounce of pathTrace (main.cpp:158-200) for every queued path. ADJ additionally
-// records each surface hit for the adjoint (triangle, radiance so far, throughput).
-template <bool ADJ>
+// records each surface hit for the adjoint (triangle, radiance so far, throughput). UNI:
+// every path belongs to scenes[0], whose descriptor is then read once as uniform data.
+template <bool ADJ, bool UNI>
 __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int qi, int bounce) {
     const uint32_t n = P.cnt[qi];
     const uint32_t* Q = P.q[qi];
@@ -332,7 +333,7 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
         uint32_t i = 0;
         if (k < n) {
             i = Q[k];
-            const SceneDesc& S = scenes[P.sid[i]];
+            const SceneDesc& S = scenes[UNI ? 0u : P.sid[i]];
             const float4 h = P.hit[i];
             const uint32_t tri = __float_as_uint(h.w);
             float4 tp = P.tp[i], L = P.L[i];
@@ -715,8 +716,15 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
                                                                                                    wait);
                 });
             }
-            if (adj) k_shade<true><<<grid, sh, 0, st>>>(d_scenes, P, qi, b);
-            else k_shade<false><<<grid, sh, 0, st>>>(d_scenes, P, qi, b);
+            const bool one = s_last - s_first == 1;
+            const SceneDesc* sd = one ? d_scenes + s_first : d_scenes;
+            if (adj) {
+                if (one) k_shade<true, true><<<grid, sh, 0, st>>>(sd, P, qi, b);
+                else k_shade<true, false><<<grid, sh, 0, st>>>(sd, P, qi, b);
+            } else {
+                if (one) k_shade<false, true><<<grid, sh, 0, st>>>(sd, P, qi, b);
+                else k_shade<false, false><<<grid, sh, 0, st>>>(sd, P, qi, b);
+            }
             k_clear1<<<1, 64, 0, st>>>(P.cnt + qi);
             qi ^= 1;
         }
