@@ -194,7 +194,11 @@ def main():
     if not (args.no_render or args.skip_extras):
         del inc, coh, hits, occ, counts
         torch.cuda.empty_cache()
+        # the batched leg tunes launch shapes for its own mixed-scene launches; the other
+        # render legs run with the launch shapes that were set before it
+        before = rtx.get_tuning()
         batched = bench_batched(stream, args.batched_scenes, tune=not args.no_autotune)
+        rtx.set_tuning(**before)
         render = {"differentiable": bench_differentiable(stream), "batched": batched,
                   "cornell": bench_cornell(stream, cpu=not args.no_cpu)}
         if not args.no_inverse:

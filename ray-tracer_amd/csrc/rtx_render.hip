@@ -53,7 +53,7 @@ struct PathState {
     uint32_t* pix;
     uint32_t* smp;
     uint32_t* q[2];
-    uint32_t* cnt;     // [0], [1] queue lengths
+    uint32_t* cnt;     // [b]: input queue length of bounce b; [ncnt/2 + b]: persistent-trace counter of bounce b
     // adjoint only: per surface hit k of path i, at [k * B + i]
     float4* vlup;      // radiance gathered up to and including hit k (xyz), triangle id (w bits)
     float4* vtp;       // throughput arriving at hit k
@@ -133,12 +133,10 @@ __global__ void k_zero_u32(uint32_t* p, uint64_t n) {
         p[i] = 0u;
 }
 
-__global__ void k_clear1(uint32_t* p) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = 0u;
-}
-
-__global__ void k_set_u32(uint32_t* p, uint32_t a, uint32_t b) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) { p[0] = a; p[1] = b; }
+// Start of a batch: the first queue holds `a` paths; every other queue length and every
+// bounce's trace counter starts at 0, so no bounce needs a clearing launch of its own.
+__global__ void k_set_counts(uint32_t* p, uint32_t n, uint32_t a) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = i == 0 ? a : 0u;
 }
 
 __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* __restrict__ off, int ns, uint64_t g0,
@@ -172,43 +170,6 @@ __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* _
 // ms), and at 8 waves the spills cost more than the wave gained.
 constexpr int kRenderTriBatch = 2;
 
-template <int DEPTH, int LOOP>
-__global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
-    k_wtrace(const SceneDesc* __restrict__ scenes, PathState P, int qi,
-                                                             uint32_t wait) {
-    __shared__ uint2 lst[DEPTH][kTraceThreads];
-    __shared__ CoopLds coop[LOOP == kLoopCoop ? kTraceThreads / 64 : 1];
-    const uint32_t n = P.cnt[qi];
-    const uint32_t* Q = P.q[qi];
-    // wave-uniform trip count (the while-while loop votes across the wave)
-    for (uint32_t wb = xcd_block(blockIdx.x) * kTraceThreads + (threadIdx.x & ~63u); wb < n;
-         wb += gridDim.x * kTraceThreads) {
-        const uint32_t k = wb + __lane_id();
-        const bool in = k < n;
-        uint32_t i = 0;
-        float4 r0 = make_float4(0.f, 0.f, 0.f, 1.f), r1 = make_float4(0.f, 0.f, 1.f, 0.f);
-        const SceneDesc* S = scenes;
-        if (in) {
-            i = Q[k];
-            r0 = P.ray0[i];
-            r1 = P.ray1[i];
-            S = scenes + P.sid[i];
-        }
-        const TraceOut h = trace_ray<false, false, DEPTH, LOOP, kRenderTriBatch, true>(
-            S->T, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w, lst, threadIdx.x, wait,
-            coop + (LOOP == kLoopCoop ? threadIdx.x / 64 : 0));
-        if (in) P.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
-    }
-}
-
-// Persistent variant (render_loop 2): a grid of waves that refill idle lanes (at least
-// kRenderRefill at a time) from the queue, so a wave does not wait for the slowest path of
-// a 64-path chunk. Each wave takes kRenderChunk queue positions per atomic on `work` and
-// serves its refills from them: with one atomic per refill the single counter serialized
-// the launch (C3 162 ms against 89 ms for the chunked kernel; 128-path chunks: 96 ms, and
-// C4 10.4 -> 8.7 ms, C5 184 -> 152 ms). The control flow follows k_trace_persist: every
-// ballot, the leader's atomic, the shuffle and the exit test are wave-uniform, and no lane
-// leaves the loop body early.
 // Where a lane finds its scene's tree (SC):
 //  - kSceneUni: every queued path belongs to scenes[0] (a launch sequence for one scene).
 //    The tree pointers are wave-uniform kernel data that stay in scalar registers.
@@ -225,13 +186,66 @@ struct SceneTrees {
     const GTri* tris;
 };
 
+// Chunked form: each wave traces 64 consecutive queue entries to the end.
+template <int DEPTH, int LOOP, int SC>
+__global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
+    k_wtrace(const SceneDesc* __restrict__ scenes, uint32_t nscenes, PathState P, int qi, int bounce, uint32_t wait) {
+    __shared__ uint2 lst[DEPTH][kTraceThreads];
+    __shared__ CoopLds coop[LOOP == kLoopCoop ? kTraceThreads / 64 : 1];
+    __shared__ SceneTrees tab[SC == kSceneLds ? kLdsScenes : 1];
+    if (SC == kSceneLds) {
+        for (uint32_t j = threadIdx.x; j < nscenes; j += kTraceThreads)
+            tab[j] = SceneTrees{scenes[j].T.nodes, scenes[j].T.leaves, scenes[j].T.tris};
+        __syncthreads();
+    }
+    const uint32_t n = P.cnt[bounce];
+    const uint32_t* Q = P.q[qi];
+    // wave-uniform trip count (the while-while loop votes across the wave)
+    for (uint32_t wb = xcd_block(blockIdx.x) * kTraceThreads + (threadIdx.x & ~63u); wb < n;
+         wb += gridDim.x * kTraceThreads) {
+        const uint32_t k = wb + __lane_id();
+        const bool in = k < n;
+        uint32_t i = 0;
+        float4 r0 = make_float4(0.f, 0.f, 0.f, 1.f), r1 = make_float4(0.f, 0.f, 1.f, 0.f);
+        const SceneDesc* S = scenes;
+        uint32_t sid = 0;
+        if (in) {
+            i = Q[k];
+            r0 = P.ray0[i];
+            r1 = P.ray1[i];
+            if (SC != kSceneUni) {
+                sid = P.sid[i];
+                S = scenes + sid;
+            }
+        }
+        TraceArgs A = SC == kSceneUni ? scenes->T : S->T;
+        if (SC == kSceneLds) {
+            A.nodes = tab[sid].nodes;
+            A.leaves = tab[sid].leaves;
+            A.tris = tab[sid].tris;
+        }
+        const TraceOut h = trace_ray<false, false, DEPTH, LOOP, kRenderTriBatch, SC != kSceneUni>(
+            A, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w, lst, threadIdx.x, wait,
+            coop + (LOOP == kLoopCoop ? threadIdx.x / 64 : 0));
+        if (in) P.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
+    }
+}
+
+// Persistent variant (render_loop 2): a grid of waves that refill idle lanes (at least
+// kRenderRefill at a time) from the queue, so a wave does not wait for the slowest path of
+// a 64-path chunk. Each wave takes kRenderChunk queue positions per atomic on `work` and
+// serves its refills from them: with one atomic per refill the single counter serialized
+// the launch (C3 162 ms against 89 ms for the chunked kernel; 128-path chunks: 96 ms, and
+// C4 10.4 -> 8.7 ms, C5 184 -> 152 ms). The control flow follows k_trace_persist: every
+// ballot, the leader's atomic, the shuffle and the exit test are wave-uniform, and no lane
+// leaves the loop body early.
 constexpr int kRenderRefill = 24;
 constexpr int kRenderChunk = 128;   // queue positions per atomic on the path counter
 
 template <int DEPTH, int LOOP, int SC>
 __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
-    k_wtrace_persist(const SceneDesc* __restrict__ scenes, uint32_t nscenes, PathState P, int qi, uint32_t wait,
-                     uint32_t* work) {
+    k_wtrace_persist(const SceneDesc* __restrict__ scenes, uint32_t nscenes, PathState P, int qi, int bounce,
+                     uint32_t wait, uint32_t* work) {
     __shared__ uint2 lst[DEPTH][kTraceThreads];
     __shared__ CoopLds coop[LOOP == kLoopCoop ? kTraceThreads / 64 : 1];
     __shared__ SceneTrees tab[SC == kSceneLds ? kLdsScenes : 1];
@@ -240,7 +254,7 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
             tab[i] = SceneTrees{scenes[i].T.nodes, scenes[i].T.leaves, scenes[i].T.tris};
         __syncthreads();
     }
-    const uint32_t n = P.cnt[qi];
+    const uint32_t n = P.cnt[bounce];
     const uint32_t* Q = P.q[qi];
     const uint32_t tid = threadIdx.x;
     const uint64_t lt = (1ull << __lane_id()) - 1ull;
@@ -318,7 +332,7 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
 // every path belongs to scenes[0], whose descriptor is then read once as uniform data.
 template <bool ADJ, bool UNI>
 __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int qi, int bounce) {
-    const uint32_t n = P.cnt[qi];
+    const uint32_t n = P.cnt[bounce];
     const uint32_t* Q = P.q[qi];
     uint32_t* Qo = P.q[qi ^ 1];
     const uint64_t lt = (1ull << __lane_id()) - 1ull;
@@ -390,7 +404,7 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
         if (threadIdx.x == 0) {
             uint32_t run = 0;
             for (uint32_t w = 0; w < nw; ++w) { const uint32_t c = wcount[w]; wcount[w] = run; run += c; }
-            bbase = run ? atomicAdd(&P.cnt[qi ^ 1], run) : 0u;
+            bbase = run ? atomicAdd(&P.cnt[bounce + 1], run) : 0u;
         }
         __syncthreads();
         if (cont) Qo[bbase + wcount[wid] + __popcll(m & lt)] = i;
@@ -604,6 +618,7 @@ struct Batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int max_b = 0;
+    uint32_t ncnt = 0;   // entries of P.cnt: queue lengths, then trace counters, one per bounce
 
     ~Batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -653,7 +668,8 @@ int Batch::setup(uint32_t batch_paths, bool adjoint) {
     P.tp = alloc<float4>(B); P.L = alloc<float4>(B);
     P.sid = alloc<uint32_t>(B); P.pix = alloc<uint32_t>(B); P.smp = alloc<uint32_t>(B);
     P.q[0] = alloc<uint32_t>(B); P.q[1] = alloc<uint32_t>(B);
-    P.cnt = alloc<uint32_t>(4);
+    ncnt = 2 * (uint32_t)(max_b + 2);
+    P.cnt = alloc<uint32_t>(ncnt);
     P.B = B;
     P.vlup = nullptr; P.vtp = nullptr; P.nv = nullptr;
     if (adjoint) {
@@ -688,32 +704,46 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
     for (uint64_t g0 = off[s_first]; g0 < off[s_last]; g0 += B) {
         const uint32_t cnt = (uint32_t)std::min<uint64_t>(B, off[s_last] - g0);
         const uint32_t cgrid = std::min<uint32_t>((cnt + 255) / 256, 65535);
-        k_set_u32<<<1, 64, 0, st>>>(P.cnt, cnt, 0u);
+        k_set_counts<<<1, 64, 0, st>>>(P.cnt, ncnt, cnt);
         k_camera<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
-        // cnt[qi] is the input queue's length; shade appends to q[qi^1] (cnt[qi^1] == 0),
-        // then the consumed queue becomes the next output and its length is cleared.
+        // bounce b traces queue q[qi] of cnt[b] paths; shade appends the survivors to q[qi^1],
+        // counting them in cnt[b+1]; the persistent trace kernel deals from cnt[ncnt/2 + b]
         int qi = 0;
         for (int b = 0; b <= mb; ++b) {
             const uint32_t loop = g_tuning.render_loop == 0 ? (uint32_t)kLoopSingle
                                   : g_tuning.render_leaf_coop ? (uint32_t)kLoopCoop : (uint32_t)kLoopWhileWhile;
-            if (g_tuning.render_loop == 2) {
-                k_clear1<<<1, 64, 0, st>>>(P.cnt + 2);   // the persistent kernel's path counter
+            // The persistent kernel hands each wave kRenderChunk paths per atomic: a batch
+            // with fewer paths than its waves can take in one round would leave most of
+            // the GPU idle, so such batches (the single scenes of a plain C3 render, say)
+            // take the chunked kernel, 64 paths per wave.
+            const bool persist =
+                g_tuning.render_loop == 2 && (uint64_t)cnt >= (uint64_t)pgrid * (kTraceThreads / 64) * kRenderChunk;
+            if (persist) {
+                uint32_t* work = P.cnt + ncnt / 2 + b;
                 with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
                     constexpr int DD = decltype(D)::value, LL = decltype(L)::value;
                     if (s_last - s_first == 1)
                         k_wtrace_persist<DD, LL, kSceneUni>
-                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes + s_first, 1u, P, qi, wait, P.cnt + 2);
+                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes + s_first, 1u, P, qi, b, wait, work);
                     else if ((uint32_t)ns <= kLdsScenes)
                         k_wtrace_persist<DD, LL, kSceneLds>
-                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, wait, P.cnt + 2);
+                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, b, wait, work);
                     else
                         k_wtrace_persist<DD, LL, kSceneGlobal>
-                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, wait, P.cnt + 2);
+                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, b, wait, work);
                 });
             } else {
                 with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
-                    k_wtrace<decltype(D)::value, decltype(L)::value><<<grid, kTraceThreads, 0, st>>>(d_scenes, P, qi,
-                                                                                                   wait);
+                    constexpr int DD = decltype(D)::value, LL = decltype(L)::value;
+                    if (s_last - s_first == 1)
+                        k_wtrace<DD, LL, kSceneUni>
+                            <<<grid, kTraceThreads, 0, st>>>(d_scenes + s_first, 1u, P, qi, b, wait);
+                    else if ((uint32_t)ns <= kLdsScenes)
+                        k_wtrace<DD, LL, kSceneLds>
+                            <<<grid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, b, wait);
+                    else
+                        k_wtrace<DD, LL, kSceneGlobal>
+                            <<<grid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, b, wait);
                 });
             }
             const bool one = s_last - s_first == 1;
@@ -725,7 +755,6 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
                 if (one) k_shade<false, true><<<grid, sh, 0, st>>>(sd, P, qi, b);
                 else k_shade<false, false><<<grid, sh, 0, st>>>(sd, P, qi, b);
             }
-            k_clear1<<<1, 64, 0, st>>>(P.cnt + qi);
             qi ^= 1;
         }
         if (adj) {

@@ -95,6 +95,27 @@ def _small_batch_scenes():
     return out
 
 
+def test_single_scene_render_variants_bit_identical():
+    """One scene per launch sequence: the traversal keeps the tree in scalar registers. The
+    persistent kernel (render_loop 2 on a batch of at least trace_grid_blocks * 512 paths),
+    the chunked loops and the single-loop form give bit-identical images."""
+    v, t, cam, p, _ = _small_batch_scenes()[1]   # 32 x 32 x 2 spp = 2048 paths
+    g = rtx.build(v, t)
+    saved = rtx.get_tuning()
+    try:
+        ref = None
+        for loop, coop, pgrid, bp in ((1, 1, 2048, 1 << 22), (2, 1, 2, 1 << 22), (2, 0, 1, 1024), (0, 0, 2048, 1 << 22),
+                                      (2, 1, 2048, 1 << 22)):
+            rtx.set_tuning(render_loop=loop, render_leaf_coop=coop, trace_grid_blocks=pgrid, batch_paths=bp)
+            print(f"[single-scene variant] loop={loop} coop={coop} pgrid={pgrid} batch={bp}", flush=True)
+            img = rtx.render(g, cam, p).cpu().numpy()
+            if ref is None:
+                ref = img
+            assert np.array_equal(img, ref), (loop, coop, pgrid, bp)
+    finally:
+        rtx.set_tuning(**saved)
+
+
 def test_batched_modes_match_single_and_oracle():
     import torch
     sc = _small_batch_scenes()
@@ -122,17 +143,20 @@ def test_render_trace_variants_bit_identical():
     saved = rtx.get_tuning()
     try:
         ref = None
-        for loop, depth, wait, grid, bp, coop in ((1, 8, 0, 2048, 1 << 22, 0), (1, 12, 16, 2048, 1 << 22, 0),
-                                                  (1, 8, 8, 8, 4096, 0), (1, 16, 32, 64, 8192, 0),
-                                                  (0, 8, 0, 512, 1 << 20, 0), (1, 8, 0, 2048, 1 << 22, 1),
-                                                  (1, 12, 32, 8, 4096, 1), (0, 8, 0, 512, 1 << 20, 1),
-                                                  (2, 8, 8, 16, 1 << 22, 1), (2, 12, 0, 16, 4096, 0),
-                                                  (2, 8, 16, 2048, 1 << 20, 1)):
-            # render_loop 2 is the persistent kernel, with trace_grid_blocks = grid here (16
-            # blocks: every lane takes many paths)
+        for loop, depth, wait, grid, bp, coop, pgrid in (
+                (1, 8, 0, 2048, 1 << 22, 0, 2048), (1, 12, 16, 2048, 1 << 22, 0, 2048),
+                (1, 8, 8, 8, 4096, 0, 2048), (1, 16, 32, 64, 8192, 0, 2048),
+                (0, 8, 0, 512, 1 << 20, 0, 2048), (1, 8, 0, 2048, 1 << 22, 1, 2048),
+                (1, 12, 32, 8, 4096, 1, 2048), (0, 8, 0, 512, 1 << 20, 1, 2048),
+                (2, 8, 8, 16, 1 << 22, 1, 2), (2, 12, 0, 16, 4096, 0, 2), (2, 16, 4, 16, 2048, 1, 1),
+                (2, 8, 16, 2048, 1 << 20, 1, 2048)):
+            # render_loop 2 runs the persistent kernel on batches of at least
+            # trace_grid_blocks * 4 waves * 128 paths (2 blocks: 1024 paths, every lane takes
+            # many paths) and the chunked kernel on smaller ones (the last variant)
             rtx.set_tuning(render_loop=loop, render_lds_depth=depth, render_wait_lanes=wait, grid_blocks=grid,
-                           batch_paths=bp, render_leaf_coop=coop, trace_grid_blocks=grid)
-            print(f"[render variant] loop={loop} depth={depth} wait={wait} grid={grid} coop={coop}", flush=True)
+                           batch_paths=bp, render_leaf_coop=coop, trace_grid_blocks=pgrid)
+            print(f"[render variant] loop={loop} depth={depth} wait={wait} grid={grid} coop={coop} pgrid={pgrid}",
+                  flush=True)
             imgs = [torch.zeros((p.height, p.width, 3), device="cuda") for _, _, _, p, _ in sc]
             b = rtx.Batch(bvhs, [c for _, _, c, _, _ in sc], [p for _, _, _, p, _ in sc], [None] * 3, [None] * 3,
                           imgs, 2)
