@@ -410,7 +410,7 @@ def bench_batched(stream, nscenes, reps=2, tune=True):
             b = rtx.Batch(bvhs, cams, params, [None] * nscenes, [None] * nscenes, imgs, 1)
             return lambda: b.launch(stream)
         t0 = time.perf_counter()
-        best, _, hist = autotune.tune(make_step, keys=autotune.RENDER_KEYS, reps=1)
+        best, _, hist = autotune.tune(make_step, keys=autotune.RENDER_KEYS, reps=2)
         rtx.set_tuning(**best)
         out["autotune"] = {"seconds": round(time.perf_counter() - t0, 2), "candidates": len(hist)}
     out["tuning"] = rtx.get_tuning()

@@ -32,7 +32,8 @@ SPACE = {
     "batch_paths": [1 << 20, 1 << 21, 1 << 22, 1 << 23],
 }
 TRACE_KEYS = ["trace_loop", "leaf_coop", "trace_lds_depth", "ray_sort", "wait_lanes", "trace_grid_blocks"]
-RENDER_KEYS = ["batch_paths", "shade_threads", "grid_blocks", "render_lds_depth", "render_loop", "render_leaf_coop",
+# loop structure first: the best block and batch sizes depend on which kernel runs
+RENDER_KEYS = ["render_loop", "render_leaf_coop", "grid_blocks", "shade_threads", "batch_paths", "render_lds_depth",
                "render_wait_lanes"]
 
 
