@@ -232,7 +232,8 @@ void rtx_batch_free(rtx_batch* batch);
  *   render_lds_depth   LDS stack entries per lane: 8, 12 or 16;
  *   render_loop        0 or 1 as trace_loop; 2 = while-while in a persistent grid of
  *                      trace_grid_blocks blocks whose waves refill idle lanes from the
- *                      bounce queue;
+ *                      bounce queue, for batches of at least trace_grid_blocks * 512
+ *                      paths (smaller batches take the loop of 1);
  *   render_wait_lanes  as wait_lanes.
  * Both:
  *   leaf_coop          1 = the while-while loops of the trace entry points (trace_loop
