@@ -133,6 +133,37 @@ def test_batched_modes_match_single_and_oracle():
             assert np.array_equal(a.cpu().numpy(), s), f"mode {mode} differs from single-scene render"
 
 
+def test_batch_of_more_scenes_than_the_lds_table():
+    """A shared batch of 70 scenes (more than the 64 the renderer's LDS scene table holds,
+    so lanes read their trees through global scene descriptors) and of 20 scenes (the LDS
+    table) give the single-scene images, with the chunked and the persistent kernel."""
+    import torch
+    sc = []
+    for i in range(70):
+        v, t = S.uv_sphere(4 + i % 5, 8 + 2 * (i % 5), 1.0, (0, 0, 0), 0.05, seed=100 + i)
+        cam = S.camera_lookat((0.05 * (i % 7), 0.3, -3.0), (0, 0, 0), 45.0)
+        sc.append((v, t, cam, rtx.make_params(8, 8, 2, 4, 3, seed=i)))
+    bvhs = [rtx.build(v, t) for v, t, _, _ in sc]
+    singles = [rtx.render(b, c, p).cpu().numpy() for b, (_, _, c, p) in zip(bvhs, sc)]
+    saved = rtx.get_tuning()
+    try:
+        for loop, pgrid in ((1, 2048), (2, 1)):
+            rtx.set_tuning(render_loop=loop, trace_grid_blocks=pgrid)
+            for count in (70, 20):
+                print(f"[many scenes] loop={loop} scenes={count}", flush=True)
+                part = sc[:count]
+                imgs = [torch.zeros((p.height, p.width, 3), device="cuda") for _, _, _, p in part]
+                b = rtx.Batch(bvhs[:count], [c for _, _, c, _ in part], [p for _, _, _, p in part], [None] * count,
+                              [None] * count, imgs, 2)
+                b.launch()
+                torch.cuda.synchronize()
+                for k, (a, ref) in enumerate(zip(imgs, singles)):
+                    assert np.array_equal(a.cpu().numpy(), ref), (loop, count, k)
+                del b
+    finally:
+        rtx.set_tuning(**saved)
+
+
 def test_render_trace_variants_bit_identical():
     """The wavefront renderer's traversal variants (one loop, while-while with any
     render_wait_lanes, cooperative leaves, the persistent kernel, stack depths, grid and
