@@ -527,7 +527,6 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C) {
     __shared__ float bbox[3 * 16][6];
     __shared__ uint32_t bc0[3 * 16], bc1[3 * 16];
     __shared__ uint32_t tsi_slot[64];   // rank exchange of the partitions
-    __shared__ float poly_lds[kPolyLdsFloats];
     __shared__ uint8_t obin[64][3];              // centroid bin per triangle and axis
     __shared__ uint8_t sb0[64][3], sb1[64][3];   // spatial entry / exit bin per triangle and axis
     __shared__ uint32_t tax[64][3];              // first clip pair of triangle i on axis a
@@ -723,10 +722,7 @@ __global__ void __launch_bounds__(64) k_sah_wave(Ctx C) {
                 const f3 p0{tv[i][0], tv[i][1], tv[i][2]}, p1{tv[i][3], tv[i][4], tv[i][5]},
                     p2{tv[i][6], tv[i][7], tv[i][8]};
                 Box cb;
-                if (!clip_tri_slab(p0, p1, p2, reg, a, cb)) {
-                    PolyLds P{poly_lds + lane};
-                    cb = clip_tri_in(p0, p1, p2, reg, P);
-                }
+                if (!clip_tri_slab(p0, p1, p2, reg, a, cb)) cb = clip_tri(p0, p1, p2, reg);
                 pbuf[r + lane][0] = cb.lo.x; pbuf[r + lane][1] = cb.lo.y; pbuf[r + lane][2] = cb.lo.z;
                 pbuf[r + lane][3] = cb.hi.x; pbuf[r + lane][4] = cb.hi.y; pbuf[r + lane][5] = cb.hi.z;
             }
