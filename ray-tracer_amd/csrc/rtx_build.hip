@@ -107,10 +107,12 @@ __device__ Box range_box(const Ctx& C, uint32_t b, uint32_t e) {
 // Sutherland-Hodgman clip of a triangle to a box, then its AABB (AABB.cpp:86-171), with
 // the reference's vertex order and float operations. The polygon (at most 9 vertices
 // after 6 planes) lives in a PolyStore:
-//  - PolyRegs: small per-lane arrays (the serial SAH path for nodes over 64 triangles);
-//  - PolyLds: a per-lane column of LDS (the wave kernels). Output positions differ
-//    between lanes, and indexing registers with a lane-varying index costs a loop over
-//    the distinct indices in the wave, while LDS addresses are per lane.
+//  - PolyRegs: small per-lane arrays, in private memory once indexed by lane-varying
+//    positions (the serial SAH path for nodes over 64 triangles, and the fallback clip of
+//    k_sah_wave: there 15 KB of LDS per wave held its occupancy to 6 waves per CU);
+//  - PolyLds: a per-lane column of LDS (k_level_apply's child clipping). Output positions
+//    differ between lanes, and indexing registers with a lane-varying index costs a loop
+//    over the distinct indices in the wave, while LDS addresses are per lane.
 constexpr int kPolyMax = 10;
 
 struct PolyRegs {
