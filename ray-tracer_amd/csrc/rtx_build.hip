@@ -128,8 +128,9 @@ struct PolyLdsT {
 using PolyLds = PolyLdsT<64>;
 constexpr int poly_floats(int cols) { return 2 * 3 * kPolyMax * cols; }
 constexpr int kPolyLdsFloats = poly_floats(64);
-// Clip pairs per chunk of k_sah_wave's spatial pass (two rounds of 64 lanes).
-constexpr int kClipChunk = 128;
+// Clip pairs per chunk of k_sah_wave's spatial pass: one round of 64 lanes. (128-pair
+// chunks took 1.5 KB more LDS per wave: 14 waves per CU instead of 17, and slower builds.)
+constexpr int kClipChunk = 64;
 
 template <class PS>
 __device__ Box clip_tri_in(f3 v0, f3 v1, f3 v2, const Box& r, PS& P) {
