@@ -295,7 +295,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
 
 }  // namespace
 
-rtx_tuning g_tuning = {1u << 23, 8192, 1024, 12, 2, 1, 24, 2048, 8, 2, 8, 1, 1};
+rtx_tuning g_tuning = {1u << 23, 8192, 1024, 12, 2, 1, 24, 2048, 8, 2, 16, 1, 1};
 
 }  // namespace rtx
 

@@ -25,7 +25,7 @@ SPACE = {
     # renderer
     "render_lds_depth": [8, 12, 16],
     "render_loop": [0, 1, 2],
-    "render_wait_lanes": [0, 4, 8, 16],
+    "render_wait_lanes": [0, 4, 8, 16, 24, 32],
     "render_leaf_coop": [0, 1],
     "grid_blocks": [1024, 2048, 4096, 8192],
     "shade_threads": [64, 128, 256, 512, 1024],
