@@ -61,6 +61,7 @@ struct Counters {
     uint32_t lvl_ls;     // the level being built: its first node and node count
     uint32_t lvl_cnt;
     uint32_t lvl_le;     // one past its last node (the next level's first)
+    uint32_t fast;       // this level's budget was resolved by the multi-block pass
 };
 
 struct Ctx {
@@ -468,6 +469,7 @@ __global__ void k_level_start(Counters* c) {
         c->lvl_cnt = le - c->lvl_le;
         c->lvl_le = le;
         c->nwave = 0;
+        c->fast = 0;
     }
 }
 
@@ -845,11 +847,81 @@ __device__ uint32_t block_scan(uint32_t v, uint32_t* ws, uint32_t& total) {
     return ex;
 }
 
+// Multi-block form of k_level_budget for wide levels. When every spatial candidate of the
+// level fits the budget, each 1024-node chunk of k_level_budget takes its scan path, so
+// the result is the exclusive scan over the whole level: per-chunk sums (k_budget_sums),
+// a scan of the chunk sums with the budget check (k_budget_scan), and the per-chunk scans
+// (k_budget_apply). Otherwise `fast` stays 0 and k_level_budget runs as for any level.
+__device__ __forceinline__ void budget_needs(const Ctx& C, uint32_t k, uint32_t cnt, bool& sp, uint32_t& nt,
+                                             uint32_t& ns, uint32_t& nch) {
+    sp = false; nt = ns = nch = 0;
+    if (k >= cnt) return;
+    const Dec d = C.dec[k];
+    sp = d.type == D_SPATIAL;
+    nt = sp ? d.nr + d.ns : 0u;
+    ns = sp ? d.ns : 0u;
+    nch = (d.type == D_MORTON || d.type == D_OBJECT || sp) ? 2u : 0u;   // every spatial split accepted
+}
+
+__global__ void __launch_bounds__(1024) k_budget_sums(Ctx C, uint32_t* part) {
+    __shared__ uint32_t ws[17];
+    const uint32_t cnt = C.ctr->lvl_cnt;
+    if (blockIdx.x * 1024u >= cnt) return;   // the whole block
+    bool sp;
+    uint32_t nt, ns, nch, tt, ts, tn;
+    budget_needs(C, blockIdx.x * 1024u + threadIdx.x, cnt, sp, nt, ns, nch);
+    block_scan(nt, ws, tt);
+    block_scan(ns, ws, ts);
+    block_scan(nch, ws, tn);
+    if (threadIdx.x == 0) { part[3 * blockIdx.x] = tt; part[3 * blockIdx.x + 1] = ts; part[3 * blockIdx.x + 2] = tn; }
+}
+
+__global__ void k_budget_scan(Ctx C, uint32_t* part) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Counters* c = C.ctr;
+    const uint32_t nb = (c->lvl_cnt + 1023u) / 1024u;
+    uint64_t t = c->tsize, sz = c->ssize, n = c->total;
+    for (uint32_t b = 0; b < nb; ++b) { t += part[3 * b]; sz += part[3 * b + 1]; }
+    if (t > C.cap || sz > C.cap) return;   // some spatial split may not fit: k_level_budget decides
+    uint32_t bt = c->tsize, bs = c->ssize, bn = c->total;
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t xt = part[3 * b], xs = part[3 * b + 1], xn = part[3 * b + 2];
+        part[3 * b] = bt; part[3 * b + 1] = bs; part[3 * b + 2] = bn;
+        bt += xt; bs += xs; bn += xn;
+    }
+    (void)n;
+    c->tsize = bt;
+    c->ssize = bs;
+    c->total = bn;
+    c->fast = 1;
+}
+
+__global__ void __launch_bounds__(1024) k_budget_apply(Ctx C, const uint32_t* part) {
+    __shared__ uint32_t ws[17];
+    if (!C.ctr->fast) return;
+    const uint32_t cnt = C.ctr->lvl_cnt;
+    if (blockIdx.x * 1024u >= cnt) return;
+    const uint32_t k = blockIdx.x * 1024u + threadIdx.x;
+    bool sp;
+    uint32_t nt, ns, nch, tt, ts, tn;
+    budget_needs(C, k, cnt, sp, nt, ns, nch);
+    const uint32_t et = block_scan(nt, ws, tt), es = block_scan(ns, ws, ts), en = block_scan(nch, ws, tn);
+    if (k < cnt) {
+        if (sp) {
+            C.toff[k] = part[3 * blockIdx.x] + et;
+            C.soff[k] = part[3 * blockIdx.x + 1] + es;
+        }
+        C.acc[k] = sp ? 1u : 0u;
+        C.child[k] = nch ? part[3 * blockIdx.x + 2] + en : kInvalid;
+    }
+}
+
 // Queue-order resolution of the spatial-split budget (bvh_sah.cpp:390-394) and child
 // numbering (s_totalNodeCount++ in FIFO order). One block of 1024 threads walks the
 // level in chunks; a chunk whose spatial candidates all fit is accepted with a scan,
 // otherwise lane 0 runs the greedy in order (a rejected candidate costs nothing).
 __global__ void __launch_bounds__(1024) k_level_budget(Ctx C) {
+    if (C.ctr->fast) return;   // resolved by k_budget_sums / k_budget_scan / k_budget_apply
     const uint32_t cnt = C.ctr->lvl_cnt;
     __shared__ uint32_t ws[17];
     __shared__ uint32_t base[3];     // total nodes, s_triIndexSize, s_sortedTrisSize
@@ -1424,6 +1496,7 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
     uint32_t* soff = S.get<uint32_t>(cap + 1);
     uint32_t* acc = S.get<uint32_t>(cap + 1);
     Counters* ctr = S.get<Counters>(1);
+    uint32_t* bpart = S.get<uint32_t>(3 * ((size_t)ncap / 1024 + 2));   // chunk sums of k_budget_*
     uint32_t* wlist = S.get<uint32_t>(cap + 1);
     Dfs D;
     uint32_t* dbuf = S.get<uint32_t>((size_t)ncap * 8);
@@ -1431,7 +1504,7 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
     B->sorted = dalloc<uint32_t>(N);
     B->verts = dalloc<float>((size_t)nv * 6);
     if (!err || !tbox || !cent || !box6 || !keysB || !keysC || !valsA || !valsB || !orand || !sorted ||
-        !tindex || !nodes || !dec || !child || !toff || !soff || !acc || !ctr || !dbuf || !B->codes || !B->sorted ||
+        !tindex || !nodes || !dec || !child || !toff || !soff || !acc || !ctr || !bpart || !dbuf || !B->codes || !B->sorted ||
         !B->verts)
         return fail(RTX_E_HIP, "device allocation failed on rtx_build()");
     D.size = dbuf; D.tcnt = dbuf + ncap; D.icnt = dbuf + 2 * (size_t)ncap; D.lcnt = dbuf + 3 * (size_t)ncap;
@@ -1492,6 +1565,12 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
         k_level_start<<<1, 64, 0, st>>>(ctr);
         k_level_decide<<<nblk(bound, 64), 64, 0, st>>>(C);
         k_sah_wave<<<bound, 64, 0, st>>>(C);
+        if (bound > 1024) {   // more than one 1024-node chunk: try the multi-block budget pass
+            const uint32_t nb = nblk(bound, 1024);
+            k_budget_sums<<<nb, 1024, 0, st>>>(C, bpart);
+            k_budget_scan<<<1, 64, 0, st>>>(C, bpart);
+            k_budget_apply<<<nb, 1024, 0, st>>>(C, bpart);
+        }
         k_level_budget<<<1, 1024, 0, st>>>(C);
         k_level_apply<<<bound, 64, 0, st>>>(C);
         RTX_HIP(hipMemcpyAsync(pc.h + slot, ctr, sizeof(Counters), hipMemcpyDeviceToHost, st));

@@ -50,6 +50,16 @@ def test_budget_exhaustion_path(suite):
     _check("straddle", v, t)
 
 
+def test_budget_exhaustion_on_wide_levels():
+    """Overlapping triangles in levels wider than one 1024-node chunk: the multi-block
+    budget pass must leave levels whose spatial splits do not all fit to the in-order
+    pass, and take the others itself."""
+    v, t = S.random_triangles(30000, seed=8, mu=-1.0, sigma=0.3)
+    o = O.build(v, t)
+    assert o.info["budget_rejected"] > 0, "scene no longer exhausts the spatial-split budget"
+    _check("straddle30k", v, t)
+
+
 def test_build_1m_random():
     """The 1M-triangle benchmark scene at full size."""
     v, t = S.random_triangles(1_000_000, seed=1)
