@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
 // C4 10.4 -> 8.7 ms, C5 184 -> 152 ms). The control flow follows k_trace_persist: every
 // ballot, the leader's atomic, the shuffle and the exit test are wave-uniform, and no lane
 // leaves the loop body early.
-constexpr int kRenderRefill = 24;
+constexpr int kRenderRefill = 32;   // 24: C5 123.7 vs 122.4 ms; 16: 126.1 ms
 constexpr int kRenderChunk = 128;   // queue positions per atomic on the path counter
 
 template <int DEPTH, int LOOP, int SC>
