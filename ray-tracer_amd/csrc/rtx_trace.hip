@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray3
 // no lane leaves the loop body early with `continue`, because the compiler may then run
 // those lanes' next iteration on their own, and a refill seen by only part of the wave
 // would split `more` and could elect a leader that never issues the atomic.
-constexpr int kRefill = 24;   // measured: 4 and 8 much slower, 16 about 1% slower, 32 level with 24
+constexpr int kRefill = 18;   // measured (Mrays/s): 12: 891, 14: 935, 16: 940, 18: 939, 20: 935, 24: 930, 32: 925
 
 template <bool ANY, bool COUNT, int DEPTH, int LOOP>
 __global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, const Ray32* __restrict__ rays,
