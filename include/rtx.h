@@ -224,7 +224,10 @@ void rtx_batch_free(rtx_batch* batch);
  *                      the previous such call on that handle, from any stream or thread;
  *   wait_lanes         0..64: the while-while leaf phase starts once at most this many lanes
  *                      of a wave are still at internal nodes (0 = wait for all);
- *   trace_grid_blocks  blocks of the persistent trace kernel.
+ *   trace_grid_blocks  blocks of the persistent trace kernel;
+ *   refill_lanes       1..64: a wave of the persistent trace kernel fetches new rays once
+ *                      at least this many of its lanes are idle (incoherent closest-hit
+ *                      rays are fastest near 18, coherent and any-hit rays near 32).
  * Renderer (rtx_render, rtx_render_adjoint, rtx_batch_*):
  *   batch_paths        paths in flight per wavefront batch (>= 1024);
  *   grid_blocks        blocks of the wavefront trace and shade kernels;
@@ -255,6 +258,7 @@ typedef struct rtx_tuning {
     uint32_t render_wait_lanes;
     uint32_t leaf_coop;
     uint32_t render_leaf_coop;
+    uint32_t refill_lanes;
 } rtx_tuning;
 int rtx_set_tuning(const rtx_tuning* t);
 int rtx_get_tuning(rtx_tuning* t);

@@ -63,13 +63,15 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray3
 }
 
 // Persistent variant (tuning trace_loop = 2): a fixed grid of waves; when at least
-// kRefill lanes of a wave are idle, one atomic on `work` hands them the next rays, so a
+// A.refill lanes of a wave are idle, one atomic on `work` hands them the next rays, so a
 // wave no longer waits for its slowest ray. Every wave-level operation (ballot, leader
 // election, shuffle, the exit test) sits in control flow that is uniform across the wave:
 // no lane leaves the loop body early with `continue`, because the compiler may then run
 // those lanes' next iteration on their own, and a refill seen by only part of the wave
 // would split `more` and could elect a leader that never issues the atomic.
-constexpr int kRefill = 18;   // measured (Mrays/s): 12: 891, 14: 935, 16: 940, 18: 939, 20: 935, 24: 930, 32: 925
+// The threshold is a tuning field. Measured (Mrays/s): incoherent closest hit 12: 891, 14: 935,
+// 16: 940, 18: 939, 20: 935, 24: 930, 32: 925; coherent closest hit 18: 1696, 24: 1932,
+// 32: 1955; incoherent any hit 18: 1205, 24: 1356, 32: 1377.
 
 template <bool ANY, bool COUNT, int DEPTH, int LOOP>
 __global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, const Ray32* __restrict__ rays,
@@ -107,7 +109,7 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, co
         }
         const uint64_t idle = __ballot(!w.live);
         const uint32_t nidle = (uint32_t)__popcll(idle);
-        if (more && nidle >= (uint32_t)kRefill) {
+        if (more && nidle >= A.refill) {
             const int leader = __ffsll((unsigned long long)idle) - 1;
             uint32_t base = 0;
             if (__lane_id() == leader) base = atomicAdd(work, nidle);
@@ -295,7 +297,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
 
 }  // namespace
 
-rtx_tuning g_tuning = {1u << 23, 8192, 1024, 12, 2, 1, 24, 2048, 8, 2, 16, 1, 1};
+rtx_tuning g_tuning = {1u << 23, 8192, 1024, 12, 2, 1, 24, 2048, 8, 2, 16, 1, 1, 18};
 
 }  // namespace rtx
 

@@ -22,6 +22,7 @@ struct TraceArgs {
     uint32_t root_ref;
     uint32_t finite;    // every node box coordinate is finite
     uint32_t wait;      // while-while: lanes still at internal nodes the leaf phase need not wait for
+    uint32_t refill;    // persistent trace kernel: idle lanes of a wave that trigger a refill
 };
 
 extern rtx_tuning g_tuning;
@@ -58,6 +59,7 @@ inline TraceArgs make_trace_args(const rtx_bvh* b) {
     A.root_ref = b->root_ref;
     A.finite = b->boxes_finite ? 1u : 0u;
     A.wait = g_tuning.wait_lanes;
+    A.refill = g_tuning.refill_lanes;
     return A;
 }
 

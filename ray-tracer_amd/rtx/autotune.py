@@ -20,6 +20,7 @@ SPACE = {
     "trace_loop": [0, 1, 2],
     "ray_sort": [0, 1],
     "wait_lanes": [0, 4, 8, 16, 24, 32],
+    "refill_lanes": [12, 18, 24, 32, 40],
     "trace_grid_blocks": [1024, 2048, 4096, 8192],
     "leaf_coop": [0, 1],
     # renderer
@@ -31,7 +32,8 @@ SPACE = {
     "shade_threads": [64, 128, 256, 512, 1024],
     "batch_paths": [1 << 20, 1 << 21, 1 << 22, 1 << 23],
 }
-TRACE_KEYS = ["trace_loop", "leaf_coop", "trace_lds_depth", "ray_sort", "wait_lanes", "trace_grid_blocks"]
+TRACE_KEYS = ["trace_loop", "leaf_coop", "trace_lds_depth", "ray_sort", "wait_lanes", "trace_grid_blocks",
+              "refill_lanes"]
 # loop structure first: the best block and batch sizes depend on which kernel runs
 RENDER_KEYS = ["render_loop", "render_leaf_coop", "grid_blocks", "shade_threads", "batch_paths", "render_lds_depth",
                "render_wait_lanes"]

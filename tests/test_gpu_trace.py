@@ -216,13 +216,15 @@ def test_persistent_variant_counts_and_any_hit(suite):
     """The persistent dynamic-fetch kernel serves closest hit, counts, normals and any-hit
     exactly while lanes take many rays each (grid forced to 16 blocks). The camera rays
     mostly miss the root box, so whole groups of lanes go idle the moment they start a
-    ray; the kernel's refill and exit must stay wave-uniform for them."""
+    ray; the kernel's refill and exit must stay wave-uniform for them, for refill
+    thresholds from 1 idle lane to all 64."""
     saved = rtx.get_tuning()
     try:
-        for name, sort, coop in (("rand20000", 0, 0), ("straddle", 0, 0), ("cornell", 0, 0), ("rand20000", 1, 0),
-                                 ("rand20000", 0, 1), ("straddle", 1, 1), ("cornell", 0, 1)):
+        for name, sort, coop, refill in (("rand20000", 0, 0, 18), ("straddle", 0, 0, 1), ("cornell", 0, 0, 64),
+                                         ("rand20000", 1, 0, 32), ("rand20000", 0, 1, 64), ("straddle", 1, 1, 1),
+                                         ("cornell", 0, 1, 18), ("cornell", 1, 1, 1)):
             rtx.set_tuning(trace_loop=2, trace_lds_depth=8, trace_grid_blocks=16, ray_sort=sort, wait_lanes=8 * sort,
-                           leaf_coop=coop)
+                           leaf_coop=coop, refill_lanes=refill)
             v, t = suite[name]
             g = rtx.build(v, t)
             o = O.build(v, t)
@@ -232,7 +234,7 @@ def test_persistent_variant_counts_and_any_hit(suite):
             cam = S.camera_lookat(tuple(c - np.array([0.0, 0.0, 6.0 * span])), tuple(c), 60.0)
             camrays = S.pinhole_rays(cam, 256, 256)
             for kind, rays in (("scattered", _rays_for(v, 30001, seed=11)), ("camera", camrays)):
-                print(f"[persistent] {name} sort={sort} {kind}", flush=True)
+                print(f"[persistent] {name} sort={sort} coop={coop} refill={refill} {kind}", flush=True)
                 h, nrm, occ, cnt = _device_trace(g, rays)
                 ref = O.trace(o, rays, counts=True, normals=True, nthreads=8)
                 if kind == "camera":
