@@ -32,8 +32,9 @@ SPACE = {
     "shade_threads": [64, 128, 256, 512, 1024],
     "batch_paths": [1 << 20, 1 << 21, 1 << 22, 1 << 23],
 }
-TRACE_KEYS = ["trace_loop", "leaf_coop", "trace_lds_depth", "ray_sort", "wait_lanes", "trace_grid_blocks",
-              "refill_lanes"]
+# the refill threshold before stack depth and leaf-phase wait: their best values depend on it
+TRACE_KEYS = ["trace_loop", "leaf_coop", "refill_lanes", "trace_lds_depth", "ray_sort", "wait_lanes",
+              "trace_grid_blocks"]
 # loop structure first: the best block and batch sizes depend on which kernel runs
 RENDER_KEYS = ["render_loop", "render_leaf_coop", "grid_blocks", "shade_threads", "batch_paths", "render_lds_depth",
                "render_wait_lanes"]
