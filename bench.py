@@ -200,7 +200,7 @@ def main():
         batched = bench_batched(stream, args.batched_scenes, tune=not args.no_autotune)
         rtx.set_tuning(**before)
         render = {"differentiable": bench_differentiable(stream), "batched": batched,
-                  "cornell": bench_cornell(stream, cpu=not args.no_cpu)}
+                  "cornell": bench_cornell(stream, cpu=not args.no_cpu, tune=not args.no_autotune)}
         if not args.no_inverse:
             render["inverse_terrain"] = bench_inverse(stream, steps=args.inverse_steps)
 
@@ -354,10 +354,12 @@ def bench_inverse(stream, steps=3):
             "texture_rmse": [rmse0, tex_rmse()]}
 
 
-def bench_cornell(stream, cpu=True, reps=3):
+def bench_cornell(stream, cpu=True, reps=3, tune=True):
     """Cornell box: open box (20 wall + 2 light + 1280 icosphere triangles), 256x256,
     16 spp, max_bounce 5, roulette from bounce 3; GPU render time, and the oracle's
-    render of the same image on the host at 1 thread and at all cores."""
+    render of the same image on the host at 1 thread and at all cores. The renderer's
+    launch shapes are autotuned for this small (1M-path) launch, in two rounds so that the
+    loop structure is revisited after the grid changes, and restored after it."""
     import torch
 
     import rtx
@@ -367,9 +369,21 @@ def bench_cornell(stream, cpu=True, reps=3):
     p = rtx.make_params(256, 256, 16, 5, 3, seed=1)
     a_t, e_t = torch.from_numpy(albedo).cuda(), torch.from_numpy(emission).cuda()
     img = torch.empty((256, 256, 3), device="cuda")
-    ms = _cuda_ms(lambda: rtx.render(g, cam, p, a_t, e_t, out=img, stream=stream), reps, stream)
+    saved = rtx.get_tuning()
+    tuning = saved
+    try:
+        if tune:
+            from rtx import autotune
+            tuning, _, _ = autotune.tune(lambda: (lambda: rtx.render(g, cam, p, a_t, e_t, out=img, stream=stream)),
+                                         keys=autotune.RENDER_KEYS, reps=2, rounds=2)
+            rtx.set_tuning(**tuning)
+        ms = _cuda_ms(lambda: rtx.render(g, cam, p, a_t, e_t, out=img, stream=stream), reps, stream)
+    finally:
+        rtx.set_tuning(**saved)
     out = {"workload": "256x256, 16 spp, max_bounce 5, roulette from bounce 3, 1302 triangles",
-           "paths": 256 * 256 * 16, "gpu_ms": round(ms, 3)}
+           "paths": 256 * 256 * 16, "gpu_ms": round(ms, 3),
+           "tuning": {k: tuning[k] for k in ("render_loop", "render_leaf_coop", "grid_blocks", "shade_threads",
+                                            "batch_paths", "render_lds_depth", "render_wait_lanes")}}
     if cpu:
         from oracle import oracle as O
         o = O.build(v, t)
