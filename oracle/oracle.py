@@ -42,31 +42,59 @@ _REF = None
 def lib():
     global _ORC
     if _ORC is None:
-        if not os.path.exists(ORACLE_SO):
-            raise RuntimeError(f"oracle not built: {ORACLE_SO} (run make -C oracle)")
-        L = C.CDLL(ORACLE_SO)
-        vp, u32, sz = C.c_void_p, C.c_uint32, C.c_size_t
-        L.orc_morton.argtypes = [vp, u32, vp, u32, vp, vp]
-        L.orc_sort.argtypes = [vp, u32, vp]
-        L.orc_build.argtypes = [vp, u32, vp, u32, C.POINTER(vp)]
-        L.orc_tree_free.argtypes = [vp]
-        for f in ("orc_tree_counts", "orc_tree_nodes", "orc_tree_tris", "orc_tree_prims",
-                  "orc_tree_sorted", "orc_tree_bfs_nodes"):
-            getattr(L, f).argtypes = [vp, vp]
-            getattr(L, f).restype = None
-        L.orc_trace.argtypes = [vp, u32, vp, u32, vp, u32, vp, sz, vp, vp, vp, C.c_int, C.c_int]
-        L.orc_sample.argtypes = [u32, u32, u32, u32]
-        L.orc_sample.restype = C.c_float
-        L.orc_sincos.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
-        L.orc_atan2.argtypes = [C.c_float, C.c_float]
-        L.orc_atan2.restype = C.c_float
-        L.orc_render.argtypes = [vp, u32, vp, vp, u32, vp, u32, vp, vp, C.POINTER(Camera),
-                                 C.POINTER(RenderParams), vp, C.c_int]
-        L.orc_render_loss_f64.argtypes = [vp, u32, vp, vp, u32, vp, u32, vp, vp, C.POINTER(Camera),
-                                          C.POINTER(RenderParams), vp, C.c_int]
-        L.orc_render_loss_f64.restype = C.c_double
-        _ORC = L
+        _ORC = _load(ORACLE_SO)
     return _ORC
+
+
+def mutant_path(k):
+    """Planted-mistake build k of the oracle (ORC_MUTANT in rtx_oracle.cpp)."""
+    return os.path.join(HERE, "_build", f"liboracle_mut{k}.so")
+
+
+class using:
+    """Context manager: route this module's oracle calls through another build of the
+    oracle library (the planted-mistake builds), restoring the real one afterwards."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _ORC
+        self.saved = lib()
+        _ORC = _load(self.path)
+        return _ORC
+
+    def __exit__(self, *exc):
+        global _ORC
+        _ORC = self.saved
+        return False
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
+    L = C.CDLL(path)
+    vp, u32, sz = C.c_void_p, C.c_uint32, C.c_size_t
+    L.orc_morton.argtypes = [vp, u32, vp, u32, vp, vp]
+    L.orc_sort.argtypes = [vp, u32, vp]
+    L.orc_build.argtypes = [vp, u32, vp, u32, C.POINTER(vp)]
+    L.orc_tree_free.argtypes = [vp]
+    for f in ("orc_tree_counts", "orc_tree_nodes", "orc_tree_tris", "orc_tree_prims",
+              "orc_tree_sorted", "orc_tree_bfs_nodes"):
+        getattr(L, f).argtypes = [vp, vp]
+        getattr(L, f).restype = None
+    L.orc_trace.argtypes = [vp, u32, vp, u32, vp, u32, vp, sz, vp, vp, vp, C.c_int, C.c_int]
+    L.orc_sample.argtypes = [u32, u32, u32, u32]
+    L.orc_sample.restype = C.c_float
+    L.orc_sincos.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_atan2.argtypes = [C.c_float, C.c_float]
+    L.orc_atan2.restype = C.c_float
+    L.orc_render.argtypes = [vp, u32, vp, vp, u32, vp, u32, vp, vp, C.POINTER(Camera),
+                             C.POINTER(RenderParams), vp, C.c_int]
+    L.orc_render_loss_f64.argtypes = [vp, u32, vp, vp, u32, vp, u32, vp, vp, C.POINTER(Camera),
+                                      C.POINTER(RenderParams), vp, C.c_int]
+    L.orc_render_loss_f64.restype = C.c_double
+    return L
 
 
 def ref_available():

@@ -745,18 +745,40 @@ inline RayF primary_ray(const orc_camera* c, int px, int py, int W, int H, float
     return RayF{{c->pos[0], c->pos[1], c->pos[2]}, d, 1e-3f, kInf};
 }
 
+// ORC_MUTANT (never set in liboracle.so): planted mistakes compiled into separate
+// test-only libraries, so the tests can show that their closed-form checks catch each one.
+//   1 uniform instead of cosine-weighted hemisphere   2 normal in the x slot of toWorld
+//   3 roulette without throughput /= p                4 roulette probability = channel mean
+//   5 tangent not normalized
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
+// sampleCosineHemisphere (main.cpp:123-131)
 inline V3 cos_hemisphere(float u1, float u2) {
     float r = sqrtf(u1);
     float theta = 6.28318548f * u2;
     float s, c;
     orc_sincos(theta, &s, &c);
+#if ORC_MUTANT == 1
+    r = sqrtf(std_max(0.f, 1.f - u1 * u1));
+    return {r * c, u1, r * s};
+#endif
     return {r * c, sqrtf(std_max(0.f, 1.f - u1)), r * s};
 }
 
+// toWorld (main.cpp:133-141)
 inline V3 to_world(V3 l, V3 n) {
     V3 up = fabsf(n.x) < 0.9f ? V3{1.f, 0.f, 0.f} : V3{0.f, 1.f, 0.f};
+#if ORC_MUTANT == 5
+    V3 t = cross(n, up);
+#else
     V3 t = normalize(cross(n, up));
+#endif
     V3 b = cross(n, t);
+#if ORC_MUTANT == 2
+    return add(add(scl(l.x, n), scl(l.y, t)), scl(l.z, b));
+#endif
     return add(add(scl(l.x, t), scl(l.y, n)), scl(l.z, b));
 }
 
@@ -856,12 +878,18 @@ Vec3T<R> path(const SceneRef& S, const orc_render_params* p, RayF ray, uint32_t 
         float u2 = orc_sample(p->seed, pixel, s, 3 + 3 * bounce);
         V3 wd = to_world(cos_hemisphere(u1, u2), h.n);
         tp = {tp.x * ax, tp.y * ay, tp.z * az};
+        // Russian roulette (main.cpp:188-193)
         if (bounce >= p->rr_bounce) {
             R pr = tp.x;
             if (pr < tp.y) pr = tp.y;
             if (pr < tp.z) pr = tp.z;
+#if ORC_MUTANT == 4
+            pr = (tp.x + tp.y + tp.z) / (R)3;
+#endif
             if ((R)orc_sample(p->seed, pixel, s, 4 + 3 * bounce) > pr) break;
+#if ORC_MUTANT != 3
             tp = {tp.x / pr, tp.y / pr, tp.z / pr};
+#endif
         }
         ray.o = add(ray.o, scl(h.t, ray.d));
         ray.d = wd;

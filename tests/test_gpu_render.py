@@ -287,3 +287,26 @@ def test_ibl_environment_matches_oracle():
     op = O.make_params(48, 40, 2, 4, 3, seed=4, env_mode=2, ibl=ibl)
     img, ref = _render_both(v, t, albedo, None, cam, p, op)
     _close(img, ref, "IBL environment")
+
+
+def test_random_walk_closed_forms_on_gpu():
+    """The GPU renderer's bounce sampling (cosine lobe, toWorld frame) and Russian roulette
+    (survival max(throughput), throughput /= p) against the closed forms of
+    tests/closed_forms.py; every image also equals the oracle's within the image tolerance."""
+    import torch
+    import closed_forms as CF
+
+    def both(v, t, albedo, emission, cam, W, H, spp, mb, rr, seed, env_mode=0, ibl=None):
+        g = rtx.build(v, t)
+        ib = None if ibl is None else torch.from_numpy(ibl).cuda()
+        p = rtx.make_params(W, H, spp, mb, rr, seed=seed, env_mode=env_mode, ibl=ib)
+        img = rtx.render(g, cam, p, _tensor(albedo), _tensor(emission)).cpu().numpy()
+        op = O.make_params(W, H, spp, mb, rr, seed=seed, env_mode=env_mode, ibl=ibl)
+        _close(img, O.render(O.build(v, t), cam, op, albedo, emission, nthreads=16), f"closed-form scene mb={mb}")
+        return img
+
+    results = CF.all_checks(both)
+    for r in results:
+        print("[closed form]", r, flush=True)
+    bad = [r for r in results if not r.ok]
+    assert not bad, bad
