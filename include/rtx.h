@@ -105,6 +105,16 @@ int rtx_debug_recip_check(uint64_t first, uint64_t count, uint64_t* mismatches, 
  * entry point n = 0 does nothing and the buffers may then be NULL. */
 int rtx_trace_closest(const rtx_bvh* bvh, const rtx_ray* rays_d, size_t n, rtx_hit* hits_d,
                       float* normals_d, void* stream);
+/* rtx_trace_closest without allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same kernels and results as rtx_trace_closest; the
+ * persistent kernel's ray counter and the ray sort's buffers live in a fixed file-scope
+ * device array (about 34 MB, held by each GPU the library runs on until the process ends)
+ * instead of on the handle.  n <= 2097152 (RTX_E_LIMIT above); the BVH's device must be
+ * current.  Every call on one GPU shares that array: calls that overlap in time on one
+ * GPU (other streams or threads) race and must be ordered by the caller. */
+int rtx_trace_closest_noalloc(const rtx_bvh* bvh, const rtx_ray* rays_d, size_t n, rtx_hit* hits_d,
+                              float* normals_d, void* stream);
 /* Any hit (occlusion): occluded_d[i] = 1 iff trace(rays[i]).hit.  No reference
  * counterpart (the reference only has closest hit); pinned by that identity. */
 int rtx_trace_any(const rtx_bvh* bvh, const rtx_ray* rays_d, size_t n, uint8_t* occluded_d, void* stream);

@@ -232,22 +232,30 @@ int check(const rtx_bvh* b, const void* rays, size_t n, const void* out, const c
 // While-while loop shape of the trace kernels: cooperative leaf phases when tuned on.
 uint32_t loop_shape() { return g_tuning.leaf_coop ? (uint32_t)kLoopCoop : (uint32_t)kLoopWhileWhile; }
 
+// Device scratch of one trace call: the persistent kernel's ray counter and the ray
+// sort's ping-pong buffers. Either the handle's (grow-only, ordered across calls with an
+// event) or caller-provided memory that needs no allocation (rtx_trace_closest_noalloc).
+struct TraceScratch {
+    uint32_t* work;
+    SortBuffers32 sort;
+};
+
 template <bool ANY, bool COUNT>
 int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float* normals, uint8_t* occl,
-           uint32_t* counts, hipStream_t s) {
+           uint32_t* counts, hipStream_t s, const TraceScratch* ext = nullptr) {
     if (n == 0) return RTX_OK;
     uint32_t blocks = (uint32_t)((n + kTraceThreads - 1) / kTraceThreads);
     const TraceArgs A = make_trace_args(b);
     const uint32_t* perm = nullptr;
     rtx_bvh* mb = const_cast<rtx_bvh*>(b);
     const bool sort = g_tuning.ray_sort && n >= kSortMinRays;
-    // The ray counter and the sort buffers belong to the handle: a call that uses them
-    // waits (on its own stream) for the previous such call, whatever stream that used.
+    // The handle's ray counter and sort buffers: a call that uses them waits (on its own
+    // stream) for the previous such call, whatever stream that used.
     std::unique_lock<std::mutex> lk(mb->scratch_mu, std::defer_lock);
     // the persistent grid pays off only with rays for many waves; small calls (the per-ray
     // bvh::trace shim) take the while-while kernel, which gives the same results
     const bool persist = g_tuning.trace_loop == 2 && n >= kPersistMinRays;
-    const bool scratch = sort || persist;
+    const bool scratch = (sort || persist) && !ext;
     if (scratch) {
         lk.lock();
         if (!mb->scratch_ev) RTX_HIP(hipEventCreateWithFlags(&mb->scratch_ev, hipEventDisableTiming));
@@ -259,10 +267,16 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
         mb->scratch_used = true;
         return RTX_OK;
     };
+    uint32_t* work = ext ? ext->work : b->work;
     if (sort) {
-        if (!reserve_sort(mb, (uint32_t)n)) return fail(RTX_E_HIP, "device allocation failed for ray sorting");
-        SortBuffers32 S{{mb->rs_keys[0], mb->rs_keys[1]}, {mb->rs_vals[0], mb->rs_vals[1]}, mb->rs_hist,
-                        mb->rs_part, 0};
+        SortBuffers32 S;
+        if (ext) {
+            S = ext->sort;
+        } else {
+            if (!reserve_sort(mb, (uint32_t)n)) return fail(RTX_E_HIP, "device allocation failed for ray sorting");
+            S = SortBuffers32{{mb->rs_keys[0], mb->rs_keys[1]}, {mb->rs_vals[0], mb->rs_vals[1]}, mb->rs_hist,
+                              mb->rs_part, 0};
+        }
         k_ray_keys<<<blocks, kTraceThreads, 0, s>>>(A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, S.keys[0],
                                                     S.vals[0]);
         // stable sort on the top 24 key bits (8 per axis): the lowest pass bought no
@@ -272,18 +286,18 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
     }
     if (persist) {
         const uint32_t pb = std::min<uint32_t>(blocks, g_tuning.trace_grid_blocks);
-        k_zero_work<<<1, 64, 0, s>>>(b->work);
-        const int pk = prof_start(s);
+        k_zero_work<<<1, 64, 0, s>>>(work);
+        const int pk = ext ? -1 : prof_start(s);
         with_trace_variant(g_tuning.trace_lds_depth, loop_shape(), [&](auto D, auto L) {
             k_trace_persist<ANY, COUNT, decltype(D)::value, decltype(L)::value><<<pb, kTraceThreads, 0, s>>>(
                 A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
-                counts, b->work, perm);
+                counts, work, perm);
         });
         prof_stop(pk, s);
         RTX_HIP(hipGetLastError());
         return release();
     }
-    const int pk = prof_start(s);
+    const int pk = ext ? -1 : prof_start(s);
     with_trace_variant(g_tuning.trace_lds_depth, g_tuning.trace_loop == 0 ? (uint32_t)kLoopSingle : loop_shape(),
                        [&](auto D, auto L) {
         k_trace<ANY, COUNT, decltype(D)::value, decltype(L)::value><<<blocks, kTraceThreads, 0, s>>>(
@@ -337,6 +351,64 @@ extern "C" int rtx_trace_closest(const rtx_bvh* b, const rtx_ray* rays, size_t n
     int rc = check(b, rays, n, hits, "rtx_trace_closest()");
     if (rc) return rc;
     return launch<false, false>(b, rays, n, hits, normals, nullptr, nullptr, (hipStream_t)stream);
+}
+
+namespace rtx {
+namespace {
+// Scratch of rtx_trace_closest_noalloc: a file-scope device array, so the entry point needs
+// no allocation. Each GPU the library runs on holds its own copy until the process ends.
+// Layout in uint32 words: [0, 64) the persistent kernel's ray counter, then the sort's
+// keys[0], keys[1], vals[0], vals[1] (kNoAllocMaxRays each), its tile histogram and its
+// scan partials.
+constexpr uint32_t kNoAllocMaxRays = 1u << 21;
+constexpr size_t kNoAllocHist = 256ull * ((kNoAllocMaxRays + 4095) / 4096);
+constexpr size_t kNoAllocPart = 64;
+constexpr size_t kNoAllocWords = 64 + 4ull * kNoAllocMaxRays + kNoAllocHist + kNoAllocPart;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+// Device address of `scratch` per device, resolved once per device with that device
+// current. In host code `scratch` names a host-side stand-in: only these addresses may be
+// passed to kernels or copies.
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+uint32_t* g_noalloc_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(int device, uint32_t** out) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+        void* p = nullptr;
+        RTX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_noalloc_ptr[device] = static_cast<uint32_t*>(p);
+    }
+    *out = g_noalloc_ptr[device];
+    return RTX_OK;
+}
+}  // namespace
+}  // namespace rtx
+
+extern "C" int rtx_trace_closest_noalloc(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits,
+                                         float* normals, void* stream) {
+    int rc = check(b, rays, n, hits, "rtx_trace_closest_noalloc()");
+    if (rc) return rc;
+    if (n > kNoAllocMaxRays)
+        return fail(RTX_E_LIMIT, "more than 2097152 rays on rtx_trace_closest_noalloc() (its fixed scratch size)");
+    if (n == 0) return RTX_OK;
+    int dev = -1;
+    RTX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(RTX_E_LIMIT, "device index out of range on rtx_trace_closest_noalloc()");
+    if (dev != b->device)
+        return fail(RTX_E_STATE, "rtx_trace_closest_noalloc() called with another device current than the BVH's");
+    uint32_t* base = nullptr;
+    rc = noalloc_scratch(dev, &base);
+    if (rc) return rc;
+    const size_t m = kNoAllocMaxRays;
+    TraceScratch S;
+    S.work = base;
+    S.sort = SortBuffers32{{base + 64, base + 64 + m}, {base + 64 + 2 * m, base + 64 + 3 * m}, base + 64 + 4 * m,
+                           base + 64 + 4 * m + kNoAllocHist, 0};
+    if (sort_hist_entries((uint32_t)n) > kNoAllocHist || sort_part_entries((uint32_t)n) > kNoAllocPart)
+        return fail(RTX_E_LIMIT, "ray sort scratch exceeds its fixed size on rtx_trace_closest_noalloc()");
+    return launch<false, false>(b, rays, n, hits, normals, nullptr, nullptr, (hipStream_t)stream, &S);
 }
 
 extern "C" int rtx_trace_any(const rtx_bvh* b, const rtx_ray* rays, size_t n, uint8_t* occluded, void* stream) {

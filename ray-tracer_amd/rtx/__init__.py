@@ -21,7 +21,7 @@ NODE48 = np.dtype([("bmin", "<f4", 4), ("bmax", "<f4", 4), ("is_leaf", "u1"), ("
 # Every symbol include/rtx.h declares.
 EXPORTS = ["rtx_last_error", "rtx_device_count", "rtx_build", "rtx_build_device", "rtx_bvh_free",
            "rtx_bvh_get_info", "rtx_bvh_export", "rtx_bvh_morton", "rtx_bvh_export_bfs", "rtx_debug_clip_slab", "rtx_debug_recip_check", "rtx_trace_closest",
-           "rtx_trace_any", "rtx_trace_counts", "rtx_trace_closest_host", "rtx_save", "rtx_load",
+           "rtx_trace_closest_noalloc", "rtx_trace_any", "rtx_trace_counts", "rtx_trace_closest_host", "rtx_save", "rtx_load",
            "rtx_scene_load", "rtx_free_host", "rtx_bake_scene", "rtx_geo_sphere", "rtx_geo_box", "rtx_render",
            "rtx_render_adjoint", "rtx_render_batched", "rtx_batch_create", "rtx_batch_launch", "rtx_batch_free",
            "rtx_set_tuning", "rtx_get_tuning", "rtx_hdr_write", "rtx_hdr_read", "rtx_profile_begin",
@@ -90,6 +90,7 @@ def lib():
         L.rtx_debug_recip_check.argtypes = [C.c_uint64, C.c_uint64, vp, vp]
         L.rtx_bvh_export_bfs.argtypes = [vp, vp, C.POINTER(u32)]
         L.rtx_trace_closest.argtypes = [vp, vp, sz, vp, vp, vp]
+        L.rtx_trace_closest_noalloc.argtypes = [vp, vp, sz, vp, vp, vp]
         L.rtx_trace_any.argtypes = [vp, vp, sz, vp, vp]
         L.rtx_trace_counts.argtypes = [vp, vp, sz, vp, vp]
         L.rtx_trace_closest_host.argtypes = [vp, vp, sz, vp, vp]
