@@ -126,6 +126,63 @@ def icosphere(subdiv, radius=1.0, center=(0, 0, 0)):
     return out, tris.astype(np.uint32)
 
 
+_ICO_T = (1.0 + 5 ** 0.5) / 2.0
+_ICO_V = [(-1, _ICO_T, 0), (1, _ICO_T, 0), (-1, -_ICO_T, 0), (1, -_ICO_T, 0), (0, -1, _ICO_T), (0, 1, _ICO_T),
+          (0, -1, -_ICO_T), (0, 1, -_ICO_T), (_ICO_T, 0, -1), (_ICO_T, 0, 1), (-_ICO_T, 0, -1), (-_ICO_T, 0, 1)]
+_ICO_F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+          (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+          (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+
+
+def geodesic_sphere(freq, radius=1.0, center=(0, 0, 0), displace=0.0, seed=0):
+    """Class-I geodesic icosphere of frequency `freq`: every icosahedron face is cut into a
+    freq x freq triangular grid (20 freq^2 triangles, 10 freq^2 + 2 shared vertices) projected
+    onto the sphere, optionally displaced along the radius by displace * fBm. Shading normals
+    are the undisplaced radial ones; triangles face outwards under the reference's back-face
+    test (see quad())."""
+    V = np.array(_ICO_V, np.float64)
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    F = np.array(_ICO_F, np.int64)
+    n = int(freq)
+    ii, jj = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    m = (ii + jj) <= n
+    i, j = ii[m], jj[m]                               # grid points of one face, weights (n-i-j, i, j)
+    local = -np.ones((n + 1, n + 1), np.int64)
+    local[i, j] = np.arange(len(i))
+    # A point is the integer combination sum_k w_k V_k over the face's corners; the same
+    # point on a shared edge or corner has the same (vertex, weight) pairs from every face.
+    w = np.stack([n - i - j, i, j], 1)                # (P, 3)
+    keys, pos = [], []
+    for a, b, c in F:
+        vid = np.array([a, b, c])[None, :].repeat(len(i), 0)
+        vid = np.where(w > 0, vid, 99)                # corners with weight 0 do not count
+        order = np.argsort(vid, 1, kind="stable")
+        keys.append(np.concatenate([np.take_along_axis(vid, order, 1), np.take_along_axis(w, order, 1)], 1))
+        pos.append((w[:, 0:1] * V[a] + w[:, 1:2] * V[b] + w[:, 2:3] * V[c]) / n)
+    keys = np.concatenate(keys)
+    pos = np.concatenate(pos)
+    _, first, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
+    inv = inv.reshape(-1)
+    P = pos[first]
+    P /= np.linalg.norm(P, axis=1, keepdims=True)
+    # triangles of one face grid: up (i,j)(i+1,j)(i,j+1) and down (i+1,j)(i+1,j+1)(i,j+1)
+    up = (i + j) <= n - 1
+    dn = (i + j) <= n - 2
+    t_up = np.stack([local[i[up], j[up]], local[i[up] + 1, j[up]], local[i[up], j[up] + 1]], 1)
+    t_dn = np.stack([local[i[dn] + 1, j[dn]], local[i[dn] + 1, j[dn] + 1], local[i[dn], j[dn] + 1]], 1)
+    tl = np.concatenate([t_up, t_dn])
+    tris = np.concatenate([inv[tl + f * len(i)] for f in range(len(F))])
+    r = radius * (1.0 + displace * fbm(P, seed=seed)) if displace else np.full(len(P), radius)
+    verts = np.zeros((len(P), 6), F32)
+    verts[:, :3] = P * r[:, None] + np.asarray(center)
+    verts[:, 3:] = P
+    e1 = P[tris[:, 1]] - P[tris[:, 0]]
+    e2 = P[tris[:, 2]] - P[tris[:, 0]]
+    outward = (np.cross(e1, e2) * P[tris].mean(1)).sum(-1) > 0
+    tris[outward] = tris[outward][:, [0, 2, 1]]
+    return verts, tris.astype(np.uint32)
+
+
 def merge(parts):
     """Concatenate (verts6, tris) meshes, offsetting indices."""
     vs, ts, off = [], [], 0
@@ -152,9 +209,10 @@ def quad(p0, p1, p2, p3, facing):
     return v, np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
 
 
-def cornell_box(seed=0, sphere_subdiv=3, sphere_radius=0.3, light_radiance=15.0, albedo_lo=0.2, albedo_hi=0.8):
+def cornell_box(seed=0, sphere_subdiv=3, sphere_radius=0.3, light_radiance=15.0, albedo_lo=0.2, albedo_hi=0.8,
+                sphere_center=(0.0, 0.0, 0.0)):
     """Open-front Cornell-style box in [-1,1]^3: 10 wall quads (20 triangles, inward
-    facing), one ceiling light quad (2 triangles) and a centred icosphere.
+    facing), one ceiling light quad (2 triangles) and an icosphere, centred by default.
     Returns verts6, tris, albedo (3 per triangle), emission (3 per triangle), camera."""
     parts = []
     # floor y=-1, ceiling y=1, back z=1, left x=-1, right x=1, each as 2 quads (split in x or z/y)
@@ -169,7 +227,7 @@ def cornell_box(seed=0, sphere_subdiv=3, sphere_radius=0.3, light_radiance=15.0,
     # ceiling light slightly below the ceiling, facing down
     parts.append(quad((-0.25, 0.99, -0.25), (0.25, 0.99, -0.25), (0.25, 0.99, 0.25), (-0.25, 0.99, 0.25),
                       (0, -1, 0)))
-    sv, st = icosphere(sphere_subdiv, sphere_radius, (0, -0.2, 0.1))
+    sv, st = icosphere(sphere_subdiv, sphere_radius, sphere_center)
     parts.append((sv, st))
     verts, tris = merge(parts)
     nt = len(tris)
@@ -259,18 +317,19 @@ def diff_scene(seed=3, rings=33, sectors=64, displace=0.08, albedo_lo=0.1, albed
 
 
 def batched_scenes(n=64, seed=2, lo=1000, hi=500_000, displace=0.05):
-    """Scenes of uneven size: fBm-displaced spheres whose triangle counts follow a
-    lognormal clamped to [lo, hi]; resolutions from {128, 256, 512}^2; spp uniform in
-    8..64; max_bounce 6. Returns a list of dicts (verts, tris, cam, width, height, spp)."""
+    """Scenes of uneven size: fBm-displaced geodesic icospheres whose triangle counts
+    20 freq^2 follow a lognormal clamped to [lo, hi] (freq 8..158: 1,280 to 499,280
+    triangles); resolutions from {128, 256, 512}^2; spp uniform in 8..64; max_bounce 6.
+    Returns a list of dicts (verts, tris, cam, width, height, spp, max_bounce)."""
     rng = np.random.Generator(np.random.PCG64(seed))
+    fmin, fmax = math.ceil(math.sqrt(lo / 20.0)), math.floor(math.sqrt(hi / 20.0))
     out = []
     for i in range(n):
         ntri = int(np.clip(np.exp(rng.normal(math.log(20000), 1.2)), lo, hi))
-        rings = max(3, int(round(math.sqrt(ntri / 4.0))))
-        sectors = 2 * rings
+        freq = int(np.clip(round(math.sqrt(ntri / 20.0)), fmin, fmax))
         res = int(rng.choice([128, 256, 512]))
         spp = int(rng.integers(8, 65))
-        v, t = uv_sphere(rings, sectors, 1.0, (0, 0, 0), displace, seed=seed * 1000 + i)
+        v, t = geodesic_sphere(freq, 1.0, (0, 0, 0), displace, seed=seed * 1000 + i)
         cam = camera_lookat((0.3 * math.sin(i), 0.4, -3.2), (0, 0, 0), 45.0)
         out.append(dict(verts=v, tris=t, cam=cam, width=res, height=res, spp=spp, max_bounce=6))
     return out
