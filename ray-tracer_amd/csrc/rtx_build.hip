@@ -1462,6 +1462,14 @@ void free_products(rtx_bvh* b) {
         if (p) (void)hipFree(p);
     if (b->scratch_ev) (void)hipEventDestroy(b->scratch_ev);
     b->scratch_ev = nullptr;
+    void* pins[] = {b->pin_rays, b->pin_hits, b->pin_nrm};
+    for (void* p : pins)
+        if (p) (void)hipHostFree(p);
+    b->pin_rays = b->pin_hits = nullptr;
+    b->pin_nrm = nullptr;
+    b->pin_cap = 0;
+    if (b->host_stream) (void)hipStreamDestroy(b->host_stream);
+    b->host_stream = nullptr;
 }
 
 }  // namespace

@@ -2,7 +2,9 @@
 // front-end with the reference's sphere/box generators.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <exception>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -134,6 +136,9 @@ int init_geos(Mesh& m) {
 int load_ply(const std::string& path, const f3& t, float scale, const Q& q, Mesh& m) {
     std::ifstream f(path);
     if (!f.is_open()) return fail(RTX_E_IO, "failed to open mesh " + path + " on loadMesh()");
+    f.seekg(0, std::ios::end);
+    const uint64_t fsize = (uint64_t)std::max<std::streamoff>(0, (std::streamoff)f.tellg());
+    f.seekg(0, std::ios::beg);
     std::string line, tok;
     std::getline(f, line);
     if (line.rfind("ply", 0) != 0) return fail(RTX_E_IO, "not a PLY file: " + path + " on loadMesh()");
@@ -169,6 +174,10 @@ int load_ply(const std::string& path, const f3& t, float scale, const Q& q, Mesh
         else if (n == "nx") inx = i; else if (n == "ny") iny = i; else if (n == "nz") inz = i;
     }
     if (ix < 0 || iy < 0 || iz < 0) return fail(RTX_E_IO, "PLY without x/y/z: " + path + " on loadMesh()");
+    // every value takes at least two bytes ("0 "): counts the file cannot hold are refused
+    // before anything is sized from them
+    if (nvert > fsize / (2 * vprops.size()) || nface > fsize / 2 || nvert > 0xFFFFFFFFull)
+        return fail(RTX_E_IO, "PLY element counts exceed the file size: " + path + " on loadMesh()");
     const bool has_n = inx >= 0 && iny >= 0 && inz >= 0;
     // rotation matrix of toMat4 (src/math/quat.hpp:44-59)
     const float xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z, xy = q.x * q.y, xz = q.x * q.z, yz = q.y * q.z;
@@ -193,6 +202,7 @@ int load_ply(const std::string& path, const f3& t, float scale, const Q& q, Mesh
     for (uint64_t i = 0; i < nface; ++i) {
         uint32_t c = 0;
         if (!(f >> c)) return fail(RTX_E_IO, "truncated PLY face list: " + path + " on loadMesh()");
+        if (c > 65536 || c > fsize / 2) return fail(RTX_E_IO, "PLY face with too many vertices: " + path + " on loadMesh()");
         std::vector<uint32_t> idx(c);
         for (uint32_t k = 0; k < c; ++k) {
             if (!(f >> idx[k])) return fail(RTX_E_IO, "truncated PLY face list: " + path + " on loadMesh()");
@@ -401,7 +411,12 @@ extern "C" int rtx_load(const char* path, rtx_bvh** outp) {
 extern "C" int rtx_scene_load(const char* scn, float** verts6, uint32_t* nv, uint32_t** tris, uint32_t* nt) {
     if (!scn || !verts6 || !nv || !tris || !nt) return fail(RTX_E_ARG, "null argument on rtx_scene_load()");
     Mesh m;
-    int rc = read_scene(scn, m);
+    int rc;
+    try {   // no exception crosses the C ABI: a scene too large for host memory is an error
+        rc = read_scene(scn, m);
+    } catch (const std::exception& e) {
+        return fail(RTX_E_IO, std::string("scene could not be read (") + e.what() + ") on rtx_scene_load()");
+    }
     if (rc) return rc;
     rc = init_geos(m);
     if (rc) return rc;

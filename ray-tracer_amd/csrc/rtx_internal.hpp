@@ -71,6 +71,13 @@ struct rtx_bvh {
     void* host_hits = nullptr;
     float* host_nrm = nullptr;
     size_t host_cap = 0;
+    // small rtx_trace_closest_host calls: pinned host buffers (hipHostMalloc) that the
+    // traversal kernel reads and writes directly, and the stream those calls run on
+    void* pin_rays = nullptr;
+    void* pin_hits = nullptr;
+    float* pin_nrm = nullptr;
+    size_t pin_cap = 0;
+    hipStream_t host_stream = nullptr;
     // diagnostics
     uint64_t* codes = nullptr;        // Morton code per input triangle
     uint32_t* sorted = nullptr;       // s_sortedTris[0..nt)

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -176,6 +177,7 @@ __global__ void k_ray_keys(TraceArgs A, const Ray32* __restrict__ rays, uint32_t
 
 
 constexpr uint32_t kSortMinRays = 1u << 16;
+constexpr size_t kPinnedMaxRays = 4096;   // rtx_trace_closest_host calls up to this size use pinned buffers
 constexpr uint32_t kPersistMinRays = 1u << 12;
 
 // Event pairs of rtx_profile_begin/end.
@@ -433,10 +435,35 @@ extern "C" int rtx_trace_closest_host(const rtx_bvh* b, const rtx_ray* rays, siz
     int rc = check(b, rays, n, hits, "rtx_trace_closest_host()");
     if (rc) return rc;
     if (n == 0) return RTX_OK;
-    // Device staging stays on the handle (grow-only), so per-ray callers such as the
-    // bvh::trace shim pay two small copies and a launch, not three allocations.
     rtx_bvh* mb = const_cast<rtx_bvh*>(b);
     std::lock_guard<std::mutex> lk(mb->host_mu);
+    if (!mb->host_stream) RTX_HIP(hipStreamCreateWithFlags(&mb->host_stream, hipStreamNonBlocking));
+    hipStream_t st = mb->host_stream;
+    // Small calls (the per-ray bvh::trace shim): the kernel reads the rays from and writes
+    // the hits to pinned host memory kept on the handle, on the handle's own stream, so a
+    // call is a host memcpy, one launch and one stream synchronization, with no DMA copies.
+    if (n <= kPinnedMaxRays) {
+        if (!mb->pin_rays) {
+            const size_t cap = kPinnedMaxRays;
+            RTX_HIP(hipHostMalloc(&mb->pin_rays, sizeof(rtx_ray) * cap, hipHostMallocDefault));
+            RTX_HIP(hipHostMalloc(&mb->pin_hits, sizeof(rtx_hit) * cap, hipHostMallocDefault));
+            RTX_HIP(hipHostMalloc((void**)&mb->pin_nrm, 12 * cap, hipHostMallocDefault));
+            mb->pin_cap = cap;
+        }
+        std::memcpy(mb->pin_rays, rays, sizeof(rtx_ray) * n);
+        void *dr = nullptr, *dh = nullptr, *dn = nullptr;
+        RTX_HIP(hipHostGetDevicePointer(&dr, mb->pin_rays, 0));
+        RTX_HIP(hipHostGetDevicePointer(&dh, mb->pin_hits, 0));
+        if (normals) RTX_HIP(hipHostGetDevicePointer(&dn, mb->pin_nrm, 0));
+        rc = launch<false, false>(b, static_cast<const rtx_ray*>(dr), n, static_cast<rtx_hit*>(dh),
+                                  static_cast<float*>(dn), nullptr, nullptr, st);
+        if (rc) return rc;
+        RTX_HIP(hipStreamSynchronize(st));
+        std::memcpy(hits, mb->pin_hits, sizeof(rtx_hit) * n);
+        if (normals) std::memcpy(normals, mb->pin_nrm, 12 * n);
+        return RTX_OK;
+    }
+    // Larger calls: device staging on the handle (grow-only) and copies on the stream.
     if (mb->host_cap < n) {
         (void)hipFree(mb->host_rays);
         (void)hipFree(mb->host_hits);
@@ -451,11 +478,12 @@ extern "C" int rtx_trace_closest_host(const rtx_bvh* b, const rtx_ray* rays, siz
     rtx_ray* dr = static_cast<rtx_ray*>(mb->host_rays);
     rtx_hit* dh = static_cast<rtx_hit*>(mb->host_hits);
     float* dn = normals ? mb->host_nrm : nullptr;
-    RTX_HIP(hipMemcpy(dr, rays, sizeof(rtx_ray) * n, hipMemcpyHostToDevice));
-    rc = launch<false, false>(b, dr, n, dh, dn, nullptr, nullptr, nullptr);
+    RTX_HIP(hipMemcpyAsync(dr, rays, sizeof(rtx_ray) * n, hipMemcpyHostToDevice, st));
+    rc = launch<false, false>(b, dr, n, dh, dn, nullptr, nullptr, st);
     if (rc) return rc;
-    RTX_HIP(hipMemcpy(hits, dh, sizeof(rtx_hit) * n, hipMemcpyDeviceToHost));
-    if (normals) RTX_HIP(hipMemcpy(normals, dn, 12 * n, hipMemcpyDeviceToHost));
+    RTX_HIP(hipMemcpyAsync(hits, dh, sizeof(rtx_hit) * n, hipMemcpyDeviceToHost, st));
+    if (normals) RTX_HIP(hipMemcpyAsync(normals, dn, 12 * n, hipMemcpyDeviceToHost, st));
+    RTX_HIP(hipStreamSynchronize(st));
     return RTX_OK;
 }
 

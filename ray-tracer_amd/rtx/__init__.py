@@ -135,6 +135,30 @@ def _dptr(t):
     return None if t is None else C.c_void_p(t.data_ptr())
 
 
+_DTYPES = {"f32": ("float32",), "u32": ("int32", "uint32"), "u8": ("uint8", "bool")}
+
+
+def _need(t, what, kind="f32", numel=None, optional=False):
+    """Check a tensor handed to the library: a contiguous CUDA tensor of the given 4-byte
+    float / 4-byte integer / byte kind holding at least `numel` elements. The kernels index
+    these buffers by ray, pixel or triangle count, so anything shorter would be read or
+    written out of bounds."""
+    if t is None:
+        if optional:
+            return None
+        raise ValueError(f"{what} is required")
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError(f"{what} must be a CUDA tensor")
+    if str(t.dtype).replace("torch.", "") not in _DTYPES[kind]:
+        raise ValueError(f"{what} must have dtype {' or '.join(_DTYPES[kind])}, not {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+    if numel is not None and t.numel() < numel:
+        raise ValueError(f"{what} holds {t.numel()} elements, needs {numel}")
+    return t
+
+
 def _stream_ptr(stream):
     if stream is None:
         return None
@@ -204,18 +228,34 @@ class BVH:
             out["normal"] = nrm
         return out
 
+    @staticmethod
+    def _rays(rays_d):
+        _need(rays_d, "rays_d")
+        if rays_d.dim() != 2 or rays_d.shape[1] != 8:
+            raise ValueError("rays_d must have shape (N, 8)")
+        return rays_d.shape[0]
+
     def trace_closest_device(self, rays_d, hits_d, normals_d=None, stream=None):
-        """Device-resident closest hit: rays_d (N, 8) float32, hits_d (N, 4) 16-byte records."""
-        n = rays_d.shape[0]
+        """Device-resident closest hit: rays_d (N, 8) float32, hits_d (N, 4) 16-byte records
+        (float32, or int32 to read the triangle index), normals_d optional (N, 3) float32."""
+        n = self._rays(rays_d)
+        if hits_d is None or str(hits_d.dtype) not in ("torch.float32", "torch.int32"):
+            raise ValueError("hits_d must be a float32 or int32 CUDA tensor")
+        _need(hits_d, "hits_d", "f32" if hits_d.dtype.is_floating_point else "u32", 4 * n)
+        _need(normals_d, "normals_d", "f32", 3 * n, optional=True)
         _check(lib().rtx_trace_closest(self._h, _dptr(rays_d), n, _dptr(hits_d), _dptr(normals_d),
                                        _stream_ptr(stream)), "rtx_trace_closest")
 
     def trace_any_device(self, rays_d, occl_d, stream=None):
-        _check(lib().rtx_trace_any(self._h, _dptr(rays_d), rays_d.shape[0], _dptr(occl_d), _stream_ptr(stream)),
+        n = self._rays(rays_d)
+        _need(occl_d, "occl_d", "u8", n)
+        _check(lib().rtx_trace_any(self._h, _dptr(rays_d), n, _dptr(occl_d), _stream_ptr(stream)),
                "rtx_trace_any")
 
     def trace_counts_device(self, rays_d, counts_d, stream=None):
-        _check(lib().rtx_trace_counts(self._h, _dptr(rays_d), rays_d.shape[0], _dptr(counts_d),
+        n = self._rays(rays_d)
+        _need(counts_d, "counts_d", "u32", 2 * n)
+        _check(lib().rtx_trace_counts(self._h, _dptr(rays_d), n, _dptr(counts_d),
                                       _stream_ptr(stream)), "rtx_trace_counts")
 
 
@@ -252,6 +292,10 @@ def build(verts6, tris):
 
 def build_device(verts_d, tris_d, stream=None):
     """GPU build from device tensors (verts (V,6) float32, tris (N,3) int32/uint32)."""
+    _need(verts_d, "verts_d")
+    _need(tris_d, "tris_d", "u32")
+    if verts_d.dim() != 2 or verts_d.shape[1] != 6 or tris_d.dim() != 2 or tris_d.shape[1] != 3:
+        raise ValueError("verts_d must have shape (V, 6) and tris_d (N, 3)")
     h = C.c_void_p()
     _check(lib().rtx_build_device(_dptr(verts_d), verts_d.shape[0], _dptr(tris_d), tris_d.shape[0],
                                   _stream_ptr(stream), C.byref(h)), "rtx_build_device")
@@ -323,6 +367,9 @@ def make_params(width, height, spp, max_bounce, rr_bounce=3, seed=0, env_mode=0,
     p.env_color[:] = [float(x) for x in env_color]
     p._keep = []
     if ibl is not None:
+        _need(ibl, "ibl")
+        if ibl.dim() != 3 or ibl.shape[2] != 3:
+            raise ValueError("ibl must have shape (h, w, 3)")
         p._keep.append(ibl)
         p.ibl_d = ibl.data_ptr()
         p.ibl_height, p.ibl_width = int(ibl.shape[0]), int(ibl.shape[1])
@@ -339,8 +386,10 @@ def make_params(width, height, spp, max_bounce, rr_bounce=3, seed=0, env_mode=0,
 def set_texture(p, uv, tex):
     """Point render params at texture albedo: uv (nverts, 2), tex (h, w, 3), contiguous
     float32 CUDA tensors (kept alive on the params)."""
-    if not (uv.is_contiguous() and tex.is_contiguous()):
-        raise ValueError("uv and tex must be contiguous")
+    _need(uv, "uv")
+    _need(tex, "tex")
+    if uv.dim() != 2 or uv.shape[1] != 2 or tex.dim() != 3 or tex.shape[2] != 3:
+        raise ValueError("uv must have shape (nverts, 2) and tex (h, w, 3)")
     p._tex_keep = (uv, tex)
     p.uv_d, p.tex_d = uv.data_ptr(), tex.data_ptr()
     p.tex_height, p.tex_width = int(tex.shape[0]), int(tex.shape[1])
@@ -398,12 +447,25 @@ def set_tuning(**kw):
     _check(lib().rtx_set_tuning(C.byref(t)), "rtx_set_tuning")
 
 
+def _check_scene(bvh, params, albedo, emission):
+    """Material and texture buffers against the tree they are indexed with."""
+    inf = bvh.info()
+    _need(albedo, "albedo", "f32", 3 * inf["input_tri_count"], optional=True)
+    _need(emission, "emission", "f32", 3 * inf["input_tri_count"], optional=True)
+    if params.tex_d:
+        uv, tex = params._tex_keep
+        if uv.shape[0] < inf["vert_count"]:
+            raise ValueError(f"uv holds {uv.shape[0]} vertices, the tree has {inf['vert_count']}")
+
+
 def render(bvh, cam, params, albedo=None, emission=None, out=None, stream=None):
     """Render into a (H, W, 3) float32 CUDA tensor. albedo/emission: (ntri, 3) CUDA tensors
     indexed by original triangle id (None = constant 0.8 / no emission)."""
     import torch
+    _check_scene(bvh, params, albedo, emission)
     if out is None:
         out = torch.empty((params.height, params.width, 3), dtype=torch.float32, device="cuda")
+    _need(out, "out", "f32", 3 * params.width * params.height)
     _check(lib().rtx_render(bvh.handle, C.byref(make_camera(cam)), C.byref(params), _dptr(albedo), _dptr(emission),
                             _dptr(out), _stream_ptr(stream)), "rtx_render")
     return out
@@ -415,12 +477,18 @@ def render_adjoint(bvh, cam, params, albedo, emission, d_image, grad_albedo=None
     gradient."""
     import torch
     textured = bool(params.tex_d)
+    _check_scene(bvh, params, albedo, emission)
+    d_image = d_image.contiguous()
+    _need(d_image, "d_image", "f32", 3 * params.width * params.height)
     if grad_albedo is None:
         grad_albedo = torch.zeros_like(params._tex_keep[1] if textured else albedo)
     if grad_emission is None and emission is not None and not textured:
         grad_emission = torch.zeros_like(emission)
+    _need(grad_albedo, "grad_albedo", "f32",
+          3 * (params.tex_width * params.tex_height if textured else bvh.info()["input_tri_count"]))
+    _need(grad_emission, "grad_emission", "f32", 3 * bvh.info()["input_tri_count"], optional=True)
     _check(lib().rtx_render_adjoint(bvh.handle, C.byref(make_camera(cam)), C.byref(params), _dptr(albedo),
-                                    _dptr(emission), _dptr(d_image.contiguous()), _dptr(grad_albedo),
+                                    _dptr(emission), _dptr(d_image), _dptr(grad_albedo),
                                     _dptr(grad_emission), _stream_ptr(stream)), "rtx_render_adjoint")
     return grad_albedo, grad_emission
 
@@ -431,6 +499,11 @@ class Batch:
 
     def __init__(self, bvhs, cams, params, albedos, emissions, images, mode):
         n = len(bvhs)
+        if not (len(cams) == len(params) == len(albedos) == len(emissions) == len(images) == n):
+            raise ValueError("one camera, params, albedo, emission and image per scene")
+        for b, p, a, e, im in zip(bvhs, params, albedos, emissions, images):
+            _check_scene(b, p, a, e)
+            _need(im, "image", "f32", 3 * p.width * p.height)
         self._keep = (bvhs, albedos, emissions, images, params)   # params hold IBL/texture tensors
         hs = (C.c_void_p * n)(*[b.handle for b in bvhs])
         cs = (Camera * n)(*[make_camera(c) for c in cams])

@@ -96,6 +96,67 @@ def clip_cases(n=6000, seed=11):
     print(f"{path}: {n} clip cases, {os.path.getsize(path)} bytes")
 
 
+BUNNY = "/root/reference/bunny.ply"
+# translate (0.5, -0.25, 1), uniform scale 6, rotation by a unit quaternion (x, y, z, w)
+BUNNY_SCN = "mesh bunny_head.ply 0.5 -0.25 1.0 6.0 q 0.1 0.2 0.3 0.9273618495495703\n"
+
+
+def bunny_head(keep=0.12):
+    """ASCII PLY text of the top `keep` fraction (by height) of the reference's bunny.ply:
+    every face whose three vertices lie above that height, with its vertices renumbered in
+    file order and the file's header layout (x y z confidence intensity, uchar/int lists)."""
+    with open(BUNNY) as f:
+        lines = f.read().split("\n")
+    end = lines.index("end_header")
+    nv = int(next(x for x in lines if x.startswith("element vertex")).split()[2])
+    nf = int(next(x for x in lines if x.startswith("element face")).split()[2])
+    vals = np.array([list(map(float, x.split())) for x in lines[end + 1:end + 1 + nv]])
+    faces = np.array([list(map(int, x.split())) for x in lines[end + 1 + nv:end + 1 + nv + nf]])
+    assert (faces[:, 0] == 3).all()
+    cut = np.quantile(vals[:, 1], 1.0 - keep)
+    f = faces[:, 1:]
+    f = f[(vals[f, 1] > cut).all(1)]
+    used = np.unique(f)
+    remap = -np.ones(nv, np.int64)
+    remap[used] = np.arange(len(used))
+    vtext = [lines[end + 1 + i] for i in used]
+    ftext = ["3 %d %d %d" % tuple(remap[x]) for x in f]
+    header = ["ply", "format ascii 1.0", f"element vertex {len(used)}", "property float x", "property float y",
+              "property float z", "property float confidence", "property float intensity",
+              f"element face {len(f)}", "property list uchar int vertex_indices", "end_header"]
+    return "\n".join(header + vtext + ftext) + "\n"
+
+
+def bunny_golden():
+    """reference_bunny_head.npz: a mesh read from PLY by rtx_scene_load (the .scn `mesh` line,
+    bvh_IO.cpp:195-231, in place of assimp's loadMesh), then built and traced by the compiled
+    reference. Holds the PLY and .scn text so the tests can load the same files."""
+    import tempfile
+    import rtx
+    ply = bunny_head()
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "bunny_head.ply"), "w") as f:
+            f.write(ply)
+        with open(os.path.join(d, "head.scn"), "w") as f:
+            f.write(BUNNY_SCN)
+        v, t = rtx.load_scene(os.path.join(d, "head.scn"))
+    r = O.ref_build(v, t)
+    out = {"ply": np.frombuffer(ply.encode(), np.uint8), "scn": np.frombuffer(BUNNY_SCN.encode(), np.uint8),
+           "verts": v, "tris": t, "codes": r.codes, "scene_box": r.scene_box, "sorted": r.sorted,
+           "nodes": r.nodes.view(np.uint8).reshape(-1, 48), "bfs_nodes": r.bfs_nodes.view(np.uint8).reshape(-1, 48),
+           "rtris": r.tris, "prims": r.prims,
+           "sizes": np.array([r.info["tindex_size"], r.info["sorted_size"]], np.uint32)}
+    for key, rs in (("rays", rays_for(v, 1000, seed=len(t))), ("edge", edge_rays(v, 1000, seed=5))):
+        h = O.ref_trace(r, rs, normals=True)
+        out[key] = rs
+        out[key + "_hit"] = np.stack([h["t"], h["u"], h["v"], h["tri"].view(np.float32)], 1)
+        out[key + "_normal"] = h["normal"]
+    path = os.path.join(HERE, "reference_bunny_head.npz")
+    np.savez_compressed(path, **out)
+    print(f"{path}: {len(t)} triangles, {len(r.nodes)} nodes, {os.path.getsize(path)} bytes")
+
+
 if __name__ == "__main__":
     main()
     clip_cases()
+    bunny_golden()

@@ -343,3 +343,60 @@ def test_empty_inputs():
     g.trace_any_device(rd, torch.zeros(0, dtype=torch.uint8, device="cuda"))
     g.trace_counts_device(rd, torch.zeros((0, 2), dtype=torch.int32, device="cuda"))
     torch.cuda.synchronize()
+
+
+def test_device_buffers_are_checked():
+    """The Python layer refuses buffers the kernels would index out of bounds or misread:
+    short outputs, wrong dtypes, non-contiguous views, host tensors, short material arrays."""
+    import torch
+    v, t = S.random_triangles(500, seed=4)
+    g = rtx.build(v, t)
+    rd = torch.from_numpy(_rays_for(v, 100, seed=1)).cuda()
+    bad = [lambda: g.trace_closest_device(rd, torch.zeros((99, 4), device="cuda")),
+           lambda: g.trace_closest_device(rd, torch.zeros((100, 4), dtype=torch.float64, device="cuda")),
+           lambda: g.trace_closest_device(rd, torch.zeros((4, 100), device="cuda").t()),
+           lambda: g.trace_closest_device(rd.cpu(), torch.zeros((100, 4), device="cuda")),
+           lambda: g.trace_closest_device(rd, torch.zeros((100, 4), device="cuda"), torch.zeros((100, 2), device="cuda")),
+           lambda: g.trace_any_device(rd, torch.zeros(50, dtype=torch.uint8, device="cuda")),
+           lambda: g.trace_counts_device(rd, torch.zeros((100, 2), device="cuda")),
+           lambda: g.trace_closest_device(rd[:, :4].contiguous(), torch.zeros((100, 4), device="cuda"))]
+    for k, f in enumerate(bad):
+        with pytest.raises(ValueError):
+            f()
+    cam = S.camera_lookat((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), 45.0)
+    p = rtx.make_params(8, 8, 1, 2, 3, seed=1)
+    with pytest.raises(ValueError):
+        rtx.render(g, cam, p, torch.zeros((499, 3), device="cuda"))
+    with pytest.raises(ValueError):
+        rtx.render(g, cam, p, torch.zeros((500, 3), dtype=torch.float64, device="cuda"))
+    with pytest.raises(ValueError):
+        rtx.render(g, cam, p, None, None, out=torch.zeros((8, 7, 3), device="cuda"))
+    img = rtx.render(g, cam, p, torch.full((500, 3), 0.5, device="cuda"))   # correct buffers pass
+    torch.cuda.synchronize()
+    assert img.shape == (8, 8, 3)
+
+
+def test_host_trace_small_and_large_calls():
+    """rtx_trace_closest_host: calls of up to 4096 rays run on pinned host buffers that the
+    kernel reads and writes directly, larger ones through device staging; both give the
+    device entry point's results, with and without normals, one ray at a time too."""
+    import torch
+    v, t = S.random_triangles(20000, seed=12)
+    g = rtx.build(v, t)
+    for n in (1, 7, 4096, 4097, 70000):
+        rays = _rays_for(v, n, seed=n)
+        h = g.trace(rays, normals=True)
+        rd = torch.from_numpy(rays).cuda()
+        hd = torch.zeros((n, 4), device="cuda")
+        nd = torch.zeros((n, 3), device="cuda")
+        g.trace_closest_device(rd, hd, nd)
+        torch.cuda.synchronize()
+        ref = hd.cpu().numpy()
+        got = np.stack([h["t"], h["u"], h["v"], h["tri"].view(np.float32)], 1)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), n
+        assert np.array_equal(h["normal"].view(np.uint32), nd.cpu().numpy().view(np.uint32)), n
+        h2 = g.trace(rays)
+        assert np.array_equal(h2["tri"], h["tri"]), n
+    one = _rays_for(v, 200, seed=3)
+    hits = [g.trace(one[i:i + 1])["tri"][0] for i in range(200)]
+    assert np.array_equal(np.array(hits, np.uint32), g.trace(one)["tri"])

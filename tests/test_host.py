@@ -92,6 +92,61 @@ def test_reference_bunny_loads(tmp_path):
     assert v.shape == (35947, 6) and t.shape == (69451, 3)
 
 
+def _golden_mesh_files(tmp_path):
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_bunny_head.npz"))
+    (tmp_path / "bunny_head.ply").write_bytes(g["ply"].tobytes())
+    (tmp_path / "head.scn").write_bytes(g["scn"].tobytes())
+    return g
+
+
+def test_ply_golden_mesh_loads_as_recorded(tmp_path):
+    """The PLY-derived mesh of reference_bunny_head.npz (the top of the reference's bunny.ply
+    under a translate * scale * rotate `mesh` line) loads into the recorded arrays bit for
+    bit, and agrees with an independent float64 model of the loader: positions
+    t + s R(q) p, smooth normals = R(q) times the sum of the unit face normals, renormalized."""
+    g = _golden_mesh_files(tmp_path)
+    v, t = rtx.load_scene(tmp_path / "head.scn")
+    assert np.array_equal(v.view(np.uint32), g["verts"].view(np.uint32)) and np.array_equal(t, g["tris"])
+    lines = g["ply"].tobytes().decode().split("\n")
+    end = lines.index("end_header")
+    nv = int(lines[2].split()[2])
+    P = np.array([list(map(float, x.split()))[:3] for x in lines[end + 1:end + 1 + nv]])
+    F = np.array([list(map(int, x.split()))[1:] for x in lines[end + 1 + nv:] if x.strip()])
+    assert np.array_equal(F, t)
+    tok = g["scn"].tobytes().decode().split()
+    tr, sc, q = np.array(tok[2:5], float), float(tok[5]), np.array(tok[7:11], float)
+    x, y, z, w = q
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                  [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                  [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+    np.testing.assert_allclose(v[:, :3], tr + sc * P @ R.T, rtol=0, atol=2e-6 * sc)
+    fn = np.cross(P[F[:, 1]] - P[F[:, 0]], P[F[:, 2]] - P[F[:, 0]])
+    fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+    N = np.zeros_like(P)
+    for k in range(3):
+        np.add.at(N, F[:, k], fn)
+    N = N @ R.T
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    np.testing.assert_allclose(v[:, 3:], N, rtol=0, atol=2e-5)
+
+
+def test_malformed_ply_is_refused(tmp_path):
+    """Header counts beyond what the file can hold, huge polygon sizes and non-numeric
+    bodies fail with an error (no allocation from unchecked counts)."""
+    head = "ply\nformat ascii 1.0\nelement vertex {nv}\nproperty float x\nproperty float y\nproperty float z\n" \
+           "element face {nf}\nproperty list uchar int vertex_indices\nend_header\n"
+    mesh = tmp_path / "m.scn"
+    mesh.write_text("mesh bad.ply 0 0 0 1 q 0 0 0 1\n")
+    cases = [head.format(nv=10 ** 15, nf=1) + "0 0 0\n",
+             head.format(nv=3, nf=10 ** 15) + "0 0 0\n1 0 0\n0 1 0\n3 0 1 2\n",
+             head.format(nv=3, nf=1) + "0 0 0\n1 0 0\n0 1 0\n4000000000 0 1 2\n",
+             head.format(nv=3, nf=1) + "0 0 0\n1 x 0\n0 1 0\n3 0 1 2\n"]
+    for body in cases:
+        (tmp_path / "bad.ply").write_text(body)
+        with pytest.raises(rtx.RtxError):
+            rtx.load_scene(mesh)
+
+
 def _write_bvh(path, version=1, truncate=0):
     nodes = np.zeros(1, O.NODE48)
     nodes["is_leaf"] = 1
