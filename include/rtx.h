@@ -237,7 +237,13 @@ void rtx_batch_free(rtx_batch* batch);
  *   trace_grid_blocks  blocks of the persistent trace kernel;
  *   refill_lanes       1..64: a wave of the persistent trace kernel fetches new rays once
  *                      at least this many of its lanes are idle (incoherent closest-hit
- *                      rays are fastest near 18, coherent and any-hit rays near 32).
+ *                      rays are fastest near 18, coherent and any-hit rays near 32);
+ *   trace_tree         0 = walk the binary tree in the reference's visiting order (hits equal
+ *                      the reference's, including which triangle wins an exact-t tie);
+ *                      1 = walk its 4-wide collapse (128-byte nodes of four child boxes, built
+ *                      with every tree): the same t for every ray, and at an exact-t tie
+ *                      between triangles of different leaves possibly another of the tied
+ *                      triangles.
  * Renderer (rtx_render, rtx_render_adjoint, rtx_batch_*):
  *   batch_paths        paths in flight per wavefront batch (>= 1024);
  *   grid_blocks        blocks of the wavefront trace and shade kernels;
@@ -269,6 +275,7 @@ typedef struct rtx_tuning {
     uint32_t leaf_coop;
     uint32_t render_leaf_coop;
     uint32_t refill_lanes;
+    uint32_t trace_tree;
 } rtx_tuning;
 int rtx_set_tuning(const rtx_tuning* t);
 int rtx_get_tuning(rtx_tuning* t);

@@ -1318,6 +1318,77 @@ __global__ void k_emit(Ctx C, uint32_t total, Dfs D, Out O) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wide traversal layout: binary nodes collapsed four children at a time, top down. A wide
+// node rooted at binary node i starts from i's two children and repeatedly replaces the
+// internal child of largest surface area by its two children (in place, so the slots keep
+// the binary tree's left-to-right order) until four slots are filled or only leaves are
+// left; its internal slots root the wide nodes of the next level. Wide node indices are the
+// binary internal indices of their roots, so no renumbering is needed. One launch per level:
+// wl[i] holds the level of the wide node rooted at i (0xFF: absorbed or not yet reached).
+// ---------------------------------------------------------------------------
+__global__ void k_wide_init(uint8_t* wl, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) wl[i] = i == 0 ? 0 : 0xFF;
+}
+
+__global__ void k_wide_level(const GNode* __restrict__ g, uint32_t n, uint32_t level, uint8_t* wl,
+                             WNode* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || wl[i] != level) return;
+    float lo[3][4], hi[3][4], area[4];
+    uint32_t rf[4];
+    auto put = [&](int s, const float* b, uint32_t r) {   // b: min xyz, max xyz
+        for (int a = 0; a < 3; ++a) { lo[a][s] = b[a]; hi[a][s] = b[3 + a]; }
+        const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+        area[s] = (dx * dy + dy * dz) + dz * dx;
+        rf[s] = r;
+    };
+    auto children = [&](const GNode& q, float* l6, float* r6) {
+        l6[0] = q.l0.x; l6[1] = q.l0.y; l6[2] = q.l0.z; l6[3] = q.l0.w; l6[4] = q.l1.x; l6[5] = q.l1.y;
+        r6[0] = q.l1.z; r6[1] = q.l1.w; r6[2] = q.r1.x; r6[3] = q.r1.y; r6[4] = q.r1.z; r6[5] = q.r1.w;
+    };
+    float l6[6], r6[6];
+    GNode q = g[i];
+    children(q, l6, r6);
+    put(0, l6, q.lref);
+    put(1, r6, q.rref);
+    int m = 2;
+    while (m < 4) {
+        int e = -1;
+        float best = -1.f;
+        for (int s = 0; s < m; ++s)
+            if (!(rf[s] & kLeafBit) && area[s] > best) { best = area[s]; e = s; }
+        if (e < 0) break;
+        q = g[rf[e]];
+        children(q, l6, r6);
+        for (int s = m; s > e + 1; --s) {   // open slot e + 1
+            for (int a = 0; a < 3; ++a) { lo[a][s] = lo[a][s - 1]; hi[a][s] = hi[a][s - 1]; }
+            area[s] = area[s - 1];
+            rf[s] = rf[s - 1];
+        }
+        put(e, l6, q.lref);
+        put(e + 1, r6, q.rref);
+        ++m;
+    }
+    for (int s = m; s < 4; ++s) {
+        for (int a = 0; a < 3; ++a) { lo[a][s] = 0.f; hi[a][s] = 0.f; }
+        rf[s] = kInvalid;
+    }
+    WNode w;
+    w.lx = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+    w.ly = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+    w.lz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+    w.hx = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+    w.hy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+    w.hz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+    w.ref = make_uint4(rf[0], rf[1], rf[2], rf[3]);
+    w.pad = make_uint4(0, 0, 0, 0);
+    out[i] = w;
+    for (int s = 0; s < m; ++s)
+        if (!(rf[s] & kLeafBit)) wl[rf[s]] = (uint8_t)(level + 1);
+}
+
 // Traversal layout from a reference-layout tree (a loaded .bvh file).
 __global__ void k_from_ref(const Node48* nodes, uint32_t nn, const uint32_t* iidx, const uint32_t* lidx,
                            GNode* gnodes, Leaf* leaves) {
@@ -1455,7 +1526,8 @@ void scratch_give(void* p, size_t bytes) {
 namespace {
 
 void free_products(rtx_bvh* b) {
-    void* ps[] = {b->verts, b->nodes48, b->tris3, b->prims, b->gnodes, b->leaves, b->gtris, b->codes, b->sorted,
+    void* ps[] = {b->verts, b->nodes48, b->tris3, b->prims, b->gnodes, b->wnodes, b->leaves, b->gtris, b->codes,
+                  b->sorted,
                   b->bfs48, b->work, b->rs_keys[0], b->rs_keys[1], b->rs_vals[0], b->rs_vals[1], b->rs_hist,
                   b->rs_part, b->host_rays, b->host_hits, b->host_nrm};
     for (void* p : ps)
@@ -1632,6 +1704,12 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
         return fail(RTX_E_HIP, "device allocation failed on rtx_build()");
     Out O{B->nodes48, B->bfs48, B->tris3, B->prims, B->gnodes, B->leaves, B->gtris};
     k_emit<<<nblk(total, 128), 128, 0, st>>>(C, total, D, O);
+    B->max_depth = (uint32_t)levels.size() - 1;
+    rc = build_wide(B, st);
+    if (rc != RTX_OK) {
+        (void)hipStreamSynchronize(st);
+        return rc;
+    }
     RTX_HIP(hipMemsetAsync(B->work + 1, 0, sizeof(uint32_t), st));
     k_boxes_nonfinite<<<nblk(total, 256), 256, 0, st>>>(B->nodes48, total, B->work + 1);
     BNode root;
@@ -1720,7 +1798,28 @@ int upload_reference_tree(rtx_bvh* B, const Node48* nodes, uint32_t nn, const ui
         if (!nodes[i].is_leaf) { st.push_back({nodes[i].a, d + 1}); st.push_back({nodes[i].b, d + 1}); }
     }
     B->max_depth = depth;
-    return RTX_OK;
+    return build_wide(B, nullptr);
+}
+
+// Widening pass shared by builds and loaded trees (see k_wide_level). The level scratch
+// comes from the scratch cache and goes back once the pass has run.
+int build_wide(rtx_bvh* B, hipStream_t st) {
+    const uint32_t n = B->ninternal;
+    if (n == 0) return RTX_OK;   // the root is a leaf
+    B->wnodes = dalloc<WNode>(n);
+    size_t got = 0;
+    uint8_t* wl = static_cast<uint8_t*>(scratch_take(n, got));
+    if (!B->wnodes || !wl) {
+        if (wl) scratch_give(wl, got);
+        return fail(RTX_E_HIP, "device allocation failed for the wide traversal nodes");
+    }
+    k_wide_init<<<nblk(n, 256), 256, 0, st>>>(wl, n);
+    // a wide level holds at least one binary level, so max_depth + 1 launches reach every node
+    for (uint32_t L = 0; L <= B->max_depth && L < 255; ++L)
+        k_wide_level<<<nblk(n, 256), 256, 0, st>>>(B->gnodes, n, L, wl, B->wnodes);
+    const bool ok = hip_ok(hipGetLastError(), "k_wide_level") && hip_ok(hipStreamSynchronize(st), "wide pass");
+    scratch_give(wl, got);
+    return ok ? RTX_OK : RTX_E_HIP;
 }
 
 }  // namespace rtx

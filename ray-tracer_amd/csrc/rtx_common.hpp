@@ -122,6 +122,19 @@ struct alignas(64) GNode {
 };
 static_assert(sizeof(GNode) == 64, "GNode layout");
 
+// Wide traversal node (trace entry points, tuning trace_tree 1): up to four children of a
+// binary node collapsed into one 128-byte line. Child boxes are the binary tree's own boxes
+// (exact floats), stored per coordinate so one float4 load brings a coordinate of all four;
+// refs use the binary child-reference encoding (internal node index, or a leaf reference),
+// kInvalid marks an empty slot (slots fill from 0, in the binary tree's left-to-right order).
+struct alignas(128) WNode {
+    float4 lx, ly, lz;   // box minima of children 0..3
+    float4 hx, hy, hz;   // box maxima
+    uint4 ref;
+    uint4 pad;
+};
+static_assert(sizeof(WNode) == 128, "WNode layout");
+
 // Traversal triangle: 36 bytes, v0 and the two edges precomputed (bit-identical to
 // computing v1-v0, v2-v0 at trace time): v0.xyz e1.xyz e2.xyz. Packed at a 36-byte
 // stride (4-byte aligned): the cooperative leaf phase loads a leaf's triangles with

@@ -47,6 +47,8 @@ struct rtx_bvh {
     uint32_t ntri = 0;
     rtx::GNode* gnodes = nullptr;     // traversal layout, internal nodes only
     uint32_t ninternal = 0;
+    rtx::WNode* wnodes = nullptr;     // 4-wide traversal layout, indexed like gnodes (entries
+                                      // of binary nodes absorbed into a wide parent are unused)
     rtx::Leaf* leaves = nullptr;
     uint32_t nleaves = 0;
     rtx::GTri* gtris = nullptr;
@@ -91,4 +93,6 @@ namespace rtx {
 // (used by rtx_load).  Defined in rtx_build.hip.
 int upload_reference_tree(rtx_bvh* b, const Node48* nodes, uint32_t nn, const uint32_t* tris3,
                           uint32_t ntri, const float* verts6, uint32_t nv);
+// Collapses the binary traversal nodes (b->gnodes) into b->wnodes. Defined in rtx_build.hip.
+int build_wide(rtx_bvh* b, hipStream_t st);
 }  // namespace rtx

@@ -803,7 +803,7 @@ extern "C" int rtx_set_tuning(const rtx_tuning* t) {
         t->trace_loop > 2 || t->ray_sort > 1 || t->wait_lanes > 64 || t->trace_grid_blocks == 0 ||
         !(t->render_lds_depth == 8 || t->render_lds_depth == 12 || t->render_lds_depth == 16) || t->render_loop > 2 ||
         t->render_wait_lanes > 64 || t->leaf_coop > 1 || t->render_leaf_coop > 1 || t->refill_lanes == 0 ||
-        t->refill_lanes > 64)
+        t->refill_lanes > 64 || t->trace_tree > 1)
         return fail(RTX_E_ARG, "bad tuning on rtx_set_tuning()");
     g_tuning = *t;
     return RTX_OK;

@@ -25,7 +25,7 @@
 namespace rtx {
 namespace {
 
-template <bool ANY, bool COUNT, int DEPTH, int LOOP>
+template <bool ANY, bool COUNT, int DEPTH, int LOOP, int TREE>
 __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray32* __restrict__ rays, uint32_t n,
                                                          Hit16* __restrict__ hits, float* __restrict__ normals,
                                                          uint8_t* __restrict__ occl, uint32_t* __restrict__ counts,
@@ -42,8 +42,9 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray3
         r0 = reinterpret_cast<const float4*>(rays)[2 * i];
         r1 = reinterpret_cast<const float4*>(rays)[2 * i + 1];
     }
-    const TraceOut h = trace_ray<ANY, COUNT, DEPTH, LOOP>(A, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w,
-                                                          lst, tid, A.wait, coop + (LOOP == kLoopCoop ? tid / 64 : 0));
+    const TraceOut h = trace_ray<ANY, COUNT, DEPTH, LOOP, kTraceTriBatch, false, TREE>(
+        A, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w, lst, tid, A.wait,
+        coop + (LOOP == kLoopCoop ? tid / 64 : 0));
     if (!in) return;
     if (ANY) {
         occl[i] = h.tri != kInvalid ? 1 : 0;
@@ -74,8 +75,10 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace(TraceArgs A, const Ray3
 // 16: 940, 18: 939, 20: 935, 24: 930, 32: 925; coherent closest hit 18: 1696, 24: 1932,
 // 32: 1955; incoherent any hit 18: 1205, 24: 1356, 32: 1377.
 
-template <bool ANY, bool COUNT, int DEPTH, int LOOP>
-__global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, const Ray32* __restrict__ rays,
+// The wide kernels are held to 6 waves per SIMD (80 VGPRs): at their natural 89 VGPRs only 5
+// fit, and 6 measured faster (1M triangles, 4M incoherent rays: 4.31 against 4.62 ms).
+template <bool ANY, bool COUNT, int DEPTH, int LOOP, int TREE>
+__global__ void __launch_bounds__(kTraceThreads, TREE == kTreeBinary ? 1 : 6) k_trace_persist(TraceArgs A, const Ray32* __restrict__ rays,
                                                                  uint32_t n, Hit16* __restrict__ hits,
                                                                  float* __restrict__ normals,
                                                                  uint8_t* __restrict__ occl,
@@ -128,7 +131,7 @@ __global__ void __launch_bounds__(kTraceThreads) k_trace_persist(TraceArgs A, co
         }
         // uniform exit test: a started ray that missed the root is written at the top
         if (!__any(w.live || id != kInvalid) && !more) break;
-        w.template node_phase<COUNT>(A, lst, tid, A.wait);
+        w.template node_phase<COUNT, TREE>(A, lst, tid, A.wait);
         if (LOOP == kLoopCoop) {   // all lanes active: the loop is wave-uniform
             const bool at = w.live && (w.ref & kLeafBit);
             if (w.template leaf_coop<ANY, COUNT, false>(A, coop + tid / 64)) w.live = false;
@@ -286,14 +289,22 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
         radix_sort_pairs(S, (uint32_t)n, 0xEu, s);
         perm = S.vals[S.cur];
     }
+    // the wide layout exists whenever the root is internal (a leaf root has one leaf to test)
+    const int tree = b->wnodes ? (int)g_tuning.trace_tree : kTreeBinary;
     if (persist) {
         const uint32_t pb = std::min<uint32_t>(blocks, g_tuning.trace_grid_blocks);
         k_zero_work<<<1, 64, 0, s>>>(work);
         const int pk = ext ? -1 : prof_start(s);
         with_trace_variant(g_tuning.trace_lds_depth, loop_shape(), [&](auto D, auto L) {
-            k_trace_persist<ANY, COUNT, decltype(D)::value, decltype(L)::value><<<pb, kTraceThreads, 0, s>>>(
-                A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
-                counts, work, perm);
+            constexpr int DD = decltype(D)::value, LL = decltype(L)::value;
+            if (tree == kTreeWide)
+                k_trace_persist<ANY, COUNT, DD, LL, kTreeWide><<<pb, kTraceThreads, 0, s>>>(
+                    A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals,
+                    occl, counts, work, perm);
+            else
+                k_trace_persist<ANY, COUNT, DD, LL, kTreeBinary><<<pb, kTraceThreads, 0, s>>>(
+                    A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals,
+                    occl, counts, work, perm);
         });
         prof_stop(pk, s);
         RTX_HIP(hipGetLastError());
@@ -302,9 +313,15 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
     const int pk = ext ? -1 : prof_start(s);
     with_trace_variant(g_tuning.trace_lds_depth, g_tuning.trace_loop == 0 ? (uint32_t)kLoopSingle : loop_shape(),
                        [&](auto D, auto L) {
-        k_trace<ANY, COUNT, decltype(D)::value, decltype(L)::value><<<blocks, kTraceThreads, 0, s>>>(
-            A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
-            counts, perm);
+        constexpr int DD = decltype(D)::value, LL = decltype(L)::value;
+        if (tree == kTreeWide)
+            k_trace<ANY, COUNT, DD, LL, kTreeWide><<<blocks, kTraceThreads, 0, s>>>(
+                A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
+                counts, perm);
+        else
+            k_trace<ANY, COUNT, DD, LL, kTreeBinary><<<blocks, kTraceThreads, 0, s>>>(
+                A, reinterpret_cast<const Ray32*>(rays), (uint32_t)n, reinterpret_cast<Hit16*>(hits), normals, occl,
+                counts, perm);
     });
     prof_stop(pk, s);
     RTX_HIP(hipGetLastError());
@@ -313,7 +330,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
 
 }  // namespace
 
-rtx_tuning g_tuning = {1u << 23, 8192, 1024, 12, 2, 1, 24, 2048, 8, 2, 16, 1, 1, 18};
+rtx_tuning g_tuning = {1u << 23, 8192, 1024, 12, 2, 1, 24, 2048, 8, 2, 16, 1, 1, 18, 0};
 
 }  // namespace rtx
 

@@ -14,6 +14,7 @@ constexpr int kTraceTriBatch = 4;   // triangles of a leaf loaded together by th
 
 struct TraceArgs {
     const GNode* nodes;
+    const WNode* wnodes;   // 4-wide layout of the same tree (trace entry points, trace_tree 1)
     const Leaf* leaves;
     const GTri* tris;
     const uint32_t* tris3;
@@ -30,6 +31,18 @@ extern rtx_tuning g_tuning;
 // Loop shapes of trace_ray: one loop over nodes and leaves; the while-while loop; the
 // while-while loop with cooperative leaf phases.
 constexpr int kLoopSingle = 0, kLoopWhileWhile = 1, kLoopCoop = 2;
+
+// Trees a trace can walk: the binary tree (the reference's visiting order, so hits equal
+// the reference's including which triangle wins an exact-t tie), or its 4-wide collapse
+// (same t for every ray; at an exact-t tie between triangles of different leaves another
+// of the tied triangles may be reported).
+constexpr int kTreeBinary = 0, kTreeWide = 1;
+
+// Order-preserving 32-bit image of a float t (t not NaN; -0 and +0 map alike).
+__device__ __forceinline__ uint32_t ord_t(float t) {
+    const uint32_t b = t == 0.f ? 0u : __float_as_uint(t);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
 
 // Calls f(integral_constant<DEPTH>, integral_constant<LOOP>) for LDS depth d (8, 12, 16)
 // and loop shape l (kLoopSingle, kLoopWhileWhile or kLoopCoop).
@@ -51,6 +64,7 @@ constexpr uint32_t kMaxTraceDepth = 8 + kSpillDepth - 2;
 inline TraceArgs make_trace_args(const rtx_bvh* b) {
     TraceArgs A;
     A.nodes = b->gnodes;
+    A.wnodes = b->wnodes;
     A.leaves = b->leaves;
     A.tris = b->gtris;
     A.tris3 = b->tris3;
@@ -92,6 +106,11 @@ __device__ __forceinline__ float4 ld_g(const float4* p) {
 __device__ __forceinline__ uint2 ld_g(const uint2* p) {
     const guint2 v = *(__attribute__((address_space(1))) const guint2*)p;
     return make_uint2(v.x, v.y);
+}
+typedef uint32_t guint4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 ld_g(const uint4* p) {
+    const guint4 v = *(__attribute__((address_space(1))) const guint4*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ float ld_g(const float* p) { return *(__attribute__((address_space(1))) const float*)p; }
 __device__ __forceinline__ uint32_t ld_g(const uint32_t* p) {
@@ -250,12 +269,71 @@ struct Walker {
             pop(lst, tid);
         }
     }
+    // One wide node: test the (up to) four child boxes, continue with the nearest child
+    // the ray enters before its best t, and push the others farthest first, each with its
+    // exact entry distance (a popped entry is skipped once that distance is >= best t).
+    // Children are ordered by 32-bit keys: the order-preserving image of the entry distance
+    // with the slot index in its two low bits, so equal distances keep the binary tree's
+    // left-to-right order (and distances within 4 ulps of each other go by slot); the order
+    // only decides which subtree is searched first, never which boxes are searched.
+    template <bool COUNT>
+    __device__ __forceinline__ void wide_step(const TraceArgs& A, uint2 (*lst)[kTraceThreads], uint32_t tid) {
+        if (COUNT) ++nI;
+        const WNode* g = A.wnodes + ref;
+        const float4 lx = ld_g(&g->lx), ly = ld_g(&g->ly), lz = ld_g(&g->lz);
+        const float4 hx = ld_g(&g->hx), hy = ld_g(&g->hy), hz = ld_g(&g->hz);
+        const uint4 rf = ld_g(&g->ref);
+        float t0, t1, t2, t3;
+        if (__all(fast)) {   // wave vote: a scalar branch, both forms give the same values
+            t0 = slab_fast(o.x, o.y, o.z, ix, iy, iz, tmin, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, bestT);
+            t1 = slab_fast(o.x, o.y, o.z, ix, iy, iz, tmin, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, bestT);
+            t2 = slab_fast(o.x, o.y, o.z, ix, iy, iz, tmin, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, bestT);
+            t3 = slab_fast(o.x, o.y, o.z, ix, iy, iz, tmin, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, bestT);
+        } else {
+            float ox = o.x, oy = o.y, oz = o.z;
+            __asm__ volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
+            t0 = slab(ox, oy, oz, ix, iy, iz, tmin, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, bestT);
+            t1 = slab(ox, oy, oz, ix, iy, iz, tmin, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, bestT);
+            t2 = slab(ox, oy, oz, ix, iy, iz, tmin, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, bestT);
+            t3 = slab(ox, oy, oz, ix, iy, iz, tmin, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, bestT);
+        }
+        // a child is entered when its box is hit before best t; empty slots never are
+        uint32_t k0 = (t0 < bestT && rf.x != kInvalid) ? ((ord_t(t0) & ~3u) | 0u) : ~0u;
+        uint32_t k1 = (t1 < bestT && rf.y != kInvalid) ? ((ord_t(t1) & ~3u) | 1u) : ~0u;
+        uint32_t k2 = (t2 < bestT && rf.z != kInvalid) ? ((ord_t(t2) & ~3u) | 2u) : ~0u;
+        uint32_t k3 = (t3 < bestT && rf.w != kInvalid) ? ((ord_t(t3) & ~3u) | 3u) : ~0u;
+        const uint32_t m = (k0 != ~0u) + (k1 != ~0u) + (k2 != ~0u) + (k3 != ~0u);
+        if (m == 0) {
+            pop(lst, tid);
+            return;
+        }
+        // Sorting network on 4 keys; each key carries its child's entry distance and ref, so
+        // no child is looked up by a lane-varying slot afterwards (the compiler would index
+        // a private array in scratch memory for that).
+        uint32_t r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+        auto ce = [](uint32_t& ka, float& ta, uint32_t& ra, uint32_t& kb, float& tb, uint32_t& rb) {
+            const bool sw = kb < ka;
+            const uint32_t k = sw ? kb : ka, r = sw ? rb : ra;
+            const float t = sw ? tb : ta;
+            kb = sw ? ka : kb; rb = sw ? ra : rb; tb = sw ? ta : tb;
+            ka = k; ra = r; ta = t;
+        };
+        ce(k0, t0, r0, k1, t1, r1);
+        ce(k2, t2, r2, k3, t3, r3);
+        ce(k0, t0, r0, k2, t2, r2);
+        ce(k1, t1, r1, k3, t3, r3);
+        ce(k1, t1, r1, k2, t2, r2);
+        if (m > 3) push(lst, tid, r3, t3);
+        if (m > 2) push(lst, tid, r2, t2);
+        if (m > 1) push(lst, tid, r1, t1);
+        ref = r0;
+    }
     // Node phase of the while-while loop: node steps for the lanes at internal nodes until
     // none is left, or until at most `wait` are left while some lane already waits at a
     // leaf (those lanes then continue in a later phase). Every vote is wave-uniform; each
     // lane's own sequence of visits is unchanged.
     // `wait` must be the same on every lane (a kernel argument or a uniform field).
-    template <bool COUNT>
+    template <bool COUNT, int TREE = kTreeBinary>
     __device__ __forceinline__ void node_phase(const TraceArgs& A, uint2 (*lst)[kTraceThreads], uint32_t tid,
                                                uint32_t wait) {
         for (;;) {
@@ -263,7 +341,10 @@ struct Walker {
             const uint64_t nm = __ballot(need);
             if (nm == 0) break;
             if ((uint32_t)__popcll(nm) <= wait && __ballot(live && (ref & kLeafBit)) != 0) break;
-            if (need) node_step<COUNT>(A, lst, tid);
+            if (need) {
+                if (TREE != kTreeBinary) wide_step<COUNT>(A, lst, tid);
+                else node_step<COUNT>(A, lst, tid);
+            }
         }
     }
     // All triangles of the current leaf, in order; true when an any-hit query is answered.
@@ -401,7 +482,8 @@ struct Walker {
 // are at leaves too (all but A.wait of them), so triangle tests run with the wave mostly
 // converged. LOOP 2 is LOOP 1 with cooperative leaf phases (leaf_coop). All visit the
 // same nodes in the same order per ray and return the same hit.
-template <bool ANY, bool COUNT, int DEPTH, int LOOP, int TB = kTraceTriBatch, bool MIXED = false>
+template <bool ANY, bool COUNT, int DEPTH, int LOOP, int TB = kTraceTriBatch, bool MIXED = false,
+          int TREE = kTreeBinary>
 __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, float tmin, float tmax,
                                               uint2 (*lst)[kTraceThreads], uint32_t tid, uint32_t wait,
                                               CoopLds* coop) {
@@ -414,13 +496,15 @@ __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, fl
             if (w.ref & kLeafBit) {
                 if (w.template leaf_step<ANY, COUNT>(A)) break;
                 w.pop(lst, tid);
+            } else if (TREE != kTreeBinary) {
+                w.template wide_step<COUNT>(A, lst, tid);
             } else {
                 w.template node_step<COUNT>(A, lst, tid);
             }
         }
     } else {
         for (;;) {
-            w.template node_phase<COUNT>(A, lst, tid, wait);
+            w.template node_phase<COUNT, TREE>(A, lst, tid, wait);
             if (!__any(w.live)) break;
             if (LOOP == kLoopCoop && __builtin_amdgcn_read_exec() == ~0ull) {
                 const bool at = w.live && (w.ref & kLeafBit);

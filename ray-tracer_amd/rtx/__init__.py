@@ -57,7 +57,8 @@ class Tuning(C.Structure):
                 ("trace_lds_depth", C.c_uint32), ("trace_loop", C.c_uint32), ("ray_sort", C.c_uint32),
                 ("wait_lanes", C.c_uint32), ("trace_grid_blocks", C.c_uint32), ("render_lds_depth", C.c_uint32),
                 ("render_loop", C.c_uint32), ("render_wait_lanes", C.c_uint32),
-                ("leaf_coop", C.c_uint32), ("render_leaf_coop", C.c_uint32), ("refill_lanes", C.c_uint32)]
+                ("leaf_coop", C.c_uint32), ("render_leaf_coop", C.c_uint32), ("refill_lanes", C.c_uint32),
+                ("trace_tree", C.c_uint32)]
 
 
 _LIB = None

@@ -23,6 +23,7 @@ SPACE = {
     "refill_lanes": [12, 18, 24, 32, 40],
     "trace_grid_blocks": [1024, 2048, 4096, 8192],
     "leaf_coop": [0, 1],
+    "trace_tree": [0, 1],
     # renderer
     "render_lds_depth": [8, 12, 16],
     "render_loop": [0, 1, 2],
@@ -33,7 +34,7 @@ SPACE = {
     "batch_paths": [1 << 20, 1 << 21, 1 << 22, 1 << 23],
 }
 # the refill threshold before stack depth and leaf-phase wait: their best values depend on it
-TRACE_KEYS = ["trace_loop", "leaf_coop", "refill_lanes", "trace_lds_depth", "ray_sort", "wait_lanes",
+TRACE_KEYS = ["trace_tree", "trace_loop", "leaf_coop", "refill_lanes", "trace_lds_depth", "ray_sort", "wait_lanes",
               "trace_grid_blocks"]
 # loop structure first: the best block and batch sizes depend on which kernel runs
 RENDER_KEYS = ["render_loop", "render_leaf_coop", "grid_blocks", "shade_threads", "batch_paths", "render_lds_depth",

@@ -257,7 +257,7 @@ def test_tuning_validation():
         for bad in (dict(trace_loop=3), dict(trace_lds_depth=10), dict(shade_threads=96),
                     dict(batch_paths=512), dict(grid_blocks=0), dict(ray_sort=2), dict(wait_lanes=65),
                     dict(trace_grid_blocks=0), dict(render_lds_depth=4), dict(render_loop=3), dict(render_wait_lanes=99),
-                    dict(refill_lanes=0), dict(refill_lanes=65),
+                    dict(refill_lanes=0), dict(refill_lanes=65), dict(trace_tree=2),
                     dict(leaf_coop=2), dict(render_leaf_coop=2)):
             with pytest.raises(rtx.RtxError):
                 rtx.set_tuning(**bad)
