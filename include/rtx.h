@@ -97,6 +97,11 @@ int rtx_debug_clip_slab(const float* v9, const float* region6, int axis, float* 
  * (0xFFFFFFFF when none).  Pins the Moller-Trumbore f = 1/a of bvh_traverse.cpp:62 and
  * the 1/dir of rayAABB (:26-28) to the reference's division.  Synchronous. */
 int rtx_debug_recip_check(uint64_t first, uint64_t count, uint64_t* mismatches, uint32_t* first_bad);
+/* Peak bytes of device scratch (build scratch, render path state) held by library calls at
+ * once since the previous call with reset != 0, which starts a new window at the bytes held
+ * now.  For the tests that check the adjoint's memory does not grow with max_bounce.  No
+ * reference counterpart. */
+int rtx_debug_scratch_peak(uint64_t* peak_bytes, int reset);
 
 /* ---- trace ------------------------------------------------------------------
  * Closest hit for n rays: replaces bvh::trace (src/bvh/bvh_traverse.cpp:237-246,
@@ -188,16 +193,34 @@ typedef struct rtx_render_params {
 int rtx_render(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_params* p,
                const float* albedo_d, const float* emission_d, float* image_d, void* stream);
 
+/* rtx_render that also stores each path's radiance (the primal of the path-replay adjoint):
+ * path_radiance_d holds 3 floats per path, width*height*spp*3 floats, path sample*W*H +
+ * pixel at 3 * (sample * width * height + pixel).  The image equals rtx_render's.  No
+ * reference counterpart. */
+int rtx_render_primal(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_params* p,
+                      const float* albedo_d, const float* emission_d, float* image_d,
+                      float* path_radiance_d, void* stream);
+
 /* Path-replay adjoint of rtx_render for the loss sum (I - target)^2: accumulates
  * dLoss/dalbedo and dLoss/demission (3 floats per original triangle, added to the
- * buffers) given d_image_d = dLoss/dI.  Paths are re-traced, not stored.  Requires
- * rr_bounce > max_bounce (no Russian roulette) so the path set does not depend on
- * albedo.  The albedo gradient is exact for albedo > 0; a path vertex on a zero albedo
- * channel adds 0 to that channel (the replay holds no radiance past it), so an optimiser
- * should keep albedo in (0, 1].  No reference counterpart. */
+ * buffers) given d_image_d = dLoss/dI.  Paths are re-traced with the same sample table,
+ * not stored: per batch of paths it runs a primal pass for each path's radiance L, then
+ * the replay, which at the k-th surface hit of a path adds dL/da_k = (L - L_upto_k) / a_k
+ * and dL/dLe_k = throughput arriving at k, where L_upto_k is the radiance gathered up to
+ * and including that hit.  Only the current bounce is held per path, so memory does not
+ * grow with max_bounce.  Requires rr_bounce > max_bounce (no Russian roulette) so the path
+ * set does not depend on albedo.  The albedo gradient is exact for albedo > 0; a path
+ * vertex on a zero albedo channel adds 0 to that channel (the replay holds no radiance past
+ * it), so an optimiser should keep albedo in (0, 1].  No reference counterpart. */
 int rtx_render_adjoint(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_params* p,
                        const float* albedo_d, const float* emission_d, const float* d_image_d,
                        float* grad_albedo_d, float* grad_emission_d, void* stream);
+/* The replay of rtx_render_adjoint alone, with each path's radiance taken from an earlier
+ * rtx_render_primal of the same scene, camera, parameters and materials (so one pass is
+ * saved).  No reference counterpart. */
+int rtx_render_adjoint_replay(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_params* p,
+                              const float* albedo_d, const float* emission_d, const float* path_radiance_d,
+                              const float* d_image_d, float* grad_albedo_d, float* grad_emission_d, void* stream);
 
 /* Batched wavefront render of many scenes.  scenes[i] renders with cams[i]/params[i]
  * into images_d[i].  mode 0 = one launch sequence per scene (plain), 1 = all scenes in

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <unordered_map>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -1454,6 +1455,15 @@ struct ScratchPool {
     std::mutex mu;
     std::vector<Block> blocks;
     size_t cached = 0;
+    // bytes requested by the blocks handed out and not yet returned, and their peak since the
+    // last reset (rtx_debug_scratch_peak)
+    std::unordered_map<void*, size_t> live;
+    size_t live_bytes = 0, peak = 0;
+    void note_take(void* p, size_t bytes) {
+        live[p] = bytes;
+        live_bytes += bytes;
+        peak = std::max(peak, live_bytes);
+    }
     void* take(int dev, size_t bytes, size_t& got) {
         {
             std::lock_guard<std::mutex> lk(mu);
@@ -1469,17 +1479,25 @@ struct ScratchPool {
                 got = blocks[bi].bytes;
                 cached -= got;
                 blocks.erase(blocks.begin() + bi);
+                note_take(p, bytes);
                 return p;
             }
         }
         void* p = nullptr;
         if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
         got = bytes;
+        std::lock_guard<std::mutex> lk(mu);
+        note_take(p, bytes);
         return p;
     }
     void give(int dev, size_t bytes, void* p) {
         {
             std::lock_guard<std::mutex> lk(mu);
+            auto it = live.find(p);
+            if (it != live.end()) {
+                live_bytes -= it->second;
+                live.erase(it);
+            }
             if (cached + bytes <= kMaxCached) {
                 blocks.push_back({dev, bytes, p});
                 cached += bytes;
@@ -1521,6 +1539,12 @@ void scratch_give(void* p, size_t bytes) {
     int dev = 0;
     (void)hipGetDevice(&dev);
     g_scratch_pool.give(dev, bytes, p);
+}
+uint64_t scratch_peak(bool reset) {
+    std::lock_guard<std::mutex> lk(g_scratch_pool.mu);
+    const uint64_t peak = g_scratch_pool.peak;
+    if (reset) g_scratch_pool.peak = g_scratch_pool.live_bytes;
+    return peak;
 }
 
 namespace {
@@ -1862,4 +1886,10 @@ extern "C" void rtx_bvh_free(rtx_bvh* b) {
     if (!b) return;
     free_products(b);
     delete b;
+}
+
+extern "C" int rtx_debug_scratch_peak(uint64_t* peak_bytes, int reset) {
+    if (!peak_bytes) return fail(RTX_E_ARG, "null argument on rtx_debug_scratch_peak()");
+    *peak_bytes = scratch_peak(reset != 0);
+    return RTX_OK;
 }

@@ -24,6 +24,8 @@ bool hip_ok(hipError_t e, const char* what);
 // the device) on every build and render.
 void* scratch_take(size_t bytes, size_t& got);
 void scratch_give(void* p, size_t bytes);
+// Peak of the bytes requested by blocks taken and not yet given back, since the last reset.
+uint64_t scratch_peak(bool reset);
 
 template <class T>
 T* dalloc(size_t n) {

@@ -54,12 +54,14 @@ struct PathState {
     uint32_t* smp;
     uint32_t* q[2];
     uint32_t* cnt;     // [b]: input queue length of bounce b; [ncnt/2 + b]: persistent-trace counter of bounce b
-    // adjoint only: per surface hit k of path i, at [k * B + i]
-    float4* vlup;      // radiance gathered up to and including hit k (xyz), triangle id (w bits)
-    float4* vtp;       // throughput arriving at hit k
-    uint32_t* nv;      // hits recorded per path
+    // adjoint replay only: the current bounce's record of path i, overwritten every bounce
+    float4* rec0;      // radiance gathered up to and including this bounce's hit (xyz); albedo entry
+                       // of the hit (w bits), kInvalid when the path hit no surface this bounce
+    float4* rec1;      // throughput arriving at the hit
     uint32_t B;
 };
+
+constexpr int kShadeForward = 0, kShadeReplay = 1;
 
 
 __device__ __forceinline__ uint32_t find_scene(const uint64_t* off, int n, uint64_t g) {
@@ -158,7 +160,15 @@ __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* _
         P.pix[i] = pixel;
         P.smp[i] = sample;
         P.q[0][i] = i;
-        if (P.nv) P.nv[i] = 0;
+    }
+}
+
+// Each path's radiance L into out[3 * (base + i) ...] (the primal radiance of the replay).
+__global__ void k_store_L(PathState P, uint32_t cnt, float* __restrict__ out, uint64_t base) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const float4 L = P.L[i];
+        float* o = out + 3 * (base + i);
+        o[0] = L.x; o[1] = L.y; o[2] = L.z;
     }
 }
 
@@ -327,11 +337,13 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
     }
 }
 
-// One bounce of pathTrace (main.cpp:158-200) for every queued path. ADJ additionally
-// records each surface hit for the adjoint (triangle, radiance so far, throughput). UNI:
-// every path belongs to scenes[0], whose descriptor is then read once as uniform data.
-template <bool ADJ, bool UNI>
+// One bounce of pathTrace (main.cpp:158-200) for every queued path. MODE kShadeReplay also
+// records this bounce's surface hit for the adjoint (albedo entry, radiance so far,
+// throughput), in place of the previous bounce's record. UNI: every path belongs to
+// scenes[0], whose descriptor is then read once as uniform data.
+template <int MODE, bool UNI>
 __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int qi, int bounce) {
+    constexpr bool ADJ = MODE == kShadeReplay;
     const uint32_t n = P.cnt[bounce];
     const uint32_t* Q = P.q[qi];
     uint32_t* Qo = P.q[qi ^ 1];
@@ -356,20 +368,18 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
             if (tri == kInvalid) {
                 f3 e = env_color(S, d);
                 L = make_float4(L.x + tp.x * e.x, L.y + tp.y * e.y, L.z + tp.z * e.z, 0.f);
+                if (ADJ) P.rec0[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kInvalid));
             } else {
                 const uint32_t prim = ld_g(S.prims + tri);
                 // albedo entry: the hit's texel (texture albedo) or its original triangle
                 const float* alb = S.tex ? S.tex : S.albedo;
                 const uint32_t aidx = S.tex ? texel_at(S, tri, h.y, h.z) : prim;
-                if (ADJ) P.vtp[(size_t)bounce * P.B + i] = tp;
+                if (ADJ) P.rec1[i] = tp;
                 if (S.emission) {
                     const float* e = S.emission + 3 * prim;
                     L = make_float4(L.x + tp.x * ld_g(e), L.y + tp.y * ld_g(e + 1), L.z + tp.z * ld_g(e + 2), 0.f);
                 }
-                if (ADJ) {
-                    P.vlup[(size_t)bounce * P.B + i] = make_float4(L.x, L.y, L.z, __uint_as_float(aidx));
-                    P.nv[i] = (uint32_t)bounce + 1;
-                }
+                if (ADJ) P.rec0[i] = make_float4(L.x, L.y, L.z, __uint_as_float(aidx));
                 if (bounce < S.max_bounce) {
                     float ax = 0.8f, ay = 0.8f, az = 0.8f;
                     if (alb) { ax = ld_g(alb + 3 * aidx); ay = ld_g(alb + 3 * aidx + 1); az = ld_g(alb + 3 * aidx + 2); }
@@ -472,22 +482,26 @@ __device__ __forceinline__ void merged_add(float* __restrict__ buf, bool has, ui
     }
 }
 
-// Gradient scatter of the path-replay adjoint. The replay itself is the forward
-// wavefront (k_wtrace + k_shade<true>), which recorded every surface hit k of path i.
-// With L = sum_m (prod_{j<m} a_j) X_m:
-//   dL/da_k = (L - L_upto_k) / a_k        dL/dLe_k = throughput arriving at k.
+// Gradient scatter of the path-replay adjoint for one bounce. The replay is the forward
+// wavefront again (the same sample table gives the same paths); at bounce b every path of
+// the bounce's queue holds the record of its hit k = b (k_shade<kShadeReplay>), and its
+// primal radiance L comes from Lfin[3 * (lbase + i) ...]. With L = sum_m (prod_{j<m} a_j) X_m:
+//   dL/da_k = (L - L_upto_k) / a_k        dL/dLe_k = throughput arriving at k,
+// where L_upto_k includes vertex k's own emission. Only the current bounce is ever stored,
+// so the state per path does not grow with max_bounce.
 // LDS=true: each block accumulates into a private copy of ONE gradient array in LDS
 // (scenes of up to kLdsGradTris triangles; ga or ge is null, the kernel runs once per
 // array) and flushes it once with global atomics; otherwise lanes add to global memory
 // after merging equal triangles within the wave.
 constexpr uint32_t kLdsGradTris = 5120;   // 3 floats each: 60 KiB of LDS
 constexpr int kGradThreads = 512;
+constexpr uint32_t kGradLdsBlocks = 128;    // LDS form: blocks per bounce, each flushing its copy once
 
 template <bool LDS>
-__global__ void __launch_bounds__(kGradThreads) k_grad(const SceneDesc* __restrict__ scenes, PathState P,
-                                                       uint32_t cnt, const float* __restrict__ dimg,
-                                                       float* __restrict__ ga, float* __restrict__ ge,
-                                                       uint32_t nprim) {
+__global__ void __launch_bounds__(kGradThreads) k_grad(const SceneDesc* __restrict__ scenes, PathState P, int qi,
+                                                       int bounce, const float* __restrict__ Lfin, uint64_t lbase,
+                                                       const float* __restrict__ dimg, float* __restrict__ ga,
+                                                       float* __restrict__ ge, uint32_t nprim) {
     extern __shared__ float lg[];   // LDS: [nprim * 3] gradient of the one non-null array
     const SceneDesc& S = scenes[0];
     const float fs = (float)S.spp;
@@ -495,50 +509,46 @@ __global__ void __launch_bounds__(kGradThreads) k_grad(const SceneDesc* __restri
         for (uint32_t j = threadIdx.x; j < 3 * nprim; j += blockDim.x) lg[j] = 0.f;
         __syncthreads();
     }
+    const uint32_t n = P.cnt[bounce];
+    const uint32_t* Q = P.q[qi];
+    const float* alb = S.tex ? S.tex : S.albedo;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t wb = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wb < cnt; wb += stride) {
-        const uint32_t i = wb + __lane_id();
-        const bool live = i < cnt;
-        uint32_t nv = 0;
-        f3 L{0.f, 0.f, 0.f}, delta{0.f, 0.f, 0.f};
-        if (live) {
-            nv = P.nv[i];
-            const float4 l = P.L[i];
-            L = f3{l.x, l.y, l.z};
-            const uint32_t pixel = P.pix[i];
-            delta = f3{dimg[3 * pixel] / fs, dimg[3 * pixel + 1] / fs, dimg[3 * pixel + 2] / fs};
-        }
-        uint32_t maxnv = nv;
-        if (!LDS)
-            for (int off = 32; off >= 1; off >>= 1) maxnv = max(maxnv, (uint32_t)__shfl_xor((int)maxnv, off));
-        for (uint32_t k = 0; k < maxnv; ++k) {
-            const bool has = k < nv;
-            uint32_t prim = 0;
-            f3 gA{0.f, 0.f, 0.f}, gE{0.f, 0.f, 0.f};
+    // wave-uniform trip count: merged_add votes across the wave
+    for (uint32_t wb = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wb < n; wb += stride) {
+        const uint32_t k = wb + __lane_id();
+        bool has = false;
+        uint32_t prim = 0;
+        f3 gA{0.f, 0.f, 0.f}, gE{0.f, 0.f, 0.f};
+        if (k < n) {
+            const uint32_t i = Q[k];
+            const float4 lu = P.rec0[i];
+            prim = __float_as_uint(lu.w);   // albedo entry: texel or original triangle
+            has = prim != kInvalid;
             if (has) {
-                const float4 lu = P.vlup[(size_t)k * P.B + i];
-                const float4 tp = P.vtp[(size_t)k * P.B + i];
-                prim = __float_as_uint(lu.w);   // albedo entry: texel or original triangle
-                const float* a = (S.tex ? S.tex : S.albedo) + 3 * prim;
+                const float4 tp = P.rec1[i];
+                const float* lf = Lfin + 3 * (lbase + i);
+                const uint32_t pixel = P.pix[i];
+                const f3 delta{dimg[3 * pixel] / fs, dimg[3 * pixel + 1] / fs, dimg[3 * pixel + 2] / fs};
+                const float* a = alb + 3 * prim;
                 // (L - lu) / a is the radiance past this vertex per unit albedo. At a zero
                 // albedo the path carried nothing past the vertex, so the replay has no
                 // record of it: that vertex contributes 0 (not 0/0).
-                gA = f3{a[0] != 0.f ? delta.x * (L.x - lu.x) / a[0] : 0.f,
-                        a[1] != 0.f ? delta.y * (L.y - lu.y) / a[1] : 0.f,
-                        a[2] != 0.f ? delta.z * (L.z - lu.z) / a[2] : 0.f};
+                gA = f3{a[0] != 0.f ? delta.x * (lf[0] - lu.x) / a[0] : 0.f,
+                        a[1] != 0.f ? delta.y * (lf[1] - lu.y) / a[1] : 0.f,
+                        a[2] != 0.f ? delta.z * (lf[2] - lu.z) / a[2] : 0.f};
                 gE = f3{delta.x * tp.x, delta.y * tp.y, delta.z * tp.z};
             }
-            if (LDS) {
-                if (has) {
-                    const f3 g = ga ? gA : gE;
-                    atomicAdd(&lg[3 * prim], g.x);
-                    atomicAdd(&lg[3 * prim + 1], g.y);
-                    atomicAdd(&lg[3 * prim + 2], g.z);
-                }
-            } else {
-                if (ga) merged_add(ga, has, prim, gA);
-                if (ge) merged_add(ge, has, prim, gE);
+        }
+        if (LDS) {
+            if (has) {
+                const f3 g = ga ? gA : gE;
+                atomicAdd(&lg[3 * prim], g.x);
+                atomicAdd(&lg[3 * prim + 1], g.y);
+                atomicAdd(&lg[3 * prim + 2], g.z);
             }
+        } else {
+            if (ga) merged_add(ga, has, prim, gA);
+            if (ge) merged_add(ge, has, prim, gE);
         }
     }
     if (LDS) {
@@ -643,10 +653,15 @@ struct Batch {
         RTX_HIP(hipEventRecord(done, s));
         return RTX_OK;
     }
-    // adjoint: dLoss/dimage in, gradients out (scene 0 only)
+    // adjoint (scene 0 only): dLoss/dimage in, gradients out. adj 0: a forward render (which
+    // also stores each path's radiance in Lpath when that is set); 1: a primal pass and the
+    // replay, per batch; 2: the replay alone, with the primal radiance read from Lpath.
+    int adj = 0;
     const float* dimg = nullptr;
     float* ga = nullptr;
     float* ge = nullptr;
+    float* Lpath = nullptr;    // 3 floats per path, indexed by the path's global number
+    float* Lbatch = nullptr;   // adj 1: the batch's primal radiance, 3 floats per batch path
     int setup(uint32_t batch_paths, bool adjoint = false);
     void enqueue_range(int s_first, int s_last, hipStream_t st);   // scenes [s_first, s_last)
     int launch(hipStream_t st);
@@ -671,12 +686,12 @@ int Batch::setup(uint32_t batch_paths, bool adjoint) {
     ncnt = 2 * (uint32_t)(max_b + 2);
     P.cnt = alloc<uint32_t>(ncnt);
     P.B = B;
-    P.vlup = nullptr; P.vtp = nullptr; P.nv = nullptr;
-    if (adjoint) {
-        P.vlup = alloc<float4>((size_t)B * (max_b + 1));
-        P.vtp = alloc<float4>((size_t)B * (max_b + 1));
-        P.nv = alloc<uint32_t>(B);
-        if (!P.vlup || !P.vtp || !P.nv) return fail(RTX_E_HIP, "device allocation failed on render setup");
+    P.rec0 = nullptr; P.rec1 = nullptr;
+    if (adjoint) {   // flat in max_bounce: one record per path, plus the batch's primal radiance
+        P.rec0 = alloc<float4>(B);
+        P.rec1 = alloc<float4>(B);
+        Lbatch = alloc<float>(3ull * B);
+        if (!P.rec0 || !P.rec1 || !Lbatch) return fail(RTX_E_HIP, "device allocation failed on render setup");
     }
     if (!d_scenes || !d_off || !P.ray0 || !P.ray1 || !P.hit || !P.tp || !P.L || !P.sid || !P.pix || !P.smp ||
         !P.q[0] || !P.q[1] || !P.cnt)
@@ -690,7 +705,6 @@ int Batch::setup(uint32_t batch_paths, bool adjoint) {
 void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
     const uint32_t grid = g_tuning.grid_blocks, sh = g_tuning.shade_threads, wait = g_tuning.render_wait_lanes;
     const uint32_t pgrid = g_tuning.trace_grid_blocks;   // blocks of the persistent trace kernels
-    const bool adj = dimg != nullptr;
     int mb = 0;
     for (int s = s_first; s < s_last; ++s) {
         const SceneDesc& S = descs[s];
@@ -701,74 +715,93 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
         mb = std::max(mb, S.max_bounce);
     }
     const int ns = (int)descs.size();
+    const bool one = s_last - s_first == 1;
+    const SceneDesc* sd = one ? d_scenes + s_first : d_scenes;
     for (uint64_t g0 = off[s_first]; g0 < off[s_last]; g0 += B) {
         const uint32_t cnt = (uint32_t)std::min<uint64_t>(B, off[s_last] - g0);
         const uint32_t cgrid = std::min<uint32_t>((cnt + 255) / 256, 65535);
-        k_set_counts<<<1, 64, 0, st>>>(P.cnt, ncnt, cnt);
-        k_camera<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
-        // bounce b traces queue q[qi] of cnt[b] paths; shade appends the survivors to q[qi^1],
-        // counting them in cnt[b+1]; the persistent trace kernel deals from cnt[ncnt/2 + b]
-        int qi = 0;
-        for (int b = 0; b <= mb; ++b) {
-            const uint32_t loop = g_tuning.render_loop == 0 ? (uint32_t)kLoopSingle
-                                  : g_tuning.render_leaf_coop ? (uint32_t)kLoopCoop : (uint32_t)kLoopWhileWhile;
-            // The persistent kernel hands each wave kRenderChunk paths per atomic: a batch
-            // with fewer paths than its waves can take in one round would leave most of
-            // the GPU idle, so such batches (the single scenes of a plain C3 render, say)
-            // take the chunked kernel, 64 paths per wave.
-            const bool persist =
-                g_tuning.render_loop == 2 && (uint64_t)cnt >= (uint64_t)pgrid * (kTraceThreads / 64) * kRenderChunk;
-            if (persist) {
-                uint32_t* work = P.cnt + ncnt / 2 + b;
-                with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
-                    constexpr int DD = decltype(D)::value, LL = decltype(L)::value;
-                    if (s_last - s_first == 1)
-                        k_wtrace_persist<DD, LL, kSceneUni>
-                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes + s_first, 1u, P, qi, b, wait, work);
-                    else if ((uint32_t)ns <= kLdsScenes)
-                        k_wtrace_persist<DD, LL, kSceneLds>
-                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, b, wait, work);
-                    else
-                        k_wtrace_persist<DD, LL, kSceneGlobal>
-                            <<<pgrid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, b, wait, work);
-                });
-            } else {
-                with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
-                    constexpr int DD = decltype(D)::value, LL = decltype(L)::value;
-                    if (s_last - s_first == 1)
-                        k_wtrace<DD, LL, kSceneUni>
-                            <<<grid, kTraceThreads, 0, st>>>(d_scenes + s_first, 1u, P, qi, b, wait);
-                    else if ((uint32_t)ns <= kLdsScenes)
-                        k_wtrace<DD, LL, kSceneLds>
-                            <<<grid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, b, wait);
-                    else
-                        k_wtrace<DD, LL, kSceneGlobal>
-                            <<<grid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, b, wait);
-                });
+        // One wavefront over the batch: camera rays, then max_bounce + 1 trace/shade rounds.
+        // The replay (kShadeReplay) scatters each bounce's gradients before the next bounce
+        // overwrites the records; Lfin + 3 * lbase is the batch's primal radiance.
+        auto wavefront = [&](int mode, const float* Lfin, uint64_t lbase) {
+            k_set_counts<<<1, 64, 0, st>>>(P.cnt, ncnt, cnt);
+            k_camera<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
+            // bounce b traces queue q[qi] of cnt[b] paths; shade appends the survivors to
+            // q[qi^1], counting them in cnt[b+1]; the persistent trace kernel deals from
+            // cnt[ncnt/2 + b]
+            int qi = 0;
+            for (int b = 0; b <= mb; ++b) {
+                const uint32_t loop = g_tuning.render_loop == 0 ? (uint32_t)kLoopSingle
+                                      : g_tuning.render_leaf_coop ? (uint32_t)kLoopCoop : (uint32_t)kLoopWhileWhile;
+                // The persistent kernel hands each wave kRenderChunk paths per atomic: a batch
+                // with fewer paths than its waves can take in one round would leave most of
+                // the GPU idle, so such batches (the single scenes of a plain C3 render, say)
+                // take the chunked kernel, 64 paths per wave.
+                const bool persist = g_tuning.render_loop == 2 &&
+                                     (uint64_t)cnt >= (uint64_t)pgrid * (kTraceThreads / 64) * kRenderChunk;
+                if (persist) {
+                    uint32_t* work = P.cnt + ncnt / 2 + b;
+                    with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
+                        constexpr int DD = decltype(D)::value, LL = decltype(L)::value;
+                        if (one)
+                            k_wtrace_persist<DD, LL, kSceneUni>
+                                <<<pgrid, kTraceThreads, 0, st>>>(d_scenes + s_first, 1u, P, qi, b, wait, work);
+                        else if ((uint32_t)ns <= kLdsScenes)
+                            k_wtrace_persist<DD, LL, kSceneLds>
+                                <<<pgrid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, b, wait, work);
+                        else
+                            k_wtrace_persist<DD, LL, kSceneGlobal>
+                                <<<pgrid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, b, wait, work);
+                    });
+                } else {
+                    with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
+                        constexpr int DD = decltype(D)::value, LL = decltype(L)::value;
+                        if (one)
+                            k_wtrace<DD, LL, kSceneUni>
+                                <<<grid, kTraceThreads, 0, st>>>(d_scenes + s_first, 1u, P, qi, b, wait);
+                        else if ((uint32_t)ns <= kLdsScenes)
+                            k_wtrace<DD, LL, kSceneLds>
+                                <<<grid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, b, wait);
+                        else
+                            k_wtrace<DD, LL, kSceneGlobal>
+                                <<<grid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, b, wait);
+                    });
+                }
+                if (mode == kShadeReplay) {
+                    if (one) k_shade<kShadeReplay, true><<<grid, sh, 0, st>>>(sd, P, qi, b);
+                    else k_shade<kShadeReplay, false><<<grid, sh, 0, st>>>(sd, P, qi, b);
+                    const uint32_t nprim = descs[0].nprim;
+                    if (nprim <= kLdsGradTris) {
+                        const uint32_t gb = std::min<uint32_t>(kGradLdsBlocks, (cnt + kGradThreads - 1) / kGradThreads);
+                        const size_t lds = 3 * nprim * sizeof(float);
+                        if (ga)
+                            k_grad<true><<<gb, kGradThreads, lds, st>>>(d_scenes, P, qi, b, Lfin, lbase, dimg, ga,
+                                                                        nullptr, nprim);
+                        if (ge)
+                            k_grad<true><<<gb, kGradThreads, lds, st>>>(d_scenes, P, qi, b, Lfin, lbase, dimg, nullptr,
+                                                                        ge, nprim);
+                    } else {
+                        k_grad<false><<<std::min<uint32_t>(4096, (cnt + kGradThreads - 1) / kGradThreads), kGradThreads,
+                                        0, st>>>(d_scenes, P, qi, b, Lfin, lbase, dimg, ga, ge, nprim);
+                    }
+                } else {
+                    if (one) k_shade<kShadeForward, true><<<grid, sh, 0, st>>>(sd, P, qi, b);
+                    else k_shade<kShadeForward, false><<<grid, sh, 0, st>>>(sd, P, qi, b);
+                }
+                qi ^= 1;
             }
-            const bool one = s_last - s_first == 1;
-            const SceneDesc* sd = one ? d_scenes + s_first : d_scenes;
-            if (adj) {
-                if (one) k_shade<true, true><<<grid, sh, 0, st>>>(sd, P, qi, b);
-                else k_shade<true, false><<<grid, sh, 0, st>>>(sd, P, qi, b);
-            } else {
-                if (one) k_shade<false, true><<<grid, sh, 0, st>>>(sd, P, qi, b);
-                else k_shade<false, false><<<grid, sh, 0, st>>>(sd, P, qi, b);
-            }
-            qi ^= 1;
+        };
+        if (adj == 0) {
+            wavefront(kShadeForward, nullptr, 0);
+            if (Lpath) k_store_L<<<cgrid, 256, 0, st>>>(P, cnt, Lpath, g0);
+            k_accum<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
+        } else if (adj == 1) {
+            wavefront(kShadeForward, nullptr, 0);
+            k_store_L<<<cgrid, 256, 0, st>>>(P, cnt, Lbatch, 0);
+            wavefront(kShadeReplay, Lbatch, 0);
+        } else {
+            wavefront(kShadeReplay, Lpath, g0);
         }
-        if (adj) {
-            const uint32_t nprim = descs[0].nprim;
-            if (nprim <= kLdsGradTris) {
-                const uint32_t gb = std::min<uint32_t>(512, (cnt + kGradThreads - 1) / kGradThreads);
-                const size_t lds = 3 * nprim * sizeof(float);
-                if (ga) k_grad<true><<<gb, kGradThreads, lds, st>>>(d_scenes, P, cnt, dimg, ga, nullptr, nprim);
-                if (ge) k_grad<true><<<gb, kGradThreads, lds, st>>>(d_scenes, P, cnt, dimg, nullptr, ge, nprim);
-            } else
-                k_grad<false><<<std::min<uint32_t>(4096, (cnt + kGradThreads - 1) / kGradThreads), kGradThreads, 0,
-                                st>>>(d_scenes, P, cnt, dimg, ga, ge, nprim);
-        }
-        else k_accum<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
     }
     if (!adj)
         for (int s = s_first; s < s_last; ++s)
@@ -876,27 +909,46 @@ extern "C" int rtx_render(const rtx_bvh* b, const rtx_camera* cam, const rtx_ren
     return rtx_render_batched(sc, 1, cam, p, al, em, im, 0, stream);
 }
 
-extern "C" int rtx_render_adjoint(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params* p,
-                                  const float* albedo_d, const float* emission_d, const float* d_image_d,
-                                  float* grad_albedo_d, float* grad_emission_d, void* stream) {
-    int rc = check_scene(b, cam, p, d_image_d, "rtx_render_adjoint()");
+extern "C" int rtx_render_primal(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params* p,
+                                 const float* albedo_d, const float* emission_d, float* image_d,
+                                 float* path_radiance_d, void* stream) {
+    int rc = check_scene(b, cam, p, image_d, "rtx_render_primal()");
     if (rc) return rc;
-    if (!albedo_d && !p->tex_d) return fail(RTX_E_ARG, "albedo is required on rtx_render_adjoint()");
+    if (!path_radiance_d) return fail(RTX_E_ARG, "null path_radiance_d on rtx_render_primal()");
+    Batch B;
+    B.descs.push_back(make_desc(b, cam, p, albedo_d, emission_d, image_d));
+    B.Lpath = path_radiance_d;
+    rc = B.setup(g_tuning.batch_paths);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    B.enqueue_range(0, 1, st);
+    rc = B.note_launch(st);
+    if (rc) return rc;
+    if (!hip_ok(hipGetLastError(), "primal launch") || !hip_ok(hipStreamSynchronize(st), "sync")) return RTX_E_HIP;
+    return RTX_OK;
+}
+
+namespace {
+int adjoint_impl(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params* p, const float* albedo_d,
+                 const float* emission_d, const float* path_radiance_d, const float* d_image_d, float* grad_albedo_d,
+                 float* grad_emission_d, void* stream, const char* fn) {
+    int rc = check_scene(b, cam, p, d_image_d, fn);
+    if (rc) return rc;
+    if (!albedo_d && !p->tex_d) return fail(RTX_E_ARG, std::string("albedo is required on ") + fn);
     if (p->tex_d && grad_emission_d)
-        return fail(RTX_E_ARG, "emission gradients are per triangle: not with texture albedo on rtx_render_adjoint()");
+        return fail(RTX_E_ARG, std::string("emission gradients are per triangle: not with texture albedo on ") + fn);
     if (p->rr_bounce <= p->max_bounce)
         return fail(RTX_E_ARG, "the adjoint needs rr_bounce > max_bounce (no Russian roulette)");
-    if (p->max_bounce > 64) return fail(RTX_E_LIMIT, "max_bounce above 64 on rtx_render_adjoint()");
+    if (p->max_bounce > 64) return fail(RTX_E_LIMIT, std::string("max_bounce above 64 on ") + fn);
     Batch B;
     B.descs.push_back(make_desc(b, cam, p, albedo_d, emission_d, nullptr));
+    B.adj = path_radiance_d ? 2 : 1;
+    B.Lpath = const_cast<float*>(path_radiance_d);
     B.dimg = d_image_d;
     B.ga = grad_albedo_d;
     B.ge = emission_d ? grad_emission_d : nullptr;
-    // the vertex records cost 32 B per hit per path: keep the batch within ~2 GiB of them
-    const uint64_t per_path = 32ull * (p->max_bounce + 1) + 96;
-    const uint32_t bp = (uint32_t)std::max<uint64_t>(1024, std::min<uint64_t>(g_tuning.batch_paths,
-                                                                               (2ull << 30) / per_path));
-    rc = B.setup(bp, true);
+    // the state per path does not depend on max_bounce: one record and the primal radiance
+    rc = B.setup(g_tuning.batch_paths, true);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     B.enqueue_range(0, 1, st);
@@ -904,4 +956,21 @@ extern "C" int rtx_render_adjoint(const rtx_bvh* b, const rtx_camera* cam, const
     if (rc) return rc;
     if (!hip_ok(hipGetLastError(), "adjoint launch") || !hip_ok(hipStreamSynchronize(st), "sync")) return RTX_E_HIP;
     return RTX_OK;
+}
+}  // namespace
+
+extern "C" int rtx_render_adjoint(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params* p,
+                                  const float* albedo_d, const float* emission_d, const float* d_image_d,
+                                  float* grad_albedo_d, float* grad_emission_d, void* stream) {
+    return adjoint_impl(b, cam, p, albedo_d, emission_d, nullptr, d_image_d, grad_albedo_d, grad_emission_d, stream,
+                        "rtx_render_adjoint()");
+}
+
+extern "C" int rtx_render_adjoint_replay(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params* p,
+                                         const float* albedo_d, const float* emission_d, const float* path_radiance_d,
+                                         const float* d_image_d, float* grad_albedo_d, float* grad_emission_d,
+                                         void* stream) {
+    if (!path_radiance_d) return fail(RTX_E_ARG, "null path_radiance_d on rtx_render_adjoint_replay()");
+    return adjoint_impl(b, cam, p, albedo_d, emission_d, path_radiance_d, d_image_d, grad_albedo_d, grad_emission_d,
+                        stream, "rtx_render_adjoint_replay()");
 }

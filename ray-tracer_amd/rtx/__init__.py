@@ -20,10 +20,11 @@ NODE48 = np.dtype([("bmin", "<f4", 4), ("bmax", "<f4", 4), ("is_leaf", "u1"), ("
 
 # Every symbol include/rtx.h declares.
 EXPORTS = ["rtx_last_error", "rtx_device_count", "rtx_build", "rtx_build_device", "rtx_bvh_free",
-           "rtx_bvh_get_info", "rtx_bvh_export", "rtx_bvh_morton", "rtx_bvh_export_bfs", "rtx_debug_clip_slab", "rtx_debug_recip_check", "rtx_trace_closest",
+           "rtx_bvh_get_info", "rtx_bvh_export", "rtx_bvh_morton", "rtx_bvh_export_bfs", "rtx_debug_clip_slab", "rtx_debug_recip_check", "rtx_debug_scratch_peak",
+           "rtx_trace_closest",
            "rtx_trace_closest_noalloc", "rtx_trace_any", "rtx_trace_counts", "rtx_trace_closest_host", "rtx_save", "rtx_load",
            "rtx_scene_load", "rtx_free_host", "rtx_bake_scene", "rtx_geo_sphere", "rtx_geo_box", "rtx_render",
-           "rtx_render_adjoint", "rtx_render_batched", "rtx_batch_create", "rtx_batch_launch", "rtx_batch_free",
+           "rtx_render_primal", "rtx_render_adjoint", "rtx_render_adjoint_replay", "rtx_render_batched", "rtx_batch_create", "rtx_batch_launch", "rtx_batch_free",
            "rtx_set_tuning", "rtx_get_tuning", "rtx_hdr_write", "rtx_hdr_read", "rtx_profile_begin",
            "rtx_profile_end"]
 
@@ -89,6 +90,7 @@ def lib():
         L.rtx_bvh_morton.argtypes = [vp, vp, vp]
         L.rtx_debug_clip_slab.argtypes = [vp, vp, C.c_int, vp]
         L.rtx_debug_recip_check.argtypes = [C.c_uint64, C.c_uint64, vp, vp]
+        L.rtx_debug_scratch_peak.argtypes = [vp, C.c_int]
         L.rtx_bvh_export_bfs.argtypes = [vp, vp, C.POINTER(u32)]
         L.rtx_trace_closest.argtypes = [vp, vp, sz, vp, vp, vp]
         L.rtx_trace_closest_noalloc.argtypes = [vp, vp, sz, vp, vp, vp]
@@ -105,7 +107,10 @@ def lib():
         L.rtx_geo_sphere.argtypes = [vp, C.c_float, i32, i32, vp, C.POINTER(u32), vp, C.POINTER(u32)]
         L.rtx_geo_box.argtypes = [vp, vp, vp, vp, C.POINTER(u32), vp, C.POINTER(u32)]
         L.rtx_render.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp]
+        L.rtx_render_primal.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp, vp]
         L.rtx_render_adjoint.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp, vp, vp]
+        L.rtx_render_adjoint_replay.argtypes = [vp, C.POINTER(Camera), C.POINTER(RenderParams), vp, vp, vp, vp, vp, vp,
+                                                vp]
         L.rtx_render_batched.argtypes = [vp, i32, vp, vp, vp, vp, vp, i32, vp]
         L.rtx_batch_create.argtypes = [vp, i32, vp, vp, vp, vp, vp, i32, C.POINTER(vp)]
         L.rtx_batch_launch.argtypes = [vp, vp]
@@ -280,6 +285,13 @@ def debug_recip_check(first=0, count=1 << 32):
     fb = C.c_uint32(0)
     _check(lib().rtx_debug_recip_check(int(first), int(count), C.byref(n), C.byref(fb)), "rtx_debug_recip_check")
     return n.value, (None if n.value == 0 else fb.value)
+
+
+def debug_scratch_peak(reset=False):
+    """Peak device scratch bytes held by library calls at once since the last reset."""
+    v = C.c_uint64(0)
+    _check(lib().rtx_debug_scratch_peak(C.byref(v), int(reset)), "rtx_debug_scratch_peak")
+    return v.value
 
 
 def build(verts6, tris):
@@ -472,12 +484,29 @@ def render(bvh, cam, params, albedo=None, emission=None, out=None, stream=None):
     return out
 
 
-def render_adjoint(bvh, cam, params, albedo, emission, d_image, grad_albedo=None, grad_emission=None, stream=None):
+def render_primal(bvh, cam, params, albedo=None, emission=None, out=None, stream=None):
+    """render() that also returns each path's radiance ((spp * H * W, 3) float32 CUDA
+    tensor, path sample * H * W + pixel): the primal that render_adjoint_replay consumes."""
+    import torch
+    _check_scene(bvh, params, albedo, emission)
+    if out is None:
+        out = torch.empty((params.height, params.width, 3), dtype=torch.float32, device="cuda")
+    _need(out, "out", "f32", 3 * params.width * params.height)
+    paths = torch.empty((params.spp * params.height * params.width, 3), dtype=torch.float32, device="cuda")
+    _check(lib().rtx_render_primal(bvh.handle, C.byref(make_camera(cam)), C.byref(params), _dptr(albedo),
+                                   _dptr(emission), _dptr(out), _dptr(paths), _stream_ptr(stream)), "rtx_render_primal")
+    return out, paths
+
+
+def render_adjoint(bvh, cam, params, albedo, emission, d_image, grad_albedo=None, grad_emission=None, stream=None,
+                   path_radiance=None):
     """Accumulate dLoss/dalbedo and dLoss/demission given dLoss/dimage (path replay). With
     texture albedo (params.tex_d set) grad_albedo is per texel and there is no emission
-    gradient."""
+    gradient. path_radiance (from render_primal of the same render) saves the primal pass."""
     import torch
     textured = bool(params.tex_d)
+    if path_radiance is not None:
+        _need(path_radiance, "path_radiance", "f32", 3 * params.spp * params.height * params.width)
     _check_scene(bvh, params, albedo, emission)
     d_image = d_image.contiguous()
     _need(d_image, "d_image", "f32", 3 * params.width * params.height)
@@ -488,9 +517,15 @@ def render_adjoint(bvh, cam, params, albedo, emission, d_image, grad_albedo=None
     _need(grad_albedo, "grad_albedo", "f32",
           3 * (params.tex_width * params.tex_height if textured else bvh.info()["input_tri_count"]))
     _need(grad_emission, "grad_emission", "f32", 3 * bvh.info()["input_tri_count"], optional=True)
-    _check(lib().rtx_render_adjoint(bvh.handle, C.byref(make_camera(cam)), C.byref(params), _dptr(albedo),
-                                    _dptr(emission), _dptr(d_image), _dptr(grad_albedo),
-                                    _dptr(grad_emission), _stream_ptr(stream)), "rtx_render_adjoint")
+    if path_radiance is None:
+        _check(lib().rtx_render_adjoint(bvh.handle, C.byref(make_camera(cam)), C.byref(params), _dptr(albedo),
+                                        _dptr(emission), _dptr(d_image), _dptr(grad_albedo),
+                                        _dptr(grad_emission), _stream_ptr(stream)), "rtx_render_adjoint")
+    else:
+        _check(lib().rtx_render_adjoint_replay(bvh.handle, C.byref(make_camera(cam)), C.byref(params), _dptr(albedo),
+                                               _dptr(emission), _dptr(path_radiance), _dptr(d_image),
+                                               _dptr(grad_albedo), _dptr(grad_emission), _stream_ptr(stream)),
+               "rtx_render_adjoint_replay")
     return grad_albedo, grad_emission
 
 
@@ -529,20 +564,21 @@ def _autograd_fn():
     import torch
 
     class RenderFunction(torch.autograd.Function):
-        """image = render(albedo, emission); backward re-traces the paths (path replay)."""
+        """image = render(albedo, emission); the forward keeps each path's radiance (12 bytes
+        per path, whatever max_bounce) and backward re-traces the paths (path replay)."""
 
         @staticmethod
         def forward(ctx, albedo, emission, bvh, cam, params):
-            img = render(bvh, cam, params, albedo, emission)
-            ctx.save_for_backward(albedo, emission)
+            img, paths = render_primal(bvh, cam, params, albedo, emission)
+            ctx.save_for_backward(albedo, emission, paths)
             ctx.scene = (bvh, cam, params)
             return img
 
         @staticmethod
         def backward(ctx, grad_img):
-            albedo, emission = ctx.saved_tensors
+            albedo, emission, paths = ctx.saved_tensors
             bvh, cam, params = ctx.scene
-            ga, ge = render_adjoint(bvh, cam, params, albedo, emission, grad_img.contiguous())
+            ga, ge = render_adjoint(bvh, cam, params, albedo, emission, grad_img.contiguous(), path_radiance=paths)
             return ga, ge, None, None, None
 
     return RenderFunction
@@ -559,17 +595,17 @@ def _texture_autograd_fn():
         def forward(ctx, tex, uv, emission, bvh, cam, params):
             tex_c = tex.detach().contiguous()
             set_texture(params, uv, tex_c)
-            img = render(bvh, cam, params, None, emission)
-            ctx.save_for_backward(tex_c)
+            img, paths = render_primal(bvh, cam, params, None, emission)
+            ctx.save_for_backward(tex_c, paths)
             ctx.scene = (uv, emission, bvh, cam, params)
             return img
 
         @staticmethod
         def backward(ctx, grad_img):
-            (tex_c,) = ctx.saved_tensors
+            tex_c, paths = ctx.saved_tensors
             uv, emission, bvh, cam, params = ctx.scene
             set_texture(params, uv, tex_c)
-            gtex, _ = render_adjoint(bvh, cam, params, None, emission, grad_img.contiguous())
+            gtex, _ = render_adjoint(bvh, cam, params, None, emission, grad_img.contiguous(), path_radiance=paths)
             return gtex, None, None, None, None, None
 
     return TextureRenderFunction

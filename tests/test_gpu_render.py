@@ -310,3 +310,50 @@ def test_random_walk_closed_forms_on_gpu():
         print("[closed form]", r, flush=True)
     bad = [r for r in results if not r.ok]
     assert not bad, bad
+
+
+def test_primal_render_and_path_radiance():
+    """rtx_render_primal gives rtx_render's image bit for bit, and each pixel is the mean of
+    its stored path radiances summed in sample order (k_accum's order)."""
+    v, t, albedo, emission, cam = S.diff_scene(seed=3)
+    g = rtx.build(v, t)
+    W, H, spp = 32, 24, 5
+    p = rtx.make_params(W, H, spp, 4, 99, seed=2)
+    a_t, e_t = _tensor(albedo), _tensor(emission)
+    ref = rtx.render(g, cam, p, a_t, e_t).cpu().numpy()
+    img, paths = rtx.render_primal(g, cam, p, a_t, e_t)
+    img, paths = img.cpu().numpy(), paths.cpu().numpy().reshape(spp, H * W, 3)
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+    acc = np.zeros((H * W, 3), np.float32)
+    for s in range(spp):
+        acc = (acc + paths[s]).astype(np.float32)
+    assert np.array_equal((acc / np.float32(spp)).astype(np.float32).reshape(H, W, 3).view(np.uint32),
+                          ref.view(np.uint32))
+
+
+def test_adjoint_replay_equals_self_contained_and_memory_is_flat():
+    """The replay from stored path radiance and the self-contained adjoint (its own primal
+    pass) give the same gradients (up to the order of float atomics), and the adjoint's
+    device scratch does not grow with max_bounce."""
+    import torch
+    v, t, albedo, emission, cam = S.diff_scene(seed=3)
+    g = rtx.build(v, t)
+    W, H, spp = 48, 40, 4
+    a_t, e_t = _tensor(albedo), _tensor(emission)
+    peaks = {}
+    for mb in (2, 30):
+        p = rtx.make_params(W, H, spp, mb, 99, seed=5)
+        img, paths = rtx.render_primal(g, cam, p, a_t, e_t)
+        d = (2 * (img - 0.3)).contiguous()
+        ga1, ge1 = rtx.render_adjoint(g, cam, p, a_t, e_t, d, path_radiance=paths)
+        torch.cuda.synchronize()
+        rtx.debug_scratch_peak(reset=True)
+        ga2, ge2 = rtx.render_adjoint(g, cam, p, a_t, e_t, d)
+        torch.cuda.synchronize()
+        peaks[mb] = rtx.debug_scratch_peak(reset=True)
+        for x, y in ((ga1, ga2), (ge1, ge2)):
+            x, y = x.cpu().numpy(), y.cpu().numpy()
+            assert np.abs(x).max() > 0
+            np.testing.assert_allclose(x, y, rtol=1e-4, atol=1e-6 * np.abs(x).max())
+    print(f"[adjoint memory] scratch peak {peaks[2]} B at max_bounce 2, {peaks[30]} B at max_bounce 30", flush=True)
+    assert peaks[2] > 0 and abs(peaks[30] - peaks[2]) <= 4096
