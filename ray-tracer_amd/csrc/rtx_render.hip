@@ -489,24 +489,29 @@ __device__ __forceinline__ void merged_add(float* __restrict__ buf, bool has, ui
 //   dL/da_k = (L - L_upto_k) / a_k        dL/dLe_k = throughput arriving at k,
 // where L_upto_k includes vertex k's own emission. Only the current bounce is ever stored,
 // so the state per path does not grow with max_bounce.
-// LDS=true: each block accumulates into a private copy of ONE gradient array in LDS
-// (scenes of up to kLdsGradTris triangles; ga or ge is null, the kernel runs once per
-// array) and flushes it once with global atomics; otherwise lanes add to global memory
-// after merging equal triangles within the wave.
-constexpr uint32_t kLdsGradTris = 5120;   // 3 floats each: 60 KiB of LDS
-constexpr int kGradThreads = 512;
-constexpr uint32_t kGradLdsBlocks = 128;    // LDS form: blocks per bounce, each flushing its copy once
+// LDS=true: each block accumulates into private copies of the gradient arrays in LDS (ga
+// at [0, 3 nprim), ge after it when both are set) and adds their nonzero entries to global
+// memory once at the end; otherwise lanes add to global memory after merging equal
+// triangles within the wave. Neighbouring queue entries are neighbouring pixels, so one
+// block's paths reach few distinct triangles and its flush is short.
+constexpr size_t kLdsGradBytes = 160 * 1024;   // LDS a gfx950 workgroup may allocate
+constexpr int kGradThreads = 1024;
+// LDS form: blocks per bounce. C4 forward+adjoint: 128 blocks of 512 threads, one launch per
+// array, 19.6 ms; 512 such blocks 16.8 ms; both arrays in one launch 16.5 ms; 256 blocks of
+// 1024 threads 16.2 ms.
+constexpr uint32_t kGradLdsBlocks = 256;
 
 template <bool LDS>
 __global__ void __launch_bounds__(kGradThreads) k_grad(const SceneDesc* __restrict__ scenes, PathState P, int qi,
                                                        int bounce, const float* __restrict__ Lfin, uint64_t lbase,
                                                        const float* __restrict__ dimg, float* __restrict__ ga,
                                                        float* __restrict__ ge, uint32_t nprim) {
-    extern __shared__ float lg[];   // LDS: [nprim * 3] gradient of the one non-null array
+    extern __shared__ float lg[];   // LDS: ga's copy (if set), then ge's (if set), 3 nprim floats each
     const SceneDesc& S = scenes[0];
     const float fs = (float)S.spp;
+    const uint32_t n3 = 3 * nprim, nlds = (ga ? n3 : 0) + (ge ? n3 : 0), eoff = ga ? n3 : 0;
     if (LDS) {
-        for (uint32_t j = threadIdx.x; j < 3 * nprim; j += blockDim.x) lg[j] = 0.f;
+        for (uint32_t j = threadIdx.x; j < nlds; j += blockDim.x) lg[j] = 0.f;
         __syncthreads();
     }
     const uint32_t n = P.cnt[bounce];
@@ -540,11 +545,15 @@ __global__ void __launch_bounds__(kGradThreads) k_grad(const SceneDesc* __restri
             }
         }
         if (LDS) {
-            if (has) {
-                const f3 g = ga ? gA : gE;
-                atomicAdd(&lg[3 * prim], g.x);
-                atomicAdd(&lg[3 * prim + 1], g.y);
-                atomicAdd(&lg[3 * prim + 2], g.z);
+            if (has && ga) {
+                atomicAdd(&lg[3 * prim], gA.x);
+                atomicAdd(&lg[3 * prim + 1], gA.y);
+                atomicAdd(&lg[3 * prim + 2], gA.z);
+            }
+            if (has && ge) {
+                atomicAdd(&lg[eoff + 3 * prim], gE.x);
+                atomicAdd(&lg[eoff + 3 * prim + 1], gE.y);
+                atomicAdd(&lg[eoff + 3 * prim + 2], gE.z);
             }
         } else {
             if (ga) merged_add(ga, has, prim, gA);
@@ -553,10 +562,9 @@ __global__ void __launch_bounds__(kGradThreads) k_grad(const SceneDesc* __restri
     }
     if (LDS) {
         __syncthreads();
-        float* out = ga ? ga : ge;
-        for (uint32_t j = threadIdx.x; j < 3 * nprim; j += blockDim.x) {
+        for (uint32_t j = threadIdx.x; j < nlds; j += blockDim.x) {
             const float x = lg[j];
-            if (x != 0.f) atomicAdd(&out[j], x);
+            if (x != 0.f) atomicAdd(j < eoff ? &ga[j] : &ge[j - eoff], x);
         }
     }
 }
@@ -688,6 +696,12 @@ int Batch::setup(uint32_t batch_paths, bool adjoint) {
     P.B = B;
     P.rec0 = nullptr; P.rec1 = nullptr;
     if (adjoint) {   // flat in max_bounce: one record per path, plus the batch's primal radiance
+        // the LDS form of k_grad may take more than the default 64 KiB of dynamic LDS
+        static const bool lds_ok = hip_ok(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_grad<true>),
+                                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                              (int)kLdsGradBytes),
+                                          "hipFuncSetAttribute");
+        if (!lds_ok) return RTX_E_HIP;
         P.rec0 = alloc<float4>(B);
         P.rec1 = alloc<float4>(B);
         Lbatch = alloc<float>(3ull * B);
@@ -771,14 +785,17 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
                     if (one) k_shade<kShadeReplay, true><<<grid, sh, 0, st>>>(sd, P, qi, b);
                     else k_shade<kShadeReplay, false><<<grid, sh, 0, st>>>(sd, P, qi, b);
                     const uint32_t nprim = descs[0].nprim;
-                    if (nprim <= kLdsGradTris) {
-                        const uint32_t gb = std::min<uint32_t>(kGradLdsBlocks, (cnt + kGradThreads - 1) / kGradThreads);
-                        const size_t lds = 3 * nprim * sizeof(float);
+                    const size_t arr = 3ull * nprim * sizeof(float);
+                    const uint32_t gb = std::min<uint32_t>(kGradLdsBlocks, (cnt + kGradThreads - 1) / kGradThreads);
+                    if ((ga ? arr : 0) + (ge ? arr : 0) <= kLdsGradBytes) {   // both arrays in one launch
+                        k_grad<true><<<gb, kGradThreads, (ga ? arr : 0) + (ge ? arr : 0), st>>>(
+                            d_scenes, P, qi, b, Lfin, lbase, dimg, ga, ge, nprim);
+                    } else if (arr <= kLdsGradBytes) {                         // one launch per array
                         if (ga)
-                            k_grad<true><<<gb, kGradThreads, lds, st>>>(d_scenes, P, qi, b, Lfin, lbase, dimg, ga,
+                            k_grad<true><<<gb, kGradThreads, arr, st>>>(d_scenes, P, qi, b, Lfin, lbase, dimg, ga,
                                                                         nullptr, nprim);
                         if (ge)
-                            k_grad<true><<<gb, kGradThreads, lds, st>>>(d_scenes, P, qi, b, Lfin, lbase, dimg, nullptr,
+                            k_grad<true><<<gb, kGradThreads, arr, st>>>(d_scenes, P, qi, b, Lfin, lbase, dimg, nullptr,
                                                                         ge, nprim);
                     } else {
                         k_grad<false><<<std::min<uint32_t>(4096, (cnt + kGradThreads - 1) / kGradThreads), kGradThreads,

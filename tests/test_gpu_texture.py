@@ -40,7 +40,8 @@ def test_textured_terrain_matches_oracle(rr):
     assert img.max() > 1.0   # the sun is seen through bounces
 
 
-@pytest.mark.parametrize("tex_res", [8, 80])   # 64 texels: LDS gradients; 6400: global atomics
+# 64 and 6400 texels: gradients accumulated in LDS (up to 160 KiB per block); 65536: global atomics
+@pytest.mark.parametrize("tex_res", [8, 80, 256])
 def test_texture_gradient_matches_finite_differences(tex_res):
     import torch
     v, t, uv, tex_true, cam, sky, sun = _setup(tex_res=tex_res)
