@@ -23,6 +23,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "ray-tracer_amd")]
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs, a wave64 VALU instruction every 4 cycles per SIMD, at
+# the 2.4 GHz peak engine clock (MI355X's published 157.3 TFLOP/s vector fp32 is this rate
+# x 64 lanes x 2 flops per FMA x 2 for packed fp32).
+VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
+
+# Counter passes of the measured kernels (each pass fits the per-block counter limits:
+# FETCH_SIZE takes 3 TCC counters, WRITE_SIZE 2).
+PMC_PASSES = [["FETCH_SIZE"],
+              ["WRITE_SIZE", "TCC_HIT_sum", "TCC_MISS_sum"],
+              ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAIT_ANY", "SQ_WAVE_CYCLES",
+               "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR"]]
 
 
 def cpu_model():
@@ -62,7 +73,17 @@ def main():
     ap.add_argument("--skip-extras", action="store_true",
                     help="time only the headline launch (used under the profiler so k_trace's average is "
                          "the incoherent launch alone)")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter passes (HBM traffic, L2 hit rate, VALU issue)")
+    ap.add_argument("--counter-legs", default="trace,c4",
+                    help="legs measured under rocprofv3: trace (headline), c4 (differentiable render), "
+                         "c5 (terrain inverse render, slow)")
+    ap.add_argument("--counters-dir", default=None, help="keep the raw rocprofv3 output here")
+    ap.add_argument("--counter-child", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.counter_child:
+        counter_child(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the traversal shards by rays with the "
               "BVH replicated, and no multi-GPU figure is produced here.", file=sys.stderr)
@@ -169,26 +190,41 @@ def main():
         ms_any, _, _ = timed(lambda: bvh.trace_any_device(inc, occ, stream=stream), max(3, args.steps // 4), 1)
         rtx.set_tuning(**trace_tuning)
 
-    # Algorithmic bytes of the incoherent launch: 32 B ray + 16 B hit + 64 B per internal
-    # node visited + 48 B per triangle tested, counts from the instrumented kernel.
+    # Hits of the timed launch against the binary walk (the reference's visiting order): t
+    # bit-identical for every ray, triangle ids equal except at exact-t ties.
+    bvh.trace_closest_device(inc, hits, stream=stream)
+    hits_bin = torch.empty_like(hits)
+    rtx.set_tuning(**dict(trace_tuning, trace_tree=0))
+    bvh.trace_closest_device(inc, hits_bin, stream=stream)
+    # Algorithmic bytes (the per-ray model: 32 B ray + 16 B hit + 64 B per internal node
+    # visited + 48 B per triangle tested), counted on the binary walk's visit sequence.
     bvh.trace_counts_device(inc, counts, stream=stream)
+    rtx.set_tuning(**trace_tuning)
     torch.cuda.synchronize()
     c = counts.to(torch.int64).sum(0).cpu().numpy()
     visits, tests = int(c[0]), int(c[1])
     alg_bytes = 48 * n + 64 * visits + 48 * tests
-    # roofline of the dominant kernel: its own average duration (HIP events around the
-    # traversal kernel on the launch stream; the ray sort before it is not included)
-    achieved = alg_bytes / (kms_inc * 1e-3) / 1e9
-
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "trace_closest_c2_traffic.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    hb, hw = hits_bin.view(torch.int32), hits.view(torch.int32)
+    diff_id = hb[:, 3] != hw[:, 3]
+    id_check = {"rays": n, "t_bit_identical": bool(torch.equal(hb[:, 0], hw[:, 0])),
+                "id_differences": int(diff_id.sum()),
+                "id_differences_at_exact_t_ties": int((diff_id & (hb[:, 0] == hw[:, 0])).sum())}
+    del hits_bin
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(bvh, v, t, inc[: 1 << 21].cpu().numpy(), coh[: 1 << 20].cpu().numpy(), args.cpu_seconds)
+        cpu = cpu_baseline(bvh, v, t, inc[: 1 << 21].cpu().numpy(), coh[: 1 << 20].cpu().numpy(), args.cpu_seconds,
+                           gpu_hits=hits[: 1 << 21].cpu().numpy())
+
+    # HBM traffic, L2 hit rate and VALU issue of the headline kernel from rocprofv3 counters
+    # of this same launch (child processes after the timed region), priced on its live
+    # kernel time above
+    counters, roofline = {}, None
+    if not args.no_counters:
+        for leg in [x for x in args.counter_legs.split(",") if x]:
+            tun = trace_tuning if leg == "trace" else None
+            counters[leg] = measure_counters(leg, tun, args)
+        roofline = trace_roofline(counters.get("trace"), kms_inc)
 
     render = None
     if not (args.no_render or args.skip_extras):
@@ -201,8 +237,12 @@ def main():
         rtx.set_tuning(**before)
         render = {"differentiable": bench_differentiable(stream), "batched": batched,
                   "cornell": bench_cornell(stream, cpu=not args.no_cpu, tune=not args.no_autotune)}
+        if "c4" in counters:
+            render["differentiable"]["roofline"] = render_roofline(counters["c4"], "k_wtrace_persist")
         if not args.no_inverse:
             render["inverse_terrain"] = bench_inverse(stream, steps=args.inverse_steps)
+            if "c5" in counters:
+                render["inverse_terrain"]["roofline"] = render_roofline(counters["c5"], "k_wtrace_persist")
 
     mrays = n / (ms_inc * 1e-3) / 1e6
     out = {
@@ -221,10 +261,15 @@ def main():
         "config": {"workload": "BVH build + traversal microbenchmark: 1M random fp32 triangles (seed 1), "
                                "4,194,304 incoherent rays, closest hit",
                    "triangles": args.tris, "rays": n, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "extra": {
+            "hit_ids_vs_binary_walk": id_check,
+            "algorithmic_model": {"bytes_per_launch": alg_bytes,
+                                  "rate_gbs": round(alg_bytes / (kms_inc * 1e-3) / 1e9, 1),
+                                  "note": "counts every node and triangle read as memory traffic, cache hits "
+                                          "included; not an HBM figure"},
+            "counters": counters or None,
             "coherent_closest_mrays_s": None if args.skip_extras else round(n / (ms_coh * 1e-3) / 1e6, 2),
             "host_buffers": host,
             "incoherent_any_hit_mrays_s": None if args.skip_extras else round(n / (ms_any * 1e-3) / 1e6, 2),
@@ -233,7 +278,6 @@ def main():
             "tree": {k: info[k] for k in ("node_count", "tri_count", "max_depth", "levels")},
             "avg_internal_visits_per_ray": round(visits / n, 2),
             "avg_triangle_tests_per_ray": round(tests / n, 2),
-            "algorithmic_bytes_per_launch": alg_bytes,
             "trace_kernel_ms": round(kms_inc, 4),
             "ray_sort_and_launch_ms": round(ms_inc - kms_inc, 4),
             "tuning": trace_tuning,
@@ -438,7 +482,211 @@ def bench_batched(stream, nscenes, reps=2, tune=True):
     return out
 
 
-def cpu_baseline(bvh, verts, tris, rays, coh_rays, seconds):
+def counter_child(args):
+    """One leg of the workload, run by measure_counters under rocprofv3: the 1 GiB device copy
+    that calibrates the HBM counters, then the leg's kernels with the given launch shapes."""
+    import numpy as np
+    import torch
+
+    import rtx
+    from rtx import scenes as S
+    dev = torch.device("cuda:0")
+    src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    if args.tuning:
+        rtx.set_tuning(**json.loads(args.tuning))
+    leg = args.counter_child
+    if leg == "trace":
+        v, t = S.random_triangles(args.tris, seed=1)
+        bvh = rtx.build_device(torch.from_numpy(v).to(dev), torch.from_numpy(t.view(np.int32)).to(dev))
+        inc = torch.from_numpy(S.incoherent_rays(args.rays, seed=1)).to(dev)
+        hits = torch.empty((args.rays, 4), dtype=torch.float32, device=dev)
+        for _ in range(3):
+            bvh.trace_closest_device(inc, hits)
+    elif leg == "c4":
+        v, t, albedo_true, emission, cam = S.diff_scene(seed=3)
+        g = rtx.build(v, t)
+        p = rtx.make_params(512, 512, 32, 4, 99, seed=3)
+        a_t, e_t = torch.from_numpy(albedo_true).cuda(), torch.from_numpy(emission).cuda()
+        target = rtx.render(g, cam, p, a_t, e_t)
+        for _ in range(2):
+            a = (a_t * 0.9).requires_grad_(True)
+            e = e_t.detach().clone().requires_grad_(True)
+            ((rtx.differentiable_render(a, e, g, cam, p) - target) ** 2).sum().backward()
+    elif leg == "c5":
+        v, t, uv = S.terrain(2048, seed=4)
+        g = rtx.build_device(torch.from_numpy(v).cuda(), torch.from_numpy(t.view(np.int32)).cuda())
+        cam, sky, sun = S.terrain_setup()
+        uv_t = torch.from_numpy(uv).cuda()
+        tex_true = torch.from_numpy(S.albedo_texture(512, seed=4)).cuda()
+        p = rtx.make_params(1920, 1080, 64, 8, 99, seed=4, env_mode=3, env_color=sky, uv=uv_t, tex=tex_true, sun=sun)
+        target = rtx.render(g, cam, p).clone()
+        x = torch.full_like(tex_true, 0.5).requires_grad_(True)
+        ((rtx.differentiable_render_texture(x, uv_t, g, cam, p) - target) ** 2).sum().backward()
+    else:
+        raise SystemExit(f"unknown counter leg {leg}")
+    torch.cuda.synchronize()
+
+
+def _kernel_family(name):
+    """Kernel name without namespaces and parameter list: k_trace_persist<false, false, 12, 2, 1>."""
+    n = name.replace("rtx::(anonymous namespace)::", "").replace("rtx::", "").replace("void ", "")
+    depth, end = 0, len(n)
+    for i, ch in enumerate(n):   # the parameter list starts at the first '(' outside <...>
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            end = i
+            break
+    return n[:end].strip()
+
+
+def measure_counters(leg, tuning, args):
+    """Runs `bench.py --counter-child leg` under rocprofv3: three --pmc passes (PMC_PASSES)
+    and one --kernel-trace pass, each in its own process (counters never combined with a
+    trace domain). Returns per-kernel dispatch averages: calibrated HBM bytes read and
+    written (FETCH_SIZE and WRITE_SIZE, each scaled by what the same counter reads for the
+    1 GiB copy that starts the child), L2 hit rate, VALU instructions and lanes active,
+    the fraction of wave time spent waiting, and the average duration."""
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    rp = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    base = args.counters_dir or tempfile.mkdtemp(prefix="rtx_counters_")
+    child = [sys.executable, os.path.abspath(__file__), "--counter-child", leg, "--tris", str(args.tris),
+             "--rays", str(args.rays)]
+    if tuning:
+        child += ["--tuning", json.dumps(tuning)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    per = {}
+
+    def run(opts, tag):
+        d = os.path.join(base, f"{leg}_{tag}")
+        cmd = [rp, *opts, "-d", d, "-o", "run", "--output-format", "csv", "--", *child]
+        pr = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                              start_new_session=True)
+        try:
+            out, _ = pr.communicate(timeout=240)
+        except subprocess.TimeoutExpired:
+            os.killpg(pr.pid, signal.SIGKILL)
+            pr.communicate()
+            raise RuntimeError(f"rocprofv3 {tag} pass of {leg} timed out")
+        files = glob.glob(os.path.join(d, "**", "*.csv"), recursive=True)
+        # rocprofv3 may end with a segmentation fault after the program has finished and
+        # its results are complete
+        if pr.returncode not in (0, 139, -11) or not files:
+            raise RuntimeError(f"rocprofv3 {tag} pass of {leg} failed ({pr.returncode}): "
+                               f"{out.decode(errors='replace')[-400:]}")
+        return d
+
+    calib = {}
+    try:
+        for k, cs in enumerate(PMC_PASSES):
+            d = run(["--pmc", *cs], f"pmc{k}")
+            sums, disp = {}, {}
+            for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+                for r in csv.DictReader(open(f)):
+                    fam, cn, val = _kernel_family(r["Kernel_Name"]), r["Counter_Name"], float(r["Counter_Value"])
+                    if "copyBuffer" in fam and cn in ("FETCH_SIZE", "WRITE_SIZE"):
+                        calib[cn] = max(calib.get(cn, 0.0), val)
+                        continue
+                    sums.setdefault(fam, {}).setdefault(cn, 0.0)
+                    sums[fam][cn] += val
+                    disp.setdefault(fam, set()).add(r.get("Dispatch_Id", len(disp.get(fam, ()))))
+            for fam, cv in sums.items():
+                e = per.setdefault(fam, {})
+                e["dispatches"] = len(disp[fam])
+                for cn, val in cv.items():
+                    e[cn] = val / len(disp[fam])
+        d = run(["--kernel-trace"], "trace")
+        for f in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                fam = _kernel_family(r["Kernel_Name"])
+                if fam in per:
+                    e = per[fam]
+                    e.setdefault("_ns", []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    except (RuntimeError, OSError) as exc:
+        return {"error": str(exc)}
+    if "FETCH_SIZE" not in calib or "WRITE_SIZE" not in calib:
+        return {"error": "no calibration copy in the counter output"}
+    ff, fw = 1048576.0 / calib["FETCH_SIZE"], 1048576.0 / calib["WRITE_SIZE"]
+    out = {"calibration": {"copy_kib": 1048576, "fetch_factor": round(ff, 4), "write_factor": round(fw, 4)},
+           "kernels": {}}
+    for fam, e in per.items():
+        ns = e.get("_ns", [])
+        if not ns or "FETCH_SIZE" not in e or "SQ_INSTS_VALU" not in e:
+            continue
+        us = sum(ns) / len(ns) / 1e3
+        fetch, write = e["FETCH_SIZE"] * 1024 * ff, e.get("WRITE_SIZE", 0.0) * 1024 * fw
+        hit, miss = e.get("TCC_HIT_sum", 0.0), e.get("TCC_MISS_sum", 0.0)
+        out["kernels"][fam] = {
+            "dispatches": e["dispatches"], "avg_us": round(us, 2),
+            "hbm_read_bytes": round(fetch), "hbm_write_bytes": round(write),
+            "hbm_gbs": round((fetch + write) / (us * 1e-6) / 1e9, 1),
+            "l2_hit_rate": round(hit / (hit + miss), 4) if hit + miss else None,
+            "valu_insts": round(e["SQ_INSTS_VALU"]),
+            "valu_issue_frac": round(e["SQ_INSTS_VALU"] / (us * 1e-6) / VALU_PEAK_PER_S, 4),
+            "lanes_active": round(e["SQ_THREAD_CYCLES_VALU"] / (64 * e["SQ_ACTIVE_INST_VALU"]), 3)
+            if e.get("SQ_ACTIVE_INST_VALU") else None,
+            "wave_wait_frac": round(e["SQ_WAIT_ANY"] / e["SQ_WAVE_CYCLES"], 3) if e.get("SQ_WAVE_CYCLES") else None,
+            "lds_insts": round(e.get("SQ_INSTS_LDS", 0.0)), "vmem_rd_insts": round(e.get("SQ_INSTS_VMEM_RD", 0.0)),
+            "vmem_wr_insts": round(e.get("SQ_INSTS_VMEM_WR", 0.0))}
+    # the busiest kernels first
+    out["kernels"] = dict(sorted(out["kernels"].items(), key=lambda kv: -kv[1]["avg_us"] * kv[1]["dispatches"])[:12])
+    return out
+
+
+def trace_roofline(ctr, kernel_ms):
+    """Roofline of the headline traversal kernel: HBM bytes per launch from the counters,
+    priced on the kernel's live HIP-event duration of the timed launches; the VALU issue
+    rate beside it (the resource that binds this kernel)."""
+    if not ctr or "kernels" not in ctr:
+        return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None,
+                "error": (ctr or {}).get("error", "counters not measured")}
+    fam = [k for k in ctr["kernels"] if k.startswith("k_trace_persist") or k.startswith("k_trace<")]
+    if not fam:
+        return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": None,
+                "error": "traversal kernel not in the counter output"}
+    k = ctr["kernels"][fam[0]]
+    traffic = k["hbm_read_bytes"] + k["hbm_write_bytes"]
+    gbs = traffic / (kernel_ms * 1e-3) / 1e9
+    valu = k["valu_insts"] / (kernel_ms * 1e-3)
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": fam[0],
+            "kernel_ms": round(kernel_ms, 4), "limiter": "valu_issue",
+            "valu_issue": {"achieved": round(valu / 1e9, 1), "peak": round(VALU_PEAK_PER_S / 1e9, 1),
+                           "unit": "G wave64 VALU instructions/s", "frac": round(valu / VALU_PEAK_PER_S, 4)},
+            "l2_hit_rate": k["l2_hit_rate"], "lanes_active": k["lanes_active"],
+            "profiled_kernel_us": k["avg_us"]}
+
+
+def render_roofline(ctr, prefix):
+    """Roofline of a render leg's dominant kernel (the first kernel family named `prefix`),
+    all figures from its counter and kernel-trace passes."""
+    if not ctr or "kernels" not in ctr:
+        return None
+    fam = [k for k in ctr["kernels"] if k.startswith(prefix)]
+    if not fam:
+        return None
+    k = ctr["kernels"][fam[0]]
+    traffic = k["hbm_read_bytes"] + k["hbm_write_bytes"]
+    return {"bound": "hbm", "achieved": k["hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(k["hbm_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": fam[0],
+            "kernel_us": k["avg_us"], "limiter": "valu_issue" if k["valu_issue_frac"] > k["hbm_gbs"] / HBM_PEAK_GBS
+            else "hbm",
+            "valu_issue": {"frac": k["valu_issue_frac"]}, "l2_hit_rate": k["l2_hit_rate"],
+            "lanes_active": k["lanes_active"]}
+
+
+def cpu_baseline(bvh, verts, tris, rays, coh_rays, seconds, gpu_hits=None):
     """The compiled reference's trace() (single thread, as the reference runs) on a bounded
     sample of the same incoherent rays over the same tree; falls back to the oracle port.
     Beside it: the all-core port, a coherent-ray sample, and the reference's serial build
@@ -450,21 +698,31 @@ def cpu_baseline(bvh, verts, tris, rays, coh_rays, seconds):
     tree = O.Tree(nodes, rtris, prims, verts)
     kind = "reference" if O.ref_available() else "port"
 
-    def rate(rs, budget):
+    ids = []
+
+    def rate(rs, budget, keep=False):
         if kind == "reference":
             O.ref_trace(tree, rs[:1])   # load the tree into the reference tracer once
         done, t0 = 0, time.perf_counter()
         chunk = 2048
         while time.perf_counter() - t0 < budget and done < len(rs):
             r = rs[done: done + chunk]
-            if kind == "reference":
-                O.ref_trace(None, r)
-            else:
-                O.trace(tree, r, nthreads=1)
+            h = O.ref_trace(None, r) if kind == "reference" else O.trace(tree, r, nthreads=1)
+            if keep:
+                ids.append(np.stack([h["t"].view(np.uint32), h["tri"]], 1))
             done += len(r)
         el = time.perf_counter() - t0
         return done, el
-    done, el = rate(rays, seconds)
+    done, el = rate(rays, seconds, keep=gpu_hits is not None)
+    match = None
+    if gpu_hits is not None:
+        # the timed GPU launch's hits against the CPU reference's on the traced sample
+        ref = np.concatenate(ids)
+        g = gpu_hits[:len(ref)].view(np.uint32)
+        same_t = ref[:, 0] == g[:, 0]
+        diff = ref[:, 1] != g[:, 3]
+        match = {"rays": int(len(ref)), "t_bit_identical": bool(same_t.all()), "id_differences": int(diff.sum()),
+                 "id_differences_at_exact_t_ties": int((diff & same_t).sum())}
     value = done / el / 1e6
     cdone, cel = rate(coh_rays, seconds / 4)
     cores = os.cpu_count() or 1
@@ -483,7 +741,7 @@ def cpu_baseline(bvh, verts, tris, rays, coh_rays, seconds):
                       f"({el:.1f} s)",
             "coherent_mrays_s": round(cdone / cel / 1e6, 4), "coherent_sample": f"{cdone} coherent rays ({cel:.1f} s)",
             "allcore_port_mrays_s": round(allcore, 4), "allcore_threads": threads, "host_cores": cores,
-            "build_1m_serial_s": build_s, "cpu_model": cpu_model()}
+            "build_1m_serial_s": build_s, "cpu_model": cpu_model(), "gpu_hits_vs_cpu": match}
 
 
 if __name__ == "__main__":
