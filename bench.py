@@ -146,11 +146,16 @@ def main():
 
     # Launch shapes for the headline: the project autotuner (coordinate descent over LDS
     # stack depth, loop structure, ray sorting, leaf-phase wait and persistent grid size).
+    # The headline walks the binary tree (trace_tree 0): the reference's visiting order, so
+    # hit ids equal the reference's bit for bit, exact-t ties included; the 4-wide layout is
+    # timed beside it below.
+    headline_keys = [k for k in autotune.TRACE_KEYS if k != "trace_tree"]
+    rtx.set_tuning(trace_tree=0)
     trace_tuning, trace_tune_s = rtx.get_tuning(), 0.0
     if not args.no_autotune:
         t_tune = time.perf_counter()
         trace_tuning, _, _ = autotune.tune(lambda: (lambda: bvh.trace_closest_device(inc, hits, stream=stream)),
-                                           keys=autotune.TRACE_KEYS, reps=3)
+                                           keys=headline_keys, reps=3)
         trace_tune_s = time.perf_counter() - t_tune
     if args.tuning:
         trace_tuning = dict(trace_tuning, **json.loads(args.tuning))
@@ -180,8 +185,9 @@ def main():
         # that are already in camera order only costs the sort, for instance)
         if not args.no_autotune:
             coh_tuning, _, _ = autotune.tune(lambda: (lambda: bvh.trace_closest_device(coh, hits, stream=stream)),
-                                             keys=autotune.TRACE_KEYS, reps=3)
+                                             keys=headline_keys, reps=3)
             rtx.set_tuning(**trace_tuning)
+            # any-hit answers do not depend on the tree walked, so the layout is tuned too
             any_tuning, _, _ = autotune.tune(lambda: (lambda: bvh.trace_any_device(inc, occ, stream=stream)),
                                              keys=autotune.TRACE_KEYS, reps=3)
         rtx.set_tuning(**coh_tuning)
@@ -190,26 +196,31 @@ def main():
         ms_any, _, _ = timed(lambda: bvh.trace_any_device(inc, occ, stream=stream), max(3, args.steps // 4), 1)
         rtx.set_tuning(**trace_tuning)
 
-    # Hits of the timed launch against the binary walk (the reference's visiting order): t
-    # bit-identical for every ray, triangle ids equal except at exact-t ties.
-    bvh.trace_closest_device(inc, hits, stream=stream)
-    hits_bin = torch.empty_like(hits)
-    rtx.set_tuning(**dict(trace_tuning, trace_tree=0))
-    bvh.trace_closest_device(inc, hits_bin, stream=stream)
     # Algorithmic bytes (the per-ray model: 32 B ray + 16 B hit + 64 B per internal node
     # visited + 48 B per triangle tested), counted on the binary walk's visit sequence.
     bvh.trace_counts_device(inc, counts, stream=stream)
-    rtx.set_tuning(**trace_tuning)
     torch.cuda.synchronize()
     c = counts.to(torch.int64).sum(0).cpu().numpy()
     visits, tests = int(c[0]), int(c[1])
     alg_bytes = 48 * n + 64 * visits + 48 * tests
-    hb, hw = hits_bin.view(torch.int32), hits.view(torch.int32)
-    diff_id = hb[:, 3] != hw[:, 3]
-    id_check = {"rays": n, "t_bit_identical": bool(torch.equal(hb[:, 0], hw[:, 0])),
-                "id_differences": int(diff_id.sum()),
-                "id_differences_at_exact_t_ties": int((diff_id & (hb[:, 0] == hw[:, 0])).sum())}
-    del hits_bin
+    # The 4-wide layout with the same launch shapes: its rate, and its hits against the
+    # headline's (t bit-identical for every ray, triangle ids equal except at exact-t ties).
+    wide = None
+    if not args.skip_extras:
+        bvh.trace_closest_device(inc, hits, stream=stream)
+        hits_w = torch.empty_like(hits)
+        rtx.set_tuning(**dict(trace_tuning, trace_tree=1))
+        ms_w, _, kms_w = timed(lambda: bvh.trace_closest_device(inc, hits_w, stream=stream), max(5, args.steps // 2), 2)
+        rtx.set_tuning(**trace_tuning)
+        torch.cuda.synchronize()
+        hb, hw = hits.view(torch.int32), hits_w.view(torch.int32)
+        diff_id = hb[:, 3] != hw[:, 3]
+        wide = {"mrays_s": round(n / (ms_w * 1e-3) / 1e6, 2), "kernel_ms": round(kms_w, 4),
+                "vs_headline": {"rays": n, "t_bit_identical": bool(torch.equal(hb[:, 0], hw[:, 0])),
+                                "id_differences": int(diff_id.sum()),
+                                "id_differences_at_exact_t_ties": int((diff_id & (hb[:, 0] == hw[:, 0])).sum())}}
+        del hits_w
+    bvh.trace_closest_device(inc, hits, stream=stream)
 
     cpu = None
     if not args.no_cpu:
@@ -264,7 +275,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extra": {
-            "hit_ids_vs_binary_walk": id_check,
+            "wide_tree": wide,
             "algorithmic_model": {"bytes_per_launch": alg_bytes,
                                   "rate_gbs": round(alg_bytes / (kms_inc * 1e-3) / 1e9, 1),
                                   "note": "counts every node and triangle read as memory traffic, cache hits "
