@@ -80,9 +80,23 @@ def main():
                          "c5 (terrain inverse render, slow)")
     ap.add_argument("--counters-dir", default=None, help="keep the raw rocprofv3 output here")
     ap.add_argument("--counter-child", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--only-leg", default=None, choices=["c1", "c3", "c4", "c5"],
+                    help="run one render leg alone (for profiling it) and print its figures")
     args = ap.parse_args()
     if args.counter_child:
         counter_child(args)
+        return
+    if args.only_leg:
+        import torch
+        import rtx
+        if args.tuning:
+            rtx.set_tuning(**json.loads(args.tuning))
+        stream = torch.cuda.current_stream()
+        leg = {"c1": lambda: bench_cornell(stream, cpu=not args.no_cpu, tune=not args.no_autotune),
+               "c3": lambda: bench_batched(stream, args.batched_scenes, tune=not args.no_autotune),
+               "c4": lambda: bench_differentiable(stream),
+               "c5": lambda: bench_inverse(stream, steps=args.inverse_steps)}[args.only_leg]()
+        print(json.dumps({"leg": args.only_leg, "result": leg}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the traversal shards by rays with the "

@@ -289,8 +289,9 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
         radix_sort_pairs(S, (uint32_t)n, 0xEu, s);
         perm = S.vals[S.cur];
     }
-    // the wide layout exists whenever the root is internal (a leaf root has one leaf to test)
-    const int tree = b->wnodes ? (int)g_tuning.trace_tree : kTreeBinary;
+    // the wide layout exists whenever the root is internal (a leaf root has one leaf to test);
+    // trees too deep for its stack bound take the binary walk
+    const int tree = b->wnodes && b->max_depth <= kMaxWideTraceDepth ? (int)g_tuning.trace_tree : kTreeBinary;
     if (persist) {
         const uint32_t pb = std::min<uint32_t>(blocks, g_tuning.trace_grid_blocks);
         k_zero_work<<<1, 64, 0, s>>>(work);

@@ -137,3 +137,40 @@ def test_noalloc_entry_with_wide_tree():
     finally:
         rtx.set_tuning(**saved)
     assert np.array_equal(h.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+def test_wide_setting_on_a_tree_too_deep_for_its_stack(tmp_path):
+    """A loaded caterpillar tree 60 levels deep (each internal node: a one-triangle leaf and
+    the next internal node) is deeper than the wide walk's stack bound, so trace_tree 1 takes
+    the binary walk: bit-identical hits, and the oracle's."""
+    D = 60
+    nodes = np.zeros(2 * D + 1, O.NODE48)
+    tris = np.arange(3 * (D + 1), dtype=np.uint32).reshape(-1, 3)
+    verts = np.zeros((3 * (D + 1), 6), np.float32)
+    for k in range(D + 1):   # triangle k in the plane z = k, facing -z
+        verts[3 * k:3 * k + 3, :3] = [[0, 0, k], [0, 1, k], [1, 0, k]]
+        verts[3 * k:3 * k + 3, 5] = -1
+    for i in range(D):
+        nodes[2 * i]["is_leaf"], nodes[2 * i]["a"], nodes[2 * i]["b"] = 0, 2 * i + 1, 2 * i + 2
+        nodes[2 * i]["bmin"][:3], nodes[2 * i]["bmax"][:3] = [0, 0, i], [1, 1, D]
+        nodes[2 * i + 1]["is_leaf"], nodes[2 * i + 1]["a"], nodes[2 * i + 1]["b"] = 1, i, 1
+        nodes[2 * i + 1]["bmin"][:3], nodes[2 * i + 1]["bmax"][:3] = [0, 0, i], [1, 1, i]
+    nodes[2 * D]["is_leaf"], nodes[2 * D]["a"], nodes[2 * D]["b"] = 1, D, 1
+    nodes[2 * D]["bmin"][:3], nodes[2 * D]["bmax"][:3] = [0, 0, D], [1, 1, D]
+    hdr = np.array([1, len(nodes), len(tris), len(verts)], np.uint32)
+    (tmp_path / "deep.bvh").write_bytes(hdr.tobytes() + nodes.tobytes() + tris.tobytes() + verts.tobytes())
+    b = rtx.load_bvh(tmp_path / "deep.bvh")
+    assert b.info()["max_depth"] == D
+    rng = np.random.default_rng(3)
+    rays = np.zeros((20000, 8), np.float32)
+    rays[:, 0:2] = rng.random((20000, 2)) * 1.2 - 0.1
+    rays[:, 2] = rng.uniform(-1, D + 1, 20000)
+    d = rng.normal(size=(20000, 3)) * [0.05, 0.05, 1]
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 7] = np.inf
+    a, _ = _trace(b, rays, 0)
+    w, _ = _trace(b, rays, 1)
+    assert np.array_equal(a.view(np.uint32), w.view(np.uint32))
+    ref = O.trace(O.Tree(nodes, tris, np.arange(len(tris), dtype=np.uint32), verts), rays)
+    assert np.array_equal(a[:, 3].view(np.uint32), ref["tri"])
+    assert (ref["tri"] != 0xFFFFFFFF).mean() > 0.2
