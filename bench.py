@@ -529,7 +529,7 @@ def counter_child(args):
         bvh = rtx.build_device(torch.from_numpy(v).to(dev), torch.from_numpy(t.view(np.int32)).to(dev))
         inc = torch.from_numpy(S.incoherent_rays(args.rays, seed=1)).to(dev)
         hits = torch.empty((args.rays, 4), dtype=torch.float32, device=dev)
-        for _ in range(3):
+        for _ in range(6):
             bvh.trace_closest_device(inc, hits)
     elif leg == "c4":
         v, t, albedo_true, emission, cam = S.diff_scene(seed=3)
@@ -574,24 +574,23 @@ def _kernel_family(name):
 def measure_counters(leg, tuning, args):
     """Runs `bench.py --counter-child leg` under rocprofv3: three --pmc passes (PMC_PASSES)
     and one --kernel-trace pass, each in its own process (counters never combined with a
-    trace domain). Returns per-kernel dispatch averages: calibrated HBM bytes read and
+    trace domain), then summarize_counters. Per kernel: calibrated HBM bytes read and
     written (FETCH_SIZE and WRITE_SIZE, each scaled by what the same counter reads for the
-    1 GiB copy that starts the child), L2 hit rate, VALU instructions and lanes active,
-    the fraction of wave time spent waiting, and the average duration."""
-    import csv
+    1 GiB copy that starts the child) and VALU instructions of the median dispatch, its
+    duration, and over all dispatches the HBM rate, L2 hit rate, VALU issue rate, lanes
+    active and the fraction of wave time spent waiting."""
     import glob
     import shutil
     import signal
     import subprocess
     import tempfile
     rp = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
-    base = args.counters_dir or tempfile.mkdtemp(prefix="rtx_counters_")
+    base = os.path.abspath(args.counters_dir) if args.counters_dir else tempfile.mkdtemp(prefix="rtx_counters_")
     child = [sys.executable, os.path.abspath(__file__), "--counter-child", leg, "--tris", str(args.tris),
              "--rays", str(args.rays)]
     if tuning:
         child += ["--tuning", json.dumps(tuning)]
     env = dict(os.environ, TMPDIR="/tmp")
-    per = {}
 
     def run(opts, tag):
         d = os.path.join(base, f"{leg}_{tag}")
@@ -612,60 +611,78 @@ def measure_counters(leg, tuning, args):
                                f"{out.decode(errors='replace')[-400:]}")
         return d
 
-    calib = {}
     try:
         for k, cs in enumerate(PMC_PASSES):
-            d = run(["--pmc", *cs], f"pmc{k}")
-            sums, disp = {}, {}
-            for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
-                for r in csv.DictReader(open(f)):
-                    fam, cn, val = _kernel_family(r["Kernel_Name"]), r["Counter_Name"], float(r["Counter_Value"])
-                    if "copyBuffer" in fam and cn in ("FETCH_SIZE", "WRITE_SIZE"):
-                        calib[cn] = max(calib.get(cn, 0.0), val)
-                        continue
-                    sums.setdefault(fam, {}).setdefault(cn, 0.0)
-                    sums[fam][cn] += val
-                    disp.setdefault(fam, set()).add(r.get("Dispatch_Id", len(disp.get(fam, ()))))
-            for fam, cv in sums.items():
-                e = per.setdefault(fam, {})
-                e["dispatches"] = len(disp[fam])
-                for cn, val in cv.items():
-                    e[cn] = val / len(disp[fam])
-        d = run(["--kernel-trace"], "trace")
-        for f in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
-            for r in csv.DictReader(open(f)):
-                fam = _kernel_family(r["Kernel_Name"])
-                if fam in per:
-                    e = per[fam]
-                    e.setdefault("_ns", []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+            run(["--pmc", *cs], f"pmc{k}")
+        run(["--kernel-trace"], "trace")
     except (RuntimeError, OSError) as exc:
         return {"error": str(exc)}
+    return summarize_counters(base, leg)
+
+
+def summarize_counters(base, leg):
+    """Reads the rocprofv3 output of measure_counters (directories <leg>_pmc<k> and
+    <leg>_trace under `base`) into the per-kernel summary it returns."""
+    import csv
+    import glob
+
+    import numpy as np
+    per, calib = {}, {}
+    for k in range(len(PMC_PASSES)):
+        d = os.path.join(base, f"{leg}_pmc{k}")
+        # per dispatch: a counter's value summed over the rows of that dispatch
+        vals = {}
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                fam, cn, val = _kernel_family(r["Kernel_Name"]), r["Counter_Name"], float(r["Counter_Value"])
+                if "copyBuffer" in fam and cn in ("FETCH_SIZE", "WRITE_SIZE"):
+                    calib[cn] = max(calib.get(cn, 0.0), val)
+                    continue
+                dv = vals.setdefault(fam, {}).setdefault(cn, {})
+                key = r.get("Dispatch_Id", str(len(dv)))
+                dv[key] = dv.get(key, 0.0) + val
+        for fam, cv in vals.items():
+            e = per.setdefault(fam, {})
+            for cn, dv in cv.items():
+                e[cn] = list(dv.values())
+    for f in glob.glob(os.path.join(base, f"{leg}_trace", "**", "*kernel_trace.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            fam = _kernel_family(r["Kernel_Name"])
+            if fam in per:
+                per[fam].setdefault("_ns", []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
     if "FETCH_SIZE" not in calib or "WRITE_SIZE" not in calib:
         return {"error": "no calibration copy in the counter output"}
     ff, fw = 1048576.0 / calib["FETCH_SIZE"], 1048576.0 / calib["WRITE_SIZE"]
     out = {"calibration": {"copy_kib": 1048576, "fetch_factor": round(ff, 4), "write_factor": round(fw, 4)},
            "kernels": {}}
+    # Per kernel: the median dispatch (the steady state of a repeated launch, not its first,
+    # cold-cache dispatch: per-launch bytes and instructions, duration), and rates over all
+    # of its dispatches together (total bytes or instructions / total duration: the figure
+    # for kernels launched at many sizes, such as one launch per bounce).
     for fam, e in per.items():
         ns = e.get("_ns", [])
         if not ns or "FETCH_SIZE" not in e or "SQ_INSTS_VALU" not in e:
             continue
-        us = sum(ns) / len(ns) / 1e3
-        fetch, write = e["FETCH_SIZE"] * 1024 * ff, e.get("WRITE_SIZE", 0.0) * 1024 * fw
-        hit, miss = e.get("TCC_HIT_sum", 0.0), e.get("TCC_MISS_sum", 0.0)
+        med = lambda cn: float(np.median(e[cn])) if cn in e else 0.0
+        tot = lambda cn: float(np.sum(e[cn])) if cn in e else 0.0
+        us, tot_s = float(np.median(ns)) / 1e3, float(np.sum(ns)) * 1e-9
+        fetch, write = med("FETCH_SIZE") * 1024 * ff, med("WRITE_SIZE") * 1024 * fw
+        tbytes = (tot("FETCH_SIZE") * ff + tot("WRITE_SIZE") * fw) * 1024
+        hit, miss = tot("TCC_HIT_sum"), tot("TCC_MISS_sum")
         out["kernels"][fam] = {
-            "dispatches": e["dispatches"], "avg_us": round(us, 2),
+            "dispatches": len(ns), "avg_us": round(us, 2), "total_us": round(tot_s * 1e6, 1),
             "hbm_read_bytes": round(fetch), "hbm_write_bytes": round(write),
-            "hbm_gbs": round((fetch + write) / (us * 1e-6) / 1e9, 1),
+            "hbm_gbs": round(tbytes / tot_s / 1e9, 1),
             "l2_hit_rate": round(hit / (hit + miss), 4) if hit + miss else None,
-            "valu_insts": round(e["SQ_INSTS_VALU"]),
-            "valu_issue_frac": round(e["SQ_INSTS_VALU"] / (us * 1e-6) / VALU_PEAK_PER_S, 4),
-            "lanes_active": round(e["SQ_THREAD_CYCLES_VALU"] / (64 * e["SQ_ACTIVE_INST_VALU"]), 3)
-            if e.get("SQ_ACTIVE_INST_VALU") else None,
-            "wave_wait_frac": round(e["SQ_WAIT_ANY"] / e["SQ_WAVE_CYCLES"], 3) if e.get("SQ_WAVE_CYCLES") else None,
-            "lds_insts": round(e.get("SQ_INSTS_LDS", 0.0)), "vmem_rd_insts": round(e.get("SQ_INSTS_VMEM_RD", 0.0)),
-            "vmem_wr_insts": round(e.get("SQ_INSTS_VMEM_WR", 0.0))}
+            "valu_insts": round(med("SQ_INSTS_VALU")),
+            "valu_issue_frac": round(tot("SQ_INSTS_VALU") / tot_s / VALU_PEAK_PER_S, 4),
+            "lanes_active": round(tot("SQ_THREAD_CYCLES_VALU") / (64 * tot("SQ_ACTIVE_INST_VALU")), 3)
+            if tot("SQ_ACTIVE_INST_VALU") else None,
+            "wave_wait_frac": round(tot("SQ_WAIT_ANY") / tot("SQ_WAVE_CYCLES"), 3) if tot("SQ_WAVE_CYCLES") else None,
+            "lds_insts": round(med("SQ_INSTS_LDS")), "vmem_rd_insts": round(med("SQ_INSTS_VMEM_RD")),
+            "vmem_wr_insts": round(med("SQ_INSTS_VMEM_WR"))}
     # the busiest kernels first
-    out["kernels"] = dict(sorted(out["kernels"].items(), key=lambda kv: -kv[1]["avg_us"] * kv[1]["dispatches"])[:12])
+    out["kernels"] = dict(sorted(out["kernels"].items(), key=lambda kv: -kv[1]["total_us"])[:12])
     return out
 
 
@@ -695,7 +712,8 @@ def trace_roofline(ctr, kernel_ms):
 
 def render_roofline(ctr, prefix):
     """Roofline of a render leg's dominant kernel (the first kernel family named `prefix`),
-    all figures from its counter and kernel-trace passes."""
+    all figures from its counter and kernel-trace passes: rates over all of its dispatches
+    (one per bounce and batch, of falling sizes); `traffic` is the median dispatch's."""
     if not ctr or "kernels" not in ctr:
         return None
     fam = [k for k in ctr["kernels"] if k.startswith(prefix)]
@@ -705,8 +723,8 @@ def render_roofline(ctr, prefix):
     traffic = k["hbm_read_bytes"] + k["hbm_write_bytes"]
     return {"bound": "hbm", "achieved": k["hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(k["hbm_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": fam[0],
-            "kernel_us": k["avg_us"], "limiter": "valu_issue" if k["valu_issue_frac"] > k["hbm_gbs"] / HBM_PEAK_GBS
-            else "hbm",
+            "kernel_us": k["avg_us"], "dispatches": k["dispatches"], "total_us": k["total_us"],
+            "limiter": "valu_issue" if k["valu_issue_frac"] > k["hbm_gbs"] / HBM_PEAK_GBS else "hbm",
             "valu_issue": {"frac": k["valu_issue_frac"]}, "l2_hit_rate": k["l2_hit_rate"],
             "lanes_active": k["lanes_active"]}
 
