@@ -75,9 +75,9 @@ def main():
                          "the incoherent launch alone)")
     ap.add_argument("--no-counters", action="store_true",
                     help="skip the rocprofv3 counter passes (HBM traffic, L2 hit rate, VALU issue)")
-    ap.add_argument("--counter-legs", default="trace,c4",
-                    help="legs measured under rocprofv3: trace (headline), c4 (differentiable render), "
-                         "c5 (terrain inverse render, slow)")
+    ap.add_argument("--counter-legs", default="trace,c3,c4,c5",
+                    help="legs measured under rocprofv3: trace (headline), c3 (batched render, plain and "
+                         "batched), c4 (differentiable render), c5 (terrain inverse render)")
     ap.add_argument("--counters-dir", default=None, help="keep the raw rocprofv3 output here")
     ap.add_argument("--counter-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--only-leg", default=None, choices=["c1", "c3", "c4", "c5"],
@@ -246,7 +246,8 @@ def main():
     # kernel time above
     counters, roofline = {}, None
     if not args.no_counters:
-        for leg in [x for x in args.counter_legs.split(",") if x]:
+        # the C3 leg runs with the launch shapes its own autotuner picks, below
+        for leg in [x for x in args.counter_legs.split(",") if x and x != "c3"]:
             tun = trace_tuning if leg == "trace" else None
             counters[leg] = measure_counters(leg, tun, args)
         roofline = trace_roofline(counters.get("trace"), kms_inc)
@@ -260,6 +261,9 @@ def main():
         before = rtx.get_tuning()
         batched = bench_batched(stream, args.batched_scenes, tune=not args.no_autotune)
         rtx.set_tuning(**before)
+        if not args.no_counters and "c3" in args.counter_legs.split(","):
+            counters["c3"] = measure_counters("c3", batched["tuning"], args)
+            batched["counters_plain_vs_batched"] = c3_comparison(counters["c3"])
         render = {"differentiable": bench_differentiable(stream), "batched": batched,
                   "cornell": bench_cornell(stream, cpu=not args.no_cpu, tune=not args.no_autotune)}
         if "c4" in counters:
@@ -541,6 +545,17 @@ def counter_child(args):
             a = (a_t * 0.9).requires_grad_(True)
             e = e_t.detach().clone().requires_grad_(True)
             ((rtx.differentiable_render(a, e, g, cam, p) - target) ** 2).sum().backward()
+    elif leg == "c3":
+        sc = S.batched_scenes(64, seed=2)
+        bvhs = [rtx.build(s["verts"], s["tris"]) for s in sc]
+        params = [rtx.make_params(s["width"], s["height"], s["spp"], s["max_bounce"], 3, seed=2 + i)
+                  for i, s in enumerate(sc)]
+        imgs = [torch.empty((s["height"], s["width"], 3), device="cuda") for s in sc]
+        for mode in (0, 1):   # plain, then the graph-batched launch
+            b = rtx.Batch(bvhs, [s["cam"] for s in sc], params, [None] * 64, [None] * 64, imgs, mode)
+            b.launch()
+            torch.cuda.synchronize()
+            del b
     elif leg == "c5":
         v, t, uv = S.terrain(2048, seed=4)
         g = rtx.build_device(torch.from_numpy(v).cuda(), torch.from_numpy(t.view(np.int32)).cuda())
@@ -727,6 +742,26 @@ def render_roofline(ctr, prefix):
             "limiter": "valu_issue" if k["valu_issue_frac"] > k["hbm_gbs"] / HBM_PEAK_GBS else "hbm",
             "valu_issue": {"frac": k["valu_issue_frac"]}, "l2_hit_rate": k["l2_hit_rate"],
             "lanes_active": k["lanes_active"]}
+
+
+def c3_comparison(ctr):
+    """The C3 counter leg's traversal and shade kernels, plain (one scene per launch
+    sequence: scene index 0 in the kernel name, shading with a uniform descriptor) against
+    batched (mixed-scene queues: scene index 1, per-lane descriptors): time over all
+    dispatches, VALU issue, lanes active and HBM rate of each."""
+    if not ctr or "kernels" not in ctr:
+        return None
+    out = {}
+    for fam, k in ctr["kernels"].items():
+        if fam.startswith("k_wtrace"):
+            side = "plain" if fam.endswith(", 0>") else "batched"
+        elif fam.startswith("k_shade"):
+            side = "plain" if fam.endswith("true>") else "batched"
+        else:
+            continue
+        out.setdefault(side, {})[fam] = {x: k[x] for x in ("dispatches", "total_us", "valu_issue_frac", "lanes_active",
+                                                           "hbm_gbs", "l2_hit_rate", "wave_wait_frac")}
+    return out
 
 
 def cpu_baseline(bvh, verts, tris, rays, coh_rays, seconds, gpu_hits=None):
