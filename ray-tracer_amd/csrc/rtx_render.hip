@@ -7,7 +7,9 @@
 // Paths of all scenes are numbered globally: scene-major, then sample, then pixel. They
 // are processed in batches of `batch_paths`; per batch:
 //   k_camera -> { k_wtrace -> k_shade } x (max_bounce + 1) -> k_accum
-// k_shade appends surviving paths to the next queue (wave ballot + one atomic per wave).
+// k_shade appends surviving paths to the next queue (an LDS scan per block, one atomic per
+// block). The adjoint's replay runs the same wavefront with k_shade<kShadeReplay>, which
+// also adds each bounce's gradient contributions (see k_shade).
 // k_accum adds each pixel's samples into the image in sample order, so the sum is the
 // reference's serial `accum += pathTrace(...)`; k_finalize divides by spp.
 // No stage needs the host: queue lengths stay in device memory, so a whole render can be
@@ -54,14 +56,25 @@ struct PathState {
     uint32_t* smp;
     uint32_t* q[2];
     uint32_t* cnt;     // [b]: input queue length of bounce b; [ncnt/2 + b]: persistent-trace counter of bounce b
-    // adjoint replay only: the current bounce's record of path i, overwritten every bounce
-    float4* rec0;      // radiance gathered up to and including this bounce's hit (xyz); albedo entry
-                       // of the hit (w bits), kInvalid when the path hit no surface this bounce
-    float4* rec1;      // throughput arriving at the hit
     uint32_t B;
 };
 
 constexpr int kShadeForward = 0, kShadeReplay = 1;
+// Where the replay adds its gradients: private copies of the arrays in the block's LDS,
+// flushed once per block; or global atomics after merging equal entries within the wave.
+constexpr int kGradNone = 0, kGradLds = 1, kGradGlobal = 2;
+
+// Inputs and outputs of the replay's gradient scatter (scene 0 only). Lfin + 3 * (lbase + i)
+// is path i's primal radiance; dimg = dLoss/dimage (3 floats per pixel); ga, ge: gradient
+// arrays (3 floats per albedo entry, per original triangle), either may be null.
+struct GradArgs {
+    const float* Lfin;
+    uint64_t lbase;
+    const float* dimg;
+    float* ga;
+    float* ge;
+    uint32_t nprim;
+};
 
 
 __device__ __forceinline__ uint32_t find_scene(const uint64_t* off, int n, uint64_t g) {
@@ -337,118 +350,6 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
     }
 }
 
-// One bounce of pathTrace (main.cpp:158-200) for every queued path. MODE kShadeReplay also
-// records this bounce's surface hit for the adjoint (albedo entry, radiance so far,
-// throughput), in place of the previous bounce's record. UNI: every path belongs to
-// scenes[0], whose descriptor is then read once as uniform data.
-template <int MODE, bool UNI>
-__global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int qi, int bounce) {
-    constexpr bool ADJ = MODE == kShadeReplay;
-    const uint32_t n = P.cnt[bounce];
-    const uint32_t* Q = P.q[qi];
-    uint32_t* Qo = P.q[qi ^ 1];
-    const uint64_t lt = (1ull << __lane_id()) - 1ull;
-    __shared__ uint32_t wcount[16];   // surviving paths per wave of the block (<= 1024 threads)
-    __shared__ uint32_t bbase;
-    const uint32_t wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    // trip count uniform per block: the queue append below is one atomic per block
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t bb = blockIdx.x * blockDim.x; bb < n; bb += stride) {
-        const uint32_t k = bb + threadIdx.x;
-        bool cont = false;
-        uint32_t i = 0;
-        if (k < n) {
-            i = Q[k];
-            const SceneDesc& S = scenes[UNI ? 0u : P.sid[i]];
-            const float4 h = P.hit[i];
-            const uint32_t tri = __float_as_uint(h.w);
-            float4 tp = P.tp[i], L = P.L[i];
-            const float4 r1 = P.ray1[i];
-            const f3 d{r1.x, r1.y, r1.z};
-            if (tri == kInvalid) {
-                f3 e = env_color(S, d);
-                L = make_float4(L.x + tp.x * e.x, L.y + tp.y * e.y, L.z + tp.z * e.z, 0.f);
-                if (ADJ) P.rec0[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kInvalid));
-            } else {
-                const uint32_t prim = ld_g(S.prims + tri);
-                // albedo entry: the hit's texel (texture albedo) or its original triangle
-                const float* alb = S.tex ? S.tex : S.albedo;
-                const uint32_t aidx = S.tex ? texel_at(S, tri, h.y, h.z) : prim;
-                if (ADJ) P.rec1[i] = tp;
-                if (S.emission) {
-                    const float* e = S.emission + 3 * prim;
-                    L = make_float4(L.x + tp.x * ld_g(e), L.y + tp.y * ld_g(e + 1), L.z + tp.z * ld_g(e + 2), 0.f);
-                }
-                if (ADJ) P.rec0[i] = make_float4(L.x, L.y, L.z, __uint_as_float(aidx));
-                if (bounce < S.max_bounce) {
-                    float ax = 0.8f, ay = 0.8f, az = 0.8f;
-                    if (alb) { ax = ld_g(alb + 3 * aidx); ay = ld_g(alb + 3 * aidx + 1); az = ld_g(alb + 3 * aidx + 2); }
-                    const uint32_t px = P.pix[i], sm = P.smp[i];
-                    const float u1 = sample01(S.seed, px, sm, 2 + 3 * bounce);
-                    const float u2 = sample01(S.seed, px, sm, 3 + 3 * bounce);
-                    const f3 nrm = hit_normal(S.T, tri, h.y, h.z);
-                    const f3 wd = bounce_dir(u1, u2, nrm);
-                    tp = make_float4(tp.x * ax, tp.y * ay, tp.z * az, 0.f);
-                    cont = true;
-                    if (bounce >= S.rr_bounce) {
-                        float pr = tp.x;
-                        if (pr < tp.y) pr = tp.y;
-                        if (pr < tp.z) pr = tp.z;
-                        if (sample01(S.seed, px, sm, 4 + 3 * bounce) > pr) cont = false;
-                        else tp = make_float4(tp.x / pr, tp.y / pr, tp.z / pr, 0.f);
-                    }
-                    if (cont) {
-                        const float4 r0 = P.ray0[i];
-                        const f3 o = add(f3{r0.x, r0.y, r0.z}, scl(h.x, d));
-                        P.ray0[i] = make_float4(o.x, o.y, o.z, 1e-4f);
-                        P.ray1[i] = make_float4(wd.x, wd.y, wd.z, kInf);
-                        P.tp[i] = tp;
-                    }
-                }
-            }
-            P.L[i] = L;
-        }
-        const uint64_t m = __ballot(cont);
-        if (__lane_id() == 0) wcount[wid] = (uint32_t)__popcll(m);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t run = 0;
-            for (uint32_t w = 0; w < nw; ++w) { const uint32_t c = wcount[w]; wcount[w] = run; run += c; }
-            bbase = run ? atomicAdd(&P.cnt[bounce + 1], run) : 0u;
-        }
-        __syncthreads();
-        if (cont) Qo[bbase + wcount[wid] + __popcll(m & lt)] = i;
-        __syncthreads();
-    }
-}
-
-// In-order accumulation: the first path of each pixel in the batch adds that pixel's
-// samples in increasing sample order.
-__global__ void k_accum(const SceneDesc* __restrict__ scenes, const uint64_t* __restrict__ off, int ns, uint64_t g0,
-                        uint32_t cnt, PathState P) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        const uint32_t s = P.sid[i];
-        const SceneDesc& S = scenes[s];
-        const uint32_t sample = P.smp[i], pixel = P.pix[i];
-        if (sample != 0 && i >= S.npix) continue;   // an earlier sample of this pixel is in the batch
-        float* img = S.image + 3 * (size_t)pixel;
-        float ax = img[0], ay = img[1], az = img[2];
-        for (uint32_t j = i, sm = sample; j < cnt && sm < (uint32_t)S.spp; j += S.npix, ++sm) {
-            const float4 L = P.L[j];
-            ax += L.x; ay += L.y; az += L.z;
-        }
-        img[0] = ax; img[1] = ay; img[2] = az;
-    }
-    (void)off; (void)ns; (void)g0;
-}
-
-__global__ void k_finalize(const SceneDesc* __restrict__ scenes, int s) {
-    const SceneDesc& S = scenes[s];
-    const float fs = (float)S.spp;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < S.npix * 3u; p += gridDim.x * blockDim.x)
-        S.image[p] = S.image[p] / fs;
-}
-
 // Atomic accumulation for the adjoint: before adding, the wave merges lanes that target
 // the same triangle (one merge round per distinct triangle, up to kMergeRounds); lanes
 // left after that add their own contributions.
@@ -482,91 +383,180 @@ __device__ __forceinline__ void merged_add(float* __restrict__ buf, bool has, ui
     }
 }
 
-// Gradient scatter of the path-replay adjoint for one bounce. The replay is the forward
-// wavefront again (the same sample table gives the same paths); at bounce b every path of
-// the bounce's queue holds the record of its hit k = b (k_shade<kShadeReplay>), and its
-// primal radiance L comes from Lfin[3 * (lbase + i) ...]. With L = sum_m (prod_{j<m} a_j) X_m:
-//   dL/da_k = (L - L_upto_k) / a_k        dL/dLe_k = throughput arriving at k,
-// where L_upto_k includes vertex k's own emission. Only the current bounce is ever stored,
-// so the state per path does not grow with max_bounce.
-// LDS=true: each block accumulates into private copies of the gradient arrays in LDS (ga
-// at [0, 3 nprim), ge after it when both are set) and adds their nonzero entries to global
-// memory once at the end; otherwise lanes add to global memory after merging equal
-// triangles within the wave. Neighbouring queue entries are neighbouring pixels, so one
-// block's paths reach few distinct triangles and its flush is short.
-constexpr size_t kLdsGradBytes = 160 * 1024;   // LDS a gfx950 workgroup may allocate
+// Dynamic LDS the replay may take for its gradient arrays (a gfx950 workgroup may allocate
+// 160 KiB; k_shade's own LDS takes under 256 bytes), and its launch shape when they fit: 256
+// blocks of 1024 threads (C4 forward+adjoint, with a separate gradient kernel: 128 blocks of
+// 512 threads one launch per array 19.6 ms; both arrays in one launch 16.5 ms; 256 blocks of
+// 1024 threads 16.2 ms).
+constexpr size_t kLdsGradBytes = 160 * 1024 - 256;
 constexpr int kGradThreads = 1024;
-// LDS form: blocks per bounce. C4 forward+adjoint: 128 blocks of 512 threads, one launch per
-// array, 19.6 ms; 512 such blocks 16.8 ms; both arrays in one launch 16.5 ms; 256 blocks of
-// 1024 threads 16.2 ms.
 constexpr uint32_t kGradLdsBlocks = 256;
 
-template <bool LDS>
-__global__ void __launch_bounds__(kGradThreads) k_grad(const SceneDesc* __restrict__ scenes, PathState P, int qi,
-                                                       int bounce, const float* __restrict__ Lfin, uint64_t lbase,
-                                                       const float* __restrict__ dimg, float* __restrict__ ga,
-                                                       float* __restrict__ ge, uint32_t nprim) {
-    extern __shared__ float lg[];   // LDS: ga's copy (if set), then ge's (if set), 3 nprim floats each
-    const SceneDesc& S = scenes[0];
-    const float fs = (float)S.spp;
-    const uint32_t n3 = 3 * nprim, nlds = (ga ? n3 : 0) + (ge ? n3 : 0), eoff = ga ? n3 : 0;
-    if (LDS) {
+// One bounce of pathTrace (main.cpp:158-200) for every queued path. UNI: every path belongs
+// to scenes[0], whose descriptor is then read once as uniform data.
+//
+// MODE kShadeReplay is the path-replay adjoint's bounce: the replay is the forward wavefront
+// again (the same sample table gives the same paths, bit for bit), and at the surface hit k
+// of path i this bounce adds, weighted by dLoss/dI of the path's pixel / spp,
+//   dL/da_k = (L - L_upto_k) / a_k   to ga[albedo entry],   dL/dLe_k = tp_k   to ge[triangle],
+// with L the path's primal radiance (G.Lfin), L_upto_k the radiance gathered up to and
+// including hit k's own emission and tp_k the throughput arriving at k: with
+// L = sum_m (prod_{j<m} a_j) X_m everything past hit k is proportional to a_k. Nothing is
+// kept per path between bounces beyond the forward path state, so memory does not grow with
+// max_bounce. At a zero albedo channel the path carried nothing past the vertex: that vertex
+// adds 0 to the channel (not 0/0). GM: kGradLds (both arrays in LDS, launched as at most
+// kGradLdsBlocks blocks of kGradThreads) or kGradGlobal.
+template <int MODE, bool UNI, int GM = kGradNone>
+__global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int qi, int bounce, GradArgs G) {
+    constexpr bool ADJ = MODE == kShadeReplay;
+    extern __shared__ float lg[];   // kGradLds: ga's copy (if set), then ge's (if set)
+    const uint32_t n3 = 3 * G.nprim, eoff = G.ga ? n3 : 0u, nlds = GM == kGradLds ? eoff + (G.ge ? n3 : 0u) : 0u;
+    if (GM == kGradLds) {
         for (uint32_t j = threadIdx.x; j < nlds; j += blockDim.x) lg[j] = 0.f;
         __syncthreads();
     }
     const uint32_t n = P.cnt[bounce];
     const uint32_t* Q = P.q[qi];
-    const float* alb = S.tex ? S.tex : S.albedo;
+    uint32_t* Qo = P.q[qi ^ 1];
+    const uint64_t lt = (1ull << __lane_id()) - 1ull;
+    __shared__ uint32_t wcount[16];   // surviving paths per wave of the block (<= 1024 threads)
+    __shared__ uint32_t bbase;
+    const uint32_t wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    // trip count uniform per block: the queue append below is one atomic per block, and the
+    // replay's wave merge votes across the wave
     const uint32_t stride = gridDim.x * blockDim.x;
-    // wave-uniform trip count: merged_add votes across the wave
-    for (uint32_t wb = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wb < n; wb += stride) {
-        const uint32_t k = wb + __lane_id();
-        bool has = false;
-        uint32_t prim = 0;
+    for (uint32_t bb = blockIdx.x * blockDim.x; bb < n; bb += stride) {
+        const uint32_t k = bb + threadIdx.x;
+        bool cont = false, has = false;
+        uint32_t i = 0, gidx = 0;
         f3 gA{0.f, 0.f, 0.f}, gE{0.f, 0.f, 0.f};
         if (k < n) {
-            const uint32_t i = Q[k];
-            const float4 lu = P.rec0[i];
-            prim = __float_as_uint(lu.w);   // albedo entry: texel or original triangle
-            has = prim != kInvalid;
-            if (has) {
-                const float4 tp = P.rec1[i];
-                const float* lf = Lfin + 3 * (lbase + i);
-                const uint32_t pixel = P.pix[i];
-                const f3 delta{dimg[3 * pixel] / fs, dimg[3 * pixel + 1] / fs, dimg[3 * pixel + 2] / fs};
-                const float* a = alb + 3 * prim;
-                // (L - lu) / a is the radiance past this vertex per unit albedo. At a zero
-                // albedo the path carried nothing past the vertex, so the replay has no
-                // record of it: that vertex contributes 0 (not 0/0).
-                gA = f3{a[0] != 0.f ? delta.x * (lf[0] - lu.x) / a[0] : 0.f,
-                        a[1] != 0.f ? delta.y * (lf[1] - lu.y) / a[1] : 0.f,
-                        a[2] != 0.f ? delta.z * (lf[2] - lu.z) / a[2] : 0.f};
-                gE = f3{delta.x * tp.x, delta.y * tp.y, delta.z * tp.z};
+            i = Q[k];
+            const SceneDesc& S = scenes[UNI ? 0u : P.sid[i]];
+            const float4 h = P.hit[i];
+            const uint32_t tri = __float_as_uint(h.w);
+            float4 tp = P.tp[i], L = P.L[i];
+            const float4 r1 = P.ray1[i];
+            const f3 d{r1.x, r1.y, r1.z};
+            if (tri == kInvalid) {
+                f3 e = env_color(S, d);
+                L = make_float4(L.x + tp.x * e.x, L.y + tp.y * e.y, L.z + tp.z * e.z, 0.f);
+            } else {
+                const uint32_t prim = ld_g(S.prims + tri);
+                // albedo entry: the hit's texel (texture albedo) or its original triangle
+                const float* alb = S.tex ? S.tex : S.albedo;
+                const uint32_t aidx = S.tex ? texel_at(S, tri, h.y, h.z) : prim;
+                const float4 tpin = tp;
+                if (S.emission) {
+                    const float* e = S.emission + 3 * prim;
+                    L = make_float4(L.x + tp.x * ld_g(e), L.y + tp.y * ld_g(e + 1), L.z + tp.z * ld_g(e + 2), 0.f);
+                }
+                float ax = 0.8f, ay = 0.8f, az = 0.8f;
+                const bool more = bounce < S.max_bounce;
+                if (alb && (more || ADJ)) {
+                    ax = ld_g(alb + 3 * aidx); ay = ld_g(alb + 3 * aidx + 1); az = ld_g(alb + 3 * aidx + 2);
+                }
+                const uint32_t px = P.pix[i];
+                if (ADJ) {
+                    has = true;
+                    gidx = aidx;
+                    const float fs = (float)S.spp;
+                    const f3 delta{ld_g(G.dimg + 3 * px) / fs, ld_g(G.dimg + 3 * px + 1) / fs,
+                                   ld_g(G.dimg + 3 * px + 2) / fs};
+                    if (G.ga) {
+                        const float* lf = G.Lfin + 3 * (G.lbase + i);
+                        gA = f3{ax != 0.f ? delta.x * (ld_g(lf) - L.x) / ax : 0.f,
+                                ay != 0.f ? delta.y * (ld_g(lf + 1) - L.y) / ay : 0.f,
+                                az != 0.f ? delta.z * (ld_g(lf + 2) - L.z) / az : 0.f};
+                    }
+                    gE = f3{delta.x * tpin.x, delta.y * tpin.y, delta.z * tpin.z};
+                }
+                if (more) {
+                    const uint32_t sm = P.smp[i];
+                    const float u1 = sample01(S.seed, px, sm, 2 + 3 * bounce);
+                    const float u2 = sample01(S.seed, px, sm, 3 + 3 * bounce);
+                    const f3 nrm = hit_normal(S.T, tri, h.y, h.z);
+                    const f3 wd = bounce_dir(u1, u2, nrm);
+                    tp = make_float4(tp.x * ax, tp.y * ay, tp.z * az, 0.f);
+                    cont = true;
+                    if (bounce >= S.rr_bounce) {
+                        float pr = tp.x;
+                        if (pr < tp.y) pr = tp.y;
+                        if (pr < tp.z) pr = tp.z;
+                        if (sample01(S.seed, px, sm, 4 + 3 * bounce) > pr) cont = false;
+                        else tp = make_float4(tp.x / pr, tp.y / pr, tp.z / pr, 0.f);
+                    }
+                    if (cont) {
+                        const float4 r0 = P.ray0[i];
+                        const f3 o = add(f3{r0.x, r0.y, r0.z}, scl(h.x, d));
+                        P.ray0[i] = make_float4(o.x, o.y, o.z, 1e-4f);
+                        P.ray1[i] = make_float4(wd.x, wd.y, wd.z, kInf);
+                        P.tp[i] = tp;
+                    }
+                }
             }
+            P.L[i] = L;
         }
-        if (LDS) {
-            if (has && ga) {
-                atomicAdd(&lg[3 * prim], gA.x);
-                atomicAdd(&lg[3 * prim + 1], gA.y);
-                atomicAdd(&lg[3 * prim + 2], gA.z);
+        if (GM == kGradLds) {
+            if (has && G.ga) {
+                atomicAdd(&lg[3 * gidx], gA.x);
+                atomicAdd(&lg[3 * gidx + 1], gA.y);
+                atomicAdd(&lg[3 * gidx + 2], gA.z);
             }
-            if (has && ge) {
-                atomicAdd(&lg[eoff + 3 * prim], gE.x);
-                atomicAdd(&lg[eoff + 3 * prim + 1], gE.y);
-                atomicAdd(&lg[eoff + 3 * prim + 2], gE.z);
+            if (has && G.ge) {
+                atomicAdd(&lg[eoff + 3 * gidx], gE.x);
+                atomicAdd(&lg[eoff + 3 * gidx + 1], gE.y);
+                atomicAdd(&lg[eoff + 3 * gidx + 2], gE.z);
             }
-        } else {
-            if (ga) merged_add(ga, has, prim, gA);
-            if (ge) merged_add(ge, has, prim, gE);
+        } else if (GM == kGradGlobal) {
+            if (G.ga) merged_add(G.ga, has, gidx, gA);
+            if (G.ge) merged_add(G.ge, has, gidx, gE);
         }
-    }
-    if (LDS) {
+        const uint64_t m = __ballot(cont);
+        if (__lane_id() == 0) wcount[wid] = (uint32_t)__popcll(m);
         __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t run = 0;
+            for (uint32_t w = 0; w < nw; ++w) { const uint32_t c = wcount[w]; wcount[w] = run; run += c; }
+            bbase = run ? atomicAdd(&P.cnt[bounce + 1], run) : 0u;
+        }
+        __syncthreads();
+        if (cont) Qo[bbase + wcount[wid] + __popcll(m & lt)] = i;
+        __syncthreads();
+    }
+    if (GM == kGradLds) {
         for (uint32_t j = threadIdx.x; j < nlds; j += blockDim.x) {
             const float x = lg[j];
-            if (x != 0.f) atomicAdd(j < eoff ? &ga[j] : &ge[j - eoff], x);
+            if (x != 0.f) atomicAdd(j < eoff ? &G.ga[j] : &G.ge[j - eoff], x);
         }
     }
+}
+
+// In-order accumulation: the first path of each pixel in the batch adds that pixel's
+// samples in increasing sample order.
+__global__ void k_accum(const SceneDesc* __restrict__ scenes, const uint64_t* __restrict__ off, int ns, uint64_t g0,
+                        uint32_t cnt, PathState P) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const uint32_t s = P.sid[i];
+        const SceneDesc& S = scenes[s];
+        const uint32_t sample = P.smp[i], pixel = P.pix[i];
+        if (sample != 0 && i >= S.npix) continue;   // an earlier sample of this pixel is in the batch
+        float* img = S.image + 3 * (size_t)pixel;
+        float ax = img[0], ay = img[1], az = img[2];
+        for (uint32_t j = i, sm = sample; j < cnt && sm < (uint32_t)S.spp; j += S.npix, ++sm) {
+            const float4 L = P.L[j];
+            ax += L.x; ay += L.y; az += L.z;
+        }
+        img[0] = ax; img[1] = ay; img[2] = az;
+    }
+    (void)off; (void)ns; (void)g0;
+}
+
+__global__ void k_finalize(const SceneDesc* __restrict__ scenes, int s) {
+    const SceneDesc& S = scenes[s];
+    const float fs = (float)S.spp;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < S.npix * 3u; p += gridDim.x * blockDim.x)
+        S.image[p] = S.image[p] / fs;
 }
 
 SceneDesc make_desc(const rtx_bvh* b, const rtx_camera* c, const rtx_render_params* p, const float* albedo,
@@ -694,18 +684,18 @@ int Batch::setup(uint32_t batch_paths, bool adjoint) {
     ncnt = 2 * (uint32_t)(max_b + 2);
     P.cnt = alloc<uint32_t>(ncnt);
     P.B = B;
-    P.rec0 = nullptr; P.rec1 = nullptr;
-    if (adjoint) {   // flat in max_bounce: one record per path, plus the batch's primal radiance
-        // the LDS form of k_grad may take more than the default 64 KiB of dynamic LDS
-        static const bool lds_ok = hip_ok(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_grad<true>),
-                                                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                              (int)kLdsGradBytes),
-                                          "hipFuncSetAttribute");
+    if (adjoint) {   // flat in max_bounce: only the batch's primal radiance is added
+        // the replay's LDS form may take more than the default 64 KiB of dynamic LDS
+        static const bool lds_ok =
+            hip_ok(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_shade<kShadeReplay, true, kGradLds>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsGradBytes),
+                   "hipFuncSetAttribute") &&
+            hip_ok(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_shade<kShadeReplay, false, kGradLds>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsGradBytes),
+                   "hipFuncSetAttribute");
         if (!lds_ok) return RTX_E_HIP;
-        P.rec0 = alloc<float4>(B);
-        P.rec1 = alloc<float4>(B);
         Lbatch = alloc<float>(3ull * B);
-        if (!P.rec0 || !P.rec1 || !Lbatch) return fail(RTX_E_HIP, "device allocation failed on render setup");
+        if (!Lbatch) return fail(RTX_E_HIP, "device allocation failed on render setup");
     }
     if (!d_scenes || !d_off || !P.ray0 || !P.ray1 || !P.hit || !P.tp || !P.L || !P.sid || !P.pix || !P.smp ||
         !P.q[0] || !P.q[1] || !P.cnt)
@@ -735,8 +725,8 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
         const uint32_t cnt = (uint32_t)std::min<uint64_t>(B, off[s_last] - g0);
         const uint32_t cgrid = std::min<uint32_t>((cnt + 255) / 256, 65535);
         // One wavefront over the batch: camera rays, then max_bounce + 1 trace/shade rounds.
-        // The replay (kShadeReplay) scatters each bounce's gradients before the next bounce
-        // overwrites the records; Lfin + 3 * lbase is the batch's primal radiance.
+        // The replay (kShadeReplay) scatters each bounce's gradients as it shades it;
+        // Lfin + 3 * lbase is the batch's primal radiance.
         auto wavefront = [&](int mode, const float* Lfin, uint64_t lbase) {
             k_set_counts<<<1, 64, 0, st>>>(P.cnt, ncnt, cnt);
             k_camera<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
@@ -782,28 +772,22 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
                     });
                 }
                 if (mode == kShadeReplay) {
-                    if (one) k_shade<kShadeReplay, true><<<grid, sh, 0, st>>>(sd, P, qi, b);
-                    else k_shade<kShadeReplay, false><<<grid, sh, 0, st>>>(sd, P, qi, b);
-                    const uint32_t nprim = descs[0].nprim;
-                    const size_t arr = 3ull * nprim * sizeof(float);
-                    const uint32_t gb = std::min<uint32_t>(kGradLdsBlocks, (cnt + kGradThreads - 1) / kGradThreads);
-                    if ((ga ? arr : 0) + (ge ? arr : 0) <= kLdsGradBytes) {   // both arrays in one launch
-                        k_grad<true><<<gb, kGradThreads, (ga ? arr : 0) + (ge ? arr : 0), st>>>(
-                            d_scenes, P, qi, b, Lfin, lbase, dimg, ga, ge, nprim);
-                    } else if (arr <= kLdsGradBytes) {                         // one launch per array
-                        if (ga)
-                            k_grad<true><<<gb, kGradThreads, arr, st>>>(d_scenes, P, qi, b, Lfin, lbase, dimg, ga,
-                                                                        nullptr, nprim);
-                        if (ge)
-                            k_grad<true><<<gb, kGradThreads, arr, st>>>(d_scenes, P, qi, b, Lfin, lbase, dimg, nullptr,
-                                                                        ge, nprim);
+                    // the gradient scatter rides on the replay's shading: both arrays in LDS
+                    // when they fit, global atomics otherwise
+                    const GradArgs G{Lfin, lbase, dimg, ga, ge, descs[0].nprim};
+                    const size_t lds = ((ga ? 1u : 0u) + (ge ? 1u : 0u)) * 3ull * descs[0].nprim * sizeof(float);
+                    if (lds <= kLdsGradBytes) {
+                        const uint32_t gb = std::min<uint32_t>(kGradLdsBlocks, (cnt + kGradThreads - 1) / kGradThreads);
+                        if (one) k_shade<kShadeReplay, true, kGradLds><<<gb, kGradThreads, lds, st>>>(sd, P, qi, b, G);
+                        else k_shade<kShadeReplay, false, kGradLds><<<gb, kGradThreads, lds, st>>>(sd, P, qi, b, G);
                     } else {
-                        k_grad<false><<<std::min<uint32_t>(4096, (cnt + kGradThreads - 1) / kGradThreads), kGradThreads,
-                                        0, st>>>(d_scenes, P, qi, b, Lfin, lbase, dimg, ga, ge, nprim);
+                        if (one) k_shade<kShadeReplay, true, kGradGlobal><<<grid, sh, 0, st>>>(sd, P, qi, b, G);
+                        else k_shade<kShadeReplay, false, kGradGlobal><<<grid, sh, 0, st>>>(sd, P, qi, b, G);
                     }
                 } else {
-                    if (one) k_shade<kShadeForward, true><<<grid, sh, 0, st>>>(sd, P, qi, b);
-                    else k_shade<kShadeForward, false><<<grid, sh, 0, st>>>(sd, P, qi, b);
+                    const GradArgs G{nullptr, 0, nullptr, nullptr, nullptr, 0};
+                    if (one) k_shade<kShadeForward, true><<<grid, sh, 0, st>>>(sd, P, qi, b, G);
+                    else k_shade<kShadeForward, false><<<grid, sh, 0, st>>>(sd, P, qi, b, G);
                 }
                 qi ^= 1;
             }
