@@ -746,17 +746,19 @@ def render_roofline(ctr, prefix):
 
 def c3_comparison(ctr):
     """The C3 counter leg's traversal and shade kernels, plain (one scene per launch
-    sequence: scene index 0 in the kernel name, shading with a uniform descriptor) against
-    batched (mixed-scene queues: scene index 1, per-lane descriptors): time over all
-    dispatches, VALU issue, lanes active and HBM rate of each."""
+    sequence: scene form 0, the last template argument of k_wtrace, and shading with a
+    uniform descriptor, the second of k_shade) against batched (mixed-scene queues: scene
+    form 1, per-lane descriptors): time over all dispatches, VALU issue, lanes active and
+    HBM rate of each."""
     if not ctr or "kernels" not in ctr:
         return None
     out = {}
     for fam, k in ctr["kernels"].items():
-        if fam.startswith("k_wtrace"):
-            side = "plain" if fam.endswith(", 0>") else "batched"
-        elif fam.startswith("k_shade"):
-            side = "plain" if fam.endswith("true>") else "batched"
+        args = [a.strip() for a in fam[fam.find("<") + 1:fam.rfind(">")].split(",")]
+        if fam.startswith("k_wtrace") and args:
+            side = "plain" if args[-1] == "0" else "batched"
+        elif fam.startswith("k_shade") and len(args) > 1:
+            side = "plain" if args[1] == "true" else "batched"
         else:
             continue
         out.setdefault(side, {})[fam] = {x: k[x] for x in ("dispatches", "total_us", "valu_issue_frac", "lanes_active",
