@@ -1563,6 +1563,7 @@ void free_products(rtx_bvh* b) {
         if (p) (void)hipHostFree(p);
     b->pin_rays = b->pin_hits = nullptr;
     b->pin_nrm = nullptr;
+    b->pin_rays_d = b->pin_hits_d = b->pin_nrm_d = nullptr;
     b->pin_cap = 0;
     if (b->host_stream) (void)hipStreamDestroy(b->host_stream);
     b->host_stream = nullptr;

@@ -80,6 +80,9 @@ struct rtx_bvh {
     void* pin_rays = nullptr;
     void* pin_hits = nullptr;
     float* pin_nrm = nullptr;
+    void* pin_rays_d = nullptr;       // their device addresses (hipHostGetDevicePointer, once)
+    void* pin_hits_d = nullptr;
+    void* pin_nrm_d = nullptr;
     size_t pin_cap = 0;
     hipStream_t host_stream = nullptr;
     // diagnostics

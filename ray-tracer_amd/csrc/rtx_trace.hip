@@ -461,20 +461,29 @@ extern "C" int rtx_trace_closest_host(const rtx_bvh* b, const rtx_ray* rays, siz
     // the hits to pinned host memory kept on the handle, on the handle's own stream, so a
     // call is a host memcpy, one launch and one stream synchronization, with no DMA copies.
     if (n <= kPinnedMaxRays) {
-        if (!mb->pin_rays) {
+        if (!mb->pin_rays) {   // allocated, and their device addresses taken, once per handle
             const size_t cap = kPinnedMaxRays;
-            RTX_HIP(hipHostMalloc(&mb->pin_rays, sizeof(rtx_ray) * cap, hipHostMallocDefault));
-            RTX_HIP(hipHostMalloc(&mb->pin_hits, sizeof(rtx_hit) * cap, hipHostMallocDefault));
-            RTX_HIP(hipHostMalloc((void**)&mb->pin_nrm, 12 * cap, hipHostMallocDefault));
+            void *r = nullptr, *h = nullptr, *nr = nullptr, *dr = nullptr, *dh = nullptr, *dn = nullptr;
+            const bool ok = hipHostMalloc(&r, sizeof(rtx_ray) * cap, hipHostMallocDefault) == hipSuccess &&
+                            hipHostMalloc(&h, sizeof(rtx_hit) * cap, hipHostMallocDefault) == hipSuccess &&
+                            hipHostMalloc(&nr, 12 * cap, hipHostMallocDefault) == hipSuccess &&
+                            hipHostGetDevicePointer(&dr, r, 0) == hipSuccess &&
+                            hipHostGetDevicePointer(&dh, h, 0) == hipSuccess &&
+                            hipHostGetDevicePointer(&dn, nr, 0) == hipSuccess;
+            if (!ok) {
+                for (void* p : {r, h, nr})
+                    if (p) (void)hipHostFree(p);
+                (void)hipGetLastError();
+                return fail(RTX_E_HIP, "pinned host allocation failed on rtx_trace_closest_host()");
+            }
+            mb->pin_rays = r; mb->pin_hits = h; mb->pin_nrm = static_cast<float*>(nr);
+            mb->pin_rays_d = dr; mb->pin_hits_d = dh; mb->pin_nrm_d = dn;
             mb->pin_cap = cap;
         }
         std::memcpy(mb->pin_rays, rays, sizeof(rtx_ray) * n);
-        void *dr = nullptr, *dh = nullptr, *dn = nullptr;
-        RTX_HIP(hipHostGetDevicePointer(&dr, mb->pin_rays, 0));
-        RTX_HIP(hipHostGetDevicePointer(&dh, mb->pin_hits, 0));
-        if (normals) RTX_HIP(hipHostGetDevicePointer(&dn, mb->pin_nrm, 0));
-        rc = launch<false, false>(b, static_cast<const rtx_ray*>(dr), n, static_cast<rtx_hit*>(dh),
-                                  static_cast<float*>(dn), nullptr, nullptr, st);
+        rc = launch<false, false>(b, static_cast<const rtx_ray*>(mb->pin_rays_d), n,
+                                  static_cast<rtx_hit*>(mb->pin_hits_d),
+                                  normals ? static_cast<float*>(mb->pin_nrm_d) : nullptr, nullptr, nullptr, st);
         if (rc) return rc;
         RTX_HIP(hipStreamSynchronize(st));
         std::memcpy(hits, mb->pin_hits, sizeof(rtx_hit) * n);
