@@ -160,9 +160,12 @@ __device__ __forceinline__ void store_gtri(GTri* p, f3 v0, f3 e1, f3 e2) {
 struct Leaf { uint32_t begin, count; };
 
 constexpr uint32_t kInlineLeaf = 0x40000000u;
+// The one inline encoding equal to kInvalid (31 triangles from 2^25 - 1), which traversal
+// keeps for a finished ray, takes the leaf-table form instead.
 RTX_HD uint32_t leaf_ref(uint32_t leaf_index, uint32_t begin, uint32_t count) {
-    return (count < 32u && begin < (1u << 25)) ? (kLeafBit | kInlineLeaf | (count << 25) | begin)
-                                               : (kLeafBit | leaf_index);
+    return (count < 32u && begin < (1u << 25) && !(count == 31u && begin == (1u << 25) - 1u))
+               ? (kLeafBit | kInlineLeaf | (count << 25) | begin)
+               : (kLeafBit | leaf_index);
 }
 
 struct Ray32 { float ox, oy, oz, tmin, dx, dy, dz, tmax; };

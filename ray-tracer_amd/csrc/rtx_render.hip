@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
     Walker<DEPTH, kRenderTriBatch> w;
     uint2 spill[kSpillDepth];
     w.spill = spill;
-    w.live = false;
+    w.kill();
     w.best = kInvalid;
     uint32_t id = kInvalid;            // path being traced
     const SceneDesc* S = scenes;       // its scene
@@ -304,11 +304,11 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
     bool gdone = false;                // `work` has passed n
     bool more = true;                  // the same on every lane: set only from wave-wide values
     for (;;) {
-        if (!w.live && id != kInvalid) {
+        if (!w.alive() && id != kInvalid) {
             P.hit[id] = make_float4(w.best != kInvalid ? w.bestT : 0.f, w.bu, w.bv, __uint_as_float(w.best));
             id = kInvalid;
         }
-        const uint64_t idle = __ballot(!w.live);
+        const uint64_t idle = __ballot(!w.alive());
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (more && nidle >= (uint32_t)kRenderRefill) {
             if (rbase == rend) {   // one atomic takes kRenderChunk queue positions
@@ -323,7 +323,7 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
             const uint32_t base = rbase;
             rbase = min(rbase + nidle, rend);
             more = !gdone || rbase < rend;
-            if (!w.live) {
+            if (!w.alive()) {
                 const uint32_t k = base + (uint32_t)__popcll(idle & lt);
                 if (k < rend) {
                     id = Q[k];
@@ -337,13 +337,13 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
             }
         }
         // uniform exit test: a started path that missed the root is written at the top
-        if (!__any(w.live || id != kInvalid) && !more) break;
+        if (!__any(w.alive() || id != kInvalid) && !more) break;
         w.template node_phase<false>(args(), lst, tid, wait);
         if (LOOP == kLoopCoop) {   // all lanes active: the loop is wave-uniform
-            const bool at = w.live && (w.ref & kLeafBit);
+            const bool at = w.alive() && (w.ref & kLeafBit);
             w.template leaf_coop<false, false, SC != kSceneUni>(args(), coop + tid / 64);
             if (at) w.pop(lst, tid);
-        } else if (w.live && (w.ref & kLeafBit)) {
+        } else if (w.alive() && (w.ref & kLeafBit)) {
             w.template leaf_step<false, false>(args());
             w.pop(lst, tid);
         }

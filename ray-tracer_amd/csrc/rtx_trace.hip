@@ -91,12 +91,12 @@ __global__ void __launch_bounds__(kTraceThreads, TREE == kTreeBinary ? 1 : 6) k_
     Walker<DEPTH> w;
     uint2 spill[kSpillDepth];
     w.spill = spill;
-    w.live = false;
+    w.kill();
     w.best = kInvalid;
     uint32_t id = kInvalid;
     bool more = true;   // the same on every lane: set only from wave-wide values
     for (;;) {
-        if (!w.live && id != kInvalid) {
+        if (!w.alive() && id != kInvalid) {
             if (ANY) {
                 occl[id] = w.best != kInvalid ? 1 : 0;
             } else {
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(kTraceThreads, TREE == kTreeBinary ? 1 : 6) k_
             }
             id = kInvalid;
         }
-        const uint64_t idle = __ballot(!w.live);
+        const uint64_t idle = __ballot(!w.alive());
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (more && nidle >= A.refill) {
             const int leader = __ffsll((unsigned long long)idle) - 1;
@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(kTraceThreads, TREE == kTreeBinary ? 1 : 6) k_
             if (__lane_id() == leader) base = atomicAdd(work, nidle);
             base = __shfl(base, leader);
             more = base + nidle < n;
-            if (!w.live) {
+            if (!w.alive()) {
                 const uint32_t k = base + (uint32_t)__popcll(idle & lt);
                 if (k < n) {
                     id = perm ? perm[k] : k;
@@ -130,14 +130,14 @@ __global__ void __launch_bounds__(kTraceThreads, TREE == kTreeBinary ? 1 : 6) k_
             }
         }
         // uniform exit test: a started ray that missed the root is written at the top
-        if (!__any(w.live || id != kInvalid) && !more) break;
+        if (!__any(w.alive() || id != kInvalid) && !more) break;
         w.template node_phase<COUNT, TREE>(A, lst, tid, A.wait);
         if (LOOP == kLoopCoop) {   // all lanes active: the loop is wave-uniform
-            const bool at = w.live && (w.ref & kLeafBit);
-            if (w.template leaf_coop<ANY, COUNT, false>(A, coop + tid / 64)) w.live = false;
+            const bool at = w.alive() && (w.ref & kLeafBit);
+            if (w.template leaf_coop<ANY, COUNT, false>(A, coop + tid / 64)) w.kill();
             else if (at) w.pop(lst, tid);
-        } else if (w.live && (w.ref & kLeafBit)) {
-            if (w.template leaf_step<ANY, COUNT>(A)) w.live = false;
+        } else if (w.alive() && (w.ref & kLeafBit)) {
+            if (w.template leaf_step<ANY, COUNT>(A)) w.kill();
             else w.pop(lst, tid);
         }
     }

@@ -201,7 +201,10 @@ struct Walker {
     float ix, iy, iz, tmin;   // tmax lives on as the starting best t (bestT <= tmax always)
     float bestT, bu, bv;
     uint32_t best, ref, top, lo, nI, nT;
-    bool live, fast;
+    bool fast;
+    // a finished (or never started) ray: ref = kInvalid, which has kLeafBit set
+    __device__ __forceinline__ bool alive() const { return ref != kInvalid; }
+    __device__ __forceinline__ void kill() { ref = kInvalid; }
     uint2* spill;   // kSpillDepth private entries; kept outside the struct so that the
                     // fields above stay in registers
 
@@ -218,11 +221,11 @@ struct Walker {
         ix = recip(d.x); iy = recip(d.y); iz = recip(d.z);
         bestT = tmax_; bu = 0.f; bv = 0.f;
         best = kInvalid; top = 0; lo = 0; nI = 0; nT = 0;
-        ref = A.root_ref;
         fast = A.finite && __builtin_isfinite(ix) && __builtin_isfinite(iy) && __builtin_isfinite(iz) &&
                ix != 0.f && iy != 0.f && iz != 0.f && __builtin_isfinite(o.x) && __builtin_isfinite(o.y) &&
                __builtin_isfinite(o.z) && !__builtin_isnan(tmin) && !__builtin_isnan(tmax_);
-        live = slab(o.x, o.y, o.z, ix, iy, iz, tmin, A.rb[0], A.rb[1], A.rb[2], A.rb[3], A.rb[4], A.rb[5], bestT) < bestT;
+        ref = slab(o.x, o.y, o.z, ix, iy, iz, tmin, A.rb[0], A.rb[1], A.rb[2], A.rb[3], A.rb[4], A.rb[5], bestT) < bestT
+                  ? A.root_ref : kInvalid;
     }
     __device__ __forceinline__ void push(uint2 (*lst)[kTraceThreads], uint32_t tid, uint32_t r, float t) {
         if (top - lo == (uint32_t)DEPTH) { spill[lo] = lst[slot(lo)][tid]; ++lo; }
@@ -231,7 +234,7 @@ struct Walker {
     }
     // pop the next entry that can still hold a closer hit
     __device__ __forceinline__ void pop(uint2 (*lst)[kTraceThreads], uint32_t tid) {
-        live = false;
+        ref = kInvalid;
         while (top > 0) {
             --top;
             uint2 e;
@@ -244,7 +247,7 @@ struct Walker {
                 const uint64_t x = wave_ld(reinterpret_cast<uint64_t*>(&lst[slot(top)][tid]));
                 e = make_uint2((uint32_t)x, (uint32_t)(x >> 32));
             }
-            if (__uint_as_float(e.y) < bestT) { ref = e.x; live = true; break; }
+            if (__uint_as_float(e.y) < bestT) { ref = e.x; break; }
         }
     }
     // one internal node: test both children, push the far one, continue with the near one
@@ -346,10 +349,10 @@ struct Walker {
     __device__ __forceinline__ void node_phase(const TraceArgs& A, uint2 (*lst)[kTraceThreads], uint32_t tid,
                                                uint32_t wait) {
         for (;;) {
-            const bool need = live && !(ref & kLeafBit);
+            const bool need = !(ref & kLeafBit);
             const uint64_t nm = __ballot(need);
             if (nm == 0) break;
-            if ((uint32_t)__popcll(nm) <= wait && __ballot(live && (ref & kLeafBit)) != 0) break;
+            if ((uint32_t)__popcll(nm) <= wait && __ballot(alive() && (ref & kLeafBit)) != 0) break;
             if (need) {
                 if (TREE != kTreeBinary) wide_step<COUNT>(A, lst, tid);
                 else node_step<COUNT>(A, lst, tid);
@@ -407,7 +410,7 @@ struct Walker {
     template <bool ANY, bool COUNT, bool MIXED>
     __device__ __forceinline__ bool leaf_coop(const TraceArgs& A, CoopLds* C) {
         const uint32_t lane = __lane_id();
-        const bool at = live && (ref & kLeafBit);
+        const bool at = alive() && (ref & kLeafBit);
         uint32_t c = 0, kb = 0;
         if (at) {
             const uint2 lf = leaf_range(A.leaves, ref);
@@ -501,7 +504,7 @@ __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, fl
     w.spill = spill;
     w.start(A, o, d, tmin, tmax);
     if (LOOP == 0) {
-        while (w.live) {
+        while (w.alive()) {
             if (w.ref & kLeafBit) {
                 if (w.template leaf_step<ANY, COUNT>(A)) break;
                 w.pop(lst, tid);
@@ -514,13 +517,13 @@ __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, fl
     } else {
         for (;;) {
             w.template node_phase<COUNT, TREE>(A, lst, tid, wait);
-            if (!__any(w.live)) break;
+            if (!__any(w.alive())) break;
             if (LOOP == kLoopCoop && __builtin_amdgcn_read_exec() == ~0ull) {
-                const bool at = w.live && (w.ref & kLeafBit);
-                if (w.template leaf_coop<ANY, COUNT, MIXED>(A, coop)) w.live = false;
+                const bool at = w.alive() && (w.ref & kLeafBit);
+                if (w.template leaf_coop<ANY, COUNT, MIXED>(A, coop)) w.kill();
                 else if (at) w.pop(lst, tid);
-            } else if (w.live && (w.ref & kLeafBit)) {
-                if (w.template leaf_step<ANY, COUNT>(A)) w.live = false;
+            } else if (w.alive() && (w.ref & kLeafBit)) {
+                if (w.template leaf_step<ANY, COUNT>(A)) w.kill();
                 else w.pop(lst, tid);
             }
         }
