@@ -252,13 +252,14 @@ struct Walker {
     }
     // one internal node: test both children, push the far one, continue with the near one
     template <bool COUNT>
-    __device__ __forceinline__ void node_step(const TraceArgs& A, uint2 (*lst)[kTraceThreads], uint32_t tid) {
+    __device__ __forceinline__ void node_step(const TraceArgs& A, uint2 (*lst)[kTraceThreads], uint32_t tid,
+                                              bool wfast) {
         if (COUNT) ++nI;
         const GNode* g = A.nodes + ref;
         const float4 l0 = ld_g(&g->l0), l1 = ld_g(&g->l1), r1b = ld_g(&g->r1);
         const uint2 refs = ld_g(reinterpret_cast<const uint2*>(&g->lref));
         float tl, tr;
-        if (__all(fast)) {   // wave vote: a scalar branch, both forms give the same values
+        if (wfast) {   // wave vote (__all(fast), uniform): a scalar branch, both forms give the same values
             tl = slab_fast(o.x, o.y, o.z, ix, iy, iz, tmin, l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, bestT);
             tr = slab_fast(o.x, o.y, o.z, ix, iy, iz, tmin, l1.z, l1.w, r1b.x, r1b.y, r1b.z, r1b.w, bestT);
         } else {
@@ -270,16 +271,17 @@ struct Walker {
             tl = slab(ox, oy, oz, ix, iy, iz, tmin, l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, bestT);
             tr = slab(ox, oy, oz, ix, iy, iz, tmin, l1.z, l1.w, r1b.x, r1b.y, r1b.z, r1b.w, bestT);
         }
-        uint32_t nearRef, farRef;
-        float tn, tf;
-        if (tl <= tr) { nearRef = refs.x; tn = tl; farRef = refs.y; tf = tr; }
-        else { nearRef = refs.y; tn = tr; farRef = refs.x; tf = tl; }
-        if (tn < bestT) {
-            if (tf < bestT) push(lst, tid, farRef, tf);
-            ref = nearRef;
-        } else {
-            pop(lst, tid);
+        const bool lf = tl <= tr;
+        const uint32_t nearRef = lf ? refs.x : refs.y, farRef = lf ? refs.y : refs.x;
+        const float tn = lf ? tl : tr, tf = lf ? tr : tl;
+        const bool go = tn < bestT, dopush = go && tf < bestT;
+        if (dopush) {
+            if (top - lo == (uint32_t)DEPTH) { spill[lo] = lst[slot(lo)][tid]; ++lo; }
+            lst[slot(top)][tid] = make_uint2(farRef, __float_as_uint(tf));
         }
+        top += dopush ? 1u : 0u;
+        ref = go ? nearRef : ref;
+        if (!go) pop(lst, tid);
     }
     // One wide node: test the (up to) four child boxes, continue with the nearest child
     // the ray enters before its best t, and push the others farthest first, each with its
@@ -348,6 +350,7 @@ struct Walker {
     template <bool COUNT, int TREE = kTreeBinary>
     __device__ __forceinline__ void node_phase(const TraceArgs& A, uint2 (*lst)[kTraceThreads], uint32_t tid,
                                                uint32_t wait) {
+        const bool wfast = __all(fast);   // every lane's fast flag is fixed while it traces a ray
         for (;;) {
             const bool need = !(ref & kLeafBit);
             const uint64_t nm = __ballot(need);
@@ -355,7 +358,7 @@ struct Walker {
             if ((uint32_t)__popcll(nm) <= wait && __ballot(alive() && (ref & kLeafBit)) != 0) break;
             if (need) {
                 if (TREE != kTreeBinary) wide_step<COUNT>(A, lst, tid);
-                else node_step<COUNT>(A, lst, tid);
+                else node_step<COUNT>(A, lst, tid, wfast);
             }
         }
     }
@@ -511,7 +514,7 @@ __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, fl
             } else if (TREE != kTreeBinary) {
                 w.template wide_step<COUNT>(A, lst, tid);
             } else {
-                w.template node_step<COUNT>(A, lst, tid);
+                w.template node_step<COUNT>(A, lst, tid, __all(w.fast));
             }
         }
     } else {
