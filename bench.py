@@ -333,7 +333,7 @@ def _cuda_ms(fn, reps, stream):
     return a.elapsed_time(b) / reps
 
 
-def bench_differentiable(stream, reps=3):
+def bench_differentiable(stream, reps=6):
     """Differentiable render: Cornell box + 4096-triangle displaced sphere, per-triangle
     albedo, 2-triangle emitter; 512x512, 32 spp, max_bounce 4, no Russian roulette (the
     adjoint needs a path set independent of albedo). Loss = sum (I - T)^2."""
@@ -360,6 +360,8 @@ def bench_differentiable(stream, reps=3):
         e = e_t.detach().clone().requires_grad_(True)
         loss = ((rtx.differentiable_render(a, e, g, cam, p) - target) ** 2).sum()
         loss.backward()
+    fwd()
+    fwd_adj()
     ms_f = _cuda_ms(fwd, reps, stream)
     ms_fa = _cuda_ms(fwd_adj, reps, stream)
     return {"workload": "512x512, 32 spp, max_bounce 4, 4118 triangles", "forward_ms": round(ms_f, 3),
