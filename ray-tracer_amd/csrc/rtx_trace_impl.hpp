@@ -121,6 +121,19 @@ __device__ __forceinline__ uint32_t ld_g(const uint32_t* p) {
     return *(__attribute__((address_space(1))) const uint32_t*)p;
 }
 
+// Inclusive sum over the 64 lanes of a wave, lane order, every lane active: four row shifts
+// within each 16-lane row, then row 0's and row 1's last lanes broadcast to the rows above
+// (DPP, no LDS round trips; checked against the serial sum on gfx950).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x111, 0xf, 0xf, false);   // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x112, 0xf, 0xf, false);   // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x114, 0xf, 0xf, false);   // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x118, 0xf, 0xf, false);   // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x142, 0xa, 0xf, false);   // row_bcast:15 into rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x143, 0xc, 0xf, false);   // row_bcast:31 into rows 2, 3
+    return x;
+}
+
 // Triangle range (begin, count) of a leaf reference: packed in the reference itself, or
 // read from the leaf table.
 __device__ __forceinline__ uint2 leaf_range(const Leaf* leaves, uint32_t ref) {
@@ -420,13 +433,8 @@ struct Walker {
             c = lf.y;
             kb = lf.x;
         }
-        uint32_t inc = c;
-#pragma unroll
-        for (int s = 1; s < 64; s <<= 1) {
-            const uint32_t y = __shfl_up(inc, s);
-            if ((int)lane >= s) inc += y;
-        }
-        const uint32_t total = __shfl(inc, 63);
+        const uint32_t inc = wave_incl_scan(c);
+        const uint32_t total = __builtin_amdgcn_readlane(inc, 63);   // uniform: a scalar loop bound
         const uint32_t start = inc - c;
         const uint32_t kbase = kb - start;   // triangle of position p is kbase + p (mod 2^32)
         if (COUNT && !ANY) nT += c;
