@@ -357,3 +357,57 @@ def test_adjoint_replay_equals_self_contained_and_memory_is_flat():
             np.testing.assert_allclose(x, y, rtol=1e-4, atol=1e-6 * np.abs(x).max())
     print(f"[adjoint memory] scratch peak {peaks[2]} B at max_bounce 2, {peaks[30]} B at max_bounce 30", flush=True)
     assert peaks[2] > 0 and abs(peaks[30] - peaks[2]) <= 4096
+
+
+def test_adjoint_over_many_batches_equals_one_batch():
+    """With paths processed in batches of 1024 (the library's smallest), the primal's stored
+    radiance and the replay's gradient scatter index each batch's paths by their global
+    number: gradients equal the one-batch run's up to the order of float atomics, for the
+    per-triangle arrays (LDS form) and for texels (global-atomics form)."""
+    import torch
+    v, t, albedo, emission, cam = S.diff_scene(seed=3)
+    g = rtx.build(v, t)
+    W, H, spp = 40, 36, 3   # 4320 paths: four full batches of 1024 and a partial one
+    p = rtx.make_params(W, H, spp, 3, 99, seed=9)
+    a_t, e_t = _tensor(albedo), _tensor(emission)
+    saved = rtx.get_tuning()
+    out = {}
+    try:
+        for bp in (1 << 23, 1024):
+            rtx.set_tuning(batch_paths=bp)
+            img, paths = rtx.render_primal(g, cam, p, a_t, e_t)
+            d = (2 * (img - 0.25)).contiguous()
+            ga, ge = rtx.render_adjoint(g, cam, p, a_t, e_t, d, path_radiance=paths)
+            ga2, ge2 = rtx.render_adjoint(g, cam, p, a_t, e_t, d)
+            torch.cuda.synchronize()
+            out[bp] = [x.cpu().numpy() for x in (img, paths, ga, ge, ga2, ge2)]
+    finally:
+        rtx.set_tuning(**saved)
+    one, many = out[1 << 23], out[1024]
+    assert np.array_equal(one[0].view(np.uint32), many[0].view(np.uint32))   # image
+    assert np.array_equal(one[1].view(np.uint32), many[1].view(np.uint32))   # path radiance
+    for x, y in zip(one[2:], many[2:]):
+        assert np.abs(x).max() > 0
+        np.testing.assert_allclose(y, x, rtol=1e-4, atol=1e-6 * np.abs(x).max())
+
+    # texels: a 512^2 texture on a small terrain takes the global-atomics form
+    tv, tt, uv = S.terrain(33, seed=4)
+    tg = rtx.build(tv, tt)
+    tcam, sky, sun = S.terrain_setup()
+    uv_t = torch.from_numpy(uv).cuda()
+    tex = torch.full((512, 512, 3), 0.5, device="cuda")
+    tp = rtx.make_params(W, H, spp, 3, 99, seed=4, env_mode=3, env_color=sky, uv=uv_t, tex=tex, sun=sun)
+    grads = {}
+    try:
+        for bp in (1 << 23, 1024):
+            rtx.set_tuning(batch_paths=bp)
+            img, paths = rtx.render_primal(tg, tcam, tp)
+            d = (2 * (img - 0.25)).contiguous()
+            gt, _ = rtx.render_adjoint(tg, tcam, tp, None, None, d, path_radiance=paths)
+            torch.cuda.synchronize()
+            grads[bp] = gt.cpu().numpy()
+    finally:
+        rtx.set_tuning(**saved)
+    x, y = grads[1 << 23], grads[1024]
+    assert np.abs(x).max() > 0
+    np.testing.assert_allclose(y, x, rtol=1e-4, atol=1e-6 * np.abs(x).max())
