@@ -304,13 +304,19 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
     bool gdone = false;                // `work` has passed n
     bool more = true;                  // the same on every lane: set only from wave-wide values
     for (;;) {
-        if (!w.alive() && id != kInvalid) {
+        const uint64_t idle = __ballot(!w.alive());
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        const bool refill = more && nidle >= (uint32_t)kRenderRefill;   // wave-uniform
+        const bool done = !more && nidle == 64u;                       // wave-uniform
+        // finished paths are written when their lanes take new ones or the wave ends, so
+        // finish()'s test runs for many lanes at once
+        if ((refill || done) && !w.alive() && id != kInvalid) {
+            w.finish(args());
             P.hit[id] = make_float4(w.best != kInvalid ? w.bestT : 0.f, w.bu, w.bv, __uint_as_float(w.best));
             id = kInvalid;
         }
-        const uint64_t idle = __ballot(!w.alive());
-        const uint32_t nidle = (uint32_t)__popcll(idle);
-        if (more && nidle >= (uint32_t)kRenderRefill) {
+        if (done) break;
+        if (refill) {
             if (rbase == rend) {   // one atomic takes kRenderChunk queue positions
                 const int leader = __ffsll((unsigned long long)idle) - 1;
                 uint32_t b = 0;
@@ -336,8 +342,7 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
                 }
             }
         }
-        // uniform exit test: a started path that missed the root is written at the top
-        if (!__any(w.alive() || id != kInvalid) && !more) break;
+        // a started path that missed the root is written at the next refill or at the end
         w.template node_phase<false>(args(), lst, tid, wait);
         if (LOOP == kLoopCoop) {   // all lanes active: the loop is wave-uniform
             const bool at = w.alive() && (w.ref & kLeafBit);

@@ -96,10 +96,17 @@ __global__ void __launch_bounds__(kTraceThreads, TREE == kTreeBinary ? 1 : 6) k_
     uint32_t id = kInvalid;
     bool more = true;   // the same on every lane: set only from wave-wide values
     for (;;) {
-        if (!w.alive() && id != kInvalid) {
+        const uint64_t idle = __ballot(!w.alive());
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        const bool refill = more && nidle >= A.refill;   // wave-uniform
+        const bool done = !more && nidle == 64u;        // wave-uniform: nothing left to trace
+        // Finished rays are written when their lanes take new rays or the wave ends (they
+        // idle until then in any case), so finish()'s test runs for many lanes at once.
+        if ((refill || done) && !w.alive() && id != kInvalid) {
             if (ANY) {
                 occl[id] = w.best != kInvalid ? 1 : 0;
             } else {
+                w.finish(A);
                 reinterpret_cast<float4*>(hits)[id] =
                     make_float4(w.best != kInvalid ? w.bestT : 0.f, w.bu, w.bv, __uint_as_float(w.best));
                 if (COUNT) { counts[2 * id] = w.nI; counts[2 * id + 1] = w.nT; }
@@ -111,9 +118,8 @@ __global__ void __launch_bounds__(kTraceThreads, TREE == kTreeBinary ? 1 : 6) k_
             }
             id = kInvalid;
         }
-        const uint64_t idle = __ballot(!w.alive());
-        const uint32_t nidle = (uint32_t)__popcll(idle);
-        if (more && nidle >= A.refill) {
+        if (done) break;
+        if (refill) {
             const int leader = __ffsll((unsigned long long)idle) - 1;
             uint32_t base = 0;
             if (__lane_id() == leader) base = atomicAdd(work, nidle);
@@ -129,8 +135,7 @@ __global__ void __launch_bounds__(kTraceThreads, TREE == kTreeBinary ? 1 : 6) k_
                 }
             }
         }
-        // uniform exit test: a started ray that missed the root is written at the top
-        if (!__any(w.alive() || id != kInvalid) && !more) break;
+        // a started ray that missed the root is written at the next refill or at the end
         w.template node_phase<COUNT, TREE>(A, lst, tid, A.wait);
         if (LOOP == kLoopCoop) {   // all lanes active: the loop is wave-uniform
             const bool at = w.alive() && (w.ref & kLeafBit);

@@ -172,9 +172,11 @@ __device__ __forceinline__ void load_tri(const GTri* p, float4& a, float4& b, fl
 // t first, then the earlier triangle. -0 and +0 compare equal, as they do as floats. t is
 // never NaN here (a NaN t fails t < best).
 __device__ __forceinline__ unsigned long long coop_key(float t, uint32_t k) {
-    uint32_t b = t == 0.f ? 0u : __float_as_uint(t);
-    b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-    return ((unsigned long long)b << 32) | k;
+    return ((unsigned long long)ord_t(t) << 32) | k;
+}
+// The float of an ord_t image (a zero comes back as +0).
+__device__ __forceinline__ float t_of_ord(uint32_t b) {
+    return __uint_as_float((b & 0x80000000u) ? (b & 0x7FFFFFFFu) : ~b);
 }
 
 // tri_test's accept/reject decision and its t up to the sign of a zero t, for the dealt
@@ -417,8 +419,8 @@ struct Walker {
     // per owner and a ballot), takes the owner's ray by shuffle and tests its triangle; an
     // LDS atomic min of coop_key(t, k) per owner keeps, among t < the owner's best t, the
     // smallest t and then the earliest triangle: what testing the leaf in order with a
-    // strict t < best update keeps. The owner re-tests that one triangle for u and v (the
-    // same inputs give the same bits). Any-hit (ANY) keeps the smallest accepted triangle
+    // strict t < best update keeps. The owner keeps that triangle and its t as its best; u
+    // and v come from finish(), once per ray. Any-hit (ANY) keeps the smallest accepted triangle
     // index instead: the first triangle in leaf order that the in-order scan would stop at.
     // Returns true on the owner lanes whose any-hit query is answered. The caller keeps all
     // 64 lanes active and the control flow uniform. MIXED: lanes may trace different trees
@@ -482,21 +484,30 @@ struct Walker {
             return hit;
         }
         if (c) {
+            // The winner and its t (up to the sign of a zero, which no later comparison
+            // sees); finish() tests the final winner again for the exact t, u and v.
             const unsigned long long mk = wave_ld(&C->mins[lane]);
             if (mk != ~0ull) {
-                const uint32_t k = (uint32_t)mk;
-                float4 a, b;
-                float cz;
-                load_tri(A.tris + k, a, b, cz);
-                float u, v;
-                // the winner passed t < best <= tmax in its dealt test
-                bestT = tri_test(o, d, tmin, kInf, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y}, f3{b.z, b.w, cz}, u, v);
-                bu = u;
-                bv = v;
-                best = k;
+                best = (uint32_t)mk;
+                bestT = t_of_ord((uint32_t)(mk >> 32));
             }
         }
         return false;
+    }
+    // Exact t, u and v of the closest hit. The cooperative leaf phases keep only each
+    // winner's triangle and its t, so the ray's final winner is tested once more here with
+    // tri_test: the inputs of the test that made it the winner, so the reference's bits (a
+    // winner of leaf_step is tested again to the same bits). It passed t < best <= tmax then.
+    // A is the lane's own tree.
+    __device__ __forceinline__ void finish(const TraceArgs& A) {
+        if (best == kInvalid) return;
+        float4 a, b;
+        float cz;
+        load_tri(A.tris + best, a, b, cz);
+        float u, v;
+        bestT = tri_test(o, d, tmin, kInf, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y}, f3{b.z, b.w, cz}, u, v);
+        bu = u;
+        bv = v;
     }
 };
 
@@ -539,6 +550,7 @@ __device__ __forceinline__ TraceOut trace_ray(const TraceArgs& A, f3 o, f3 d, fl
             }
         }
     }
+    if (!ANY) w.finish(A);
     TraceOut r;
     r.t = w.best != kInvalid ? w.bestT : 0.f;
     r.u = w.bu;
