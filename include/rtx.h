@@ -193,19 +193,21 @@ typedef struct rtx_render_params {
 int rtx_render(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_params* p,
                const float* albedo_d, const float* emission_d, float* image_d, void* stream);
 
-/* rtx_render that also stores each path's radiance (the primal of the path-replay adjoint):
- * path_radiance_d holds 3 floats per path, width*height*spp*3 floats, path sample*W*H +
- * pixel at 3 * (sample * width * height + pixel).  The image equals rtx_render's.  No
- * reference counterpart. */
+/* rtx_render that also stores each path's record (the primal of the path-replay adjoint):
+ * path_records_d (16-byte aligned) holds 8 floats per path, width*height*spp*8 floats; path
+ * g = sample*width*height + pixel has its radiance L at 8g..8g+2 (8g+3 is 0) and the hit of
+ * its camera ray at 8g+4..8g+7 (t, u, v, triangle index bits; 0xFFFFFFFF on a miss).  32
+ * bytes per path whatever max_bounce is; with the camera rays' hits stored the replay does
+ * not trace them again.  The image equals rtx_render's.  No reference counterpart. */
 int rtx_render_primal(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_params* p,
                       const float* albedo_d, const float* emission_d, float* image_d,
-                      float* path_radiance_d, void* stream);
+                      float* path_records_d, void* stream);
 
 /* Path-replay adjoint of rtx_render for the loss sum (I - target)^2: accumulates
  * dLoss/dalbedo and dLoss/demission (3 floats per original triangle, added to the
  * buffers) given d_image_d = dLoss/dI.  Paths are re-traced with the same sample table,
- * not stored: per batch of paths it runs a primal pass for each path's radiance L, then
- * the replay, which at the k-th surface hit of a path adds dL/da_k = (L - L_upto_k) / a_k
+ * not stored: per batch of paths it runs a primal pass for each path's radiance L and
+ * camera-ray hit (a 32-byte record per path), then the replay, which at the k-th surface hit of a path adds dL/da_k = (L - L_upto_k) / a_k
  * and dL/dLe_k = throughput arriving at k, where L_upto_k is the radiance gathered up to
  * and including that hit.  Only the current bounce is held per path, so memory does not
  * grow with max_bounce.  Requires rr_bounce > max_bounce (no Russian roulette) so the path
@@ -215,11 +217,12 @@ int rtx_render_primal(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_rende
 int rtx_render_adjoint(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_params* p,
                        const float* albedo_d, const float* emission_d, const float* d_image_d,
                        float* grad_albedo_d, float* grad_emission_d, void* stream);
-/* The replay of rtx_render_adjoint alone, with each path's radiance taken from an earlier
- * rtx_render_primal of the same scene, camera, parameters and materials (so one pass is
- * saved).  No reference counterpart. */
+/* The replay of rtx_render_adjoint alone, with each path's record (radiance and camera-ray
+ * hit) taken from an earlier rtx_render_primal of the same scene, camera, parameters and
+ * materials (so the primal pass and the camera rays' trace are saved).  No reference
+ * counterpart. */
 int rtx_render_adjoint_replay(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_params* p,
-                              const float* albedo_d, const float* emission_d, const float* path_radiance_d,
+                              const float* albedo_d, const float* emission_d, const float* path_records_d,
                               const float* d_image_d, float* grad_albedo_d, float* grad_emission_d, void* stream);
 
 /* Batched wavefront render of many scenes.  scenes[i] renders with cams[i]/params[i]

@@ -64,11 +64,11 @@ constexpr int kShadeForward = 0, kShadeReplay = 1;
 // flushed once per block; or global atomics after merging equal entries within the wave.
 constexpr int kGradNone = 0, kGradLds = 1, kGradGlobal = 2;
 
-// Inputs and outputs of the replay's gradient scatter (scene 0 only). Lfin + 3 * (lbase + i)
-// is path i's primal radiance; dimg = dLoss/dimage (3 floats per pixel); ga, ge: gradient
+// Inputs and outputs of the replay's gradient scatter (scene 0 only). rec[2 * (lbase + i)]
+// holds path i's primal radiance (its path record, see PathRecord); dimg = dLoss/dimage (3 floats per pixel); ga, ge: gradient
 // arrays (3 floats per albedo entry, per original triangle), either may be null.
 struct GradArgs {
-    const float* Lfin;
+    const float4* rec;
     uint64_t lbase;
     const float* dimg;
     float* ga;
@@ -176,13 +176,27 @@ __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* _
     }
 }
 
-// Each path's radiance L into out[3 * (base + i) ...] (the primal radiance of the replay).
-__global__ void k_store_L(PathState P, uint32_t cnt, float* __restrict__ out, uint64_t base) {
+// Path records of the primal render, the replay's only input per path: two float4 per path,
+// rec[2 * g] = (radiance L, 0) and rec[2 * g + 1] = the path's first hit (t, u, v, triangle
+// bits), g the path's global number. 32 bytes per path, whatever max_bounce is; with the first
+// hit stored the replay skips the trace of the camera rays.
+// Each path's radiance L into rec[2 * (base + i)].
+__global__ void k_store_L(PathState P, uint32_t cnt, float4* __restrict__ rec, uint64_t base) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         const float4 L = P.L[i];
-        float* o = out + 3 * (base + i);
-        o[0] = L.x; o[1] = L.y; o[2] = L.z;
+        rec[2 * (base + i)] = make_float4(L.x, L.y, L.z, 0.f);
     }
+}
+// After the camera rays' trace (queue 0 is the identity): path i's hit into rec[2 * (base + i) + 1].
+__global__ void k_store_hit(PathState P, uint32_t cnt, float4* __restrict__ rec, uint64_t base) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
+        rec[2 * (base + i) + 1] = P.hit[i];
+}
+// The replay's camera rays take their hits from the records instead of a trace: the same
+// rays (the same sample table) give the same hits.
+__global__ void k_load_hit(PathState P, uint32_t cnt, const float4* __restrict__ rec, uint64_t base) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
+        P.hit[i] = rec[2 * (base + i) + 1];
 }
 
 // The wavefront trace loads leaf triangles two at a time and is held to 64 VGPRs so that
@@ -404,7 +418,7 @@ constexpr uint32_t kGradLdsBlocks = 256;
 // again (the same sample table gives the same paths, bit for bit), and at the surface hit k
 // of path i this bounce adds, weighted by dLoss/dI of the path's pixel / spp,
 //   dL/da_k = (L - L_upto_k) / a_k   to ga[albedo entry],   dL/dLe_k = tp_k   to ge[triangle],
-// with L the path's primal radiance (G.Lfin), L_upto_k the radiance gathered up to and
+// with L the path's primal radiance (from G.rec), L_upto_k the radiance gathered up to and
 // including hit k's own emission and tp_k the throughput arriving at k: with
 // L = sum_m (prod_{j<m} a_j) X_m everything past hit k is proportional to a_k. Nothing is
 // kept per path between bounces beyond the forward path state, so memory does not grow with
@@ -469,10 +483,10 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
                     const f3 delta{ld_g(G.dimg + 3 * px) / fs, ld_g(G.dimg + 3 * px + 1) / fs,
                                    ld_g(G.dimg + 3 * px + 2) / fs};
                     if (G.ga) {
-                        const float* lf = G.Lfin + 3 * (G.lbase + i);
-                        gA = f3{ax != 0.f ? delta.x * (ld_g(lf) - L.x) / ax : 0.f,
-                                ay != 0.f ? delta.y * (ld_g(lf + 1) - L.y) / ay : 0.f,
-                                az != 0.f ? delta.z * (ld_g(lf + 2) - L.z) / az : 0.f};
+                        const float4 lf = ld_g(G.rec + 2 * (G.lbase + i));
+                        gA = f3{ax != 0.f ? delta.x * (lf.x - L.x) / ax : 0.f,
+                                ay != 0.f ? delta.y * (lf.y - L.y) / ay : 0.f,
+                                az != 0.f ? delta.z * (lf.z - L.z) / az : 0.f};
                     }
                     gE = f3{delta.x * tpin.x, delta.y * tpin.y, delta.z * tpin.z};
                 }
@@ -657,14 +671,14 @@ struct Batch {
         return RTX_OK;
     }
     // adjoint (scene 0 only): dLoss/dimage in, gradients out. adj 0: a forward render (which
-    // also stores each path's radiance in Lpath when that is set); 1: a primal pass and the
-    // replay, per batch; 2: the replay alone, with the primal radiance read from Lpath.
+    // also stores each path's record in rec when that is set); 1: a primal pass and the
+    // replay, per batch; 2: the replay alone, with the path records read from rec.
     int adj = 0;
     const float* dimg = nullptr;
     float* ga = nullptr;
     float* ge = nullptr;
-    float* Lpath = nullptr;    // 3 floats per path, indexed by the path's global number
-    float* Lbatch = nullptr;   // adj 1: the batch's primal radiance, 3 floats per batch path
+    float4* rec = nullptr;     // path records (k_store_L), indexed by the path's global number
+    float4* recb = nullptr;    // adj 1: the batch's path records
     int setup(uint32_t batch_paths, bool adjoint = false);
     void enqueue_range(int s_first, int s_last, hipStream_t st);   // scenes [s_first, s_last)
     int launch(hipStream_t st);
@@ -689,7 +703,7 @@ int Batch::setup(uint32_t batch_paths, bool adjoint) {
     ncnt = 2 * (uint32_t)(max_b + 2);
     P.cnt = alloc<uint32_t>(ncnt);
     P.B = B;
-    if (adjoint) {   // flat in max_bounce: only the batch's primal radiance is added
+    if (adjoint) {   // flat in max_bounce: only the batch's path records are added
         // the replay's LDS form may take more than the default 64 KiB of dynamic LDS
         static const bool lds_ok =
             hip_ok(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_shade<kShadeReplay, true, kGradLds>),
@@ -699,8 +713,8 @@ int Batch::setup(uint32_t batch_paths, bool adjoint) {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsGradBytes),
                    "hipFuncSetAttribute");
         if (!lds_ok) return RTX_E_HIP;
-        Lbatch = alloc<float>(3ull * B);
-        if (!Lbatch) return fail(RTX_E_HIP, "device allocation failed on render setup");
+        recb = alloc<float4>(2ull * B);
+        if (!recb) return fail(RTX_E_HIP, "device allocation failed on render setup");
     }
     if (!d_scenes || !d_off || !P.ray0 || !P.ray1 || !P.hit || !P.tp || !P.L || !P.sid || !P.pix || !P.smp ||
         !P.q[0] || !P.q[1] || !P.cnt)
@@ -730,9 +744,10 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
         const uint32_t cnt = (uint32_t)std::min<uint64_t>(B, off[s_last] - g0);
         const uint32_t cgrid = std::min<uint32_t>((cnt + 255) / 256, 65535);
         // One wavefront over the batch: camera rays, then max_bounce + 1 trace/shade rounds.
-        // The replay (kShadeReplay) scatters each bounce's gradients as it shades it;
-        // Lfin + 3 * lbase is the batch's primal radiance.
-        auto wavefront = [&](int mode, const float* Lfin, uint64_t lbase) {
+        // The replay (kShadeReplay) scatters each bounce's gradients as it shades it and takes
+        // its camera rays' hits from the batch's path records, recs + 2 * rbase; the forward
+        // wavefront stores its camera rays' hits there when recs is set.
+        auto wavefront = [&](int mode, float4* recs, uint64_t rbase) {
             k_set_counts<<<1, 64, 0, st>>>(P.cnt, ncnt, cnt);
             k_camera<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
             // bounce b traces queue q[qi] of cnt[b] paths; shade appends the survivors to
@@ -748,7 +763,9 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
                 // take the chunked kernel, 64 paths per wave.
                 const bool persist = g_tuning.render_loop == 2 &&
                                      (uint64_t)cnt >= (uint64_t)pgrid * (kTraceThreads / 64) * kRenderChunk;
-                if (persist) {
+                if (mode == kShadeReplay && b == 0) {
+                    k_load_hit<<<cgrid, 256, 0, st>>>(P, cnt, recs, rbase);
+                } else if (persist) {
                     uint32_t* work = P.cnt + ncnt / 2 + b;
                     with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
                         constexpr int DD = decltype(D)::value, LL = decltype(L)::value;
@@ -776,10 +793,11 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
                                 <<<grid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, b, wait);
                     });
                 }
+                if (mode == kShadeForward && b == 0 && recs) k_store_hit<<<cgrid, 256, 0, st>>>(P, cnt, recs, rbase);
                 if (mode == kShadeReplay) {
                     // the gradient scatter rides on the replay's shading: both arrays in LDS
                     // when they fit, global atomics otherwise
-                    const GradArgs G{Lfin, lbase, dimg, ga, ge, descs[0].nprim};
+                    const GradArgs G{recs, rbase, dimg, ga, ge, descs[0].nprim};
                     const size_t lds = ((ga ? 1u : 0u) + (ge ? 1u : 0u)) * 3ull * descs[0].nprim * sizeof(float);
                     if (lds <= kLdsGradBytes) {
                         const uint32_t gb = std::min<uint32_t>(kGradLdsBlocks, (cnt + kGradThreads - 1) / kGradThreads);
@@ -798,15 +816,15 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
             }
         };
         if (adj == 0) {
-            wavefront(kShadeForward, nullptr, 0);
-            if (Lpath) k_store_L<<<cgrid, 256, 0, st>>>(P, cnt, Lpath, g0);
+            wavefront(kShadeForward, rec, g0);
+            if (rec) k_store_L<<<cgrid, 256, 0, st>>>(P, cnt, rec, g0);
             k_accum<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
         } else if (adj == 1) {
-            wavefront(kShadeForward, nullptr, 0);
-            k_store_L<<<cgrid, 256, 0, st>>>(P, cnt, Lbatch, 0);
-            wavefront(kShadeReplay, Lbatch, 0);
+            wavefront(kShadeForward, recb, 0);
+            k_store_L<<<cgrid, 256, 0, st>>>(P, cnt, recb, 0);
+            wavefront(kShadeReplay, recb, 0);
         } else {
-            wavefront(kShadeReplay, Lpath, g0);
+            wavefront(kShadeReplay, rec, g0);
         }
     }
     if (!adj)
@@ -917,13 +935,15 @@ extern "C" int rtx_render(const rtx_bvh* b, const rtx_camera* cam, const rtx_ren
 
 extern "C" int rtx_render_primal(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params* p,
                                  const float* albedo_d, const float* emission_d, float* image_d,
-                                 float* path_radiance_d, void* stream) {
+                                 float* path_records_d, void* stream) {
     int rc = check_scene(b, cam, p, image_d, "rtx_render_primal()");
     if (rc) return rc;
-    if (!path_radiance_d) return fail(RTX_E_ARG, "null path_radiance_d on rtx_render_primal()");
+    if (!path_records_d) return fail(RTX_E_ARG, "null path_records_d on rtx_render_primal()");
+    if (reinterpret_cast<uintptr_t>(path_records_d) % 16)
+        return fail(RTX_E_ARG, "path_records_d is not 16-byte aligned on rtx_render_primal()");
     Batch B;
     B.descs.push_back(make_desc(b, cam, p, albedo_d, emission_d, image_d));
-    B.Lpath = path_radiance_d;
+    B.rec = reinterpret_cast<float4*>(path_records_d);
     rc = B.setup(g_tuning.batch_paths);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
@@ -936,10 +956,12 @@ extern "C" int rtx_render_primal(const rtx_bvh* b, const rtx_camera* cam, const 
 
 namespace {
 int adjoint_impl(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params* p, const float* albedo_d,
-                 const float* emission_d, const float* path_radiance_d, const float* d_image_d, float* grad_albedo_d,
+                 const float* emission_d, const float* path_records_d, const float* d_image_d, float* grad_albedo_d,
                  float* grad_emission_d, void* stream, const char* fn) {
     int rc = check_scene(b, cam, p, d_image_d, fn);
     if (rc) return rc;
+    if (reinterpret_cast<uintptr_t>(path_records_d) % 16)
+        return fail(RTX_E_ARG, std::string("path_records_d is not 16-byte aligned on ") + fn);
     if (!albedo_d && !p->tex_d) return fail(RTX_E_ARG, std::string("albedo is required on ") + fn);
     if (p->tex_d && grad_emission_d)
         return fail(RTX_E_ARG, std::string("emission gradients are per triangle: not with texture albedo on ") + fn);
@@ -948,12 +970,12 @@ int adjoint_impl(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_param
     if (p->max_bounce > 64) return fail(RTX_E_LIMIT, std::string("max_bounce above 64 on ") + fn);
     Batch B;
     B.descs.push_back(make_desc(b, cam, p, albedo_d, emission_d, nullptr));
-    B.adj = path_radiance_d ? 2 : 1;
-    B.Lpath = const_cast<float*>(path_radiance_d);
+    B.adj = path_records_d ? 2 : 1;
+    B.rec = reinterpret_cast<float4*>(const_cast<float*>(path_records_d));
     B.dimg = d_image_d;
     B.ga = grad_albedo_d;
     B.ge = emission_d ? grad_emission_d : nullptr;
-    // the state per path does not depend on max_bounce: one record and the primal radiance
+    // the state per path does not depend on max_bounce: the path state and one path record
     rc = B.setup(g_tuning.batch_paths, true);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
@@ -973,10 +995,10 @@ extern "C" int rtx_render_adjoint(const rtx_bvh* b, const rtx_camera* cam, const
 }
 
 extern "C" int rtx_render_adjoint_replay(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params* p,
-                                         const float* albedo_d, const float* emission_d, const float* path_radiance_d,
+                                         const float* albedo_d, const float* emission_d, const float* path_records_d,
                                          const float* d_image_d, float* grad_albedo_d, float* grad_emission_d,
                                          void* stream) {
-    if (!path_radiance_d) return fail(RTX_E_ARG, "null path_radiance_d on rtx_render_adjoint_replay()");
-    return adjoint_impl(b, cam, p, albedo_d, emission_d, path_radiance_d, d_image_d, grad_albedo_d, grad_emission_d,
+    if (!path_records_d) return fail(RTX_E_ARG, "null path_records_d on rtx_render_adjoint_replay()");
+    return adjoint_impl(b, cam, p, albedo_d, emission_d, path_records_d, d_image_d, grad_albedo_d, grad_emission_d,
                         stream, "rtx_render_adjoint_replay()");
 }

@@ -485,28 +485,30 @@ def render(bvh, cam, params, albedo=None, emission=None, out=None, stream=None):
 
 
 def render_primal(bvh, cam, params, albedo=None, emission=None, out=None, stream=None):
-    """render() that also returns each path's radiance ((spp * H * W, 3) float32 CUDA
-    tensor, path sample * H * W + pixel): the primal that render_adjoint_replay consumes."""
+    """render() that also returns each path's record ((spp * H * W, 8) float32 CUDA tensor,
+    row sample * H * W + pixel: the path's radiance in columns 0-2, 0, then its camera ray's
+    hit t, u, v and triangle index bits): the primal that render_adjoint consumes."""
     import torch
     _check_scene(bvh, params, albedo, emission)
     if out is None:
         out = torch.empty((params.height, params.width, 3), dtype=torch.float32, device="cuda")
     _need(out, "out", "f32", 3 * params.width * params.height)
-    paths = torch.empty((params.spp * params.height * params.width, 3), dtype=torch.float32, device="cuda")
+    paths = torch.empty((params.spp * params.height * params.width, 8), dtype=torch.float32, device="cuda")
     _check(lib().rtx_render_primal(bvh.handle, C.byref(make_camera(cam)), C.byref(params), _dptr(albedo),
                                    _dptr(emission), _dptr(out), _dptr(paths), _stream_ptr(stream)), "rtx_render_primal")
     return out, paths
 
 
 def render_adjoint(bvh, cam, params, albedo, emission, d_image, grad_albedo=None, grad_emission=None, stream=None,
-                   path_radiance=None):
+                   path_records=None):
     """Accumulate dLoss/dalbedo and dLoss/demission given dLoss/dimage (path replay). With
     texture albedo (params.tex_d set) grad_albedo is per texel and there is no emission
-    gradient. path_radiance (from render_primal of the same render) saves the primal pass."""
+    gradient. path_records (from render_primal of the same render) saves the primal pass
+    and the trace of the camera rays."""
     import torch
     textured = bool(params.tex_d)
-    if path_radiance is not None:
-        _need(path_radiance, "path_radiance", "f32", 3 * params.spp * params.height * params.width)
+    if path_records is not None:
+        _need(path_records, "path_records", "f32", 8 * params.spp * params.height * params.width)
     _check_scene(bvh, params, albedo, emission)
     d_image = d_image.contiguous()
     _need(d_image, "d_image", "f32", 3 * params.width * params.height)
@@ -517,13 +519,13 @@ def render_adjoint(bvh, cam, params, albedo, emission, d_image, grad_albedo=None
     _need(grad_albedo, "grad_albedo", "f32",
           3 * (params.tex_width * params.tex_height if textured else bvh.info()["input_tri_count"]))
     _need(grad_emission, "grad_emission", "f32", 3 * bvh.info()["input_tri_count"], optional=True)
-    if path_radiance is None:
+    if path_records is None:
         _check(lib().rtx_render_adjoint(bvh.handle, C.byref(make_camera(cam)), C.byref(params), _dptr(albedo),
                                         _dptr(emission), _dptr(d_image), _dptr(grad_albedo),
                                         _dptr(grad_emission), _stream_ptr(stream)), "rtx_render_adjoint")
     else:
         _check(lib().rtx_render_adjoint_replay(bvh.handle, C.byref(make_camera(cam)), C.byref(params), _dptr(albedo),
-                                               _dptr(emission), _dptr(path_radiance), _dptr(d_image),
+                                               _dptr(emission), _dptr(path_records), _dptr(d_image),
                                                _dptr(grad_albedo), _dptr(grad_emission), _stream_ptr(stream)),
                "rtx_render_adjoint_replay")
     return grad_albedo, grad_emission
@@ -564,8 +566,9 @@ def _autograd_fn():
     import torch
 
     class RenderFunction(torch.autograd.Function):
-        """image = render(albedo, emission); the forward keeps each path's radiance (12 bytes
-        per path, whatever max_bounce) and backward re-traces the paths (path replay)."""
+        """image = render(albedo, emission); the forward keeps each path's record (radiance
+        and camera-ray hit, 32 bytes per path, whatever max_bounce) and backward re-traces the
+        paths from their first hits (path replay)."""
 
         @staticmethod
         def forward(ctx, albedo, emission, bvh, cam, params):
@@ -578,7 +581,7 @@ def _autograd_fn():
         def backward(ctx, grad_img):
             albedo, emission, paths = ctx.saved_tensors
             bvh, cam, params = ctx.scene
-            ga, ge = render_adjoint(bvh, cam, params, albedo, emission, grad_img.contiguous(), path_radiance=paths)
+            ga, ge = render_adjoint(bvh, cam, params, albedo, emission, grad_img.contiguous(), path_records=paths)
             return ga, ge, None, None, None
 
     return RenderFunction
@@ -605,7 +608,7 @@ def _texture_autograd_fn():
             tex_c, paths = ctx.saved_tensors
             uv, emission, bvh, cam, params = ctx.scene
             set_texture(params, uv, tex_c)
-            gtex, _ = render_adjoint(bvh, cam, params, None, emission, grad_img.contiguous(), path_radiance=paths)
+            gtex, _ = render_adjoint(bvh, cam, params, None, emission, grad_img.contiguous(), path_records=paths)
             return gtex, None, None, None, None, None
 
     return TextureRenderFunction

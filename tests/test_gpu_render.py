@@ -312,9 +312,35 @@ def test_random_walk_closed_forms_on_gpu():
     assert not bad, bad
 
 
-def test_primal_render_and_path_radiance():
-    """rtx_render_primal gives rtx_render's image bit for bit, and each pixel is the mean of
-    its stored path radiances summed in sample order (k_accum's order)."""
+def _camera_rays(cam, W, H, spp, seed):
+    """The renderer's camera rays (primaryRay, main.cpp:69-83, with the sample table's jitter)
+    in float32 and the renderer's operation order, path sample * H * W + pixel."""
+    f = np.float32
+    n = W * H
+    rays = np.zeros((spp * n, 8), f)
+    fwd, rt, up = (np.asarray(cam[k], f) for k in ("fwd", "right", "up"))
+    th = f(cam["tan_half_fov_y"])
+    aspect = f(W) / f(H)
+    for s in range(spp):
+        for pix in range(n):
+            x, y = pix % W, pix // W
+            jx, jy = f(O.sample(seed, pix, s, 0)), f(O.sample(seed, pix, s, 1))
+            u = ((f(x) + jx) / f(W) * f(2) - f(1)) * aspect
+            v = (f(y) + jy) / f(H) * f(2) - f(1)
+            d = (fwd + (u * th) * rt) + (v * th) * up
+            inv = f(1) / np.sqrt((d[0] * d[0] + d[1] * d[1]) + (d[2] * d[2] + f(0)))
+            rays[s * n + pix, 0:3] = cam["pos"]
+            rays[s * n + pix, 3] = f(1e-3)
+            rays[s * n + pix, 4:7] = inv * d
+            rays[s * n + pix, 7] = np.inf
+    return rays
+
+
+def test_primal_render_and_path_records():
+    """rtx_render_primal gives rtx_render's image bit for bit; each pixel is the mean of its
+    stored path radiances summed in sample order (k_accum's order); and each record's hit is
+    the oracle's closest hit of that path's camera ray, bit for bit (the replay uses it in
+    place of tracing the camera rays again)."""
     v, t, albedo, emission, cam = S.diff_scene(seed=3)
     g = rtx.build(v, t)
     W, H, spp = 32, 24, 5
@@ -322,13 +348,23 @@ def test_primal_render_and_path_radiance():
     a_t, e_t = _tensor(albedo), _tensor(emission)
     ref = rtx.render(g, cam, p, a_t, e_t).cpu().numpy()
     img, paths = rtx.render_primal(g, cam, p, a_t, e_t)
-    img, paths = img.cpu().numpy(), paths.cpu().numpy().reshape(spp, H * W, 3)
+    img, rec = img.cpu().numpy(), paths.cpu().numpy()
+    assert rec.shape == (spp * H * W, 8)
+    rad = rec[:, 0:3].reshape(spp, H * W, 3)
     assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
     acc = np.zeros((H * W, 3), np.float32)
     for s in range(spp):
-        acc = (acc + paths[s]).astype(np.float32)
+        acc = (acc + rad[s]).astype(np.float32)
     assert np.array_equal((acc / np.float32(spp)).astype(np.float32).reshape(H, W, 3).view(np.uint32),
                           ref.view(np.uint32))
+    assert not rec[:, 3].any()
+    hits = O.trace(O.build(v, t), _camera_rays(cam, W, H, spp, seed=2))
+    tri = rec[:, 7].view(np.uint32)
+    assert np.array_equal(tri, hits["tri"])
+    hit = tri != 0xFFFFFFFF
+    assert hit.mean() > 0.9
+    for k, c in (("t", 4), ("u", 5), ("v", 6)):
+        assert np.array_equal(rec[hit, c].view(np.uint32), hits[k][hit].view(np.uint32)), k
 
 
 def test_adjoint_replay_equals_self_contained_and_memory_is_flat():
@@ -345,7 +381,7 @@ def test_adjoint_replay_equals_self_contained_and_memory_is_flat():
         p = rtx.make_params(W, H, spp, mb, 99, seed=5)
         img, paths = rtx.render_primal(g, cam, p, a_t, e_t)
         d = (2 * (img - 0.3)).contiguous()
-        ga1, ge1 = rtx.render_adjoint(g, cam, p, a_t, e_t, d, path_radiance=paths)
+        ga1, ge1 = rtx.render_adjoint(g, cam, p, a_t, e_t, d, path_records=paths)
         torch.cuda.synchronize()
         rtx.debug_scratch_peak(reset=True)
         ga2, ge2 = rtx.render_adjoint(g, cam, p, a_t, e_t, d)
@@ -377,7 +413,7 @@ def test_adjoint_over_many_batches_equals_one_batch():
             rtx.set_tuning(batch_paths=bp)
             img, paths = rtx.render_primal(g, cam, p, a_t, e_t)
             d = (2 * (img - 0.25)).contiguous()
-            ga, ge = rtx.render_adjoint(g, cam, p, a_t, e_t, d, path_radiance=paths)
+            ga, ge = rtx.render_adjoint(g, cam, p, a_t, e_t, d, path_records=paths)
             ga2, ge2 = rtx.render_adjoint(g, cam, p, a_t, e_t, d)
             torch.cuda.synchronize()
             out[bp] = [x.cpu().numpy() for x in (img, paths, ga, ge, ga2, ge2)]
@@ -403,7 +439,7 @@ def test_adjoint_over_many_batches_equals_one_batch():
             rtx.set_tuning(batch_paths=bp)
             img, paths = rtx.render_primal(tg, tcam, tp)
             d = (2 * (img - 0.25)).contiguous()
-            gt, _ = rtx.render_adjoint(tg, tcam, tp, None, None, d, path_radiance=paths)
+            gt, _ = rtx.render_adjoint(tg, tcam, tp, None, None, d, path_records=paths)
             torch.cuda.synchronize()
             grads[bp] = gt.cpu().numpy()
     finally:
