@@ -402,6 +402,55 @@ __device__ __forceinline__ void merged_add(float* __restrict__ buf, bool has, ui
     }
 }
 
+// Sum over the 64 lanes of a wave (every lane active): DPP row shifts and row broadcasts
+// into lane 63 (the order of wave_incl_scan), read back as a uniform value.
+__device__ __forceinline__ float wave_total(float x) {
+#define RTX_DPP_ADD(ctrl, rows) x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), ctrl, rows, 0xf, false))
+    RTX_DPP_ADD(0x111, 0xf);   // row_shr:1
+    RTX_DPP_ADD(0x112, 0xf);   // row_shr:2
+    RTX_DPP_ADD(0x114, 0xf);   // row_shr:4
+    RTX_DPP_ADD(0x118, 0xf);   // row_shr:8
+    RTX_DPP_ADD(0x142, 0xa);   // row_bcast:15 into rows 1, 3
+    RTX_DPP_ADD(0x143, 0xc);   // row_bcast:31 into rows 2, 3
+#undef RTX_DPP_ADD
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
+
+// The replay's gradient adds into the block's LDS copies (ga at lg, ge at lg + eoff; ua / ue:
+// the array is kept). Lanes of a wave often add to the same entry (neighbouring paths hit the
+// same wall), and LDS atomics on one address serialize, so per round the lanes sharing the
+// first pending lane's entry are summed by wave_total and added once; a round whose entry has
+// a single lane is that lane's own add. After kLdsMergeRounds rounds the lanes left add
+// their own contributions. Every lane of the wave must be active.
+constexpr int kLdsMergeRounds = 4;
+__device__ __forceinline__ void lds_grad_add(float* lg, uint32_t eoff, bool ua, bool ue, bool has, uint32_t idx,
+                                             f3 gA, f3 gE) {
+    auto add = [&](uint32_t e, f3 a, f3 g) {
+        if (ua) { atomicAdd(&lg[3 * e], a.x); atomicAdd(&lg[3 * e + 1], a.y); atomicAdd(&lg[3 * e + 2], a.z); }
+        if (ue) {
+            atomicAdd(&lg[eoff + 3 * e], g.x); atomicAdd(&lg[eoff + 3 * e + 1], g.y); atomicAdd(&lg[eoff + 3 * e + 2], g.z);
+        }
+    };
+    uint64_t pending = __ballot(has);
+    const uint64_t me = 1ull << __lane_id();
+    for (int round = 0; pending && round < kLdsMergeRounds; ++round) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;   // uniform
+        const uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)idx, leader);
+        const bool in = (pending & me) && idx == p;
+        const uint64_t m = __ballot(in);
+        pending &= ~m;
+        if (__popcll(m) == 1) {
+            if (in) add(p, gA, gE);
+            continue;
+        }
+        f3 sa{0.f, 0.f, 0.f}, se{0.f, 0.f, 0.f};
+        if (ua) sa = f3{wave_total(in ? gA.x : 0.f), wave_total(in ? gA.y : 0.f), wave_total(in ? gA.z : 0.f)};
+        if (ue) se = f3{wave_total(in ? gE.x : 0.f), wave_total(in ? gE.y : 0.f), wave_total(in ? gE.z : 0.f)};
+        if (__lane_id() == leader) add(p, sa, se);
+    }
+    if (pending & me) add(idx, gA, gE);
+}
+
 // Dynamic LDS the replay may take for its gradient arrays (a gfx950 workgroup may allocate
 // 160 KiB; k_shade's own LDS takes under 256 bytes), and its launch shape when they fit: 256
 // blocks of 1024 threads (C4 forward+adjoint, with a separate gradient kernel: 128 blocks of
@@ -517,16 +566,7 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
             P.L[i] = L;
         }
         if (GM == kGradLds) {
-            if (has && G.ga) {
-                atomicAdd(&lg[3 * gidx], gA.x);
-                atomicAdd(&lg[3 * gidx + 1], gA.y);
-                atomicAdd(&lg[3 * gidx + 2], gA.z);
-            }
-            if (has && G.ge) {
-                atomicAdd(&lg[eoff + 3 * gidx], gE.x);
-                atomicAdd(&lg[eoff + 3 * gidx + 1], gE.y);
-                atomicAdd(&lg[eoff + 3 * gidx + 2], gE.z);
-            }
+            lds_grad_add(lg, eoff, G.ga != nullptr, G.ge != nullptr, has, gidx, gA, gE);
         } else if (GM == kGradGlobal) {
             if (G.ga) merged_add(G.ga, has, gidx, gA);
             if (G.ge) merged_add(G.ge, has, gidx, gE);
