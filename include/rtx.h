@@ -194,11 +194,12 @@ int rtx_render(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_param
                const float* albedo_d, const float* emission_d, float* image_d, void* stream);
 
 /* rtx_render that also stores each path's record (the primal of the path-replay adjoint):
- * path_records_d (16-byte aligned) holds 8 floats per path, width*height*spp*8 floats; path
- * g = sample*width*height + pixel has its radiance L at 8g..8g+2 (8g+3 is 0) and the hit of
- * its camera ray at 8g+4..8g+7 (t, u, v, triangle index bits; 0xFFFFFFFF on a miss).  32
- * bytes per path whatever max_bounce is; with the camera rays' hits stored the replay does
- * not trace them again.  The image equals rtx_render's.  No reference counterpart. */
+ * path_records_d (16-byte aligned) holds 8 floats per path in two halves, N = width*height*
+ * spp paths: path g = sample*width*height + pixel has its radiance L at 4g..4g+2 (4g+3 is 0)
+ * and the hit of its camera ray at 4N+4g..4N+4g+3 (t, u, v, triangle index bits; 0xFFFFFFFF
+ * on a miss).  32 bytes per path whatever max_bounce is; with the camera rays' hits stored
+ * the replay does not trace them again.  The image equals rtx_render's.  No reference
+ * counterpart. */
 int rtx_render_primal(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_params* p,
                       const float* albedo_d, const float* emission_d, float* image_d,
                       float* path_records_d, void* stream);
@@ -271,7 +272,8 @@ void rtx_batch_free(rtx_batch* batch);
  *                      between triangles of different leaves possibly another of the tied
  *                      triangles.
  * Renderer (rtx_render, rtx_render_adjoint, rtx_batch_*):
- *   batch_paths        paths in flight per wavefront batch (>= 1024);
+ *   batch_paths        paths in flight per wavefront batch (>= 1024; default 2^26, 100
+ *                      bytes of path state per path, 32 more for the adjoint's records);
  *   grid_blocks        blocks of the wavefront trace and shade kernels;
  *   shade_threads      threads per shade block (multiple of 64, <= 1024);
  *   render_lds_depth   LDS stack entries per lane: 8, 12 or 16;

@@ -1444,9 +1444,11 @@ struct PinnedCounters {
 // unmaps, and freeing them cost more than the build's kernels. Blocks go back to the cache
 // once their build's stream is idle, and the next build takes the smallest cached block of
 // at least the size it needs (at most twice that). At most kMaxCached bytes stay cached;
-// beyond that blocks are freed.
+// beyond that blocks are freed. 16 GiB holds a build of 40M triangles or the path state of
+// a 2^26-path render batch (6.7 GB) with its adjoint records, a few percent of the 288 GB
+// of an MI355X.
 struct ScratchPool {
-    static constexpr size_t kMaxCached = size_t(4) << 30;
+    static constexpr size_t kMaxCached = size_t(16) << 30;
     struct Block {
         int dev;
         size_t bytes;

@@ -64,16 +64,17 @@ constexpr int kShadeForward = 0, kShadeReplay = 1;
 // flushed once per block; or global atomics after merging equal entries within the wave.
 constexpr int kGradNone = 0, kGradLds = 1, kGradGlobal = 2;
 
-// Inputs and outputs of the replay's gradient scatter (scene 0 only). rec[2 * (lbase + i)]
-// holds path i's primal radiance (its path record, see PathRecord); dimg = dLoss/dimage (3 floats per pixel); ga, ge: gradient
+// Inputs and outputs of the replay's gradient scatter (scene 0 only). recL[lbase + i] holds
+// path i's primal radiance (its path record, see k_store_L); dimg = dLoss/dimage (3 floats per pixel); ga, ge: gradient
 // arrays (3 floats per albedo entry, per original triangle), either may be null.
 struct GradArgs {
-    const float4* rec;
+    const float4* recL;
     uint64_t lbase;
     const float* dimg;
     float* ga;
     float* ge;
     uint32_t nprim;
+    float4* hit_out;   // forward shading of bounce 0: path i's camera-ray hit to hit_out[lbase + i]
 };
 
 
@@ -176,27 +177,23 @@ __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* _
     }
 }
 
-// Path records of the primal render, the replay's only input per path: two float4 per path,
-// rec[2 * g] = (radiance L, 0) and rec[2 * g + 1] = the path's first hit (t, u, v, triangle
-// bits), g the path's global number. 32 bytes per path, whatever max_bounce is; with the first
-// hit stored the replay skips the trace of the camera rays.
-// Each path's radiance L into rec[2 * (base + i)].
-__global__ void k_store_L(PathState P, uint32_t cnt, float4* __restrict__ rec, uint64_t base) {
+// Path records of the primal render, the replay's only input per path, in two halves of a
+// render's N paths: recL[g] = (radiance L, 0) and recH[g] = the hit of the path's camera ray
+// (t, u, v, triangle bits), g the path's global number, recH = recL + N. 32 bytes per path,
+// whatever max_bounce is; with the camera rays' hits stored the replay does not trace them
+// again. recH is written by the forward shading of bounce 0 (GradArgs::hit_out).
+// Each path's radiance L into recL[base + i].
+__global__ void k_store_L(PathState P, uint32_t cnt, float4* __restrict__ recL, uint64_t base) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         const float4 L = P.L[i];
-        rec[2 * (base + i)] = make_float4(L.x, L.y, L.z, 0.f);
+        recL[base + i] = make_float4(L.x, L.y, L.z, 0.f);
     }
 }
-// After the camera rays' trace (queue 0 is the identity): path i's hit into rec[2 * (base + i) + 1].
-__global__ void k_store_hit(PathState P, uint32_t cnt, float4* __restrict__ rec, uint64_t base) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
-        rec[2 * (base + i) + 1] = P.hit[i];
-}
 // The replay's camera rays take their hits from the records instead of a trace: the same
-// rays (the same sample table) give the same hits.
-__global__ void k_load_hit(PathState P, uint32_t cnt, const float4* __restrict__ rec, uint64_t base) {
+// rays (the same sample table) give the same hits. Queue 0 is the identity.
+__global__ void k_load_hit(PathState P, uint32_t cnt, const float4* __restrict__ recH, uint64_t base) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
-        P.hit[i] = rec[2 * (base + i) + 1];
+        P.hit[i] = recH[base + i];
 }
 
 // The wavefront trace loads leaf triangles two at a time and is held to 64 VGPRs so that
@@ -467,7 +464,7 @@ constexpr uint32_t kGradLdsBlocks = 256;
 // again (the same sample table gives the same paths, bit for bit), and at the surface hit k
 // of path i this bounce adds, weighted by dLoss/dI of the path's pixel / spp,
 //   dL/da_k = (L - L_upto_k) / a_k   to ga[albedo entry],   dL/dLe_k = tp_k   to ge[triangle],
-// with L the path's primal radiance (from G.rec), L_upto_k the radiance gathered up to and
+// with L the path's primal radiance (from G.recL), L_upto_k the radiance gathered up to and
 // including hit k's own emission and tp_k the throughput arriving at k: with
 // L = sum_m (prod_{j<m} a_j) X_m everything past hit k is proportional to a_k. Nothing is
 // kept per path between bounces beyond the forward path state, so memory does not grow with
@@ -502,6 +499,7 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
             i = Q[k];
             const SceneDesc& S = scenes[UNI ? 0u : P.sid[i]];
             const float4 h = P.hit[i];
+            if (!ADJ && G.hit_out) G.hit_out[G.lbase + i] = h;   // bounce 0: queue 0 is the identity
             const uint32_t tri = __float_as_uint(h.w);
             float4 tp = P.tp[i], L = P.L[i];
             const float4 r1 = P.ray1[i];
@@ -532,7 +530,7 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
                     const f3 delta{ld_g(G.dimg + 3 * px) / fs, ld_g(G.dimg + 3 * px + 1) / fs,
                                    ld_g(G.dimg + 3 * px + 2) / fs};
                     if (G.ga) {
-                        const float4 lf = ld_g(G.rec + 2 * (G.lbase + i));
+                        const float4 lf = ld_g(G.recL + G.lbase + i);
                         gA = f3{ax != 0.f ? delta.x * (lf.x - L.x) / ax : 0.f,
                                 ay != 0.f ? delta.y * (lf.y - L.y) / ay : 0.f,
                                 az != 0.f ? delta.z * (lf.z - L.z) / az : 0.f};
@@ -712,13 +710,14 @@ struct Batch {
     }
     // adjoint (scene 0 only): dLoss/dimage in, gradients out. adj 0: a forward render (which
     // also stores each path's record in rec when that is set); 1: a primal pass and the
-    // replay, per batch; 2: the replay alone, with the path records read from rec.
+    // replay, per batch; 2: the replay alone, with the path records read from rec. A record
+    // array holds the radiance half (one float4 per path) and then the hit half (k_store_L).
     int adj = 0;
     const float* dimg = nullptr;
     float* ga = nullptr;
     float* ge = nullptr;
-    float4* rec = nullptr;     // path records (k_store_L), indexed by the path's global number
-    float4* recb = nullptr;    // adj 1: the batch's path records
+    float4* rec = nullptr;     // the render's path records (2 * paths float4), by global path number
+    float4* recb = nullptr;    // adj 1: the batch's path records (2 * B float4)
     int setup(uint32_t batch_paths, bool adjoint = false);
     void enqueue_range(int s_first, int s_last, hipStream_t st);   // scenes [s_first, s_last)
     int launch(hipStream_t st);
@@ -784,10 +783,10 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
         const uint32_t cnt = (uint32_t)std::min<uint64_t>(B, off[s_last] - g0);
         const uint32_t cgrid = std::min<uint32_t>((cnt + 255) / 256, 65535);
         // One wavefront over the batch: camera rays, then max_bounce + 1 trace/shade rounds.
-        // The replay (kShadeReplay) scatters each bounce's gradients as it shades it and takes
-        // its camera rays' hits from the batch's path records, recs + 2 * rbase; the forward
-        // wavefront stores its camera rays' hits there when recs is set.
-        auto wavefront = [&](int mode, float4* recs, uint64_t rbase) {
+        // The replay (kShadeReplay) scatters each bounce's gradients as it shades it, reads the
+        // paths' radiance from recL + rbase and takes its camera rays' hits from recH + rbase;
+        // the forward wavefront writes its camera rays' hits to recH + rbase when recH is set.
+        auto wavefront = [&](int mode, const float4* recL, float4* recH, uint64_t rbase) {
             k_set_counts<<<1, 64, 0, st>>>(P.cnt, ncnt, cnt);
             k_camera<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
             // bounce b traces queue q[qi] of cnt[b] paths; shade appends the survivors to
@@ -804,7 +803,7 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
                 const bool persist = g_tuning.render_loop == 2 &&
                                      (uint64_t)cnt >= (uint64_t)pgrid * (kTraceThreads / 64) * kRenderChunk;
                 if (mode == kShadeReplay && b == 0) {
-                    k_load_hit<<<cgrid, 256, 0, st>>>(P, cnt, recs, rbase);
+                    k_load_hit<<<cgrid, 256, 0, st>>>(P, cnt, recH, rbase);
                 } else if (persist) {
                     uint32_t* work = P.cnt + ncnt / 2 + b;
                     with_trace_variant(g_tuning.render_lds_depth, loop, [&](auto D, auto L) {
@@ -833,11 +832,10 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
                                 <<<grid, kTraceThreads, 0, st>>>(d_scenes, (uint32_t)ns, P, qi, b, wait);
                     });
                 }
-                if (mode == kShadeForward && b == 0 && recs) k_store_hit<<<cgrid, 256, 0, st>>>(P, cnt, recs, rbase);
                 if (mode == kShadeReplay) {
                     // the gradient scatter rides on the replay's shading: both arrays in LDS
                     // when they fit, global atomics otherwise
-                    const GradArgs G{recs, rbase, dimg, ga, ge, descs[0].nprim};
+                    const GradArgs G{recL, rbase, dimg, ga, ge, descs[0].nprim, nullptr};
                     const size_t lds = ((ga ? 1u : 0u) + (ge ? 1u : 0u)) * 3ull * descs[0].nprim * sizeof(float);
                     if (lds <= kLdsGradBytes) {
                         const uint32_t gb = std::min<uint32_t>(kGradLdsBlocks, (cnt + kGradThreads - 1) / kGradThreads);
@@ -848,23 +846,24 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
                         else k_shade<kShadeReplay, false, kGradGlobal><<<grid, sh, 0, st>>>(sd, P, qi, b, G);
                     }
                 } else {
-                    const GradArgs G{nullptr, 0, nullptr, nullptr, nullptr, 0};
+                    const GradArgs G{nullptr, rbase, nullptr, nullptr, nullptr, 0, b == 0 ? recH : nullptr};
                     if (one) k_shade<kShadeForward, true><<<grid, sh, 0, st>>>(sd, P, qi, b, G);
                     else k_shade<kShadeForward, false><<<grid, sh, 0, st>>>(sd, P, qi, b, G);
                 }
                 qi ^= 1;
             }
         };
+        float4* recH = rec ? rec + off[s_last] : nullptr;   // one scene: off[s_last] = its paths
         if (adj == 0) {
-            wavefront(kShadeForward, rec, g0);
+            wavefront(kShadeForward, nullptr, recH, g0);
             if (rec) k_store_L<<<cgrid, 256, 0, st>>>(P, cnt, rec, g0);
             k_accum<<<cgrid, 256, 0, st>>>(d_scenes, d_off, ns, g0, cnt, P);
         } else if (adj == 1) {
-            wavefront(kShadeForward, recb, 0);
+            wavefront(kShadeForward, nullptr, recb + B, 0);
             k_store_L<<<cgrid, 256, 0, st>>>(P, cnt, recb, 0);
-            wavefront(kShadeReplay, recb, 0);
+            wavefront(kShadeReplay, recb, recb + B, 0);
         } else {
-            wavefront(kShadeReplay, rec, g0);
+            wavefront(kShadeReplay, rec, recH, g0);
         }
     }
     if (!adj)

@@ -485,15 +485,15 @@ def render(bvh, cam, params, albedo=None, emission=None, out=None, stream=None):
 
 
 def render_primal(bvh, cam, params, albedo=None, emission=None, out=None, stream=None):
-    """render() that also returns each path's record ((spp * H * W, 8) float32 CUDA tensor,
-    row sample * H * W + pixel: the path's radiance in columns 0-2, 0, then its camera ray's
-    hit t, u, v and triangle index bits): the primal that render_adjoint consumes."""
+    """render() that also returns each path's record ((2, spp * H * W, 4) float32 CUDA
+    tensor, path sample * H * W + pixel: [0, g] = its radiance and 0, [1, g] = its camera
+    ray's hit t, u, v and triangle index bits): the primal that render_adjoint consumes."""
     import torch
     _check_scene(bvh, params, albedo, emission)
     if out is None:
         out = torch.empty((params.height, params.width, 3), dtype=torch.float32, device="cuda")
     _need(out, "out", "f32", 3 * params.width * params.height)
-    paths = torch.empty((params.spp * params.height * params.width, 8), dtype=torch.float32, device="cuda")
+    paths = torch.empty((2, params.spp * params.height * params.width, 4), dtype=torch.float32, device="cuda")
     _check(lib().rtx_render_primal(bvh.handle, C.byref(make_camera(cam)), C.byref(params), _dptr(albedo),
                                    _dptr(emission), _dptr(out), _dptr(paths), _stream_ptr(stream)), "rtx_render_primal")
     return out, paths

@@ -31,7 +31,7 @@ SPACE = {
     "render_leaf_coop": [0, 1],
     "grid_blocks": [1024, 2048, 4096, 8192],
     "shade_threads": [64, 128, 256, 512, 1024],
-    "batch_paths": [1 << 20, 1 << 21, 1 << 22, 1 << 23],
+    "batch_paths": [1 << 20, 1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26],
 }
 # the refill threshold before stack depth and leaf-phase wait: their best values depend on it
 TRACE_KEYS = ["trace_tree", "trace_loop", "leaf_coop", "refill_lanes", "trace_lds_depth", "ray_sort", "wait_lanes",

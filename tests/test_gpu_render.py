@@ -349,22 +349,22 @@ def test_primal_render_and_path_records():
     ref = rtx.render(g, cam, p, a_t, e_t).cpu().numpy()
     img, paths = rtx.render_primal(g, cam, p, a_t, e_t)
     img, rec = img.cpu().numpy(), paths.cpu().numpy()
-    assert rec.shape == (spp * H * W, 8)
-    rad = rec[:, 0:3].reshape(spp, H * W, 3)
+    assert rec.shape == (2, spp * H * W, 4)
+    rad = rec[0, :, 0:3].reshape(spp, H * W, 3)
     assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
     acc = np.zeros((H * W, 3), np.float32)
     for s in range(spp):
         acc = (acc + rad[s]).astype(np.float32)
     assert np.array_equal((acc / np.float32(spp)).astype(np.float32).reshape(H, W, 3).view(np.uint32),
                           ref.view(np.uint32))
-    assert not rec[:, 3].any()
+    assert not rec[0, :, 3].any()
     hits = O.trace(O.build(v, t), _camera_rays(cam, W, H, spp, seed=2))
-    tri = rec[:, 7].view(np.uint32)
+    tri = rec[1, :, 3].view(np.uint32)
     assert np.array_equal(tri, hits["tri"])
     hit = tri != 0xFFFFFFFF
     assert hit.mean() > 0.9
-    for k, c in (("t", 4), ("u", 5), ("v", 6)):
-        assert np.array_equal(rec[hit, c].view(np.uint32), hits[k][hit].view(np.uint32)), k
+    for k, c in (("t", 0), ("u", 1), ("v", 2)):
+        assert np.array_equal(rec[1, hit, c].view(np.uint32), hits[k][hit].view(np.uint32)), k
 
 
 def test_adjoint_replay_equals_self_contained_and_memory_is_flat():
