@@ -195,11 +195,13 @@ int rtx_render(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_param
 
 /* rtx_render that also stores each path's record (the primal of the path-replay adjoint):
  * path_records_d (16-byte aligned) holds 8 floats per path in two halves, N = width*height*
- * spp paths: path g = sample*width*height + pixel has its radiance L at 4g..4g+2 (4g+3 is 0)
- * and the hit of its camera ray at 4N+4g..4N+4g+3 (t, u, v, triangle index bits; 0xFFFFFFFF
- * on a miss).  32 bytes per path whatever max_bounce is; with the camera rays' hits stored
- * the replay does not trace them again.  The image equals rtx_render's.  No reference
- * counterpart. */
+ * spp paths: path g = sample*width*height + k has its radiance L at 4g..4g+2 (4g+3 is 0) and
+ * the hit of its camera ray at 4N+4g..4N+4g+3 (t, u, v, triangle index bits; 0xFFFFFFFF on a
+ * miss).  Slot k of a sample is a pixel in 8 x 8 tile order: bands of 8 rows from y = 0,
+ * tiles of 8 columns from x = 0 along a band, row-major inside a tile (the last band and a
+ * band's last tile may be narrower).  32 bytes per path whatever max_bounce is; with the
+ * camera rays' hits stored the replay does not trace them again.  The image equals
+ * rtx_render's.  No reference counterpart. */
 int rtx_render_primal(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_params* p,
                       const float* albedo_d, const float* emission_d, float* image_d,
                       float* path_records_d, void* stream);

@@ -4,7 +4,8 @@
 // Every random number comes from the counter-based sample table (seed, pixel, sample,
 // dim): dims 0/1 jitter, then per bounce b: 2+3b, 3+3b (direction), 4+3b (roulette).
 //
-// Paths of all scenes are numbered globally: scene-major, then sample, then pixel. They
+// Paths of all scenes are numbered globally: scene-major, then sample, then pixel in 8 x 8
+// tile order (tile_pixel). They
 // are processed in batches of `batch_paths`; per batch:
 //   k_camera -> { k_wtrace -> k_shade } x (max_bounce + 1) -> k_accum
 // k_shade appends surviving paths to the next queue (an LDS scan per block, one atomic per
@@ -64,17 +65,17 @@ constexpr int kShadeForward = 0, kShadeReplay = 1;
 // flushed once per block; or global atomics after merging equal entries within the wave.
 constexpr int kGradNone = 0, kGradLds = 1, kGradGlobal = 2;
 
-// Inputs and outputs of the replay's gradient scatter (scene 0 only). recL[lbase + i] holds
+// Inputs and outputs of the replay's gradient scatter (scene 0 only). recL[rbase + i] holds
 // path i's primal radiance (its path record, see k_store_L); dimg = dLoss/dimage (3 floats per pixel); ga, ge: gradient
 // arrays (3 floats per albedo entry, per original triangle), either may be null.
 struct GradArgs {
     const float4* recL;
-    uint64_t lbase;
     const float* dimg;
     float* ga;
     float* ge;
     uint32_t nprim;
-    float4* hit_out;   // forward shading of bounce 0: path i's camera-ray hit to hit_out[lbase + i]
+    float4* hit_out;   // forward shading of bounce 0: path i's camera-ray hit to hit_out[rbase + i]
+    uint64_t rbase;    // record of the batch's first path
 };
 
 
@@ -155,6 +156,27 @@ __global__ void k_set_counts(uint32_t* p, uint32_t n, uint32_t a) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = i == 0 ? a : 0u;
 }
 
+// Pixel of slot q of a sample's W x H slots: 8 x 8 tiles, tile by tile along each band of 8
+// rows, row-major inside a tile (the last band and each band's last tile may be narrower).
+// Consecutive paths, and so the 64 of a wave, then cover a compact patch of the image, and
+// their bounce rays start from neighbouring surface points. A bijection of [0, W H), so each
+// pixel's samples keep their order; results do not depend on it.
+__device__ __forceinline__ uint32_t tile_pixel(uint32_t q, uint32_t W, uint32_t H) {
+    const uint32_t band = q / (8u * W), y0 = 8u * band, hb = min(8u, H - y0);
+    const uint32_t r = q - band * 8u * W, full = W / 8u;
+    uint32_t row, col;
+    if (r < full * 8u * hb) {
+        const uint32_t t = r / (8u * hb), k = r - t * 8u * hb;
+        row = k / 8u;
+        col = 8u * t + (k & 7u);
+    } else {
+        const uint32_t wl = W - 8u * full, k = r - full * 8u * hb;
+        row = k / wl;
+        col = 8u * full + k % wl;
+    }
+    return (y0 + row) * W + col;
+}
+
 __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* __restrict__ off, int ns, uint64_t g0,
                          uint32_t cnt, PathState P) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
@@ -162,7 +184,8 @@ __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* _
         const uint32_t s = find_scene(off, ns, g);
         const SceneDesc& S = scenes[s];
         const uint64_t q = g - S.path_begin;
-        const uint32_t sample = (uint32_t)(q / S.npix), pixel = (uint32_t)(q % S.npix);
+        const uint32_t sample = (uint32_t)(q / S.npix),
+                       pixel = tile_pixel((uint32_t)(q % S.npix), (uint32_t)S.W, (uint32_t)S.H);
         const int x = (int)(pixel % (uint32_t)S.W), y = (int)(pixel / (uint32_t)S.W);
         f3 o, d;
         primary_ray(S, x, y, sample01(S.seed, pixel, sample, 0), sample01(S.seed, pixel, sample, 1), o, d);
@@ -177,11 +200,12 @@ __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* _
     }
 }
 
-// Path records of the primal render, the replay's only input per path, in two halves of a
-// render's N paths: recL[g] = (radiance L, 0) and recH[g] = the hit of the path's camera ray
-// (t, u, v, triangle bits), g the path's global number, recH = recL + N. 32 bytes per path,
-// whatever max_bounce is; with the camera rays' hits stored the replay does not trace them
-// again. recH is written by the forward shading of bounce 0 (GradArgs::hit_out).
+// Path records of the primal render (one scene), the replay's only input per path, in two
+// halves of the render's N paths: recL[g] = (radiance L, 0) and recH[g] = the hit of the
+// path's camera ray (t, u, v, triangle bits), g the path's number (sample * npix + slot, the
+// slot's pixel by tile_pixel), recH = recL + N. 32 bytes per path, whatever max_bounce is;
+// with the camera rays' hits stored the replay does not trace them again. recH is written by
+// the forward shading of bounce 0 (GradArgs::hit_out).
 // Each path's radiance L into recL[base + i].
 __global__ void k_store_L(PathState P, uint32_t cnt, float4* __restrict__ recL, uint64_t base) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
@@ -499,7 +523,7 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
             i = Q[k];
             const SceneDesc& S = scenes[UNI ? 0u : P.sid[i]];
             const float4 h = P.hit[i];
-            if (!ADJ && G.hit_out) G.hit_out[G.lbase + i] = h;   // bounce 0: queue 0 is the identity
+            if (!ADJ && G.hit_out) G.hit_out[G.rbase + i] = h;   // bounce 0: queue 0 is the identity
             const uint32_t tri = __float_as_uint(h.w);
             float4 tp = P.tp[i], L = P.L[i];
             const float4 r1 = P.ray1[i];
@@ -530,7 +554,7 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
                     const f3 delta{ld_g(G.dimg + 3 * px) / fs, ld_g(G.dimg + 3 * px + 1) / fs,
                                    ld_g(G.dimg + 3 * px + 2) / fs};
                     if (G.ga) {
-                        const float4 lf = ld_g(G.recL + G.lbase + i);
+                        const float4 lf = ld_g(G.recL + G.rbase + i);
                         gA = f3{ax != 0.f ? delta.x * (lf.x - L.x) / ax : 0.f,
                                 ay != 0.f ? delta.y * (lf.y - L.y) / ay : 0.f,
                                 az != 0.f ? delta.z * (lf.z - L.z) / az : 0.f};
@@ -835,7 +859,7 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
                 if (mode == kShadeReplay) {
                     // the gradient scatter rides on the replay's shading: both arrays in LDS
                     // when they fit, global atomics otherwise
-                    const GradArgs G{recL, rbase, dimg, ga, ge, descs[0].nprim, nullptr};
+                    const GradArgs G{recL, dimg, ga, ge, descs[0].nprim, nullptr, rbase};
                     const size_t lds = ((ga ? 1u : 0u) + (ge ? 1u : 0u)) * 3ull * descs[0].nprim * sizeof(float);
                     if (lds <= kLdsGradBytes) {
                         const uint32_t gb = std::min<uint32_t>(kGradLdsBlocks, (cnt + kGradThreads - 1) / kGradThreads);
@@ -846,14 +870,15 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
                         else k_shade<kShadeReplay, false, kGradGlobal><<<grid, sh, 0, st>>>(sd, P, qi, b, G);
                     }
                 } else {
-                    const GradArgs G{nullptr, rbase, nullptr, nullptr, nullptr, 0, b == 0 ? recH : nullptr};
+                    const GradArgs G{nullptr, nullptr, nullptr, nullptr, 0, b == 0 ? recH : nullptr, rbase};
                     if (one) k_shade<kShadeForward, true><<<grid, sh, 0, st>>>(sd, P, qi, b, G);
                     else k_shade<kShadeForward, false><<<grid, sh, 0, st>>>(sd, P, qi, b, G);
                 }
                 qi ^= 1;
             }
         };
-        float4* recH = rec ? rec + off[s_last] : nullptr;   // one scene: off[s_last] = its paths
+        // the caller's records (one scene): off[s_last] = the scene's paths
+        float4* recH = rec ? rec + off[s_last] : nullptr;
         if (adj == 0) {
             wavefront(kShadeForward, nullptr, recH, g0);
             if (rec) k_store_L<<<cgrid, 256, 0, st>>>(P, cnt, rec, g0);

@@ -486,8 +486,9 @@ def render(bvh, cam, params, albedo=None, emission=None, out=None, stream=None):
 
 def render_primal(bvh, cam, params, albedo=None, emission=None, out=None, stream=None):
     """render() that also returns each path's record ((2, spp * H * W, 4) float32 CUDA
-    tensor, path sample * H * W + pixel: [0, g] = its radiance and 0, [1, g] = its camera
-    ray's hit t, u, v and triangle index bits): the primal that render_adjoint consumes."""
+    tensor, path g = sample * H * W + slot, the slot's pixel in 8 x 8 tile order (tile_pixel):
+    [0, g] = its radiance and 0, [1, g] = its camera ray's hit t, u, v and triangle index
+    bits): the primal that render_adjoint consumes."""
     import torch
     _check_scene(bvh, params, albedo, emission)
     if out is None:
@@ -497,6 +498,26 @@ def render_primal(bvh, cam, params, albedo=None, emission=None, out=None, stream
     _check(lib().rtx_render_primal(bvh.handle, C.byref(make_camera(cam)), C.byref(params), _dptr(albedo),
                                    _dptr(emission), _dptr(out), _dptr(paths), _stream_ptr(stream)), "rtx_render_primal")
     return out, paths
+
+
+def tile_pixel(k, width, height):
+    """Pixel (y * width + x) of path slot k of a sample: 8 x 8 tiles, tile by tile along each
+    band of 8 rows, row-major inside a tile (the renderer's path order; numpy arrays work)."""
+    import numpy as np
+    k = np.asarray(k, np.int64)
+    band = k // (8 * width)
+    y0 = 8 * band
+    hb = np.minimum(8, height - y0)
+    r = k - band * 8 * width
+    full = width // 8
+    in_full = r < full * 8 * hb
+    t = r // (8 * hb)
+    kk = r - t * 8 * hb
+    wl = max(width - 8 * full, 1)
+    k2 = r - full * 8 * hb
+    row = np.where(in_full, kk // 8, k2 // wl)
+    col = np.where(in_full, 8 * t + (kk & 7), 8 * full + k2 % wl)
+    return (y0 + row) * width + col
 
 
 def render_adjoint(bvh, cam, params, albedo, emission, d_image, grad_albedo=None, grad_emission=None, stream=None,

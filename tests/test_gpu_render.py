@@ -314,25 +314,28 @@ def test_random_walk_closed_forms_on_gpu():
 
 def _camera_rays(cam, W, H, spp, seed):
     """The renderer's camera rays (primaryRay, main.cpp:69-83, with the sample table's jitter)
-    in float32 and the renderer's operation order, path sample * H * W + pixel."""
+    in float32 and the renderer's operation order, path sample * H * W + slot (the slot's
+    pixel by rtx.tile_pixel)."""
     f = np.float32
     n = W * H
     rays = np.zeros((spp * n, 8), f)
     fwd, rt, up = (np.asarray(cam[k], f) for k in ("fwd", "right", "up"))
     th = f(cam["tan_half_fov_y"])
     aspect = f(W) / f(H)
+    slot_pixel = rtx.tile_pixel(np.arange(n), W, H)
     for s in range(spp):
-        for pix in range(n):
+        for slot in range(n):
+            pix = int(slot_pixel[slot])
             x, y = pix % W, pix // W
             jx, jy = f(O.sample(seed, pix, s, 0)), f(O.sample(seed, pix, s, 1))
             u = ((f(x) + jx) / f(W) * f(2) - f(1)) * aspect
             v = (f(y) + jy) / f(H) * f(2) - f(1)
             d = (fwd + (u * th) * rt) + (v * th) * up
             inv = f(1) / np.sqrt((d[0] * d[0] + d[1] * d[1]) + (d[2] * d[2] + f(0)))
-            rays[s * n + pix, 0:3] = cam["pos"]
-            rays[s * n + pix, 3] = f(1e-3)
-            rays[s * n + pix, 4:7] = inv * d
-            rays[s * n + pix, 7] = np.inf
+            rays[s * n + slot, 0:3] = cam["pos"]
+            rays[s * n + slot, 3] = f(1e-3)
+            rays[s * n + slot, 4:7] = inv * d
+            rays[s * n + slot, 7] = np.inf
     return rays
 
 
@@ -350,7 +353,8 @@ def test_primal_render_and_path_records():
     img, paths = rtx.render_primal(g, cam, p, a_t, e_t)
     img, rec = img.cpu().numpy(), paths.cpu().numpy()
     assert rec.shape == (2, spp * H * W, 4)
-    rad = rec[0, :, 0:3].reshape(spp, H * W, 3)
+    rad = np.empty((spp, H * W, 3), np.float32)
+    rad[:, rtx.tile_pixel(np.arange(H * W), W, H)] = rec[0, :, 0:3].reshape(spp, H * W, 3)   # by pixel
     assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
     acc = np.zeros((H * W, 3), np.float32)
     for s in range(spp):

@@ -266,3 +266,16 @@ def test_tuning_validation():
         assert rtx.get_tuning() == dict(saved, ray_sort=1, trace_lds_depth=12, trace_loop=0)
     finally:
         rtx.set_tuning(**saved)
+
+
+@pytest.mark.parametrize("wh", [(1, 1), (7, 3), (8, 8), (9, 17), (33, 5), (1920, 1080), (17, 1), (1, 17)])
+def test_tile_pixel_order_is_a_bijection(wh):
+    """The renderer's path order within a sample (8 x 8 tiles along bands of 8 rows) visits
+    every pixel once; a full tile is 64 consecutive slots covering an 8 x 8 block."""
+    W, H = wh
+    k = np.arange(W * H)
+    p = rtx.tile_pixel(k, W, H)
+    assert np.array_equal(np.sort(p), k)
+    if W >= 8 and H >= 8:
+        blk = p[:64]
+        assert sorted(set(blk % W)) == list(range(8)) and sorted(set(blk // W)) == list(range(8))
