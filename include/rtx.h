@@ -189,7 +189,10 @@ typedef struct rtx_render_params {
  * 339-365) with every random number drawn from the counter-based sample table.
  * albedo_d / emission_d: 3 floats per ORIGINAL triangle (NULL = reference constant 0.8 /
  * no emission).  image_d: width*height*3 floats, row y = 0 at the bottom like the
- * reference's pre-flip buffer. */
+ * reference's pre-flip buffer.  rtx_render, rtx_render_primal, rtx_render_adjoint and
+ * rtx_render_adjoint_replay queue their work on `stream` and return without waiting for it
+ * (the parameters are copied at the call); the caller synchronizes before reading results
+ * on the host. */
 int rtx_render(const rtx_bvh* bvh, const rtx_camera* cam, const rtx_render_params* p,
                const float* albedo_d, const float* emission_d, float* image_d, void* stream);
 

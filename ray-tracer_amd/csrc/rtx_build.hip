@@ -1439,23 +1439,27 @@ struct PinnedCounters {
     }
 };
 
-// Process-wide cache of build scratch blocks, per device. A build allocates about 380
+// Process-wide cache of device scratch blocks, per device. A build allocates about 380
 // bytes of scratch per triangle in some twenty blocks; hipFree synchronizes the device and
-// unmaps, and freeing them cost more than the build's kernels. Blocks go back to the cache
-// once their build's stream is idle, and the next build takes the smallest cached block of
-// at least the size it needs (at most twice that). At most kMaxCached bytes stay cached;
-// beyond that blocks are freed. 16 GiB holds a build of 40M triangles or the path state of
-// a 2^26-path render batch (6.7 GB) with its adjoint records, a few percent of the 288 GB
-// of an MI355X.
+// unmaps, and freeing them cost more than the build's kernels. A block goes back to the cache
+// either idle, or with an event recorded on the stream whose queued work may still use it
+// (a render returns its path state that way, without waiting for the GPU); whoever takes it
+// next makes its own stream wait for that event. The next user takes the smallest cached
+// block of at least the size it needs (at most twice that). At most kMaxCached bytes stay
+// cached; beyond that blocks are freed. 16 GiB holds a build of 40M triangles or the path
+// state of a 2^26-path render batch (6.7 GB) with its adjoint records, a few percent of the
+// 288 GB of an MI355X.
 struct ScratchPool {
     static constexpr size_t kMaxCached = size_t(16) << 30;
     struct Block {
         int dev;
         size_t bytes;
         void* p;
+        hipEvent_t ev;   // recorded after the last work that used the block, or null
     };
     std::mutex mu;
     std::vector<Block> blocks;
+    std::vector<std::pair<int, hipEvent_t>> spare;   // events to record again, per device
     size_t cached = 0;
     // bytes requested by the blocks handed out and not yet returned, and their peak since the
     // last reset (rtx_debug_scratch_peak)
@@ -1466,7 +1470,9 @@ struct ScratchPool {
         live_bytes += bytes;
         peak = std::max(peak, live_bytes);
     }
-    void* take(int dev, size_t bytes, size_t& got) {
+    void* take(int dev, size_t bytes, size_t& got, hipStream_t st) {
+        void* p = nullptr;
+        hipEvent_t ev = nullptr;
         {
             std::lock_guard<std::mutex> lk(mu);
             int bi = -1;
@@ -1477,22 +1483,33 @@ struct ScratchPool {
                     bi = i;
             }
             if (bi >= 0) {
-                void* p = blocks[bi].p;
+                p = blocks[bi].p;
+                ev = blocks[bi].ev;
                 got = blocks[bi].bytes;
                 cached -= got;
                 blocks.erase(blocks.begin() + bi);
                 note_take(p, bytes);
-                return p;
             }
         }
-        void* p = nullptr;
+        if (p) {
+            if (ev) {
+                // the wait takes the event's current record; the event can be recorded again
+                if (hipStreamWaitEvent(st, ev, 0) != hipSuccess) (void)hipEventSynchronize(ev);
+                std::lock_guard<std::mutex> lk(mu);
+                spare.push_back({dev, ev});
+            }
+            return p;
+        }
         if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
         got = bytes;
         std::lock_guard<std::mutex> lk(mu);
         note_take(p, bytes);
         return p;
     }
-    void give(int dev, size_t bytes, void* p) {
+    // pending: work queued on `after` may still use the block (otherwise it is idle)
+    void give(int dev, size_t bytes, void* p, hipStream_t after, bool pending) {
+        hipEvent_t ev = nullptr;
+        bool keep = false;
         {
             std::lock_guard<std::mutex> lk(mu);
             auto it = live.find(p);
@@ -1500,13 +1517,31 @@ struct ScratchPool {
                 live_bytes -= it->second;
                 live.erase(it);
             }
-            if (cached + bytes <= kMaxCached) {
-                blocks.push_back({dev, bytes, p});
-                cached += bytes;
-                return;
+            keep = cached + bytes <= kMaxCached;
+            if (keep) cached += bytes;   // reserved: the block is added below
+            if (keep && pending)
+                for (size_t i = 0; i < spare.size(); ++i)
+                    if (spare[i].first == dev) {
+                        ev = spare[i].second;
+                        spare.erase(spare.begin() + i);
+                        break;
+                    }
+        }
+        if (!keep) {
+            if (pending) (void)hipStreamSynchronize(after);
+            (void)hipFree(p);
+            return;
+        }
+        if (pending) {
+            if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+            if (!ev || hipEventRecord(ev, after) != hipSuccess) {   // no event: wait here instead
+                if (ev) (void)hipEventDestroy(ev);
+                ev = nullptr;
+                (void)hipStreamSynchronize(after);
             }
         }
-        (void)hipFree(p);
+        std::lock_guard<std::mutex> lk(mu);
+        blocks.push_back({dev, bytes, p, ev});
     }
 };
 ScratchPool g_scratch_pool;
@@ -1520,7 +1555,7 @@ struct Scratch {
     template <class T>
     T* get(size_t n) {
         size_t got = 0;
-        T* p = (T*)scratch_take(std::max<size_t>(n, 1) * sizeof(T), got);
+        T* p = (T*)scratch_take(std::max<size_t>(n, 1) * sizeof(T), got, st);
         if (p) blocks.push_back({p, got});
         return p;
     }
@@ -1532,15 +1567,20 @@ struct Scratch {
 
 }  // namespace
 
-void* scratch_take(size_t bytes, size_t& got) {
+void* scratch_take(size_t bytes, size_t& got, hipStream_t st) {
     int dev = 0;
     (void)hipGetDevice(&dev);
-    return g_scratch_pool.take(dev, bytes, got);
+    return g_scratch_pool.take(dev, bytes, got, st);
 }
 void scratch_give(void* p, size_t bytes) {
     int dev = 0;
     (void)hipGetDevice(&dev);
-    g_scratch_pool.give(dev, bytes, p);
+    g_scratch_pool.give(dev, bytes, p, nullptr, false);
+}
+void scratch_give_after(void* p, size_t bytes, hipStream_t st) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    g_scratch_pool.give(dev, bytes, p, st, true);
 }
 uint64_t scratch_peak(bool reset) {
     std::lock_guard<std::mutex> lk(g_scratch_pool.mu);
@@ -1835,7 +1875,7 @@ int build_wide(rtx_bvh* B, hipStream_t st) {
     if (n == 0) return RTX_OK;   // the root is a leaf
     B->wnodes = dalloc<WNode>(n);
     size_t got = 0;
-    uint8_t* wl = static_cast<uint8_t*>(scratch_take(n, got));
+    uint8_t* wl = static_cast<uint8_t*>(scratch_take(n, got, st));
     if (!B->wnodes || !wl) {
         if (wl) scratch_give(wl, got);
         return fail(RTX_E_HIP, "device allocation failed for the wide traversal nodes");

@@ -19,11 +19,13 @@ bool hip_ok(hipError_t e, const char* what);
     } while (0)
 
 // Process-wide cache of device scratch blocks (rtx_build.hip): scratch_take returns a
-// cached block of at least `bytes` (its size in `got`) or a new allocation; scratch_give
-// returns a block once no queued work uses it. Saves hipMalloc/hipFree (which synchronizes
-// the device) on every build and render.
-void* scratch_take(size_t bytes, size_t& got);
+// cached block of at least `bytes` (its size in `got`) or a new allocation, and orders the
+// work later queued on `st` after the block's last user; scratch_give returns an idle block,
+// scratch_give_after one that work queued on `st` may still use. Saves hipMalloc/hipFree
+// (which synchronizes the device) on every build and render.
+void* scratch_take(size_t bytes, size_t& got, hipStream_t st);
 void scratch_give(void* p, size_t bytes);
+void scratch_give_after(void* p, size_t bytes, hipStream_t st);
 // Peak of the bytes requested by blocks taken and not yet given back, since the last reset.
 uint64_t scratch_peak(bool reset);
 

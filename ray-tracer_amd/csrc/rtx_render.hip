@@ -633,6 +633,27 @@ __global__ void k_accum(const SceneDesc* __restrict__ scenes, const uint64_t* __
     (void)off; (void)ns; (void)g0;
 }
 
+// Device copies of the scene table and path offsets, written by kernels that carry them as
+// launch arguments: ordered on the render's stream like its other work, with no host copy
+// to wait for and no host buffer to keep alive.
+constexpr int kPutDescs = 8;     // 8 descriptors per launch (kernel arguments stay under 4 KB)
+constexpr int kPutOffs = 64;
+struct DescChunk {
+    SceneDesc d[kPutDescs];
+    uint32_t n;
+};
+struct OffChunk {
+    uint64_t v[kPutOffs];
+    uint32_t n;
+};
+__global__ void k_put_descs(SceneDesc* __restrict__ dst, DescChunk c) {
+    for (uint32_t i = threadIdx.x; i < c.n; i += blockDim.x) dst[i] = c.d[i];
+}
+__global__ void k_put_offs(uint64_t* __restrict__ dst, OffChunk c) {
+    for (uint32_t i = threadIdx.x; i < c.n; i += blockDim.x) dst[i] = c.v[i];
+}
+static_assert(sizeof(DescChunk) < 4096 && sizeof(OffChunk) < 4096, "kernel arguments");
+
 __global__ void k_finalize(const SceneDesc* __restrict__ scenes, int s) {
     const SceneDesc& S = scenes[s];
     const float fs = (float)S.spp;
@@ -709,27 +730,36 @@ struct Batch {
     int max_b = 0;
     uint32_t ncnt = 0;   // entries of P.cnt: queue lengths, then trace counters, one per bounce
 
+    hipStream_t sst = nullptr;    // stream of setup's uploads
+    hipStream_t last = nullptr;   // stream of the latest launch
+
     ~Batch() {
+        // A graph batch may still run: wait for it before its graph and blocks go. Otherwise
+        // the blocks go back to the cache ordered after the latest launch's stream: whoever
+        // takes one next waits for that work on the GPU, not here (launches of one batch are
+        // ordered by its caller: they share the path state).
+        if (exec && done) (void)hipEventSynchronize(done);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        // blocks go back to the cache only once no launch of this batch can still use them
-        // (launches of one batch are ordered by its caller: they share the path state)
-        if (done) {
-            (void)hipEventSynchronize(done);
-            (void)hipEventDestroy(done);
+        const hipStream_t after = last ? last : sst;   // setup's uploads may be the last work
+        const bool pending = !exec && (done || sst);
+        if (done) (void)hipEventDestroy(done);
+        for (auto& a : allocs) {
+            if (pending) scratch_give_after(a.first, a.second, after);
+            else scratch_give(a.first, a.second);
         }
-        for (auto& a : allocs) scratch_give(a.first, a.second);
     }
     template <class T>
     T* alloc(size_t n) {
         size_t got = 0;
-        T* p = (T*)scratch_take(std::max<size_t>(n, 1) * sizeof(T), got);
+        T* p = (T*)scratch_take(std::max<size_t>(n, 1) * sizeof(T), got, sst);
         if (p) allocs.push_back({p, got});
         return p;
     }
     int note_launch(hipStream_t s) {
         if (!done) RTX_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
         RTX_HIP(hipEventRecord(done, s));
+        last = s;
         return RTX_OK;
     }
     // adjoint (scene 0 only): dLoss/dimage in, gradients out. adj 0: a forward render (which
@@ -742,12 +772,14 @@ struct Batch {
     float* ge = nullptr;
     float4* rec = nullptr;     // the render's path records (2 * paths float4), by global path number
     float4* recb = nullptr;    // adj 1: the batch's path records (2 * B float4)
-    int setup(uint32_t batch_paths, bool adjoint = false);
+    // Takes the path state from the scratch cache and uploads the scene table, ordered on st.
+    int setup(uint32_t batch_paths, hipStream_t st, bool adjoint = false);
     void enqueue_range(int s_first, int s_last, hipStream_t st);   // scenes [s_first, s_last)
     int launch(hipStream_t st);
 };
 
-int Batch::setup(uint32_t batch_paths, bool adjoint) {
+int Batch::setup(uint32_t batch_paths, hipStream_t st, bool adjoint) {
+    sst = st;
     const int ns = (int)descs.size();
     off.assign(ns + 1, 0);
     for (int s = 0; s < ns; ++s) {
@@ -782,8 +814,19 @@ int Batch::setup(uint32_t batch_paths, bool adjoint) {
     if (!d_scenes || !d_off || !P.ray0 || !P.ray1 || !P.hit || !P.tp || !P.L || !P.sid || !P.pix || !P.smp ||
         !P.q[0] || !P.q[1] || !P.cnt)
         return fail(RTX_E_HIP, "device allocation failed on render setup");
-    RTX_HIP(hipMemcpy(d_scenes, descs.data(), sizeof(SceneDesc) * ns, hipMemcpyHostToDevice));
-    RTX_HIP(hipMemcpy(d_off, off.data(), sizeof(uint64_t) * (ns + 1), hipMemcpyHostToDevice));
+    for (int s0 = 0; s0 < ns; s0 += kPutDescs) {
+        DescChunk c{};
+        c.n = (uint32_t)std::min(kPutDescs, ns - s0);
+        for (uint32_t k = 0; k < c.n; ++k) c.d[k] = descs[s0 + k];
+        k_put_descs<<<1, 64, 0, st>>>(d_scenes + s0, c);
+    }
+    for (int s0 = 0; s0 <= ns; s0 += kPutOffs) {
+        OffChunk c{};
+        c.n = (uint32_t)std::min(kPutOffs, ns + 1 - s0);
+        for (uint32_t k = 0; k < c.n; ++k) c.v[k] = off[s0 + k];
+        k_put_offs<<<1, 64, 0, st>>>(d_off + s0, c);
+    }
+    RTX_HIP(hipGetLastError());
     return RTX_OK;
 }
 
@@ -951,20 +994,28 @@ extern "C" int rtx_batch_create(const rtx_bvh* const* scenes, int n, const rtx_c
         B.descs.push_back(make_desc(scenes[s], &cams[s], &params[s], albedo_d ? albedo_d[s] : nullptr,
                                     emission_d ? emission_d[s] : nullptr, images_d[s]));
     }
-    int rc = B.setup(g_tuning.batch_paths);
-    if (rc) { delete bt; return rc; }
+    // A batch may be launched on any stream: its setup runs on a stream of its own and is
+    // complete when creation returns.
+    hipStream_t cs;
+    if (!hip_ok(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "stream")) { delete bt; return RTX_E_HIP; }
+    int rc = B.setup(g_tuning.batch_paths, cs);
+    bool ok = hip_ok(hipStreamSynchronize(cs), "batch setup");
+    B.sst = nullptr;
+    if (rc || !ok) {
+        (void)hipStreamDestroy(cs);
+        delete bt;
+        return rc ? rc : RTX_E_HIP;
+    }
     if (mode == 1) {
-        hipStream_t cs;
-        if (!hip_ok(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "stream")) { delete bt; return RTX_E_HIP; }
-        bool ok = hip_ok(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
+        ok = hip_ok(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
         if (ok) {
             B.enqueue_range(0, n, cs);
             ok = hip_ok(hipStreamEndCapture(cs, &B.graph), "end capture");
         }
         if (ok) ok = hip_ok(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0), "instantiate");
-        (void)hipStreamDestroy(cs);
-        if (!ok) { delete bt; return RTX_E_HIP; }
     }
+    (void)hipStreamDestroy(cs);
+    if (!ok) { delete bt; return RTX_E_HIP; }
     *out = bt;
     return RTX_OK;
 }
@@ -988,13 +1039,29 @@ extern "C" int rtx_render_batched(const rtx_bvh* const* scenes, int n, const rtx
     return rc;
 }
 
+namespace {
+// One scene's render, queued on `stream` and not waited for; the path state goes back to the
+// scratch cache ordered after it.
+int render_one(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params* p, const float* albedo_d,
+               const float* emission_d, float* image_d, float* path_records_d, void* stream) {
+    Batch B;
+    B.descs.push_back(make_desc(b, cam, p, albedo_d, emission_d, image_d));
+    B.rec = reinterpret_cast<float4*>(path_records_d);
+    hipStream_t st = (hipStream_t)stream;
+    int rc = B.setup(g_tuning.batch_paths, st);
+    if (rc) return rc;
+    B.enqueue_range(0, 1, st);
+    rc = B.note_launch(st);
+    if (rc) return rc;
+    return hip_ok(hipGetLastError(), "render launch") ? RTX_OK : RTX_E_HIP;
+}
+}  // namespace
+
 extern "C" int rtx_render(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params* p, const float* albedo_d,
                           const float* emission_d, float* image_d, void* stream) {
-    const rtx_bvh* sc[1] = {b};
-    const float* al[1] = {albedo_d};
-    const float* em[1] = {emission_d};
-    float* im[1] = {image_d};
-    return rtx_render_batched(sc, 1, cam, p, al, em, im, 0, stream);
+    int rc = check_scene(b, cam, p, image_d, "rtx_render()");
+    if (rc) return rc;
+    return render_one(b, cam, p, albedo_d, emission_d, image_d, nullptr, stream);
 }
 
 extern "C" int rtx_render_primal(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params* p,
@@ -1005,17 +1072,7 @@ extern "C" int rtx_render_primal(const rtx_bvh* b, const rtx_camera* cam, const 
     if (!path_records_d) return fail(RTX_E_ARG, "null path_records_d on rtx_render_primal()");
     if (reinterpret_cast<uintptr_t>(path_records_d) % 16)
         return fail(RTX_E_ARG, "path_records_d is not 16-byte aligned on rtx_render_primal()");
-    Batch B;
-    B.descs.push_back(make_desc(b, cam, p, albedo_d, emission_d, image_d));
-    B.rec = reinterpret_cast<float4*>(path_records_d);
-    rc = B.setup(g_tuning.batch_paths);
-    if (rc) return rc;
-    hipStream_t st = (hipStream_t)stream;
-    B.enqueue_range(0, 1, st);
-    rc = B.note_launch(st);
-    if (rc) return rc;
-    if (!hip_ok(hipGetLastError(), "primal launch") || !hip_ok(hipStreamSynchronize(st), "sync")) return RTX_E_HIP;
-    return RTX_OK;
+    return render_one(b, cam, p, albedo_d, emission_d, image_d, path_records_d, stream);
 }
 
 namespace {
@@ -1039,15 +1096,15 @@ int adjoint_impl(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_param
     B.dimg = d_image_d;
     B.ga = grad_albedo_d;
     B.ge = emission_d ? grad_emission_d : nullptr;
-    // the state per path does not depend on max_bounce: the path state and one path record
-    rc = B.setup(g_tuning.batch_paths, true);
-    if (rc) return rc;
+    // the state per path does not depend on max_bounce: the path state and one path record;
+    // queued on the stream and not waited for, like render_one
     hipStream_t st = (hipStream_t)stream;
+    rc = B.setup(g_tuning.batch_paths, st, true);
+    if (rc) return rc;
     B.enqueue_range(0, 1, st);
     rc = B.note_launch(st);
     if (rc) return rc;
-    if (!hip_ok(hipGetLastError(), "adjoint launch") || !hip_ok(hipStreamSynchronize(st), "sync")) return RTX_E_HIP;
-    return RTX_OK;
+    return hip_ok(hipGetLastError(), "adjoint launch") ? RTX_OK : RTX_E_HIP;
 }
 }  // namespace
 
