@@ -1040,6 +1040,18 @@ extern "C" int rtx_render_batched(const rtx_bvh* const* scenes, int n, const rtx
 }
 
 namespace {
+// The single-scene entry points take their path state from the scratch cache and give it
+// back as soon as their work is queued, so their launches cannot be captured into a graph
+// (a replay would use blocks that belong to later calls): a capturing stream is refused.
+// rtx_batch_create's mode 1 is the graph form.
+int refuse_capture(hipStream_t st, const char* fn) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    RTX_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        return fail(RTX_E_STATE, std::string("stream capture of ") + fn + " (use rtx_batch_create mode 1 for a graph)");
+    return RTX_OK;
+}
+
 // One scene's render, queued on `stream` and not waited for; the path state goes back to the
 // scratch cache ordered after it.
 int render_one(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params* p, const float* albedo_d,
@@ -1048,7 +1060,9 @@ int render_one(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_params*
     B.descs.push_back(make_desc(b, cam, p, albedo_d, emission_d, image_d));
     B.rec = reinterpret_cast<float4*>(path_records_d);
     hipStream_t st = (hipStream_t)stream;
-    int rc = B.setup(g_tuning.batch_paths, st);
+    int rc = refuse_capture(st, "a render");
+    if (rc) return rc;
+    rc = B.setup(g_tuning.batch_paths, st);
     if (rc) return rc;
     B.enqueue_range(0, 1, st);
     rc = B.note_launch(st);
@@ -1099,6 +1113,8 @@ int adjoint_impl(const rtx_bvh* b, const rtx_camera* cam, const rtx_render_param
     // the state per path does not depend on max_bounce: the path state and one path record;
     // queued on the stream and not waited for, like render_one
     hipStream_t st = (hipStream_t)stream;
+    rc = refuse_capture(st, fn);
+    if (rc) return rc;
     rc = B.setup(g_tuning.batch_paths, st, true);
     if (rc) return rc;
     B.enqueue_range(0, 1, st);

@@ -222,6 +222,44 @@ def test_graph_replays_match_eager():
         torch.cuda.synchronize()
 
 
+def test_render_is_stream_ordered_and_refuses_capture():
+    """rtx.render returns before its work is done (stream-ordered); renders queued back to
+    back on one stream, with their path state recycled through the scratch cache, equal the
+    same renders run one at a time; and a render into a capturing stream is refused (its path
+    state goes back to the cache once queued, so a replay would use blocks of later calls)."""
+    import torch
+    sc = _small_batch_scenes()
+    bvhs = [rtx.build(v, t) for v, t, _, _, _ in sc]
+    one = []
+    for b, (_, _, c, p, _) in zip(bvhs, sc):
+        one.append(rtx.render(b, c, p).cpu().numpy())
+        torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    outs = []
+    with torch.cuda.stream(s):
+        for rep in range(4):
+            for b, (_, _, c, p, _) in zip(bvhs, sc):
+                outs.append(rtx.render(b, c, p, stream=s))
+    s.synchronize()
+    for k, o in enumerate(outs):
+        assert np.array_equal(o.cpu().numpy(), one[k % 3]), f"queued render {k} differs"
+    v, t, c, p, _ = sc[0]
+    img = torch.zeros((p.height, p.width, 3), device="cuda")
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        g.capture_begin()
+        try:
+            with pytest.raises(rtx.RtxError, match="capture"):
+                rtx.render(bvhs[0], c, p, out=img, stream=s)
+            img.add_(1.0)   # some captured work
+        finally:
+            g.capture_end()
+    s.synchronize()
+    g.replay()
+    torch.cuda.synchronize()
+    assert float(img.min()) == 1.0 and float(img.max()) == 1.0
+
+
 def test_adjoint_matches_finite_differences():
     """Path-replay gradients of sum((I - T)^2) w.r.t. per-triangle albedo and emission
     against central differences (eps 1e-3) of the oracle renderer, 1e-2 relative."""
