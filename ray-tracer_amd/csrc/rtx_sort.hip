@@ -12,6 +12,18 @@
 namespace rtx {
 namespace {
 
+// LDS words shared by the lanes of one wave: relaxed wave-scope atomics keep every access
+// (a lane reads what another lane wrote) as a plain ds_* instruction; a wave's LDS operations
+// complete in issue order.
+template <class T>
+__device__ __forceinline__ void wave_st(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+template <class T>
+__device__ __forceinline__ T wave_ld(T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 constexpr int kRsThreads = 256, kRsItems = 16, kRsTile = kRsThreads * kRsItems;
 constexpr int kScanThreads = 1024, kScanItems = 4, kScanSeg = kScanThreads * kScanItems;
 
@@ -88,34 +100,38 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_add(uint32_t* data, uint3
         if (b + k < m) data[b + k] += add;
 }
 
-// Scatter of one pass. Ranks are stable within the tile: by item round, then wave, then
-// lane (ballots give each lane its rank among the wave's lanes holding the same digit).
-// The tile is first placed in LDS in digit order and then written out slot by slot, so
-// consecutive lanes store consecutive elements of a digit's run (4-byte stores to unrelated
-// addresses cost a memory transaction each).
+// Scatter of one pass. Each wave ranks its own contiguous chunk of the tile (kRsItems rounds
+// of 64 keys): within a round, ballots give each lane its rank among the lanes holding the
+// same digit, and a per-wave LDS count per digit carries the ranks from round to round, so
+// the rounds need no block barrier. A key's place in the tile is then: the tile's start of
+// its digit, plus the counts of that digit in the earlier waves' chunks, plus its rank in its
+// own chunk; that is index order within each digit, so the sort is stable. The tile is placed
+// in LDS in digit order and written out slot by slot, so consecutive lanes store consecutive
+// elements of a digit's run (4-byte stores to unrelated addresses cost a memory transaction
+// each). (The earlier form interleaved the waves' rounds and needed three block barriers per
+// round.)
 template <class K>
 __global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const K* kin, const uint32_t* vin, K* kout,
                                                            uint32_t* vout, uint32_t n, int shift,
                                                            const uint32_t* hist, uint32_t ntiles) {
-    __shared__ uint32_t run[256];      // elements of digit d seen so far in this tile
-    __shared__ uint32_t gbase[256];    // global position of this tile's first digit-d element
-    __shared__ uint32_t lstart[256];   // tile-local position of the first digit-d element
-    __shared__ uint32_t wcnt[kRsThreads / 64][256];
-    __shared__ uint32_t wsum[kRsThreads / 64];
+    constexpr int kWaves = kRsThreads / 64;
+    __shared__ uint32_t wcnt[kWaves][256];   // per wave: digit-d keys ranked so far, then the wave's offset in the digit's run
+    __shared__ uint32_t gbase[256];          // global position of this tile's first digit-d element
+    __shared__ uint32_t lstart[256];         // tile-local position of the first digit-d element
+    __shared__ uint32_t wsum[kWaves];
     __shared__ K skey[kRsTile];
     __shared__ uint32_t sval[kRsTile];
     const int lane = __lane_id(), w = threadIdx.x >> 6;
     gbase[threadIdx.x] = hist[threadIdx.x * ntiles + blockIdx.x];
-    run[threadIdx.x] = 0;
+    for (int ww = 0; ww < kWaves; ++ww) wcnt[ww][threadIdx.x] = 0;
+    __syncthreads();
     const uint64_t lt = (1ull << lane) - 1ull;
-    const uint32_t base = blockIdx.x * kRsTile;
+    const uint32_t base = blockIdx.x * kRsTile + w * (kRsItems * 64);   // this wave's chunk
     K keys[kRsItems];
     uint32_t vals[kRsItems], lrank[kRsItems];
 #pragma unroll
     for (int r = 0; r < kRsItems; ++r) {
-        for (int ww = 0; ww < kRsThreads / 64; ++ww) wcnt[ww][threadIdx.x] = 0;
-        __syncthreads();
-        const uint32_t i = base + r * kRsThreads + threadIdx.x;
+        const uint32_t i = base + r * 64 + lane;
         const bool valid = i < n;
         const K k = valid ? kin[i] : (K)0;
         const uint32_t v = valid ? vin[i] : 0u;
@@ -126,21 +142,23 @@ __global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const K* kin, const u
             mask &= ((d >> bit) & 1u) ? bb : ~bb;
         }
         const uint32_t rank = __popcll(mask & lt);
-        if (valid && rank == 0) wcnt[w][d] = __popcll(mask);
-        __syncthreads();
-        uint32_t lr = run[d] + rank;
-        for (int ww = 0; ww < w; ++ww) lr += wcnt[ww][d];
+        // every lane reads its digit's count before the first lane of each digit writes it (a
+        // wave's LDS accesses run in issue order)
+        const uint32_t before = valid ? wave_ld(&wcnt[w][d]) : 0u;
+        if (valid && rank == 0) wave_st(&wcnt[w][d], before + (uint32_t)__popcll(mask));
         keys[r] = k;
         vals[r] = v;
-        lrank[r] = lr;
-        __syncthreads();
-        uint32_t add_ = 0;
-        for (int ww = 0; ww < kRsThreads / 64; ++ww) add_ += wcnt[ww][threadIdx.x];
-        run[threadIdx.x] += add_;
+        lrank[r] = before + rank;
     }
     __syncthreads();
+    // digit d = this thread: the waves' counts become offsets, their sum the tile's count
+    uint32_t c = 0;
+    for (int ww = 0; ww < kWaves; ++ww) {
+        const uint32_t x = wcnt[ww][threadIdx.x];
+        wcnt[ww][threadIdx.x] = c;
+        c += x;
+    }
     // tile-local digit starts: exclusive scan of the tile's digit counts (one digit per thread)
-    const uint32_t c = run[threadIdx.x];
     uint32_t inc = c;
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t y = __shfl_up(inc, o);
@@ -154,16 +172,17 @@ __global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const K* kin, const u
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kRsItems; ++r) {
-        const uint32_t i = base + r * kRsThreads + threadIdx.x;
+        const uint32_t i = base + r * 64 + lane;
         if (i < n) {
             const uint32_t d = (uint32_t)(keys[r] >> shift) & 255u;
-            const uint32_t slot = lstart[d] + lrank[r];
+            const uint32_t slot = lstart[d] + wcnt[w][d] + lrank[r];
             skey[slot] = keys[r];
             sval[slot] = vals[r];
         }
     }
     __syncthreads();
-    const uint32_t m = n - base < (uint32_t)kRsTile ? n - base : (uint32_t)kRsTile;
+    const uint32_t tb = blockIdx.x * kRsTile;
+    const uint32_t m = n - tb < (uint32_t)kRsTile ? n - tb : (uint32_t)kRsTile;
     for (uint32_t j = threadIdx.x; j < m; j += kRsThreads) {
         const K k = skey[j];
         const uint32_t d = (uint32_t)(k >> shift) & 255u;
