@@ -1693,7 +1693,7 @@ static int build_impl(const float* verts_d, uint32_t nv, const uint32_t* tris_d,
         if ((vary >> (8 * pass)) & 255u) pass_mask |= 1u << pass;
     RTX_HIP(hipMemcpyAsync(keysB, B->codes, sizeof(uint64_t) * N, hipMemcpyDeviceToDevice, st));
     SortBuffers SB{{keysB, keysC}, {valsA, valsB}, hist, spart, 0};
-    radix_sort_pairs(SB, N, pass_mask, st);
+    if (radix_sort_pairs(SB, N, pass_mask, st)) return fail(RTX_E_HIP, "Morton code sort failed on rtx_build()");
     const uint64_t* scodes = SB.keys[SB.cur];
     const uint32_t* vin = SB.vals[SB.cur];
     RTX_HIP(hipMemcpyAsync(sorted, vin, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, st));

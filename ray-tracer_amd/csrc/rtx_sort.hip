@@ -1,16 +1,28 @@
-// Stable LSD radix sort of (uint64 key, uint32 value) pairs on 8-bit digits, shared by
-// the builder (Morton codes, bvh_morton.cpp:109-159) and the ray reordering of traces.
+// Stable LSD radix sort of (uint64 or uint32 key, uint32 value) pairs on 8-bit digits,
+// shared by the builder (Morton codes, bvh_morton.cpp:109-159) and the ray reordering of
+// traces.
 //
-// Per pass: k_rs_hist counts digits per 4096-key tile in LDS; a three-kernel scan turns
-// the digit-major histogram into global offsets; k_rs_scatter ranks keys stably inside a
-// tile with wave ballots (lanes holding the same digit) and per-wave LDS counts, then
-// writes the tile out in digit order from LDS.
+// Each pass is three kernels over 4096-key tiles:
+// - k_rs_count: the digit counts of every tile, digit-major (count[d][tile]);
+// - k_rs_rowscan: one block per digit turns its row into exclusive prefixes over the tiles
+//   and stores the row total;
+// - k_rs_scatter: one block per tile ranks its keys stably (wave ballots and per-wave LDS
+//   counts), places the tile in digit order in LDS and writes each digit's run to the start
+//   of that digit (a scan of the row totals) plus the row prefix of its tile.
+// No block waits on another. (A single-kernel pass chains each tile's offsets to its
+// predecessors' through device-wide flags; with every tile of a 4M-key pass resident at once,
+// that chain and the tile counter it needs cost more than the two extra launches.)
 #include <hip/hip_runtime.h>
 
 #include "rtx_sort.hpp"
 
 namespace rtx {
 namespace {
+
+constexpr int kRsThreads = 256, kRsItems = 16, kRsTile = kRsThreads * kRsItems;
+constexpr int kWaves = kRsThreads / 64;
+constexpr int kScanItems = 4;   // row entries per thread and step of k_rs_rowscan
+static_assert(kRsTile == (int)kSortTile, "scatter tile size");
 
 // LDS words shared by the lanes of one wave: relaxed wave-scope atomics keep every access
 // (a lane reads what another lane wrote) as a plain ds_* instruction; a wave's LDS operations
@@ -24,118 +36,118 @@ __device__ __forceinline__ T wave_ld(T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
-constexpr int kRsThreads = 256, kRsItems = 16, kRsTile = kRsThreads * kRsItems;
-constexpr int kScanThreads = 1024, kScanItems = 4, kScanSeg = kScanThreads * kScanItems;
-
-template <class K>
-__global__ void k_rs_hist(const K* keys, uint32_t n, int shift, uint32_t* hist, uint32_t ntiles) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    uint32_t base = blockIdx.x * kRsTile;
-    for (int r = 0; r < kRsItems; ++r) {
-        uint32_t i = base + r * kRsThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
-}
-
-// Block-wide exclusive scan (1024 threads); returns the block total in *total.
-__device__ uint32_t block_excl(uint32_t v, uint32_t* ws, uint32_t* total) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t s = v;
+// Inclusive scan of one value per thread over the 256-thread block; wsum holds kWaves words,
+// which after the call hold the waves' totals.
+__device__ __forceinline__ uint32_t block_incl(uint32_t v, uint32_t* wsum) {
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    uint32_t inc = v;
     for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(s, o);
-        if (lane >= o) s += y;
+        const uint32_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
     }
-    if (lane == 63) ws[w] = s;
+    if (lane == 63) wsum[w] = inc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (int i = 0; i < kScanThreads / 64; ++i) { uint32_t x = ws[i]; ws[i] = run; run += x; }
-        ws[kScanThreads / 64] = run;
+    for (int ww = 0; ww < w; ++ww) inc += wsum[ww];
+    return inc;
+}
+
+// The keys of one 4096-key tile, kRsItems per lane: wave w holds the contiguous chunk
+// [w * 1024, (w + 1) * 1024) of the tile, round r the 64 keys at r * 64. Every load is issued
+// before any key is used.
+template <class K>
+__device__ __forceinline__ void load_chunk(const K* kin, uint32_t base, uint32_t n, K (&keys)[kRsItems]) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int r = 0; r < kRsItems; ++r) {
+        const uint32_t i = base + r * 64 + lane;
+        keys[r] = i < n ? kin[i] : (K)0;
     }
+}
+
+// count[d * ntiles + tile] = keys of digit d in the tile.
+template <class K>
+__global__ void __launch_bounds__(kRsThreads) k_rs_count(const K* keys, uint32_t n, int shift, uint32_t* count,
+                                                         uint32_t ntiles) {
+    __shared__ uint32_t h[kWaves][256];   // per wave: fewer lanes collide on one counter
+    const int w = threadIdx.x >> 6, d0 = threadIdx.x;
+    for (int ww = 0; ww < kWaves; ++ww) h[ww][d0] = 0;
     __syncthreads();
-    uint32_t ex = ws[w] + s - v;
-    *total = ws[kScanThreads / 64];
+    const uint32_t base = blockIdx.x * kRsTile + w * (kRsItems * 64);
+    K k[kRsItems];
+    load_chunk(keys, base, n, k);
+#pragma unroll
+    for (int r = 0; r < kRsItems; ++r)
+        if (base + r * 64 + __lane_id() < n) atomicAdd(&h[w][(uint32_t)(k[r] >> shift) & 255u], 1u);
     __syncthreads();
-    return ex;
+    uint32_t c = 0;
+    for (int ww = 0; ww < kWaves; ++ww) c += h[ww][d0];
+    count[(size_t)d0 * ntiles + blockIdx.x] = c;
 }
 
-// Exclusive scan of segments of kScanSeg entries in place; segment totals to part[].
-__global__ void __launch_bounds__(kScanThreads) k_scan_seg(uint32_t* data, uint32_t m, uint32_t* part) {
-    __shared__ uint32_t ws[kScanThreads / 64 + 1];
-    const uint32_t b = blockIdx.x * kScanSeg + threadIdx.x * kScanItems;
-    uint32_t v[kScanItems], s = 0;
-    for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (b + k < m) ? data[b + k] : 0u;
-        s += v[k];
+// Row d of count (one block per digit) to exclusive prefixes over the tiles; the row's total
+// to total[d].
+__global__ void __launch_bounds__(kRsThreads) k_rs_rowscan(uint32_t* count, uint32_t ntiles, uint32_t* total) {
+    __shared__ uint32_t wsum[kWaves];
+    uint32_t* row = count + (size_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < ntiles; b += kRsThreads * kScanItems) {
+        const uint32_t i0 = b + threadIdx.x * kScanItems;
+        uint32_t v[kScanItems], s = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            v[k] = i0 + k < ntiles ? row[i0 + k] : 0u;
+            s += v[k];
+        }
+        uint32_t run = carry + block_incl(s, wsum) - s;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (i0 + k < ntiles) row[i0 + k] = run;
+            run += v[k];
+        }
+        for (int ww = 0; ww < kWaves; ++ww) carry += wsum[ww];
+        __syncthreads();   // wsum is rewritten by the next step
     }
-    uint32_t tot;
-    uint32_t run = block_excl(s, ws, &tot);
-    for (int k = 0; k < kScanItems; ++k) {
-        if (b + k < m) data[b + k] = run;
-        run += v[k];
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+    if (threadIdx.x == 0) total[blockIdx.x] = carry;
 }
 
-// Exclusive scan of the segment totals (any count, one block, serial per thread chunk).
-__global__ void __launch_bounds__(kScanThreads) k_scan_parts(uint32_t* part, uint32_t np) {
-    __shared__ uint32_t ws[kScanThreads / 64 + 1];
-    const uint32_t chunk = (np + kScanThreads - 1) / kScanThreads;
-    const uint32_t b = threadIdx.x * chunk, e = min(np, b + chunk);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += part[i];
-    uint32_t tot;
-    uint32_t run = block_excl(s, ws, &tot);
-    for (uint32_t i = b; i < e; ++i) { uint32_t x = part[i]; part[i] = run; run += x; }
-}
-
-__global__ void __launch_bounds__(kScanThreads) k_scan_add(uint32_t* data, uint32_t m, const uint32_t* part) {
-    const uint32_t add = part[blockIdx.x];
-    const uint32_t b = blockIdx.x * kScanSeg + threadIdx.x * kScanItems;
-    for (int k = 0; k < kScanItems; ++k)
-        if (b + k < m) data[b + k] += add;
-}
-
-// Scatter of one pass. Each wave ranks its own contiguous chunk of the tile (kRsItems rounds
-// of 64 keys): within a round, ballots give each lane its rank among the lanes holding the
-// same digit, and a per-wave LDS count per digit carries the ranks from round to round, so
-// the rounds need no block barrier. A key's place in the tile is then: the tile's start of
-// its digit, plus the counts of that digit in the earlier waves' chunks, plus its rank in its
-// own chunk; that is index order within each digit, so the sort is stable. The tile is placed
-// in LDS in digit order and written out slot by slot, so consecutive lanes store consecutive
-// elements of a digit's run (4-byte stores to unrelated addresses cost a memory transaction
-// each). (The earlier form interleaved the waves' rounds and needed three block barriers per
-// round.)
+// Scatter of one pass (digit at bit `shift`), one block per tile. Each wave ranks its own
+// contiguous chunk of the tile (kRsItems rounds of 64 keys): within a round, ballots give
+// each lane its rank among the lanes holding the same digit, and a per-wave LDS count per
+// digit carries the ranks from round to round, so the rounds need no block barrier. A key's
+// place in the tile is the tile's start of its digit, plus the counts of that digit in the
+// earlier waves' chunks, plus its rank in its own chunk: index order within each digit, so
+// the sort is stable. The tile is placed in LDS in digit order and written out slot by slot,
+// so consecutive lanes store consecutive elements of a digit's run.
 template <class K>
 __global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const K* kin, const uint32_t* vin, K* kout,
                                                            uint32_t* vout, uint32_t n, int shift,
-                                                           const uint32_t* hist, uint32_t ntiles) {
-    constexpr int kWaves = kRsThreads / 64;
+                                                           const uint32_t* count, const uint32_t* total,
+                                                           uint32_t ntiles) {
     __shared__ uint32_t wcnt[kWaves][256];   // per wave: digit-d keys ranked so far, then the wave's offset in the digit's run
     __shared__ uint32_t gbase[256];          // global position of this tile's first digit-d element
     __shared__ uint32_t lstart[256];         // tile-local position of the first digit-d element
-    __shared__ uint32_t wsum[kWaves];
+    __shared__ uint32_t wsum[2][kWaves];
     __shared__ K skey[kRsTile];
     __shared__ uint32_t sval[kRsTile];
-    const int lane = __lane_id(), w = threadIdx.x >> 6;
-    gbase[threadIdx.x] = hist[threadIdx.x * ntiles + blockIdx.x];
-    for (int ww = 0; ww < kWaves; ++ww) wcnt[ww][threadIdx.x] = 0;
+    const int lane = __lane_id(), w = threadIdx.x >> 6, d0 = threadIdx.x;
+    for (int ww = 0; ww < kWaves; ++ww) wcnt[ww][d0] = 0;
+    // this tile's digit offsets do not depend on the ranking: load them first
+    const uint32_t tot = total[d0], pre = count[(size_t)d0 * ntiles + blockIdx.x];
     __syncthreads();
     const uint64_t lt = (1ull << lane) - 1ull;
     const uint32_t base = blockIdx.x * kRsTile + w * (kRsItems * 64);   // this wave's chunk
     K keys[kRsItems];
     uint32_t vals[kRsItems], lrank[kRsItems];
+    load_chunk(kin, base, n, keys);
 #pragma unroll
     for (int r = 0; r < kRsItems; ++r) {
         const uint32_t i = base + r * 64 + lane;
-        const bool valid = i < n;
-        const K k = valid ? kin[i] : (K)0;
-        const uint32_t v = valid ? vin[i] : 0u;
-        const uint32_t d = (uint32_t)(k >> shift) & 255u;
+        vals[r] = i < n ? vin[i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kRsItems; ++r) {
+        const bool valid = base + r * 64 + lane < n;
+        const uint32_t d = (uint32_t)(keys[r] >> shift) & 255u;
         uint64_t mask = __ballot(valid);
         for (int bit = 0; bit < 8; ++bit) {
             const uint64_t bb = __ballot(valid && ((d >> bit) & 1u));
@@ -146,29 +158,19 @@ __global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const K* kin, const u
         // wave's LDS accesses run in issue order)
         const uint32_t before = valid ? wave_ld(&wcnt[w][d]) : 0u;
         if (valid && rank == 0) wave_st(&wcnt[w][d], before + (uint32_t)__popcll(mask));
-        keys[r] = k;
-        vals[r] = v;
         lrank[r] = before + rank;
     }
     __syncthreads();
-    // digit d = this thread: the waves' counts become offsets, their sum the tile's count
+    // digit d0 = this thread: the waves' counts become offsets, their sum the tile's count
     uint32_t c = 0;
     for (int ww = 0; ww < kWaves; ++ww) {
-        const uint32_t x = wcnt[ww][threadIdx.x];
-        wcnt[ww][threadIdx.x] = c;
+        const uint32_t x = wcnt[ww][d0];
+        wcnt[ww][d0] = c;
         c += x;
     }
-    // tile-local digit starts: exclusive scan of the tile's digit counts (one digit per thread)
-    uint32_t inc = c;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(inc, o);
-        if (lane >= o) inc += y;
-    }
-    if (lane == 63) wsum[w] = inc;
-    __syncthreads();
-    uint32_t off = 0;
-    for (int ww = 0; ww < w; ++ww) off += wsum[ww];
-    lstart[threadIdx.x] = off + inc - c;
+    // global digit starts and tile-local digit starts (exclusive scans over the digits)
+    gbase[d0] = block_incl(tot, wsum[0]) - tot + pre;
+    lstart[d0] = block_incl(c, wsum[1]) - c;
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kRsItems; ++r) {
@@ -194,28 +196,21 @@ __global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const K* kin, const u
 
 }  // namespace
 
-size_t sort_hist_entries(uint32_t n) { return 256ull * ((n + kRsTile - 1) / kRsTile); }
-size_t sort_part_entries(uint32_t n) { return (sort_hist_entries(n) + kScanSeg - 1) / kScanSeg + 1; }
-
 template <class K>
 int sort_impl(SortBuffersT<K>& S, uint32_t n, uint32_t pass_mask, hipStream_t st) {
     S.cur = 0;
     if (n == 0) return 0;
     const uint32_t ntiles = (n + kRsTile - 1) / kRsTile;
-    const uint32_t m = 256 * ntiles;
-    const uint32_t nseg = (m + kScanSeg - 1) / kScanSeg;
     for (int pass = 0; pass < (int)sizeof(K); ++pass) {
         if (!((pass_mask >> pass) & 1u)) continue;
         const int shift = 8 * pass;
-        k_rs_hist<K><<<ntiles, kRsThreads, 0, st>>>(S.keys[S.cur], n, shift, S.hist, ntiles);
-        k_scan_seg<<<nseg, kScanThreads, 0, st>>>(S.hist, m, S.part);
-        k_scan_parts<<<1, kScanThreads, 0, st>>>(S.part, nseg);
-        k_scan_add<<<nseg, kScanThreads, 0, st>>>(S.hist, m, S.part);
+        k_rs_count<K><<<ntiles, kRsThreads, 0, st>>>(S.keys[S.cur], n, shift, S.hist, ntiles);
+        k_rs_rowscan<<<256, kRsThreads, 0, st>>>(S.hist, ntiles, S.part);
         k_rs_scatter<K><<<ntiles, kRsThreads, 0, st>>>(S.keys[S.cur], S.vals[S.cur], S.keys[S.cur ^ 1],
-                                                       S.vals[S.cur ^ 1], n, shift, S.hist, ntiles);
+                                                       S.vals[S.cur ^ 1], n, shift, S.hist, S.part, ntiles);
         S.cur ^= 1;
     }
-    return 0;
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int radix_sort_pairs(SortBuffers& S, uint32_t n, uint32_t pass_mask, hipStream_t st) {

@@ -291,7 +291,7 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
                                                     S.vals[0]);
         // stable sort on the top 24 key bits (8 per axis): the lowest pass bought no
         // locality the traversal could measure, sorting on 16 bits lost some
-        radix_sort_pairs(S, (uint32_t)n, 0xEu, s);
+        if (radix_sort_pairs(S, (uint32_t)n, 0xEu, s)) return fail(RTX_E_HIP, "ray sort failed");
         perm = S.vals[S.cur];
     }
     // the wide layout exists whenever the root is internal (a leaf root has one leaf to test);
@@ -386,11 +386,11 @@ namespace {
 // Scratch of rtx_trace_closest_noalloc: a file-scope device array, so the entry point needs
 // no allocation. Each GPU the library runs on holds its own copy until the process ends.
 // Layout in uint32 words: [0, 64) the persistent kernel's ray counter, then the sort's
-// keys[0], keys[1], vals[0], vals[1] (kNoAllocMaxRays each), its tile histogram and its
-// scan partials.
+// keys[0], keys[1], vals[0], vals[1] (kNoAllocMaxRays each), its per-tile digit counts and
+// its digit totals.
 constexpr uint32_t kNoAllocMaxRays = 1u << 21;
-constexpr size_t kNoAllocHist = 256ull * ((kNoAllocMaxRays + 4095) / 4096);
-constexpr size_t kNoAllocPart = 64;
+constexpr size_t kNoAllocHist = sort_hist_entries(kNoAllocMaxRays);
+constexpr size_t kNoAllocPart = sort_part_entries(kNoAllocMaxRays);
 constexpr size_t kNoAllocWords = 64 + 4ull * kNoAllocMaxRays + kNoAllocHist + kNoAllocPart;
 static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
 
