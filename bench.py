@@ -195,6 +195,16 @@ def main():
             bvh.trace(one)
         host = {"pcie_inclusive_mrays_s": round(n / host_s / 1e6, 2),
                 "single_ray_call_us": round((time.perf_counter() - t_h) / 200 * 1e6, 1)}
+        # one call per ray over 1,000 different rays, and calls of 256 rays (calls of at most
+        # 256 rays take the one-wave-per-ray kernel)
+        t_h = time.perf_counter()
+        for i in range(1000):
+            bvh.trace(inc_h[i:i + 1])
+        host["single_ray_call_us_1000_rays"] = round((time.perf_counter() - t_h) / 1000 * 1e6, 1)
+        t_h = time.perf_counter()
+        for i in range(50):
+            bvh.trace(inc_h[256 * i:256 * (i + 1)])
+        host["call_256_rays_us"] = round((time.perf_counter() - t_h) / 50 * 1e6, 1)
         # the coherent and any-hit launches get launch shapes of their own (sorting rays
         # that are already in camera order only costs the sort, for instance)
         if not args.no_autotune:

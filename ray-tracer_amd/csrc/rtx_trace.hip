@@ -148,6 +148,287 @@ __global__ void __launch_bounds__(kTraceThreads, TREE == kTreeBinary ? 1 : 6) k_
     }
 }
 
+// One wave per ray, for calls of a few rays (the per-ray bvh::trace shim, small batches),
+// whose time is the ray's chain of dependent loads: the reference's depth-first walk
+// (bvh_traverse.cpp:91-178) visits about 78 nodes one after another. Here the wave walks
+// the tree breadth-first, a level per step, each lane testing the two children of one node
+// of the level, and the leaves met at a level are tested together, their (ray, triangle)
+// pairs dealt over the lanes. The chain shrinks to the tree's depth.
+// The hit is still the reference's. Each node carries a path key: one bit per level, set
+// when the node is its parent's far child (near = the smaller entry distance, the left one
+// on a tie, as the reference orders them), so keys order leaves as the reference visits
+// them. The reference keeps, among the triangles with the smallest t, the first it tests:
+// the one with the smallest (key, triangle index). A subtree is dropped once its entry
+// distance exceeds the best t, or equals it with a key after the best's: it can then hold no
+// such triangle. (Where the two walks cull a child differently, that child's entry distance
+// exceeds the smallest t, so no candidate depends on how its parent's children were ordered.)
+// A level of more nodes or leaves than the LDS lists hold (or a tree deeper than the key's
+// 64 bits) sends the ray through the usual one-lane walk instead.
+constexpr uint32_t kWaveList = 512;       // nodes and leaves per level
+constexpr uint32_t kWaveMaxDepth = 60;    // tree depth the 64-bit path keys cover
+constexpr size_t kWaveRayMax = 256;       // calls of at most this many rays take the wave kernel
+struct WaveLds {
+    uint32_t ref[2][kWaveList];
+    float tn[2][kWaveList];
+    unsigned long long key[2][kWaveList];
+    uint32_t lref[kWaveList];
+    unsigned long long lkey[kWaveList];
+    uint8_t mark[64];
+};
+// A candidate hit ordered as the reference ranks equal-t hits: t (ord_t image), path key,
+// triangle index.
+struct WaveCand {
+    uint32_t ot;
+    unsigned long long key;
+    uint32_t k;
+};
+__device__ __forceinline__ bool cand_less(const WaveCand& a, const WaveCand& b) {
+    return a.ot != b.ot ? a.ot < b.ot : a.key != b.key ? a.key < b.key : a.k < b.k;
+}
+__device__ __forceinline__ unsigned long long shfl64(unsigned long long x, int src) {
+    const uint32_t lo = __shfl((uint32_t)x, src), hi = __shfl((uint32_t)(x >> 32), src);
+    return ((unsigned long long)hi << 32) | lo;
+}
+// Smallest value over the wave: a DPP min scan (row shifts, then row broadcasts), read from
+// the last lane.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+#define RTX_DPP_MIN(ctrl, rmask) \
+    x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(~0u, x, ctrl, rmask, 0xf, false))
+    RTX_DPP_MIN(0x111, 0xf);   // row_shr:1
+    RTX_DPP_MIN(0x112, 0xf);   // row_shr:2
+    RTX_DPP_MIN(0x114, 0xf);   // row_shr:4
+    RTX_DPP_MIN(0x118, 0xf);   // row_shr:8
+    RTX_DPP_MIN(0x142, 0xa);   // row_bcast:15 into rows 1, 3
+    RTX_DPP_MIN(0x143, 0xc);   // row_bcast:31 into rows 2, 3
+#undef RTX_DPP_MIN
+    return __builtin_amdgcn_readlane(x, 63);
+}
+__device__ __forceinline__ WaveCand wave_min_cand(WaveCand c) {
+    for (int o = 1; o < 64; o <<= 1) {
+        const int src = (int)(__lane_id() ^ (uint32_t)o);
+        WaveCand x;
+        x.ot = __shfl(c.ot, src);
+        x.key = shfl64(c.key, src);
+        x.k = __shfl(c.k, src);
+        if (cand_less(x, c)) c = x;
+    }
+    return c;
+}
+
+template <bool ANY>
+__global__ void __launch_bounds__(64) k_trace_wave(TraceArgs A, const Ray32* __restrict__ rays,
+                                                   Hit16* __restrict__ hits, float* __restrict__ normals,
+                                                   uint8_t* __restrict__ occl, uint32_t max_depth) {
+    __shared__ WaveLds L;
+    __shared__ uint2 lst[12][kTraceThreads];   // the one-lane walk's stack, when it is needed
+    const uint32_t lane = __lane_id(), r = blockIdx.x;
+    const uint64_t lt = (1ull << lane) - 1ull, upto = (2ull << lane) - 1ull;
+    const float4 r0 = reinterpret_cast<const float4*>(rays)[2 * r];
+    const float4 r1 = reinterpret_cast<const float4*>(rays)[2 * r + 1];
+    const float ox = r0.x, oy = r0.y, oz = r0.z, tmin = r0.w, tmax = r1.w;
+    const float ix = recip(r1.x), iy = recip(r1.y), iz = recip(r1.z);
+    const f3 o{ox, oy, oz}, d{r1.x, r1.y, r1.z};
+    // the best hit so far (wave-uniform); bestT is its t, or tmax before any hit
+    bool found = false;
+    WaveCand best{~0u, ~0ull, ~0u};
+    float bestT = tmax;
+    // entry distance tn and path key of a subtree that may still hold the reference's hit
+    auto keep = [&](float tn, unsigned long long key) {
+        return tn < bestT || (found && tn == bestT && key < best.key);
+    };
+    bool overflow = max_depth > kWaveMaxDepth;
+    uint32_t ncur = 0, nleaf = 0, cur = 0, depth = 0;
+    const float troot = slab(ox, oy, oz, ix, iy, iz, tmin, A.rb[0], A.rb[1], A.rb[2], A.rb[3], A.rb[4], A.rb[5], tmax);
+    if (!overflow && troot < tmax) {
+        if (lane == 0) {
+            if (A.root_ref & kLeafBit) { L.lref[0] = A.root_ref; L.lkey[0] = 0ull; }
+            else { L.ref[0][0] = A.root_ref; L.tn[0][0] = troot; L.key[0][0] = 0ull; }
+        }
+        if (A.root_ref & kLeafBit) nleaf = 1;
+        else ncur = 1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // A node of the level, read from the list, and its 64-byte record. The first 64 nodes of a
+    // level are read before the leaves met at the previous level are tested, so their loads
+    // are in flight together with the triangle loads.
+    struct NodeIn {
+        uint32_t ref;
+        float tn;
+        unsigned long long key;
+        float4 l0, l1, rb;
+        uint2 refs;
+    };
+    auto read_node = [&](uint32_t i) {
+        NodeIn x{0u, kInf, 0ull, make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f),
+                 make_float4(0.f, 0.f, 0.f, 0.f), make_uint2(kInvalid, kInvalid)};
+        if (i < ncur) {
+            x.ref = L.ref[cur][i];
+            x.tn = L.tn[cur][i];
+            x.key = L.key[cur][i];
+            const GNode* g = A.nodes + x.ref;
+            x.l0 = ld_g(&g->l0);
+            x.l1 = ld_g(&g->l1);
+            x.rb = ld_g(&g->r1);
+            x.refs = ld_g(reinterpret_cast<const uint2*>(&g->lref));
+        }
+        return x;
+    };
+    for (;;) {
+        const NodeIn first = read_node(lane);
+        // the leaves met at this level: every (leaf, triangle) pair, 64 per round
+        for (uint32_t lb = 0; lb < nleaf; lb += 64) {
+            const uint32_t j = lb + lane;
+            uint32_t c = 0, kb = 0;
+            unsigned long long lk = 0ull;
+            if (j < nleaf) {
+                const uint2 lr = leaf_range(A.leaves, L.lref[j]);
+                kb = lr.x;
+                c = lr.y;
+                lk = L.lkey[j];
+            }
+            const uint32_t inc = wave_incl_scan(c);
+            const uint32_t total = __builtin_amdgcn_readlane(inc, 63);
+            const uint32_t start = inc - c;
+            WaveCand mine{~0u, ~0ull, ~0u};
+            for (uint32_t W = 0; W < total; W += 64) {
+                wave_st(&L.mark[lane], kNoLane);
+                __builtin_amdgcn_wave_barrier();
+                if (c && start < W + 64u && start + c > W) wave_st(&L.mark[(start > W ? start : W) - W], (uint8_t)lane);
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t m = wave_ld(&L.mark[lane]);
+                const uint64_t starts = __ballot(m != kNoLane);
+                const int src = 63 - __clzll((long long)(starts & upto));
+                const uint32_t ow = __shfl(m, src);
+                const uint32_t okb = __shfl(kb, ow), ostart = __shfl(start, ow);
+                const unsigned long long okey = shfl64(lk, ow);
+                if (W + lane < total) {
+                    const uint32_t k = okb + (W + lane - ostart);
+                    float4 a, b;
+                    float cz;
+                    load_tri(A.tris + k, a, b, cz);
+                    const float t = tri_test_t(o, d, tmin, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y}, f3{b.z, b.w, cz});
+                    const WaveCand x{ord_t(t), okey, k};
+                    // only a hit that can still become (or tie) the best
+                    if (t < tmax && (t < bestT || (found && t == bestT)) && cand_less(x, mine)) mine = x;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            // the wave's smallest t; usually one lane holds it, else the full (t, key, index) order
+            const uint32_t mo = wave_min_u32(mine.ot);
+            if (mo != ~0u) {
+                const uint64_t tied = __ballot(mine.ot == mo);
+                WaveCand w;
+                if (__popcll(tied) == 1) {
+                    const int h = __builtin_ctzll(tied);
+                    w.ot = mo;
+                    w.key = ((unsigned long long)__builtin_amdgcn_readlane((uint32_t)(mine.key >> 32), h) << 32) |
+                            __builtin_amdgcn_readlane((uint32_t)mine.key, h);
+                    w.k = __builtin_amdgcn_readlane(mine.k, h);
+                } else {
+                    w = wave_min_cand(mine);
+                }
+                if (!found || cand_less(w, best)) {
+                    best = w;
+                    found = true;
+                    bestT = t_of_ord(w.ot);
+                }
+            }
+        }
+        if ((ANY && found) || ncur == 0 || overflow) break;
+        // the nodes of this level: both children of each, kept ones into the next level's lists
+        uint32_t nnext = 0;
+        nleaf = 0;
+        const unsigned long long bit = 1ull << (63u - depth);
+        for (uint32_t base = 0; base < ncur; base += 64) {
+            const uint32_t i = base + lane;
+            const NodeIn x = base == 0 ? first : read_node(i);
+            const bool act = i < ncur && keep(x.tn, x.key);
+            const unsigned long long key = x.key;
+            float tl = kInf, tr = kInf;
+            const uint2 refs = x.refs;
+            if (act) {
+                tl = slab(ox, oy, oz, ix, iy, iz, tmin, x.l0.x, x.l0.y, x.l0.z, x.l0.w, x.l1.x, x.l1.y, bestT);
+                tr = slab(ox, oy, oz, ix, iy, iz, tmin, x.l1.z, x.l1.w, x.rb.x, x.rb.y, x.rb.z, x.rb.w, bestT);
+            }
+            const bool lf = tl <= tr;
+            const uint32_t nref = lf ? refs.x : refs.y, fref = lf ? refs.y : refs.x;
+            const float ntn = lf ? tl : tr, ftn = lf ? tr : tl;
+            const unsigned long long nkey = key, fkey = key | bit;
+            const bool en = act && keep(ntn, nkey), ef = act && keep(ftn, fkey);
+            const bool in_ = en && !(nref & kLeafBit), if_ = ef && !(fref & kLeafBit);
+            const bool ln = en && (nref & kLeafBit), lf_ = ef && (fref & kLeafBit);
+            const uint64_t bin = __ballot(in_), bif = __ballot(if_), bln = __ballot(ln), blf = __ballot(lf_);
+            const uint32_t cin = (uint32_t)__popcll(bin), cif = (uint32_t)__popcll(bif);
+            const uint32_t cln = (uint32_t)__popcll(bln), clf = (uint32_t)__popcll(blf);
+            if (nnext + cin + cif > kWaveList || nleaf + cln + clf > kWaveList) {
+                overflow = true;
+                break;
+            }
+            const uint32_t nx = cur ^ 1u;
+            if (in_) {
+                const uint32_t q = nnext + (uint32_t)__popcll(bin & lt);
+                L.ref[nx][q] = nref; L.tn[nx][q] = ntn; L.key[nx][q] = nkey;
+            }
+            if (if_) {
+                const uint32_t q = nnext + cin + (uint32_t)__popcll(bif & lt);
+                L.ref[nx][q] = fref; L.tn[nx][q] = ftn; L.key[nx][q] = fkey;
+            }
+            if (ln) {
+                const uint32_t q = nleaf + (uint32_t)__popcll(bln & lt);
+                L.lref[q] = nref; L.lkey[q] = nkey;
+            }
+            if (lf_) {
+                const uint32_t q = nleaf + cln + (uint32_t)__popcll(blf & lt);
+                L.lref[q] = fref; L.lkey[q] = fkey;
+            }
+            nnext += cin + cif;
+            nleaf += cln + clf;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (overflow) break;
+        cur ^= 1u;
+        ncur = nnext;
+        ++depth;
+    }
+    if (overflow) {
+        // the usual walk on one lane
+        if (lane == 0) {
+            const TraceOut h = trace_ray<ANY, false, 12, kLoopSingle, kTraceTriBatch, false, kTreeBinary>(
+                A, o, d, tmin, tmax, lst, 0u, 0u, nullptr);
+            if (ANY) {
+                occl[r] = h.tri != kInvalid ? 1 : 0;
+            } else {
+                reinterpret_cast<float4*>(hits)[r] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
+                if (normals) {
+                    f3 nn{0.f, 0.f, 0.f};
+                    if (h.tri != kInvalid) nn = hit_normal(A, h.tri, h.u, h.v);
+                    normals[3 * r] = nn.x; normals[3 * r + 1] = nn.y; normals[3 * r + 2] = nn.z;
+                }
+            }
+        }
+        return;
+    }
+    if (lane != 0) return;
+    if (ANY) {
+        occl[r] = found ? 1 : 0;
+        return;
+    }
+    // exact t, u and v of the winner: the reference's test on the reference's inputs
+    float t = 0.f, u = 0.f, v = 0.f;
+    if (found) {
+        float4 a, b;
+        float cz;
+        load_tri(A.tris + best.k, a, b, cz);
+        t = tri_test(o, d, tmin, kInf, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y}, f3{b.z, b.w, cz}, u, v);
+    }
+    reinterpret_cast<float4*>(hits)[r] = make_float4(t, u, v, __uint_as_float(found ? best.k : kInvalid));
+    if (normals) {
+        f3 nn{0.f, 0.f, 0.f};
+        if (found) nn = hit_normal(A, best.k, u, v);
+        normals[3 * r] = nn.x; normals[3 * r + 1] = nn.y; normals[3 * r + 2] = nn.z;
+    }
+}
+
 __global__ void k_zero_work(uint32_t* w) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *w = 0u;
 }
@@ -265,6 +546,16 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
     // the persistent grid pays off only with rays for many waves; small calls (the per-ray
     // bvh::trace shim) take the while-while kernel, which gives the same results
     const bool persist = g_tuning.trace_loop == 2 && n >= kPersistMinRays;
+    // calls of a few rays: one wave per ray, breadth-first (not the counting launch, whose
+    // per-ray visit counts are those of the depth-first walk)
+    if (!COUNT && n <= kWaveRayMax && b->max_depth <= kWaveMaxDepth) {
+        const int pk = ext ? -1 : prof_start(s);
+        k_trace_wave<ANY><<<(uint32_t)n, 64, 0, s>>>(A, reinterpret_cast<const Ray32*>(rays),
+                                                    reinterpret_cast<Hit16*>(hits), normals, occl, b->max_depth);
+        prof_stop(pk, s);
+        RTX_HIP(hipGetLastError());
+        return RTX_OK;
+    }
     const bool scratch = (sort || persist) && !ext;
     if (scratch) {
         lk.lock();

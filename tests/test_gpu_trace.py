@@ -400,3 +400,62 @@ def test_host_trace_small_and_large_calls():
     one = _rays_for(v, 200, seed=3)
     hits = [g.trace(one[i:i + 1])["tri"][0] for i in range(200)]
     assert np.array_equal(np.array(hits, np.uint32), g.trace(one)["tri"])
+
+
+def _chunked_trace(g, rays, chunk):
+    """Closest hits, normals and any-hit flags of rays traced in calls of `chunk` rays (calls
+    of at most 256 rays take the one-wave-per-ray kernel)."""
+    import torch
+    rd = torch.from_numpy(rays).cuda()
+    hits = torch.zeros((len(rays), 4), dtype=torch.float32, device="cuda")
+    nrm = torch.zeros((len(rays), 3), dtype=torch.float32, device="cuda")
+    occ = torch.zeros(len(rays), dtype=torch.uint8, device="cuda")
+    for a in range(0, len(rays), chunk):
+        b = min(a + chunk, len(rays))
+        g.trace_closest_device(rd[a:b], hits[a:b], nrm[a:b])
+        g.trace_any_device(rd[a:b], occ[a:b])
+    torch.cuda.synchronize()
+    return hits.cpu().numpy(), nrm.cpu().numpy(), occ.cpu().numpy()
+
+
+@pytest.mark.parametrize("name", ["rand1", "rand17", "rand20000", "cornell", "test_traverse", "straddle", "dupcodes",
+                                  "degenerate", "grid"])
+def test_wave_per_ray_kernel_matches_oracle(suite, name):
+    """Calls of at most 256 rays walk each ray's tree breadth-first with one wave and keep
+    the reference's tie order by path keys: ids, t, u, v and normals equal the oracle's bit for
+    bit on random and edge-case rays (the grid scene's rays hit shared edges and vertices,
+    the straddle scene's duplicated triangles), in calls of 256, 37 and 1 rays."""
+    from scenes_util import edge_rays
+    v, t = suite[name]
+    g = rtx.build(v, t)
+    o = O.build(v, t)
+    rays = np.concatenate([_rays_for(v, 3000, seed=len(t) + 1), edge_rays(v, 3000, seed=5)])
+    ref = O.trace(o, rays, normals=True, nthreads=8)
+    hit = ref["tri"] != 0xFFFFFFFF
+    for chunk in (256, 37, 1):
+        print(f"[wave kernel] {name} chunk={chunk}", flush=True)
+        sel = slice(0, 300) if chunk == 1 else slice(0, len(rays))
+        h, nrm, occ = _chunked_trace(g, rays[sel], chunk)
+        r_tri, r_hit = ref["tri"][sel], hit[sel]
+        assert np.array_equal(h[:, 3].view(np.uint32), r_tri), f"{name}/{chunk}: ids differ on {(h[:, 3].view(np.uint32) != r_tri).sum()} rays"
+        for k, c in (("t", 0), ("u", 1), ("v", 2)):
+            assert np.array_equal(h[r_hit, c].view(np.uint32), ref[k][sel][r_hit].view(np.uint32)), (name, chunk, k)
+        assert np.array_equal(occ.astype(bool), r_hit), (name, chunk)
+        np.testing.assert_allclose(nrm, ref["normal"][sel], rtol=0, atol=1e-6)
+
+
+def test_wave_per_ray_kernel_1m_and_ties():
+    """The 1M-triangle scene: 4,096 incoherent and 4,096 coherent rays in calls of 256 rays
+    equal the oracle bit for bit, and equal one large call of the same rays."""
+    v, t = S.random_triangles(1_000_000, seed=1)
+    g = rtx.build(v, t)
+    o = O.build(v, t)
+    cam = S.camera_lookat((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), 45.0)
+    rays = np.concatenate([S.incoherent_rays(4096, seed=7), S.pinhole_rays(cam, 2048, 2048)[::1024]])
+    ref = O.trace(o, rays, nthreads=16)
+    h, _, occ = _chunked_trace(g, rays, 256)
+    big, _, occ_big = _chunked_trace(g, rays, len(rays))
+    assert np.array_equal(h[:, 3].view(np.uint32), ref["tri"])
+    assert np.array_equal(h[:, 0].view(np.uint32), ref["t"].view(np.uint32))
+    assert np.array_equal(h.view(np.uint32), big.view(np.uint32))
+    assert np.array_equal(occ, occ_big)
