@@ -196,7 +196,7 @@ def main():
         host = {"pcie_inclusive_mrays_s": round(n / host_s / 1e6, 2),
                 "single_ray_call_us": round((time.perf_counter() - t_h) / 200 * 1e6, 1)}
         # one call per ray over 1,000 different rays, and calls of 256 rays (calls of at most
-        # 256 rays take the one-wave-per-ray kernel)
+        # 16,384 rays take the one-wave-per-ray kernel)
         t_h = time.perf_counter()
         for i in range(1000):
             bvh.trace(inc_h[i:i + 1])

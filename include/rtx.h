@@ -107,9 +107,9 @@ int rtx_debug_scratch_peak(uint64_t* peak_bytes, int reset);
  * Closest hit for n rays: replaces bvh::trace (src/bvh/bvh_traverse.cpp:237-246,
  * traverse :91-178).  rays_d/hits_d device pointers; normals_d optional (3 floats per
  * ray, the interpolated shading normal of bvh_traverse.cpp:135-139).  On every trace
- * entry point n = 0 does nothing and the buffers may then be NULL.  Calls of at most 256
- * rays (closest and any hit) walk each ray's tree breadth-first with one wave per ray,
- * which keeps the reference's hit, including which triangle wins an exact-t tie. */
+ * entry point n = 0 does nothing and the buffers may then be NULL.  Calls of at most
+ * 16,384 rays (closest and any hit) walk each ray's tree breadth-first with one wave per
+ * ray, which keeps the reference's hit, including which triangle wins an exact-t tie. */
 int rtx_trace_closest(const rtx_bvh* bvh, const rtx_ray* rays_d, size_t n, rtx_hit* hits_d,
                       float* normals_d, void* stream);
 /* rtx_trace_closest without allocation, for callers that cannot allocate device memory

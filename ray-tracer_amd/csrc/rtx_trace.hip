@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(kTraceThreads, TREE == kTreeBinary ? 1 : 6) k_
     }
 }
 
-// One wave per ray, for calls of a few rays (the per-ray bvh::trace shim, small batches),
+// One wave per ray, for calls of few rays (the per-ray bvh::trace shim, small batches),
 // whose time is the ray's chain of dependent loads: the reference's depth-first walk
 // (bvh_traverse.cpp:91-178) visits about 78 nodes one after another. Here the wave walks
 // the tree breadth-first, a level per step, each lane testing the two children of one node
@@ -166,7 +166,10 @@ __global__ void __launch_bounds__(kTraceThreads, TREE == kTreeBinary ? 1 : 6) k_
 // 64 bits) sends the ray through the usual one-lane walk instead.
 constexpr uint32_t kWaveList = 512;       // nodes and leaves per level
 constexpr uint32_t kWaveMaxDepth = 60;    // tree depth the 64-bit path keys cover
-constexpr size_t kWaveRayMax = 256;       // calls of at most this many rays take the wave kernel
+// Calls of at most this many rays take the wave kernel: on the 1M-triangle tree it is ahead
+// of the persistent kernel up to about 25k rays (16,384 rays: 386 against 557 us per call;
+// 32,768: 714 against 589).
+constexpr size_t kWaveRayMax = 16384;
 struct WaveLds {
     uint32_t ref[2][kWaveList];
     float tn[2][kWaveList];
@@ -219,8 +222,12 @@ template <bool ANY>
 __global__ void __launch_bounds__(64) k_trace_wave(TraceArgs A, const Ray32* __restrict__ rays,
                                                    Hit16* __restrict__ hits, float* __restrict__ normals,
                                                    uint8_t* __restrict__ occl, uint32_t max_depth) {
-    __shared__ WaveLds L;
-    __shared__ uint2 lst[12][kTraceThreads];   // the one-lane walk's stack, when it is needed
+    // the lists, and in the same bytes the one-lane walk's stack when that is needed (it uses
+    // column 0 of the 12-entry ring once the lists are abandoned)
+    constexpr size_t kStackBytes = sizeof(uint2) * 12 * kTraceThreads;
+    __shared__ __align__(16) unsigned char lds[sizeof(WaveLds) > kStackBytes ? sizeof(WaveLds) : kStackBytes];
+    WaveLds& L = *reinterpret_cast<WaveLds*>(lds);
+    auto lst = reinterpret_cast<uint2(*)[kTraceThreads]>(lds);
     const uint32_t lane = __lane_id(), r = blockIdx.x;
     const uint64_t lt = (1ull << lane) - 1ull, upto = (2ull << lane) - 1ull;
     const float4 r0 = reinterpret_cast<const float4*>(rays)[2 * r];

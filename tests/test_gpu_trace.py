@@ -404,7 +404,7 @@ def test_host_trace_small_and_large_calls():
 
 def _chunked_trace(g, rays, chunk):
     """Closest hits, normals and any-hit flags of rays traced in calls of `chunk` rays (calls
-    of at most 256 rays take the one-wave-per-ray kernel)."""
+    of at most 16,384 rays take the one-wave-per-ray kernel)."""
     import torch
     rd = torch.from_numpy(rays).cuda()
     hits = torch.zeros((len(rays), 4), dtype=torch.float32, device="cuda")
@@ -421,7 +421,7 @@ def _chunked_trace(g, rays, chunk):
 @pytest.mark.parametrize("name", ["rand1", "rand17", "rand20000", "cornell", "test_traverse", "straddle", "dupcodes",
                                   "degenerate", "grid"])
 def test_wave_per_ray_kernel_matches_oracle(suite, name):
-    """Calls of at most 256 rays walk each ray's tree breadth-first with one wave and keep
+    """Calls of at most 16,384 rays walk each ray's tree breadth-first with one wave and keep
     the reference's tie order by path keys: ids, t, u, v and normals equal the oracle's bit for
     bit on random and edge-case rays (the grid scene's rays hit shared edges and vertices,
     the straddle scene's duplicated triangles), in calls of 256, 37 and 1 rays."""
@@ -445,13 +445,14 @@ def test_wave_per_ray_kernel_matches_oracle(suite, name):
 
 
 def test_wave_per_ray_kernel_1m_and_ties():
-    """The 1M-triangle scene: 4,096 incoherent and 4,096 coherent rays in calls of 256 rays
-    equal the oracle bit for bit, and equal one large call of the same rays."""
+    """The 1M-triangle scene: 16,384 incoherent and 4,096 coherent rays in calls of 256 rays
+    equal the oracle bit for bit, and equal one large call of the same rays (the persistent
+    kernel)."""
     v, t = S.random_triangles(1_000_000, seed=1)
     g = rtx.build(v, t)
     o = O.build(v, t)
     cam = S.camera_lookat((0.5, 0.5, -1.5), (0.5, 0.5, 0.5), 45.0)
-    rays = np.concatenate([S.incoherent_rays(4096, seed=7), S.pinhole_rays(cam, 2048, 2048)[::1024]])
+    rays = np.concatenate([S.incoherent_rays(16384, seed=7), S.pinhole_rays(cam, 2048, 2048)[::1024]])
     ref = O.trace(o, rays, nthreads=16)
     h, _, occ = _chunked_trace(g, rays, 256)
     big, _, occ_big = _chunked_trace(g, rays, len(rays))
