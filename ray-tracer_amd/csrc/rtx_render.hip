@@ -377,8 +377,10 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
                 }
             }
         }
-        // a started path that missed the root is written at the next refill or at the end
-        w.template node_phase<false>(args(), lst, tid, wait);
+        // a started path that missed the root is written at the next refill or at the end;
+        // once the queue is empty the leaf-phase wait scales with the lanes still tracing
+        const uint32_t wt = more ? wait : (wait * (uint32_t)__popcll(__ballot(w.alive()))) >> 6;
+        w.template node_phase<false>(args(), lst, tid, wt);
         if (LOOP == kLoopCoop) {   // all lanes active: the loop is wave-uniform
             const bool at = w.alive() && (w.ref & kLeafBit);
             w.template leaf_coop<false, false, SC != kSceneUni>(args(), coop + tid / 64);

@@ -135,8 +135,12 @@ __global__ void __launch_bounds__(kTraceThreads, TREE == kTreeBinary ? 1 : 6) k_
                 }
             }
         }
-        // a started ray that missed the root is written at the next refill or at the end
-        w.template node_phase<COUNT, TREE>(A, lst, tid, A.wait);
+        // A started ray that missed the root is written at the next refill or at the end.
+        // Once the queue is empty, the leaf-phase wait scales with the lanes still tracing: a
+        // wave's last rays no longer stop each other's node phases at every leaf one of them
+        // reaches (4M incoherent rays: about 1% faster).
+        const uint32_t wait = more ? A.wait : (A.wait * (uint32_t)__popcll(__ballot(w.alive()))) >> 6;
+        w.template node_phase<COUNT, TREE>(A, lst, tid, wait);
         if (LOOP == kLoopCoop) {   // all lanes active: the loop is wave-uniform
             const bool at = w.alive() && (w.ref & kLeafBit);
             if (w.template leaf_coop<ANY, COUNT, false>(A, coop + tid / 64)) w.kill();
