@@ -460,3 +460,39 @@ def test_wave_per_ray_kernel_1m_and_ties():
     assert np.array_equal(h[:, 0].view(np.uint32), ref["t"].view(np.uint32))
     assert np.array_equal(h.view(np.uint32), big.view(np.uint32))
     assert np.array_equal(occ, occ_big)
+
+
+def test_wave_per_ray_kernel_falls_back_on_wide_levels():
+    """A ray along a stack of 40,000 parallel squares meets every node's box: its
+    breadth-first walk reaches 1,024 internal nodes at one level (counted on the oracle's
+    tree) before any leaf, more than the 512-entry lists hold, so the ray takes the one-lane walk:
+    the hits still equal the oracle's (rays along the stack and ordinary ones, in small and
+    large calls)."""
+    n = 40000
+    z = np.linspace(0.0, 1.0, n, dtype=np.float32)
+    quad = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    verts = np.zeros((4 * n, 6), np.float32)
+    verts[:, 0:2] = np.tile(quad, (n, 1))
+    verts[:, 2] = np.repeat(z, 4)
+    verts[:, 5] = -1.0
+    base = 4 * np.arange(n, dtype=np.uint32)
+    tris = np.concatenate([np.stack([base, base + 2, base + 1], 1), np.stack([base, base + 3, base + 2], 1)])
+    g = rtx.build(verts, tris)
+    o = O.build(verts, tris)
+    rng = np.random.default_rng(3)
+    along = np.zeros((64, 8), np.float32)
+    along[:, 0:2] = rng.uniform(0.05, 0.95, (64, 2))
+    along[:, 2] = 1.5
+    along[:, 6] = -1.0          # straight down the stack: every square is hit
+    along[:, 7] = np.inf
+    rays = np.concatenate([along, _rays_for(verts, 1000, seed=4)])
+    ref = O.trace(o, rays, nthreads=8)
+    assert (ref["tri"][:64] != 0xFFFFFFFF).all()
+    for chunk in (len(rays), 64, 1):
+        print(f"[wave fallback] chunk={chunk}", flush=True)
+        sel = slice(0, 128) if chunk == 1 else slice(0, len(rays))
+        h, _, occ = _chunked_trace(g, rays[sel], chunk)
+        assert np.array_equal(h[:, 3].view(np.uint32), ref["tri"][sel]), chunk
+        hit = ref["tri"][sel] != 0xFFFFFFFF
+        assert np.array_equal(h[hit, 0].view(np.uint32), ref["t"][sel][hit].view(np.uint32)), chunk
+        assert np.array_equal(occ.astype(bool), hit), chunk
