@@ -259,7 +259,9 @@ void rtx_batch_free(rtx_batch* batch);
  *   trace_loop         0 = one node/leaf loop, 1 = while-while (triangle tests run with the
  *                      wave mostly converged), 2 = while-while in a persistent grid of
  *                      trace_grid_blocks blocks where idle lanes fetch new rays (calls
- *                      with fewer than 4096 rays run loop 1);
+ *                      with fewer than 4096 rays run loop 1). Closest- and any-hit calls
+ *                      of at most 16,384 rays take the one-wave-per-ray walk instead,
+ *                      whatever this field (rtx_trace_counts never does);
  *   ray_sort           1 = sort rays (>= 65536 of them) by a Morton key of their origins
  *                      and trace in that order, writing results back at each ray's index.
  *                      The sort's scratch lives on the BVH handle and grows to the largest

@@ -554,8 +554,8 @@ int launch(const rtx_bvh* b, const rtx_ray* rays, size_t n, rtx_hit* hits, float
     // The handle's ray counter and sort buffers: a call that uses them waits (on its own
     // stream) for the previous such call, whatever stream that used.
     std::unique_lock<std::mutex> lk(mb->scratch_mu, std::defer_lock);
-    // the persistent grid pays off only with rays for many waves; small calls (the per-ray
-    // bvh::trace shim) take the while-while kernel, which gives the same results
+    // the persistent grid pays off only with rays for many waves; smaller counting calls take
+    // the while-while kernel, which gives the same results
     const bool persist = g_tuning.trace_loop == 2 && n >= kPersistMinRays;
     // calls of a few rays: one wave per ray, breadth-first (not the counting launch, whose
     // per-ray visit counts are those of the depth-first walk)
