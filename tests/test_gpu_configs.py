@@ -88,6 +88,34 @@ def test_terrain_render_matches_oracle(terrain):
     del torch
 
 
+def test_terrain_traces_wave_per_ray_and_persistent(terrain):
+    """The terrain's 28-level tree: camera rays and rays scattered over the terrain box give
+    the same hits in one large call (sorted persistent traversal) and in calls of 4,096 rays
+    (one wave per ray), and a subset equals the oracle's."""
+    import torch
+    v, t, uv, g, o = terrain
+    cam, _, _ = S.terrain_setup()
+    cr = S.pinhole_rays(cam, 160, 128)
+    sc = S.incoherent_rays(20480, seed=12)
+    lo, hi = v[:, :3].min(0), v[:, :3].max(0)
+    sc[:, :3] = lo + (hi - lo) * sc[:, :3]
+    rays = np.concatenate([cr, sc])
+    rd = torch.from_numpy(rays).cuda()
+    big = torch.zeros((len(rays), 4), device="cuda")
+    small = torch.zeros((len(rays), 4), device="cuda")
+    g.trace_closest_device(rd, big)
+    for a in range(0, len(rays), 4096):
+        g.trace_closest_device(rd[a:a + 4096], small[a:a + 4096])
+    torch.cuda.synchronize()
+    hb, hs = big.cpu().numpy(), small.cpu().numpy()
+    assert np.array_equal(hb.view(np.uint32), hs.view(np.uint32))
+    sel = np.r_[0:1000, len(cr):len(cr) + 1000]
+    ref = O.trace(o, rays[sel], nthreads=16)
+    assert np.array_equal(hs[sel, 3].view(np.uint32), ref["tri"])
+    assert np.array_equal(hs[sel, 0].view(np.uint32), ref["t"].view(np.uint32))
+    assert (ref["tri"] != 0xFFFFFFFF).mean() > 0.5
+
+
 # ---- batched scenes -------------------------------------------------------------------
 
 @pytest.fixture(scope="module")
