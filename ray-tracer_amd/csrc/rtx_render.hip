@@ -189,15 +189,22 @@ __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* _
         const int x = (int)(pixel % (uint32_t)S.W), y = (int)(pixel / (uint32_t)S.W);
         f3 o, d;
         primary_ray(S, x, y, sample01(S.seed, pixel, sample, 0), sample01(S.seed, pixel, sample, 1), o, d);
-        P.ray0[i] = make_float4(o.x, o.y, o.z, 1e-3f);
         P.ray1[i] = make_float4(d.x, d.y, d.z, kInf);
-        P.tp[i] = make_float4(1.f, 1.f, 1.f, 0.f);
-        P.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         P.sid[i] = s;
         P.pix[i] = pixel;
         P.smp[i] = sample;
-        P.q[0][i] = i;
     }
+}
+
+// What bounce 0 does not read from the path state, because it is the same for every camera
+// ray: the origin and tmin (the scene's camera position and 1e-3, as primary_ray gives
+// them), the throughput (1, 1, 1), the radiance (0, 0, 0) and the queue (the identity: the
+// batch's paths in order). k_camera writes only the direction, scene, pixel and sample.
+__device__ __forceinline__ float4 ray_origin(const PathState& P, const SceneDesc& S, uint32_t i, int bounce) {
+    return bounce == 0 ? make_float4(S.pos[0], S.pos[1], S.pos[2], 1e-3f) : P.ray0[i];
+}
+__device__ __forceinline__ uint32_t queued_path(const uint32_t* Q, uint32_t k, int bounce) {
+    return bounce == 0 ? k : Q[k];
 }
 
 // Path records of the primal render (one scene), the replay's only input per path, in two
@@ -268,13 +275,13 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
         const SceneDesc* S = scenes;
         uint32_t sid = 0;
         if (in) {
-            i = Q[k];
-            r0 = P.ray0[i];
+            i = queued_path(Q, k, bounce);
             r1 = P.ray1[i];
             if (SC != kSceneUni) {
                 sid = P.sid[i];
                 S = scenes + sid;
             }
+            r0 = ray_origin(P, *S, i, bounce);
         }
         TraceArgs A = SC == kSceneUni ? scenes->T : S->T;
         if (SC == kSceneLds) {
@@ -367,12 +374,12 @@ __global__ void __launch_bounds__(kTraceThreads, LOOP == kLoopCoop ? 7 : 8)
             if (!w.alive()) {
                 const uint32_t k = base + (uint32_t)__popcll(idle & lt);
                 if (k < rend) {
-                    id = Q[k];
+                    id = queued_path(Q, k, bounce);
                     if (SC != kSceneUni) {
                         sid = P.sid[id];
                         S = scenes + sid;
                     }
-                    const float4 r0 = P.ray0[id], r1 = P.ray1[id];
+                    const float4 r0 = ray_origin(P, *S, id, bounce), r1 = P.ray1[id];
                     w.start(SC == kSceneUni ? TU : S->T, f3{r0.x, r0.y, r0.z}, f3{r1.x, r1.y, r1.z}, r0.w, r1.w);
                 }
             }
@@ -522,12 +529,16 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
         uint32_t i = 0, gidx = 0;
         f3 gA{0.f, 0.f, 0.f}, gE{0.f, 0.f, 0.f};
         if (k < n) {
-            i = Q[k];
+            i = queued_path(Q, k, bounce);
             const SceneDesc& S = scenes[UNI ? 0u : P.sid[i]];
             const float4 h = P.hit[i];
             if (!ADJ && G.hit_out) G.hit_out[G.rbase + i] = h;   // bounce 0: queue 0 is the identity
             const uint32_t tri = __float_as_uint(h.w);
-            float4 tp = P.tp[i], L = P.L[i];
+            float4 tp = make_float4(1.f, 1.f, 1.f, 0.f), L = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (bounce > 0) {
+                tp = P.tp[i];
+                L = P.L[i];
+            }
             const float4 r1 = P.ray1[i];
             const f3 d{r1.x, r1.y, r1.z};
             if (tri == kInvalid) {
@@ -579,7 +590,7 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
                         else tp = make_float4(tp.x / pr, tp.y / pr, tp.z / pr, 0.f);
                     }
                     if (cont) {
-                        const float4 r0 = P.ray0[i];
+                        const float4 r0 = ray_origin(P, S, i, bounce);
                         const f3 o = add(f3{r0.x, r0.y, r0.z}, scl(h.x, d));
                         P.ray0[i] = make_float4(o.x, o.y, o.z, 1e-4f);
                         P.ray1[i] = make_float4(wd.x, wd.y, wd.z, kInf);
