@@ -53,11 +53,10 @@ struct PathState {
     float4* tp;
     float4* L;
     uint32_t* sid;
-    uint32_t* pix;
-    uint32_t* smp;
     uint32_t* q[2];
     uint32_t* cnt;     // [b]: input queue length of bounce b; [ncnt/2 + b]: persistent-trace counter of bounce b
     uint32_t B;
+    uint64_t g0;       // path number of the batch's first path (its paths are g0, g0 + 1, ...)
 };
 
 constexpr int kShadeForward = 0, kShadeReplay = 1;
@@ -191,9 +190,16 @@ __global__ void k_camera(const SceneDesc* __restrict__ scenes, const uint64_t* _
         primary_ray(S, x, y, sample01(S.seed, pixel, sample, 0), sample01(S.seed, pixel, sample, 1), o, d);
         P.ray1[i] = make_float4(d.x, d.y, d.z, kInf);
         P.sid[i] = s;
-        P.pix[i] = pixel;
-        P.smp[i] = sample;
     }
+}
+
+// Pixel and sample of batch path i, which belongs to scene S: as k_camera derives them from
+// the path's number, so they are not stored.
+__device__ __forceinline__ void path_pixel_sample(const PathState& P, const SceneDesc& S, uint32_t i, uint32_t& pixel,
+                                                  uint32_t& sample) {
+    const uint64_t q = P.g0 + i - S.path_begin;
+    sample = (uint32_t)(q / S.npix);
+    pixel = tile_pixel((uint32_t)(q % S.npix), (uint32_t)S.W, (uint32_t)S.H);
 }
 
 // What bounce 0 does not read from the path state, because it is the same for every camera
@@ -559,7 +565,8 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
                 if (alb && (more || ADJ)) {
                     ax = ld_g(alb + 3 * aidx); ay = ld_g(alb + 3 * aidx + 1); az = ld_g(alb + 3 * aidx + 2);
                 }
-                const uint32_t px = P.pix[i];
+                uint32_t px, sm;
+                path_pixel_sample(P, S, i, px, sm);
                 if (ADJ) {
                     has = true;
                     gidx = aidx;
@@ -575,7 +582,6 @@ __global__ void k_shade(const SceneDesc* __restrict__ scenes, PathState P, int q
                     gE = f3{delta.x * tpin.x, delta.y * tpin.y, delta.z * tpin.z};
                 }
                 if (more) {
-                    const uint32_t sm = P.smp[i];
                     const float u1 = sample01(S.seed, px, sm, 2 + 3 * bounce);
                     const float u2 = sample01(S.seed, px, sm, 3 + 3 * bounce);
                     const f3 nrm = hit_normal(S.T, tri, h.y, h.z);
@@ -633,7 +639,8 @@ __global__ void k_accum(const SceneDesc* __restrict__ scenes, const uint64_t* __
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         const uint32_t s = P.sid[i];
         const SceneDesc& S = scenes[s];
-        const uint32_t sample = P.smp[i], pixel = P.pix[i];
+        uint32_t pixel, sample;
+        path_pixel_sample(P, S, i, pixel, sample);
         if (sample != 0 && i >= S.npix) continue;   // an earlier sample of this pixel is in the batch
         float* img = S.image + 3 * (size_t)pixel;
         float ax = img[0], ay = img[1], az = img[2];
@@ -806,7 +813,7 @@ int Batch::setup(uint32_t batch_paths, hipStream_t st, bool adjoint) {
     d_off = alloc<uint64_t>(ns + 1);
     P.ray0 = alloc<float4>(B); P.ray1 = alloc<float4>(B); P.hit = alloc<float4>(B);
     P.tp = alloc<float4>(B); P.L = alloc<float4>(B);
-    P.sid = alloc<uint32_t>(B); P.pix = alloc<uint32_t>(B); P.smp = alloc<uint32_t>(B);
+    P.sid = alloc<uint32_t>(B);
     P.q[0] = alloc<uint32_t>(B); P.q[1] = alloc<uint32_t>(B);
     ncnt = 2 * (uint32_t)(max_b + 2);
     P.cnt = alloc<uint32_t>(ncnt);
@@ -824,7 +831,7 @@ int Batch::setup(uint32_t batch_paths, hipStream_t st, bool adjoint) {
         recb = alloc<float4>(2ull * B);
         if (!recb) return fail(RTX_E_HIP, "device allocation failed on render setup");
     }
-    if (!d_scenes || !d_off || !P.ray0 || !P.ray1 || !P.hit || !P.tp || !P.L || !P.sid || !P.pix || !P.smp ||
+    if (!d_scenes || !d_off || !P.ray0 || !P.ray1 || !P.hit || !P.tp || !P.L || !P.sid ||
         !P.q[0] || !P.q[1] || !P.cnt)
         return fail(RTX_E_HIP, "device allocation failed on render setup");
     for (int s0 = 0; s0 < ns; s0 += kPutDescs) {
@@ -861,6 +868,7 @@ void Batch::enqueue_range(int s_first, int s_last, hipStream_t st) {
     const SceneDesc* sd = one ? d_scenes + s_first : d_scenes;
     for (uint64_t g0 = off[s_first]; g0 < off[s_last]; g0 += B) {
         const uint32_t cnt = (uint32_t)std::min<uint64_t>(B, off[s_last] - g0);
+        P.g0 = g0;   // launches take P by value: this batch's kernels see this batch's g0
         const uint32_t cgrid = std::min<uint32_t>((cnt + 255) / 256, 65535);
         // One wavefront over the batch: camera rays, then max_bounce + 1 trace/shade rounds.
         // The replay (kShadeReplay) scatters each bounce's gradients as it shades it, reads the
