@@ -281,7 +281,7 @@ void rtx_batch_free(rtx_batch* batch);
  *                      between triangles of different leaves possibly another of the tied
  *                      triangles.
  * Renderer (rtx_render, rtx_render_adjoint, rtx_batch_*):
- *   batch_paths        paths in flight per wavefront batch (>= 1024; default 2^26, 100
+ *   batch_paths        paths in flight per wavefront batch (>= 1024; default 2^26, 92
  *                      bytes of path state per path, 32 more for the adjoint's records);
  *   grid_blocks        blocks of the wavefront trace and shade kernels;
  *   shade_threads      threads per shade block (multiple of 64, <= 1024);
