@@ -205,7 +205,7 @@ __device__ __forceinline__ void path_pixel_sample(const PathState& P, const Scen
 // What bounce 0 does not read from the path state, because it is the same for every camera
 // ray: the origin and tmin (the scene's camera position and 1e-3, as primary_ray gives
 // them), the throughput (1, 1, 1), the radiance (0, 0, 0) and the queue (the identity: the
-// batch's paths in order). k_camera writes only the direction, scene, pixel and sample.
+// batch's paths in order). k_camera writes only the direction and the scene.
 __device__ __forceinline__ float4 ray_origin(const PathState& P, const SceneDesc& S, uint32_t i, int bounce) {
     return bounce == 0 ? make_float4(S.pos[0], S.pos[1], S.pos[2], 1e-3f) : P.ray0[i];
 }
